@@ -1,0 +1,198 @@
+"""ctypes binding of libcpuvh_amd.so (include/cpuvh_amd.h).
+
+Python-side convenience over the C ABI of the MI355X viscous-hydro step:
+tests and bench.py drive the product through exactly the entry points a
+C/C++ caller uses.  Loading fails loudly when the built library is missing;
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpuvh_amd.so")
+
+FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn",
+               "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
+               "up_t", "up_x", "up_y", "up_n", "regfac"]
+FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
+
+VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NOMEM, VH_ERR_NAN, VH_ERR_STATE, VH_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
+EOS_CONFORMAL, EOS_TABLE = 0, 1
+
+# The exported entry points, as declared in include/cpuvh_amd.h.
+EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
+           "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag",
+           "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
+           "vh_status_string", "vh_build_info"]
+
+
+class VhConfig(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("dt", ctypes.c_double), ("t0", ctypes.c_double), ("etabar", ctypes.c_double),
+                ("T_reg_GeV", ctypes.c_double), ("bulk", ctypes.c_int), ("eos_kind", ctypes.c_int),
+                ("eos_table", ctypes.c_void_p),
+                ("etas_Tk_GeV", ctypes.c_double), ("etas_slope_lo", ctypes.c_double),
+                ("etas_slope_hi", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class VhError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: status {status} ({_lib().vh_status_string(status).decode()})")
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        D = ctypes.c_double
+        L.vh_create.argtypes = [ctypes.POINTER(VhConfig), ctypes.POINTER(P)]
+        L.vh_destroy.argtypes = [P]
+        L.vh_upload_ic.argtypes = [P] + [P] * 8
+        L.vh_init_conserved.argtypes = [P, D]
+        L.vh_step.argtypes = [P, D]
+        L.vh_steps.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.vh_sync.argtypes = [P]
+        L.vh_nan_info.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]
+        L.vh_download.argtypes = [P, ctypes.c_int, P]
+        L.vh_upload_field.argtypes = [P, ctypes.c_int, P]
+        L.vh_count_above.argtypes = [P, D, ctypes.POINTER(ctypes.c_int64)]
+        L.vh_fo_flag.argtypes = [P, D, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        for f in ("vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature"):
+            getattr(L, f).argtypes = [P, D]
+            getattr(L, f).restype = D
+        L.vh_last_kernel_ms.argtypes = [P]
+        L.vh_last_kernel_ms.restype = D
+        L.vh_status_string.argtypes = [ctypes.c_int]
+        L.vh_status_string.restype = ctypes.c_char_p
+        L.vh_build_info.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def load_eos_table(path):
+    raw = np.fromfile(path, dtype="<f8")
+    if raw.size != 400000:
+        raise ValueError(f"{path}: expected 100000 rows of (e, P, s, T)")
+    return np.ascontiguousarray(raw)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Hydro:
+    """One lattice on one GPU (a vh_handle)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV=0.0, bulk=False,
+                 eos_table=None, etas_Tk_GeV=0.154, etas_slope_lo=0.0, etas_slope_hi=0.0, device=0):
+        L = _lib()
+        self.shape = (nz, ny, nx)
+        self.N = nx * ny * nz
+        self.nc = 15 if bulk else 14
+        self._table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)
+        cfg = VhConfig(nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV, int(bool(bulk)),
+                       EOS_CONFORMAL if self._table is None else EOS_TABLE,
+                       None if self._table is None else self._table.ctypes.data,
+                       etas_Tk_GeV, etas_slope_lo, etas_slope_hi, device)
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        self._check(L.vh_create(ctypes.byref(cfg), ctypes.byref(h)), "vh_create")
+        self.h = h
+
+    @staticmethod
+    def _check(st, what):
+        if st != VH_OK:
+            raise VhError(st, what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().vh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_ic(self, e, p, ut, ux, uy, un, pi10, Pi=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (e, p, ut, ux, uy, un)]
+        pi = np.ascontiguousarray(pi10, dtype=np.float64).ravel()
+        if any(a.size != self.N for a in arrs) or pi.size != 10 * self.N:
+            raise ValueError("initial-condition arrays do not match the lattice")
+        Pa = None if Pi is None else np.ascontiguousarray(Pi, dtype=np.float64).ravel()
+        self._check(_lib().vh_upload_ic(self.h, *[_ptr(a) for a in arrs], _ptr(pi), _ptr(Pa)), "vh_upload_ic")
+
+    def init_conserved(self, t):
+        self._check(_lib().vh_init_conserved(self.h, t), "vh_init_conserved")
+
+    def step(self, t):
+        self._check(_lib().vh_step(self.h, t), "vh_step")
+
+    def steps(self, n_first, count):
+        self._check(_lib().vh_steps(self.h, n_first, count), "vh_steps")
+
+    def sync(self):
+        self._check(_lib().vh_sync(self.h), "vh_sync")
+
+    def nan_info(self):
+        site = ctypes.c_int()
+        cell = ctypes.c_int64()
+        self._check(_lib().vh_nan_info(self.h, ctypes.byref(site), ctypes.byref(cell)), "vh_nan_info")
+        return site.value, cell.value
+
+    def get(self, field):
+        f = FIELD[field] if isinstance(field, str) else field
+        out = np.empty(self.N, dtype=np.float64)
+        self._check(_lib().vh_download(self.h, f, _ptr(out)), "vh_download")
+        return out
+
+    def set(self, field, values):
+        f = FIELD[field] if isinstance(field, str) else field
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self._check(_lib().vh_upload_field(self.h, f, _ptr(a)), "vh_upload_field")
+
+    def count_above(self, eF):
+        c = ctypes.c_int64()
+        self._check(_lib().vh_count_above(self.h, eF, ctypes.byref(c)), "vh_count_above")
+        return c.value
+
+    def fo_flag_ids(self, eF, cap=None, with_corners=False):
+        nx, ny, nz = self.cfg.nx, self.cfg.ny, self.cfg.nz
+        m = (nx - 1) * (ny - 1) * (nz - 1 if nz > 1 else 1)
+        cap = m if cap is None else cap
+        ids = np.empty(max(cap, 1), dtype=np.int64)
+        per = 16 if nz > 1 else 8
+        corners = np.empty(max(cap, 1) * per, dtype=np.float64) if with_corners else None
+        n = ctypes.c_int64()
+        self._check(_lib().vh_fo_flag(self.h, eF, _ptr(ids), _ptr(corners), cap, ctypes.byref(n)), "vh_fo_flag")
+        ids = ids[:n.value].copy()
+        if with_corners:
+            return ids, corners[:n.value * per].reshape(n.value, per).copy()
+        return ids
+
+    def eos_energy_density(self, T):
+        return _lib().vh_eos_energy_density(self.h, T)
+
+    def eos_pressure(self, e):
+        return _lib().vh_eos_pressure(self.h, e)
+
+    def eos_temperature(self, e):
+        return _lib().vh_eos_temperature(self.h, e)
+
+    def last_kernel_ms(self):
+        return _lib().vh_last_kernel_ms(self.h)
+
+
+def build_info():
+    return _lib().vh_build_info().decode()

@@ -1,0 +1,223 @@
+// Device-side physics of the viscous-hydro step (gfx950, FP64).
+//
+// Arithmetic is written to round exactly like cpu-vh's compiled C++: every
+// expression keeps the reference's operand order, the file is compiled with
+// -ffp-contract=off (no fused multiply-add unless written as fma()), and the
+// reference's single-precision islands (sqrtf/fabsf/powf on double operands)
+// are reproduced bit for bit:
+//   * sqrtf(x)   == (float)sqrt((double)x): the double square root of a float
+//                   rounded back to float is the correctly rounded float root;
+//   * powf(x,.25f) (glibc) == the correctly rounded fourth root, except on a
+//                   list of inputs where glibc's result differs; that list is
+//                   computed at start-up from the host's own glibc and looked
+//                   up here (eos_fourth_root).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vh {
+
+constexpr double HBARC = 0.197326938;   // rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:886
+constexpr double EOS_FACTOR = 13.8997;  // rhic/include/EquationOfState.h:19
+constexpr double THETA = 1.1;           // rhic/include/FluxLimiter.h:13
+constexpr int EOS_LEN = 100000;         // rhic/src/EquationOfState.cpp:295
+constexpr int MAX_ITERS_NR = 10000;     // rhic/src/EnergyMomentumTensor.cpp:18-20
+constexpr int MAX_ITERS_BI = 1000000;
+constexpr double ACC = 1.0e-3;
+constexpr double delta_pipi = 1.33333;  // rhic/src/SourceTerms.cpp:50-53
+constexpr double tau_pipi = 1.42857;
+constexpr double delta_PiPi = 0.666667;
+constexpr double lambda_piPi = 1.2;
+
+enum : int { NAN_NONE = 0, NAN_ENERGY = 1, NAN_ROOT = 2, NAN_REG_SHEAR = 3, NAN_REG_BULK = 4 };
+
+struct EosDev {
+    int kind;        // 0 conformal, 1 table
+    int cs2_const;   // table: host verified that the reference's c_s^2 is 1/3 for every e
+    double e_min, e_spacing, e_max;
+    const double *p_tb;
+    const double *T_tb;
+    // glibc powf(x, 0.25f) exceptions: sorted (x bits, result bits) pairs,
+    // bucketed by (xbits - exc_lo) >> exc_shift.
+    const uint2 *exc;
+    const uint32_t *exc_bucket;
+    uint32_t exc_lo, exc_hi;
+    int exc_shift;
+};
+
+__device__ __forceinline__ float f32_sqrt(float x) { return (float)__builtin_sqrt((double)x); }
+__device__ __forceinline__ float f32_fabs(float x) { return __builtin_fabsf(x); }
+
+// glibc powf(x, 0.25f) for the conformal temperature (EquationOfState.cpp:196).
+__device__ __forceinline__ float eos_fourth_root(const EosDev &E, float x) {
+    float r = (float)__builtin_sqrt(__builtin_sqrt((double)x));
+    uint32_t xb = __float_as_uint(x);
+    if (xb >= E.exc_lo && xb < E.exc_hi) {
+        uint32_t b = (xb - E.exc_lo) >> E.exc_shift;
+        uint32_t lo = E.exc_bucket[b], hi = E.exc_bucket[b + 1];
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            uint32_t k = E.exc[mid].x;
+            if (k < xb) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < E.exc_bucket[b + 1] && E.exc[lo].x == xb) r = __uint_as_float(E.exc[lo].y);
+    }
+    return r;
+}
+
+// rhic/src/EquationOfState.cpp:329-360
+__device__ __forceinline__ double eos_interp(const EosDev &E, double e, const double *__restrict__ tb) {
+    const double e0 = E.e_min, de = E.e_spacing;
+    int idx = (int)((e - e0) / de);
+    idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
+    idx = idx > 0 ? idx : 0;
+    const double frac = (e - (idx * de + e0)) / de;
+    double t1 = tb[idx], t2 = tb[idx + 1];
+    return t1 * (1. - frac) + t2 * frac;
+}
+
+template <bool TABLE>
+__device__ __forceinline__ double eos_pressure(const EosDev &E, double e) {
+    if constexpr (TABLE) {
+        double f = eos_interp(E, e, E.p_tb);
+        return 1.e-15 < f ? f : 1.e-15;  // std::max(1.e-15, f)
+    } else {
+        return e / 3.;
+    }
+}
+
+template <bool TABLE>
+__device__ __forceinline__ double eos_cs2(const EosDev &E, double e) {
+    if constexpr (TABLE) {
+        if (E.cs2_const) return 1. / 3.;
+        double v_min = 0.01, v_max = 1. / 3.;
+        double delta_e = E.e_spacing;
+        double p = eos_pressure<true>(E, e);
+        double p_r = eos_pressure<true>(E, e + delta_e);
+        double delta_p = p_r - p;
+        double v = delta_e / delta_p;
+        v = v_max < v ? v_max : v;  // std::min(v_max, v)
+        v = v_min < v ? v : v_min;  // std::max(v_min, .)
+        return v;
+    } else {
+        return 1. / 3.;
+    }
+}
+
+template <bool TABLE>
+__device__ __forceinline__ double eos_temperature(const EosDev &E, double e) {
+    if constexpr (TABLE) {
+        double f = eos_interp(E, e, E.T_tb);
+        return 1.e-15 < f ? f : 1.e-15;
+    } else {
+        return (double)eos_fourth_root(E, (float)(e / EOS_FACTOR));
+    }
+}
+
+// rhic/src/FluxLimiter.cpp:14-32: sign(0) = +1, minmod = (sx+sy)*min(|x|,|y|)/2
+__device__ __forceinline__ int isign(double x) { return x < 0 ? -1 : 1; }
+__device__ __forceinline__ double minmod(double x, double y) {
+    return (isign(x) + isign(y)) * __builtin_fmin(__builtin_fabs(x), __builtin_fabs(y)) / 2.0;
+}
+__device__ __forceinline__ double approx_deriv(double x, double y, double z) {
+    double l = THETA * (y - x);
+    double c = (z - x) / 2.0;
+    double r = THETA * (z - y);
+    return minmod(l, minmod(c, r));
+}
+
+// rhic/src/EnergyMomentumTensor.cpp:22-115
+template <bool TABLE>
+__device__ double energy_from_conserved(const EosDev &E, double ePrev, double M0, double M, double Pi, int &nan) {
+    if constexpr (!TABLE) {
+        (void)E; (void)ePrev; (void)Pi; (void)nan;
+        return (double)f32_fabs((float)((double)f32_sqrt(f32_fabs((float)(4 * M0 * M0 - 3 * M))) - M0));
+    } else {
+        double e0 = ePrev;
+        for (int j = 0; j < MAX_ITERS_NR; ++j) {
+            double p = eos_pressure<true>(E, e0);
+            double cs2 = eos_cs2<true>(E, e0);
+            double cst2 = p / e0;
+            double A = M0 * (1.0 - cst2) + Pi;
+            double B = M0 * (M0 + Pi) - M;
+            double H = (double)f32_sqrt(f32_fabs((float)(A * A + 4 * cst2 * B)));
+            double D = (A - H) / (2 * cst2);
+            double f = e0 + D;
+            double fp = 1.0 - ((cs2 - cst2) * (B + D * H - ((cs2 - cst2) * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
+            if (__builtin_isnan(f / fp)) { nan = NAN_ROOT; return ePrev; }
+            double e = e0 - f / fp;
+            if ((double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e)) return e;
+            e0 = e;
+        }
+        // bisection fallback (:48-111)
+        double e_lo = 0.0, e_mid = ePrev, e_hi = 2.0 * (ePrev + 1.0);
+        for (int j = 0; j < MAX_ITERS_BI; ++j) {
+            double p_lo = eos_pressure<true>(E, e_lo), p_mid = eos_pressure<true>(E, e_mid),
+                   p_hi = eos_pressure<true>(E, e_hi);
+            double cst2_lo = p_lo / e_lo, cst2_mid = p_mid / e_mid, cst2_hi = p_hi / e_hi;
+            double A_lo = M0 * (1.0 - cst2_lo) + Pi;
+            double A_mid = M0 * (1.0 - cst2_mid) + Pi;
+            double A_hi = M0 * (1.0 - cst2_hi) + Pi;
+            double B = M0 * (M0 + Pi) - M;
+            double H_lo = (double)f32_sqrt(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
+            double H_mid = (double)f32_sqrt(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
+            double H_hi = (double)f32_sqrt(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
+            double D_lo = (A_lo - H_lo) / (2 * cst2_lo);
+            double D_mid = (A_mid - H_mid) / (2 * cst2_mid);
+            double D_hi = (A_hi - H_hi) / (2 * cst2_hi);
+            double f_lo = e_lo + D_lo, f_mid = e_mid + D_mid, f_hi = e_hi + D_hi;
+            if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return e_mid;
+            if (f_lo * f_mid < 0.0) {
+                e_hi = e_mid;
+                e_mid = (e_lo + e_hi) / 2.0;
+            } else if (f_mid * f_hi < 0.0) {
+                e_lo = e_mid;
+                e_mid = (e_lo + e_hi) / 2.0;
+            }
+        }
+        return ePrev;
+    }
+}
+
+struct Prim {
+    double e, p, ut, ux, uy, un;
+};
+
+// getInferredVariables, rhic/src/EnergyMomentumTensor.cpp:117-233.
+// ttt..pitn are q[0..7]; Pi = q[14] when bulk is evolved.
+template <bool TABLE>
+__device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
+                                        double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
+                                        int &nan) {
+    double M0 = ttt - pitt;
+    double M1 = ttx - pitx;
+    double M2 = tty - pity;
+    double M3 = ttn - pitn;
+    double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+    Prim r;
+    if (ePrev <= 0.1) r.e = M0 - M / M0;
+    else r.e = energy_from_conserved<TABLE>(E, ePrev, M0, M, Pi, nan);
+    if (__builtin_isnan(r.e)) nan = NAN_ENERGY;
+    r.p = eos_pressure<TABLE>(E, r.e);
+    if (r.e < 1.e-7) {
+        r.e = 1.e-7;
+        r.p = 1.e-7;
+    }
+    double P = r.p + Pi;
+    double Ei = 1.0 / (r.e + P);
+    r.ut = (double)f32_sqrt(f32_fabs((float)((M0 + P) * Ei)));
+    double E2 = Ei / r.ut;
+    r.ux = M1 * E2;
+    r.uy = M2 * E2;
+    r.un = M3 * E2;
+    if (r.ut > 1.0e3) {
+        r.ut = 1.0;
+        r.ux = 0.0;
+        r.uy = 0.0;
+        r.un = 0.0;
+    }
+    return r;
+}
+
+}  // namespace vh

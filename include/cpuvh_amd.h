@@ -1,0 +1,142 @@
+/*
+ * cpuvh_amd.h — C ABI of the MI355X (gfx950) viscous-hydro step.
+ *
+ * Drop-in boundary for cpu-vh's rungeKutta2 hot path.  The reference operates
+ * on global SoA arrays with two ghost layers per side
+ * (rhic/include/DynamicalVariables.h:47-113, rhic/src/DynamicalVariables.cpp:35-167);
+ * here the state lives in HBM behind an opaque handle and crosses the boundary
+ * as interior-only, x-fastest host arrays (index i + nx*(j + ny*k), the
+ * reference's columnMajorLinearIndex order without ghosts).  Units as in the
+ * reference: e, p in fm^-4, u^mu contravariant in (tau, x, y, eta_s).
+ *
+ * Field ids (vh_download / vh_upload_field):
+ *   0..13  ttt ttx tty ttn pitt pitx pity pitn pixx pixy pixn piyy piyn pinn
+ *   14     Pi (bulk builds only)
+ *   15 e   16 p   17..20 ut ux uy un   21..24 previous-step ut ux uy un
+ *   25     regulation factor of the last stage (regFacShear)
+ *
+ * All entry points return a vh_status; none aborts the process.
+ */
+#ifndef CPUVH_AMD_H
+#define CPUVH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VH_OK = 0,
+    VH_ERR_ARG = -1,        /* invalid argument or configuration */
+    VH_ERR_HIP = -2,        /* HIP runtime error */
+    VH_ERR_NOMEM = -3,      /* device allocation failed */
+    VH_ERR_NAN = -4,        /* NaN in recovery or regulation (reference: printf + exit(-1)) */
+    VH_ERR_STATE = -5,      /* call out of order (e.g. step before init) */
+    VH_ERR_CAPACITY = -6    /* output buffer too small; required size returned */
+} vh_status;
+
+/* Where the NaN of VH_ERR_NAN came from (reference message in brackets). */
+typedef enum {
+    VH_NAN_NONE = 0,
+    VH_NAN_ENERGY = 1,      /* "Found e nan inside getInferredVariables" EnergyMomentumTensor.cpp:197-203 */
+    VH_NAN_ROOT = 2,        /* "f/fp is nan" EnergyMomentumTensor.cpp:42 */
+    VH_NAN_REG_SHEAR = 3,   /* "Found facShear nan" FullyDiscreteKurganovTadmorScheme.cpp:849 */
+    VH_NAN_REG_BULK = 4     /* "Found facBulk nan" FullyDiscreteKurganovTadmorScheme.cpp:854 */
+} vh_nan_site;
+
+typedef enum { VH_EOS_CONFORMAL = 0, VH_EOS_TABLE = 1 } vh_eos_kind;
+
+/* Replaces the reference's LatticeParameters + HydroParameters structs
+ * (rhic/include/LatticeParameters.h:17-34, rhic/include/HydroParameters.h:15-23)
+ * plus the compile-time switches PI (bulk) and *_EOS
+ * (rhic/include/DynamicalVariables.h:13-15, rhic/include/EquationOfState.h:13-15). */
+typedef struct {
+    int nx, ny, nz;            /* numLatticePointsX/Y/Rapidity (interior points) */
+    double dx, dy, dz;         /* latticeSpacingX/Y/Rapidity */
+    double dt;                 /* latticeSpacingProperTime */
+    double t0;                 /* initialProperTimePoint */
+    double etabar;             /* shearViscosityToEntropyDensity (eta/s, or its minimum) */
+    double T_reg_GeV;          /* regulationTemperatureGeV */
+    int bulk;                  /* 1: evolve Pi (reference built with -DPI) */
+    int eos_kind;              /* vh_eos_kind */
+    const double *eos_table;   /* VH_EOS_TABLE: 4*100000 little-endian doubles (e, P, s, T in GeV units),
+                                  the layout of EoS/hrg_hotqcd_eos_binary.dat; copied */
+    /* eta/s(T) extension (no reference counterpart: the reference's eta/s is constant,
+       rhic/src/SourceTerms.cpp:69).  eta/s = etabar + slope_hi*(T-Tk) above Tk and
+       etabar + slope_lo*(Tk-T) below, T in GeV.  Both slopes 0 reproduces the reference. */
+    double etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    int device;                /* HIP device ordinal */
+} vh_config;
+
+typedef struct vh_handle vh_handle;
+
+/* Allocates the device state (allocateHostMemory, DynamicalVariables.cpp:43-167). */
+vh_status vh_create(const vh_config *cfg, vh_handle **out);
+vh_status vh_destroy(vh_handle *h);
+
+/* Initial state, the type-11 reader's assignments (rhic/src/InitialConditions.cpp:41-105):
+ * e, p, u^mu, previous u^mu := u^mu, pi^{mu nu} (10 arrays back to back, pitt..pinn),
+ * Pi (may be NULL: zero).  Host arrays of nx*ny*nz doubles. */
+vh_status vh_upload_ic(vh_handle *h, const double *e, const double *p, const double *ut, const double *ux,
+                       const double *uy, const double *un, const double *pi10, const double *Pi);
+
+/* setConservedVariables (rhic/src/DynamicalVariables.cpp:169-218) followed by the
+ * ghost refresh of HydroPlugin.cpp:204 (implicit here: clamp-to-edge indexing). */
+vh_status vh_init_conserved(vh_handle *h, double t);
+
+/* rungeKutta2(t, dt, q, Q, ...) + setCurrentConservedVariables()
+ * (rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:866-924, rhic/src/HydroPlugin.cpp:292,300).
+ * Asynchronous on the handle's stream; a NaN is reported by the next synchronising call. */
+vh_status vh_step(vh_handle *h, double t);
+
+/* count steps n = n_first .. n_first+count-1 of the driver loop, each at
+ * t = t0 + (n-1)*dt as HydroPlugin.cpp:226,307 advances it, replayed from a
+ * captured HIP graph. */
+vh_status vh_steps(vh_handle *h, int n_first, int count);
+
+/* Waits for the stream and returns VH_ERR_NAN if any stage hit a NaN. */
+vh_status vh_sync(vh_handle *h);
+vh_status vh_nan_info(vh_handle *h, int *site, int64_t *cell);
+
+/* Host copy of one field (ids above), nx*ny*nz doubles.  Synchronising. */
+vh_status vh_download(vh_handle *h, int field, double *out);
+vh_status vh_upload_field(vh_handle *h, int field, const double *in);
+
+/* checkForCellsAboveTc (freezeout/FreezeOut.cpp:364-381): number of interior
+ * cells with e > eF. */
+vh_status vh_count_above(vh_handle *h, double eF, int64_t *count);
+
+/* Freezeout cell flagging for one coarse step: the reference keeps the
+ * previous and current coarse e slices (swapAndSetHydroVariables,
+ * freezeout/FreezeOut.cpp:57-97; the previous slice starts as zeros,
+ * HydroPlugin.cpp:159) and Cornelius builds elements only in cubes with
+ * 0 < #(corners >= eF) < 2^dim (cornelius.cpp:2201-2212, :2257-2272) over
+ * ix<nx-1, iy<ny-1 (, iz<nz-1) (FreezeOut.cpp:207-212, :288-291).
+ * This call shifts the slices (previous := last current, current := e) and
+ * writes the flagged cube ids (ix + (nx-1)*(iy + (ny-1)*iz)) in increasing
+ * order into ids (capacity cap), the count into *n_flagged, and, when corners
+ * is not NULL, the 2^dim corner energies of each flagged cube (previous slice
+ * first, then current; within a slice x fastest).  VH_ERR_CAPACITY if cap is
+ * too small (*n_flagged still set). */
+vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int64_t cap, int64_t *n_flagged);
+
+/* Host-side EoS of the handle (EquationOfState.cpp:44-290): used by drivers for
+ * e_F = e(T_F) and by tests. */
+double vh_eos_energy_density(const vh_handle *h, double T);
+double vh_eos_pressure(const vh_handle *h, double e);
+double vh_eos_temperature(const vh_handle *h, double e);
+
+/* Average duration (ms) of the last timed vh_steps call's stage kernels,
+ * from HIP events on the handle's stream; 0 if none. */
+double vh_last_kernel_ms(const vh_handle *h);
+
+const char *vh_status_string(vh_status s);
+const char *vh_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPUVH_AMD_H */
