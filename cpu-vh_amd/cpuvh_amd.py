@@ -25,7 +25,8 @@ EOS_CONFORMAL, EOS_TABLE = 0, 1
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
-           "vh_status_string", "vh_build_info"]
+           "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
+           "vh_prepare_steps"]
 
 
 class VhConfig(ctypes.Structure):
@@ -36,6 +37,17 @@ class VhConfig(ctypes.Structure):
                 ("eos_table", ctypes.c_void_p),
                 ("etas_Tk_GeV", ctypes.c_double), ("etas_slope_lo", ctypes.c_double),
                 ("etas_slope_hi", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class GlauberParams(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_int), ("R", ctypes.c_double), ("a", ctypes.c_double),
+                ("sigma_nn_fm2", ctypes.c_double), ("b", ctypes.c_double), ("b_max", ctypes.c_double),
+                ("width", ctypes.c_double), ("eta_flat", ctypes.c_double), ("eta_variance", ctypes.c_double),
+                ("e0", ctypes.c_double), ("e_floor", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class GlauberInfo(ctypes.Structure):
+    _fields_ = [("b", ctypes.c_double), ("n_part", ctypes.c_int), ("n_coll", ctypes.c_int)]
 
 
 class VhError(RuntimeError):
@@ -75,6 +87,11 @@ def _lib():
         L.vh_status_string.argtypes = [ctypes.c_int]
         L.vh_status_string.restype = ctypes.c_char_p
         L.vh_build_info.restype = ctypes.c_char_p
+        L.vh_ic_mc_glauber.argtypes = [ctypes.POINTER(GlauberParams), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       D, D, D, P, ctypes.POINTER(GlauberInfo)]
+        L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
+        L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
+        L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -142,6 +159,9 @@ class Hydro:
     def steps(self, n_first, count):
         self._check(_lib().vh_steps(self.h, n_first, count), "vh_steps")
 
+    def prepare_steps(self, count):
+        self._check(_lib().vh_prepare_steps(self.h, count), "vh_prepare_steps")
+
     def sync(self):
         self._check(_lib().vh_sync(self.h), "vh_sync")
 
@@ -196,3 +216,36 @@ class Hydro:
 
 def build_info():
     return _lib().vh_build_info().decode()
+
+
+def mc_glauber(nx, ny, nz, dx, dy, dz, seed, A=197, R=6.38, a=0.535, sigma_nn_fm2=4.2, b=-1.0, b_max=10.0,
+               width=0.5, eta_flat=4.0, eta_variance=1.96, e0=10.0, e_floor=1e-3):
+    """Seeded MC-Glauber energy density (vh_ic_mc_glauber); returns (e, info)."""
+    gp = GlauberParams(A, R, a, sigma_nn_fm2, b, b_max, width, eta_flat, eta_variance, e0, e_floor, seed)
+    info = GlauberInfo()
+    e = np.empty(nx * ny * nz, dtype=np.float64)
+    st = _lib().vh_ic_mc_glauber(ctypes.byref(gp), nx, ny, nz, dx, dy, dz, _ptr(e), ctypes.byref(info))
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_mc_glauber")
+    return e, {"b": info.b, "n_part": info.n_part, "n_coll": info.n_coll}
+
+
+def read_tmunu(path, nx, ny, nz):
+    """IC type 11 reader (vh_ic_read_tmunu): dict of x-fastest arrays."""
+    N = nx * ny * nz
+    out = {k: np.empty(N) for k in ("e", "p", "ut", "ux", "uy", "un", "Pi")}
+    out["pi"] = np.empty(10 * N)
+    st = _lib().vh_ic_read_tmunu(path.encode(), nx, ny, nz, *[_ptr(out[k]) for k in
+                                                              ("e", "p", "ut", "ux", "uy", "un", "pi", "Pi")])
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_read_tmunu")
+    return out
+
+
+def write_tmunu(path, nx, ny, nz, dx, dy, dz, ic):
+    arrs = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("e", "p", "ut", "ux", "uy", "un", "pi")]
+    Pi = ic.get("Pi")
+    Pi = None if Pi is None else np.ascontiguousarray(Pi, dtype=np.float64)
+    st = _lib().vh_ic_write_tmunu(path.encode(), nx, ny, nz, dx, dy, dz, *[_ptr(a) for a in arrs], _ptr(Pi))
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_write_tmunu")

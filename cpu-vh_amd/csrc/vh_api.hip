@@ -243,6 +243,7 @@ vh_status vh_create(const vh_config *cfg, vh_handle **out) {
         E.cs2_const = h->heos.cs2_is_one_third() ? 1 : 0;
         E.e_min = h->heos.e_min;
         E.e_spacing = h->heos.e_spacing;
+        E.r_spacing = 1.0 / h->heos.e_spacing;
         E.e_max = h->heos.e_max;
         const size_t tb = sizeof(double) * h->heos.p_tb.size();
         if (hipMalloc(&h->d_ptb, tb) != hipSuccess || hipMalloc(&h->d_Ttb, tb) != hipSuccess)
@@ -347,16 +348,8 @@ vh_status vh_step(vh_handle *h, double t) {
     return enqueue_step(h, st, nullptr, 0);
 }
 
-vh_status vh_steps(vh_handle *h, int n_first, int count) {
-    if (!h || n_first < 1 || count < 0) return VH_ERR_ARG;
-    if (!h->conserved_ready) return VH_ERR_STATE;
-    if (count == 0) return VH_OK;
-    HIPCHK(hipSetDevice(h->dev));
-    vh_status s = ensure_times(h, n_first + count);
-    if (s != VH_OK) return s;
-    HIPCHK(hipStreamSynchronize(h->stream));
-    *h->h_base = n_first;
-    HIPCHK(hipMemcpyAsync(h->d_base, h->h_base, sizeof(int), hipMemcpyHostToDevice, h->stream));
+// Graph of [stage 1, stage 2] x count for the current buffer-role state.
+static vh_status get_graph(vh_handle *h, int count, hipGraphExec_t *out) {
     const int parity = h->iq_cur * 3 + h->iu_cur;  // buffer-role state the graph was captured in
     const auto key = std::make_pair(count, parity);
     auto it = h->graphs.find(key);
@@ -377,8 +370,34 @@ vh_status vh_steps(vh_handle *h, int n_first, int count) {
         if (ie != hipSuccess) return hip_fail(ie, "hipGraphInstantiate");
         it = h->graphs.emplace(key, ge).first;
     }
+    *out = it->second;
+    return VH_OK;
+}
+
+vh_status vh_prepare_steps(vh_handle *h, int count) {
+    if (!h || count < 1) return VH_ERR_ARG;
+    if (!h->conserved_ready) return VH_ERR_STATE;
+    HIPCHK(hipSetDevice(h->dev));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    hipGraphExec_t ge;
+    return get_graph(h, count, &ge);
+}
+
+vh_status vh_steps(vh_handle *h, int n_first, int count) {
+    if (!h || n_first < 1 || count < 0) return VH_ERR_ARG;
+    if (!h->conserved_ready) return VH_ERR_STATE;
+    if (count == 0) return VH_OK;
+    HIPCHK(hipSetDevice(h->dev));
+    vh_status s = ensure_times(h, n_first + count);
+    if (s != VH_OK) return s;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *h->h_base = n_first;
+    HIPCHK(hipMemcpyAsync(h->d_base, h->h_base, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    hipGraphExec_t ge = nullptr;
+    s = get_graph(h, count, &ge);
+    if (s != VH_OK) return s;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    HIPCHK(hipGraphLaunch(it->second, h->stream));
+    HIPCHK(hipGraphLaunch(ge, h->stream));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     for (int k = 0; k < count; ++k) {
         std::swap(h->iq_cur, h->iq_next);

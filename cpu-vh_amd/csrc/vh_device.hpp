@@ -31,10 +31,53 @@ constexpr double lambda_piPi = 1.2;
 
 enum : int { NAN_NONE = 0, NAN_ENERGY = 1, NAN_ROOT = 2, NAN_REG_SHEAR = 3, NAN_REG_BULK = 4 };
 
+// Division by a divisor used several times: one correctly rounded reciprocal
+// r = RN(1/b) and, per numerator, q0 = RN(a*r), rem = a - b*q0 (exact with
+// FMA), q = RN(q0 + rem*r).  By Markstein's theorem (IBM J. Res. Dev. 34,
+// 1990; Muller et al., Handbook of Floating-Point Arithmetic, ch. 5) q is the
+// correctly rounded a/b, i.e. the division operator's value, when nothing
+// underflows or overflows.  Sufficient here: |b| in [2^-100, 2^100] (so r is
+// normal) and |q0| in [2^-860, 2^1000] (then rem is a multiple of 2^-1064,
+// exactly representable, and q is normal); a == +-0 gives a*r, the
+// correctly signed zero.
+//
+// DivFast uses the reciprocal form and, for the lanes whose quotient leaves
+// that range (tiny or huge values, NaN), the division operator; DivExact
+// always uses the division operator.  Either way every result equals the
+// division operator's.
+struct Recip {
+    double b, r;
+    bool ok;
+};
+__device__ __forceinline__ Recip make_recip(double b) {
+    Recip R;
+    R.b = b;
+    R.r = 1.0 / b;
+    const double ab = __builtin_fabs(b);
+    R.ok = ab >= 0x1p-100 && ab <= 0x1p100;
+    return R;
+}
+struct DivFast {
+    __device__ __forceinline__ double operator()(double a, const Recip &R) {
+        const double q0 = a * R.r;
+        const double rem = __builtin_fma(-R.b, q0, a);
+        double q = __builtin_fma(rem, R.r, q0);
+        const double aq = __builtin_fabs(q0);
+        const bool zero = a == 0.0;
+        if (zero) q = q0;
+        if (__builtin_expect(!(R.ok && ((aq >= 0x1p-860 && aq <= 0x1p1000) || zero)), 0)) q = a / R.b;
+        return q;
+    }
+};
+struct DivExact {
+    __device__ __forceinline__ double operator()(double a, const Recip &R) { return a / R.b; }
+};
+
 struct EosDev {
     int kind;        // 0 conformal, 1 table
     int cs2_const;   // table: host verified that the reference's c_s^2 is 1/3 for every e
     double e_min, e_spacing, e_max;
+    double r_spacing;  // RN(1/e_spacing)
     const double *p_tb;
     const double *T_tb;
     // glibc powf(x, 0.25f) exceptions: sorted (x bits, result bits) pairs,
@@ -67,34 +110,36 @@ __device__ __forceinline__ float eos_fourth_root(const EosDev &E, float x) {
 }
 
 // rhic/src/EquationOfState.cpp:329-360
-__device__ __forceinline__ double eos_interp(const EosDev &E, double e, const double *__restrict__ tb) {
+template <class DV>
+__device__ __forceinline__ double eos_interp(const EosDev &E, double e, const double *__restrict__ tb, DV &div) {
     const double e0 = E.e_min, de = E.e_spacing;
-    int idx = (int)((e - e0) / de);
+    const Recip Rde{de, E.r_spacing, true};
+    int idx = (int)div(e - e0, Rde);
     idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
     idx = idx > 0 ? idx : 0;
-    const double frac = (e - (idx * de + e0)) / de;
+    const double frac = div(e - (idx * de + e0), Rde);
     double t1 = tb[idx], t2 = tb[idx + 1];
     return t1 * (1. - frac) + t2 * frac;
 }
 
-template <bool TABLE>
-__device__ __forceinline__ double eos_pressure(const EosDev &E, double e) {
+template <bool TABLE, class DV>
+__device__ __forceinline__ double eos_pressure(const EosDev &E, double e, DV &div) {
     if constexpr (TABLE) {
-        double f = eos_interp(E, e, E.p_tb);
+        double f = eos_interp(E, e, E.p_tb, div);
         return 1.e-15 < f ? f : 1.e-15;  // std::max(1.e-15, f)
     } else {
-        return e / 3.;
+        return div(e, Recip{3., 1. / 3., true});
     }
 }
 
-template <bool TABLE>
-__device__ __forceinline__ double eos_cs2(const EosDev &E, double e) {
+template <bool TABLE, class DV>
+__device__ __forceinline__ double eos_cs2(const EosDev &E, double e, DV &div) {
     if constexpr (TABLE) {
         if (E.cs2_const) return 1. / 3.;
         double v_min = 0.01, v_max = 1. / 3.;
         double delta_e = E.e_spacing;
-        double p = eos_pressure<true>(E, e);
-        double p_r = eos_pressure<true>(E, e + delta_e);
+        double p = eos_pressure<true>(E, e, div);
+        double p_r = eos_pressure<true>(E, e + delta_e, div);
         double delta_p = p_r - p;
         double v = delta_e / delta_p;
         v = v_max < v ? v_max : v;  // std::min(v_max, v)
@@ -105,10 +150,10 @@ __device__ __forceinline__ double eos_cs2(const EosDev &E, double e) {
     }
 }
 
-template <bool TABLE>
-__device__ __forceinline__ double eos_temperature(const EosDev &E, double e) {
+template <bool TABLE, class DV>
+__device__ __forceinline__ double eos_temperature(const EosDev &E, double e, DV &div) {
     if constexpr (TABLE) {
-        double f = eos_interp(E, e, E.T_tb);
+        double f = eos_interp(E, e, E.T_tb, div);
         return 1.e-15 < f ? f : 1.e-15;
     } else {
         return (double)eos_fourth_root(E, (float)(e / EOS_FACTOR));
@@ -127,56 +172,71 @@ __device__ __forceinline__ double approx_deriv(double x, double y, double z) {
     return minmod(l, minmod(c, r));
 }
 
-// rhic/src/EnergyMomentumTensor.cpp:22-115
+// Newton-Raphson + bisection root solve of the table EoS,
+// rhic/src/EnergyMomentumTensor.cpp:26-111.  One out-of-line body serves all
+// thirteen call sites of a stage; it uses the division operator throughout.
+struct RootResult {
+    double e;
+    int nan;
+};
+__device__ __noinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
+    DivExact dx;
+    double e0 = ePrev;
+    for (int j = 0; j < MAX_ITERS_NR; ++j) {
+        double p = eos_pressure<true>(E, e0, dx);
+        double cs2 = eos_cs2<true>(E, e0, dx);
+        double cst2 = p / e0;
+        double A = M0 * (1.0 - cst2) + Pi;
+        double B = M0 * (M0 + Pi) - M;
+        double H = (double)f32_sqrt(f32_fabs((float)(A * A + 4 * cst2 * B)));
+        double D = (A - H) / (2 * cst2);
+        double f = e0 + D;
+        double fp = 1.0 - ((cs2 - cst2) * (B + D * H - ((cs2 - cst2) * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
+        if (__builtin_isnan(f / fp)) return RootResult{ePrev, NAN_ROOT};
+        double e = e0 - f / fp;
+        if ((double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e)) return RootResult{e, NAN_NONE};
+        e0 = e;
+    }
+    // bisection fallback (:48-111)
+    double e_lo = 0.0, e_mid = ePrev, e_hi = 2.0 * (ePrev + 1.0);
+    for (int j = 0; j < MAX_ITERS_BI; ++j) {
+        double p_lo = eos_pressure<true>(E, e_lo, dx), p_mid = eos_pressure<true>(E, e_mid, dx),
+               p_hi = eos_pressure<true>(E, e_hi, dx);
+        double cst2_lo = p_lo / e_lo, cst2_mid = p_mid / e_mid, cst2_hi = p_hi / e_hi;
+        double A_lo = M0 * (1.0 - cst2_lo) + Pi;
+        double A_mid = M0 * (1.0 - cst2_mid) + Pi;
+        double A_hi = M0 * (1.0 - cst2_hi) + Pi;
+        double B = M0 * (M0 + Pi) - M;
+        double H_lo = (double)f32_sqrt(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
+        double H_mid = (double)f32_sqrt(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
+        double H_hi = (double)f32_sqrt(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
+        double D_lo = (A_lo - H_lo) / (2 * cst2_lo);
+        double D_mid = (A_mid - H_mid) / (2 * cst2_mid);
+        double D_hi = (A_hi - H_hi) / (2 * cst2_hi);
+        double f_lo = e_lo + D_lo, f_mid = e_mid + D_mid, f_hi = e_hi + D_hi;
+        if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return RootResult{e_mid, NAN_NONE};
+        if (f_lo * f_mid < 0.0) {
+            e_hi = e_mid;
+            e_mid = (e_lo + e_hi) / 2.0;
+        } else if (f_mid * f_hi < 0.0) {
+            e_lo = e_mid;
+            e_mid = (e_lo + e_hi) / 2.0;
+        }
+    }
+    return RootResult{ePrev, NAN_NONE};
+}
+
+// energyDensityFromConservedVariables, rhic/src/EnergyMomentumTensor.cpp:22-115
 template <bool TABLE>
-__device__ double energy_from_conserved(const EosDev &E, double ePrev, double M0, double M, double Pi, int &nan) {
+__device__ __forceinline__ double energy_from_conserved(const EosDev &E, double ePrev, double M0, double M, double Pi,
+                                                        int &nan) {
     if constexpr (!TABLE) {
         (void)E; (void)ePrev; (void)Pi; (void)nan;
         return (double)f32_fabs((float)((double)f32_sqrt(f32_fabs((float)(4 * M0 * M0 - 3 * M))) - M0));
     } else {
-        double e0 = ePrev;
-        for (int j = 0; j < MAX_ITERS_NR; ++j) {
-            double p = eos_pressure<true>(E, e0);
-            double cs2 = eos_cs2<true>(E, e0);
-            double cst2 = p / e0;
-            double A = M0 * (1.0 - cst2) + Pi;
-            double B = M0 * (M0 + Pi) - M;
-            double H = (double)f32_sqrt(f32_fabs((float)(A * A + 4 * cst2 * B)));
-            double D = (A - H) / (2 * cst2);
-            double f = e0 + D;
-            double fp = 1.0 - ((cs2 - cst2) * (B + D * H - ((cs2 - cst2) * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
-            if (__builtin_isnan(f / fp)) { nan = NAN_ROOT; return ePrev; }
-            double e = e0 - f / fp;
-            if ((double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e)) return e;
-            e0 = e;
-        }
-        // bisection fallback (:48-111)
-        double e_lo = 0.0, e_mid = ePrev, e_hi = 2.0 * (ePrev + 1.0);
-        for (int j = 0; j < MAX_ITERS_BI; ++j) {
-            double p_lo = eos_pressure<true>(E, e_lo), p_mid = eos_pressure<true>(E, e_mid),
-                   p_hi = eos_pressure<true>(E, e_hi);
-            double cst2_lo = p_lo / e_lo, cst2_mid = p_mid / e_mid, cst2_hi = p_hi / e_hi;
-            double A_lo = M0 * (1.0 - cst2_lo) + Pi;
-            double A_mid = M0 * (1.0 - cst2_mid) + Pi;
-            double A_hi = M0 * (1.0 - cst2_hi) + Pi;
-            double B = M0 * (M0 + Pi) - M;
-            double H_lo = (double)f32_sqrt(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
-            double H_mid = (double)f32_sqrt(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
-            double H_hi = (double)f32_sqrt(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
-            double D_lo = (A_lo - H_lo) / (2 * cst2_lo);
-            double D_mid = (A_mid - H_mid) / (2 * cst2_mid);
-            double D_hi = (A_hi - H_hi) / (2 * cst2_hi);
-            double f_lo = e_lo + D_lo, f_mid = e_mid + D_mid, f_hi = e_hi + D_hi;
-            if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return e_mid;
-            if (f_lo * f_mid < 0.0) {
-                e_hi = e_mid;
-                e_mid = (e_lo + e_hi) / 2.0;
-            } else if (f_mid * f_hi < 0.0) {
-                e_lo = e_mid;
-                e_mid = (e_lo + e_hi) / 2.0;
-            }
-        }
-        return ePrev;
+        const RootResult r = table_root(E, ePrev, M0, M, Pi);
+        if (r.nan) nan = r.nan;
+        return r.e;
     }
 }
 
@@ -186,10 +246,10 @@ struct Prim {
 
 // getInferredVariables, rhic/src/EnergyMomentumTensor.cpp:117-233.
 // ttt..pitn are q[0..7]; Pi = q[14] when bulk is evolved.
-template <bool TABLE>
+template <bool TABLE, class DV>
 __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
                                         double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
-                                        int &nan) {
+                                        int &nan, DV &div) {
     double M0 = ttt - pitt;
     double M1 = ttx - pitx;
     double M2 = tty - pity;
@@ -199,7 +259,7 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
     if (ePrev <= 0.1) r.e = M0 - M / M0;
     else r.e = energy_from_conserved<TABLE>(E, ePrev, M0, M, Pi, nan);
     if (__builtin_isnan(r.e)) nan = NAN_ENERGY;
-    r.p = eos_pressure<TABLE>(E, r.e);
+    r.p = eos_pressure<TABLE>(E, r.e, div);
     if (r.e < 1.e-7) {
         r.e = 1.e-7;
         r.p = 1.e-7;

@@ -1,0 +1,176 @@
+// Host-side initial conditions of the product (C ABI in include/cpuvh_amd.h).
+//
+// * Seeded Monte-Carlo Glauber energy density: Woods-Saxon nucleons, wounded
+//   nucleons from a black-disk sigma_NN cut, one Gaussian per participant, and
+//   the reference's longitudinal profile (a plateau of full width
+//   rapidityMean with Gaussian tails of variance rapidityVariance,
+//   rhic/src/InitialConditions.cpp:1046-1064) plus the reference's 1e-3
+//   floor (:1134).  The reference's own generator uses glibc rand() with a
+//   fixed seed and hard-coded nuclear parameters
+//   (rhic/src/MonteCarloGlauberModel.cpp:18-111); this one takes every
+//   parameter and a 64-bit seed so events are reproducible and independent.
+// * Type-11 file reader/writer: input/Tmunu.dat, 20 columns per line read
+//   through float (rhic/src/InitialConditions.cpp:41-105).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cpuvh_amd.h"
+
+namespace {
+
+// SplitMix64 (Steele, Lea & Flood 2014): a 64-bit counter passed through a
+// fixed bijective mixer.
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+void sample_nucleus(SplitMix64 &rng, int A, double R, double a, double *x, double *y) {
+    const double rmax = R + 12 * a;
+    double M = 0;  // max of r^2 rho(r) on [0, rmax]
+    for (int i = 0; i <= 4000; ++i) {
+        const double r = rmax * i / 4000.0;
+        M = std::fmax(M, r * r / (1 + std::exp((r - R) / a)));
+    }
+    M *= 1.01;
+    int m = 0;
+    while (m < A) {
+        const double r = rmax * rng.uniform();
+        const double u = rng.uniform();
+        if (u * M < r * r / (1 + std::exp((r - R) / a))) {
+            const double ct = 2 * rng.uniform() - 1, phi = 2 * M_PI * rng.uniform();
+            const double st = std::sqrt(1 - ct * ct);
+            x[m] = r * st * std::cos(phi);
+            y[m] = r * st * std::sin(phi);
+            ++m;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" vh_status vh_ic_mc_glauber(const vh_glauber_params *gp, int nx, int ny, int nz, double dx, double dy,
+                                      double dz, double *e_out, vh_glauber_info *info) {
+    if (!gp || !e_out || nx < 1 || ny < 1 || nz < 1 || gp->A < 1 || !(gp->sigma_nn_fm2 > 0) || !(gp->width > 0))
+        return VH_ERR_ARG;
+    SplitMix64 rng{gp->seed};
+    const int A = gp->A;
+    double b = gp->b;
+    if (b < 0) b = gp->b_max * rng.uniform();
+    std::vector<double> x1(A), y1(A), x2(A), y2(A);
+    sample_nucleus(rng, A, gp->R, gp->a, x1.data(), y1.data());
+    sample_nucleus(rng, A, gp->R, gp->a, x2.data(), y2.data());
+    for (int i = 0; i < A; ++i) {
+        x1[i] -= b / 2;
+        x2[i] += b / 2;
+    }
+    const double d2 = gp->sigma_nn_fm2 / M_PI;
+    std::vector<int> w1(A, 0), w2(A, 0);
+    int ncoll = 0;
+    for (int i = 0; i < A; ++i)
+        for (int j = 0; j < A; ++j) {
+            const double ddx = x1[i] - x2[j], ddy = y1[i] - y2[j];
+            if (ddx * ddx + ddy * ddy < d2) {
+                ++w1[i];
+                ++w2[j];
+                ++ncoll;
+            }
+        }
+    std::vector<double> px, py;
+    for (int i = 0; i < A; ++i) {
+        if (w1[i]) { px.push_back(x1[i]); py.push_back(y1[i]); }
+        if (w2[i]) { px.push_back(x2[i]); py.push_back(y2[i]); }
+    }
+    // transverse profile: normalised Gaussians of width w around each participant
+    std::vector<double> eT((size_t)nx * ny, 0.0);
+    const double w = gp->width, norm = 1.0 / (2 * M_PI * w * w), cut = 6 * w;
+    for (size_t m = 0; m < px.size(); ++m) {
+        for (int j = 0; j < ny; ++j) {
+            const double y = (j - (ny - 1) / 2.0) * dy;
+            if (std::fabs(y - py[m]) > cut) continue;
+            for (int i = 0; i < nx; ++i) {
+                const double x = (i - (nx - 1) / 2.0) * dx;
+                if (std::fabs(x - px[m]) > cut) continue;
+                const double r2 = (x - px[m]) * (x - px[m]) + (y - py[m]) * (y - py[m]);
+                eT[(size_t)i + (size_t)nx * j] += norm * std::exp(-r2 / (2 * w * w));
+            }
+        }
+    }
+    // longitudinal profile, InitialConditions.cpp:1046-1064 (integer (nz-1)/2 as there)
+    std::vector<double> eL(nz);
+    for (int k = 0; k < nz; ++k) {
+        const double eta = (k - (nz - 1) / 2) * dz;
+        const double es = std::fabs(eta) - gp->eta_flat / 2;
+        const double arg = -es * es / gp->eta_variance / 2 * (es < 0 ? 0.0 : 1.0);
+        eL[k] = std::exp(arg);
+    }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                e_out[(size_t)i + (size_t)nx * (j + (size_t)ny * k)] =
+                    gp->e0 * eT[(size_t)i + (size_t)nx * j] * eL[k] + gp->e_floor;
+    if (info) {
+        info->b = b;
+        info->n_part = (int)px.size();
+        info->n_coll = ncoll;
+    }
+    return VH_OK;
+}
+
+// rhic/src/InitialConditions.cpp:41-105: values pass through float, lines in
+// i-outer, k-inner order; columns x y z e p ut ux uy un pitt..pinn Pi.
+extern "C" vh_status vh_ic_read_tmunu(const char *path, int nx, int ny, int nz, double *e, double *p, double *ut,
+                                      double *ux, double *uy, double *un, double *pi10, double *Pi) {
+    if (!path || !e || !p || !ut || !ux || !uy || !un || !pi10) return VH_ERR_ARG;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return VH_ERR_ARG;
+    const size_t N = (size_t)nx * ny * nz;
+    float v[20];
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                if (std::fscanf(f, "%f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f\n", &v[0], &v[1],
+                                &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12],
+                                &v[13], &v[14], &v[15], &v[16], &v[17], &v[18], &v[19]) != 20) {
+                    std::fclose(f);
+                    return VH_ERR_ARG;
+                }
+                const size_t s = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                e[s] = v[3]; p[s] = v[4]; ut[s] = v[5]; ux[s] = v[6]; uy[s] = v[7]; un[s] = v[8];
+                for (int c = 0; c < 10; ++c) pi10[c * N + s] = v[9 + c];
+                if (Pi) Pi[s] = v[19];
+            }
+    std::fclose(f);
+    return VH_OK;
+}
+
+extern "C" vh_status vh_ic_write_tmunu(const char *path, int nx, int ny, int nz, double dx, double dy, double dz,
+                                       const double *e, const double *p, const double *ut, const double *ux,
+                                       const double *uy, const double *un, const double *pi10, const double *Pi) {
+    if (!path || !e || !p || !ut || !ux || !uy || !un || !pi10) return VH_ERR_ARG;
+    FILE *f = std::fopen(path, "w");
+    if (!f) return VH_ERR_ARG;
+    const size_t N = (size_t)nx * ny * nz;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const size_t s = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                std::fprintf(f, "%.9g %.9g %.9g", (i - (nx - 1) / 2.0) * dx, (j - (ny - 1) / 2.0) * dy,
+                             (k - (nz - 1) / 2.0) * dz);
+                const double vals[7] = {e[s], p[s], ut[s], ux[s], uy[s], un[s], 0.0};
+                for (int c = 0; c < 6; ++c) std::fprintf(f, " %.9g", (double)(float)vals[c]);
+                for (int c = 0; c < 10; ++c) std::fprintf(f, " %.9g", (double)(float)pi10[c * N + s]);
+                std::fprintf(f, " %.9g\n", Pi ? (double)(float)Pi[s] : 0.0);
+            }
+    std::fclose(f);
+    return VH_OK;
+}

@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256) void fo_corners_kernel(const double *__restric
 template <int NC, bool TABLE, bool DIM3>
 static hipError_t launch_stage_t(const StageParams &P, const EosDev &E, hipStream_t st) {
     const long N = (long)P.nx * P.ny * P.nz;
-    const unsigned grid = (unsigned)((N + 255) / 256);
-    hipLaunchKernelGGL((stage_kernel<NC, TABLE, DIM3>), dim3(grid), dim3(256), 0, st, P, E);
+    const unsigned grid = (unsigned)((N + CELLS_PER_BLOCK - 1) / CELLS_PER_BLOCK);
+    const unsigned threads = (DIM3 ? 4 : 3) * CELLS_PER_BLOCK;
+    hipLaunchKernelGGL((stage_kernel<NC, TABLE, DIM3>), dim3(grid), dim3(threads), 0, st, P, E);
     return hipGetLastError();
 }
 

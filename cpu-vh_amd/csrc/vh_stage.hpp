@@ -61,44 +61,64 @@ __device__ __forceinline__ double etabar_of_T(const StageParams &P, double T) {
     return P.etas_min + P.etas_slope_lo * (P.etas_Tk_GeV - TGeV);
 }
 
-// KT central-upwind flux at the face between points b and c of the line
-// (a, b, c, d): rhic/src/SemiDiscreteKurganovTadmorScheme.cpp:14-62 with the
-// half-site extrapolations of rhic/src/HalfSiteExtrapolation.cpp:12-26
-// (forward face of cell i: a,b,c,d = i-1,i,i+1,i+2; backward: i-2,i-1,i,i+1).
-template <int NC, int DIR, bool TABLE>
-__device__ __forceinline__ void face_flux(const StageParams &P, const EosDev &E, long a, long b, long c, long d,
-                                          double ePrev, double t, double (&H)[NC], int &nan) {
-    double qR[NC], qL[NC];
-#pragma unroll
-    for (int n = 0; n < NC; ++n) {
-        const double *__restrict__ f = P.q[n];
-        double xa = f[a], xb = f[b], xc = f[c], xd = f[d];
-        qR[n] = xc - approx_deriv(xb, xc, xd) / 2;
-        qL[n] = xb + approx_deriv(xa, xb, xc) / 2;
-    }
-    const double PiR = NC == 15 ? qR[NC - 1] : 0.0, PiL = NC == 15 ? qL[NC - 1] : 0.0;
-    Prim R = recover<TABLE>(E, t, qR[0], qR[1], qR[2], qR[3], qR[4], qR[5], qR[6], qR[7], PiR, ePrev, nan);
-    Prim L = recover<TABLE>(E, t, qL[0], qL[1], qL[2], qL[3], qL[4], qL[5], qL[6], qL[7], PiL, ePrev, nan);
-    const double vR = DIR == 0 ? R.ux : (DIR == 1 ? R.uy : R.un);
-    const double vL = DIR == 0 ? L.ux : (DIR == 1 ? L.uy : L.un);
-    // localPropagationSpeed: fmax(rho(left), rho(right)), SpectralRadius.cpp:14-24
-    const double amax = __builtin_fmax(__builtin_fabs(vL / L.ut), __builtin_fabs(vR / R.ut));
-#pragma unroll
-    for (int n = 0; n < NC; ++n) {
-        double FqR = vR * qR[n] / R.ut;  // FluxFunctions.cpp:15-25
-        double FqL = vL * qL[n] / L.ut;
-        double res = FqR + FqL - amax * (qR[n] - qL[n]);
-        res /= 2;
-        H[n] = res;
-    }
+// Primitive state on one side of a face, with the reciprocal of u^tau used by
+// every flux of that side.
+struct Side {
+    double v;  // u^x, u^y or u^eta (direction of the face normal)
+    Recip ut;
+};
+
+// One face's KT speed: localPropagationSpeed = fmax(rho(left), rho(right))
+// (LocalPropagationSpeed.cpp:14-22, SpectralRadius.cpp:14-24).
+template <class DV>
+__device__ __forceinline__ double face_speed(const Side &R, const Side &L, DV &div) {
+    return __builtin_fmax(__builtin_fabs(div(L.v, L.ut)), __builtin_fabs(div(R.v, R.ut)));
 }
 
+// Flux of one variable at one face: (F(qR) + F(qL) - a (qR - qL)) / 2 with
+// F(q) = v q / u^tau (SemiDiscreteKurganovTadmorScheme.cpp:53-61,
+// FluxFunctions.cpp:15-25).
+template <class DV>
+__device__ __forceinline__ double face_var_flux(const Side &R, const Side &L, double a, double qR, double qL, DV &div) {
+    const double FqR = div(R.v * qR, R.ut);
+    const double FqL = div(L.v * qL, L.ut);
+    double res = FqR + FqL - a * (qR - qL);
+    res /= 2;
+    return res;
+}
+
+template <int DIR>
+__device__ __forceinline__ Side make_side(const Prim &w) {
+    return Side{DIR == 0 ? w.ux : (DIR == 1 ? w.uy : w.un), make_recip(w.ut)};
+}
+
+// The three limited slopes of one variable on the 5-point line
+// (qmm, qm, q, qp, qpp): D(qmm,qm,q), D(qm,q,qp), D(q,qp,qpp).  The four
+// half-site values of the two faces follow from them exactly as
+// HalfSiteExtrapolation.cpp:12-26 forms them (the middle slope is the one both
+// the forward-left and the backward-right extrapolations evaluate).
+struct Line {
+    double qm, q, qp, Dm, Dc, Dp;
+};
+__device__ __forceinline__ Line make_line(const double *__restrict__ f, long smm, long sm, long s, long sp,
+                                          long spp) {
+    const double xmm = f[smm], xm = f[sm], x = f[s], xp = f[sp], xpp = f[spp];
+    return Line{xm, x, xp, approx_deriv(xmm, xm, x), approx_deriv(xm, x, xp), approx_deriv(x, xp, xpp)};
+}
+__device__ __forceinline__ double right_fwd(const Line &l) { return l.qp - l.Dp / 2; }
+__device__ __forceinline__ double left_fwd(const Line &l) { return l.q + l.Dc / 2; }
+__device__ __forceinline__ double right_bwd(const Line &l) { return l.q - l.Dc / 2; }
+__device__ __forceinline__ double left_bwd(const Line &l) { return l.qm + l.Dm / 2; }
+
 // One direction's contribution (eulerStepKernelX/Y/Z, :300-634, with
-// loadSourceTermsX/Y/Z, rhic/src/SourceTerms.cpp:316-419), added into acc.
-template <int NC, int DIR, bool TABLE>
+// loadSourceTermsX/Y/Z, rhic/src/SourceTerms.cpp:316-419), added into acc:
+// r = ((-H_{+1/2} + H_{-1/2}) / d [+ S_d]) * dt, acc += r.  Every face flux
+// uses the owning cell's e as the root solve's previous value, as the
+// reference does.
+template <int NC, int DIR, bool TABLE, class DV, class SINK>
 __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &E, long s, int coord, int ncoord,
-                                            long stride, double ePrev, double t, double t2pow, double ut_c,
-                                            double v_c, double (&acc)[NC], int &nan) {
+                                            long stride, double ePrev, double t, const Recip &rt2pow,
+                                            const Recip &rut, double v_c, SINK &&sink, int &nan, DV &div) {
     auto nb = [&](int o) -> long {
         int c = coord + o;
         c = c < 0 ? 0 : (c >= ncoord ? ncoord - 1 : c);
@@ -106,25 +126,32 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
     };
     const long sm2 = nb(-2), sm1 = nb(-1), sp1 = nb(1), sp2 = nb(2);
     const double d = DIR == 0 ? P.dx : (DIR == 1 ? P.dy : P.dz);
-    double r[NC];
+    const Recip rd = make_recip(d);
+
+    // Half-site primitive states of the two faces (four root solves).
+    Side Rf, Lf, Rb, Lb;
     {
-        double H[NC];
-        face_flux<NC, DIR, TABLE>(P, E, sm1, s, sp1, sp2, ePrev, t, H, nan);
+        double qRf[9], qLf[9], qRb[9], qLb[9];
 #pragma unroll
-        for (int n = 0; n < NC; ++n) r[n] = -H[n];
-    }
-    {
-        double H[NC];
-        face_flux<NC, DIR, TABLE>(P, E, sm2, sm1, s, sp1, ePrev, t, H, nan);
-#pragma unroll
-        for (int n = 0; n < NC; ++n) {
-            r[n] += H[n];
-            r[n] /= d;
+        for (int m = 0; m < 9; ++m) {
+            const int n = m < 8 ? m : NC - 1;
+            if (m == 8 && NC == 14) { qRf[m] = qLf[m] = qRb[m] = qLb[m] = 0.0; continue; }
+            const Line l = make_line(P.q[n], sm2, sm1, s, sp1, sp2);
+            qRf[m] = right_fwd(l); qLf[m] = left_fwd(l);
+            qRb[m] = right_bwd(l); qLb[m] = left_bwd(l);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        Rf = make_side<DIR>(recover<TABLE>(E, t, qRf[0], qRf[1], qRf[2], qRf[3], qRf[4], qRf[5], qRf[6], qRf[7], qRf[8], ePrev, nan, div));
+        Lf = make_side<DIR>(recover<TABLE>(E, t, qLf[0], qLf[1], qLf[2], qLf[3], qLf[4], qLf[5], qLf[6], qLf[7], qLf[8], ePrev, nan, div));
+        Rb = make_side<DIR>(recover<TABLE>(E, t, qRb[0], qRb[1], qRb[2], qRb[3], qRb[4], qRb[5], qRb[6], qRb[7], qRb[8], ePrev, nan, div));
+        Lb = make_side<DIR>(recover<TABLE>(E, t, qLb[0], qLb[1], qLb[2], qLb[3], qLb[4], qLb[5], qLb[6], qLb[7], qLb[8], ePrev, nan, div));
     }
+    const double af = face_speed(Rf, Lf, div), ab = face_speed(Rb, Lb, div);
+
+    // pi-gradient terms of T^{tau mu} (central differences, fac = 1/d/2)
     const double fac = 1 / d / 2;
     auto dq = [&](int n) { return (P.q[n][sp1] - P.q[n][sm1]) * fac; };
-    const double v = v_c / ut_c;
+    const double v = div(v_c, rut);
     double S0, S1, S2, S3;
     if constexpr (DIR == 0) {
         double dxpitt = dq(4), dxpitx = dq(5), dxpity = dq(6), dxpitn = dq(7), dxpixx = dq(8), dxpixy = dq(9),
@@ -161,26 +188,42 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
         } else {
             double dnPi = dq(14);
             S0 = dnpitt * v - dnpitn - v * dnPi;
-            S3 = dnpitn * v - dnpinn - dnPi / t2pow;
+            S3 = dnpitn * v - dnpinn - div(dnPi, rt2pow);
         }
         S1 = dnpitx * v - dnpixn;
         S2 = dnpity * v - dnpiyn;
     }
-    r[0] += S0; r[0] *= P.dt;
-    r[1] += S1; r[1] *= P.dt;
-    r[2] += S2; r[2] *= P.dt;
-    r[3] += S3; r[3] *= P.dt;
+    // Per-variable fluxes of both faces, then r and the accumulation.
 #pragma unroll
-    for (int n = 4; n < NC; ++n) r[n] *= P.dt;
-#pragma unroll
-    for (int n = 0; n < NC; ++n) acc[n] += r[n];
+    for (int n = 0; n < NC; ++n) {
+        const Line l = make_line(P.q[n], sm2, sm1, s, sp1, sp2);
+        const double Hf = face_var_flux(Rf, Lf, af, right_fwd(l), left_fwd(l), div);
+        const double Hb = face_var_flux(Rb, Lb, ab, right_bwd(l), left_bwd(l), div);
+        double r = -Hf;
+        r += Hb;
+        r = div(r, rd);
+        if (n == 0) r += S0;
+        if (n == 1) r += S1;
+        if (n == 2) r += S2;
+        if (n == 3) r += S3;
+        r *= P.dt;
+        sink(n, r);
+        __builtin_amdgcn_sched_barrier(0);  // keep one variable's loads live at a time
+    }
 }
 
 // loadSourceTerms2 + setPimunuSourceTerms (rhic/src/SourceTerms.cpp:421-612,
 // :55-313), then acc = q + dt*S (eulerStepKernelSource :233-296).
-template <int NC, bool TABLE>
+// Divisions by the cell's u^tau, by t, t^2, t^3, 2t, pow(t,2), the lattice
+// spacings, dt, 3 and 5 go through correctly rounded reciprocals (divr),
+// which return the division operator's value.
+template <int NC, bool TABLE, class DV>
 __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &E, long s, int i, int j, int k,
-                                            double t, double t2pow, double ecell, double (&acc)[NC]) {
+                                            double t, const Recip &rt2pow, const Recip &rut, double ecell,
+                                            double (&acc)[NC], DV &div) {
+    const Recip rdx = make_recip(P.dx), rdy = make_recip(P.dy), rdz = make_recip(P.dz);
+    const Recip rt = make_recip(t), rdt = make_recip(P.dt);
+    const Recip r3{3.0, 1.0 / 3.0, true}, r5{5.0, 1.0 / 5.0, true};
     const double ttt = acc[0], ttx = acc[1], tty = acc[2], ttn = acc[3];
     const double pitt = acc[4], pitx = acc[5], pity = acc[6], pitn = acc[7], pixx = acc[8], pixy = acc[9],
                  pixn = acc[10], piyy = acc[11], piyn = acc[12], pinn = acc[13];
@@ -197,46 +240,46 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
     const long sl = i > 0 ? s - 1 : s, sr = i < P.nx - 1 ? s + 1 : s;
     const long sb = j > 0 ? s - strY : s, stp = j < P.ny - 1 ? s + strY : s;
     const long sk = k > 0 ? s - strZ : s, sf = k < P.nz - 1 ? s + strZ : s;
-    const double dxut = approx_deriv(utv[sl], ut, utv[sr]) / P.dx;
-    const double dxux = approx_deriv(uxv[sl], ux, uxv[sr]) / P.dx;
-    const double dxuy = approx_deriv(uyv[sl], uy, uyv[sr]) / P.dx;
-    const double dxun = approx_deriv(unv[sl], un, unv[sr]) / P.dx;
-    const double dyut = approx_deriv(utv[sb], ut, utv[stp]) / P.dy;
-    const double dyux = approx_deriv(uxv[sb], ux, uxv[stp]) / P.dy;
-    const double dyuy = approx_deriv(uyv[sb], uy, uyv[stp]) / P.dy;
-    const double dyun = approx_deriv(unv[sb], un, unv[stp]) / P.dy;
-    const double dnut = approx_deriv(utv[sk], ut, utv[sf]) / P.dz;
-    const double dnux = approx_deriv(uxv[sk], ux, uxv[sf]) / P.dz;
-    const double dnuy = approx_deriv(uyv[sk], uy, uyv[sf]) / P.dz;
-    const double dnun = approx_deriv(unv[sk], un, unv[sf]) / P.dz;
-    const double dxp = approx_deriv(pv[sl], p, pv[sr]) / P.dx;
-    const double dyp = approx_deriv(pv[sb], p, pv[stp]) / P.dy;
-    const double dnp = approx_deriv(pv[sk], p, pv[sf]) / P.dz;
+    const double dxut = div(approx_deriv(utv[sl], ut, utv[sr]), rdx);
+    const double dxux = div(approx_deriv(uxv[sl], ux, uxv[sr]), rdx);
+    const double dxuy = div(approx_deriv(uyv[sl], uy, uyv[sr]), rdx);
+    const double dxun = div(approx_deriv(unv[sl], un, unv[sr]), rdx);
+    const double dyut = div(approx_deriv(utv[sb], ut, utv[stp]), rdy);
+    const double dyux = div(approx_deriv(uxv[sb], ux, uxv[stp]), rdy);
+    const double dyuy = div(approx_deriv(uyv[sb], uy, uyv[stp]), rdy);
+    const double dyun = div(approx_deriv(unv[sb], un, unv[stp]), rdy);
+    const double dnut = div(approx_deriv(utv[sk], ut, utv[sf]), rdz);
+    const double dnux = div(approx_deriv(uxv[sk], ux, uxv[sf]), rdz);
+    const double dnuy = div(approx_deriv(uyv[sk], uy, uyv[sf]), rdz);
+    const double dnun = div(approx_deriv(unv[sk], un, unv[sf]), rdz);
+    const double dxp = div(approx_deriv(pv[sl], p, pv[sr]), rdx);
+    const double dyp = div(approx_deriv(pv[sb], p, pv[stp]), rdy);
+    const double dnp = div(approx_deriv(pv[sk], p, pv[sf]), rdz);
 
     double S[NC];
     const double pP = p + Pi;
-    const double tnn = (ecell + pP) * un * un + pP / t / t + pinn;  // Tnn(e, p+Pi, un, pinn, t)
-    const double vx = ux / ut, vy = uy / ut, vn = un / ut;
-    const double dxvx = (dxux - vx * dxut) / ut;
-    const double dyvy = (dyuy - vy * dyut) / ut;
-    const double dnvn = (dnun - vn * dnut) / ut;
+    const double tnn = (ecell + pP) * un * un + div(div(pP, rt), rt) + pinn;  // Tnn(e, p+Pi, un, pinn, t)
+    const double vx = div(ux, rut), vy = div(uy, rut), vn = div(un, rut);
+    const double dxvx = div(dxux - vx * dxut, rut);
+    const double dyvy = div(dyuy - vy * dyut, rut);
+    const double dnvn = div(dnun - vn * dnut, rut);
     const double dkvk = dxvx + dyvy + dnvn;
     const double src0 = 0.0;  // dynamical sources are zero outside IC type 12
-    S[0] = src0 - (ttt / t + t * tnn) + dkvk * (pitt - p - Pi) - vx * dxp - vy * dyp - vn * dnp;
-    S[1] = src0 - ttx / t - dxp + dkvk * pitx;
-    S[2] = src0 - tty / t - dyp + dkvk * pity;
-    S[3] = src0 - 3 * ttn / t - dnp / t2pow + dkvk * pitn;
+    S[0] = src0 - (div(ttt, rt) + t * tnn) + dkvk * (pitt - p - Pi) - vx * dxp - vy * dyp - vn * dnp;
+    S[1] = src0 - div(ttx, rt) - dxp + dkvk * pitx;
+    S[2] = src0 - div(tty, rt) - dyp + dkvk * pity;
+    S[3] = src0 - div(3 * ttn, rt) - div(dnp, rt2pow) + dkvk * pitn;
 
     const double e = ecell;
-    const double T = eos_temperature<TABLE>(E, e);
+    const double T = eos_temperature<TABLE>(E, e, div);
     const double d_etabar = etabar_of_T(P, T);
-    const double taupiInv = T / 5 / d_etabar;
-    const double beta_pi = (e + p) / 5;
-    const double cs2 = eos_cs2<TABLE>(E, e);
+    const double taupiInv = div(T, r5) / d_etabar;
+    const double beta_pi = div(e + p, r5);
+    const double cs2 = eos_cs2<TABLE>(E, e, div);
     const double a = 1.0 / 3.0 - cs2;
     const double a2 = a * a;
     const double beta_Pi = 15 * a2 * (e + p);
-    const double lambda_Pipi = 8 * a / 5;
+    const double lambda_Pipi = div(8 * a, r5);
     double tauPiInv = 0.0;
     if constexpr (NC == 15) {
         // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48); with a == 0 the
@@ -250,41 +293,41 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
         }
     }
     const double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
-    const double d_dt = P.dt;
-    const double dtut = (ut - utp) / d_dt;
-    const double dtux = (ux - uxp) / d_dt;
-    const double dtuy = (uy - uyp) / d_dt;
-    const double dtun = (un - unp) / d_dt;
+    const Recip rt2 = make_recip(t2), rt3 = make_recip(t3), r2t = make_recip(2.0 * t);
+        const double dtut = div(ut - utp, rdt);
+    const double dtux = div(ux - uxp, rdt);
+    const double dtuy = div(uy - uyp, rdt);
+    const double dtun = div(un - unp, rdt);
     const double Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
     const double DuxUpper = ut * dtux + ux * dxux + uy * dyux + un * dnux;
     const double Dux = -DuxUpper;
     const double DuyUpper = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
     const double Duy = -DuyUpper;
-    const double DunUpper = ut * dtun + ux * dxun + uy * dyun + un * dnun + 2 * ut * un / t;
+    const double DunUpper = ut * dtun + ux * dxun + uy * dyun + un * dnun + div(2 * ut * un, rt);
     const double Dun = -t2 * DunUpper;
     const double dut = Dut - t * un * un;
     const double dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
     const double duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
     const double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
-    const double theta = ut / t + dtut + dxux + dyuy + dnun;
-    const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1) * theta / 3.0;
-    const double stx = -(t * un2 * ux) / 2.0 + (dtux - dxut) / 2.0 - (ux * dut + ut * dux) / 2.0 + ut * ux * theta / 3.0;
-    const double sty = -(t * un2 * uy) / 2.0 + (dtuy - dyut) / 2.0 - (uy * dut + ut * duy) / 2.0 + ut * uy * theta / 3.0;
-    const double stn = -un * (2.0 * ut2 + t2 * un2) / (2.0 * t) + (dtun - dnut / t2) / 2.0 -
-                       (un * dut + ut * dun) / 2.0 + ut * un * theta / 3.0;
-    const double sxx = -(dxux + ux * dux) + (1.0 + ux * ux) * theta / 3.0;
-    const double sxy = -(dxuy + dyux) / 2.0 - (uy * dux + ux * duy) / 2.0 + ux * uy * theta / 3.0;
-    const double sxn = -ut * ux * un / t - (dxun + dnux / t2) / 2.0 - (un * dux + ux * dun) / 2.0 + ux * un * theta / 3.0;
-    const double syy = -(dyuy + uy * duy) + (1.0 + uy * uy) * theta / 3.0;
-    const double syn = -ut * uy * un / t - (dyun + dnuy / t2) / 2.0 - (un * duy + uy * dun) / 2.0 + uy * un * theta / 3.0;
-    const double snn = -ut * (1.0 + 2.0 * t2 * un2) / t3 - dnun / t2 - un * dun + (1.0 / t2 + un2) * theta / 3.0;
+    const double theta = div(ut, rt) + dtut + dxux + dyuy + dnun;
+    const double stt = -t * ut * un2 + (dtut - ut * dut) + div((ut2 - 1) * theta, r3);
+    const double stx = -(t * un2 * ux) / 2.0 + (dtux - dxut) / 2.0 - (ux * dut + ut * dux) / 2.0 + div(ut * ux * theta, r3);
+    const double sty = -(t * un2 * uy) / 2.0 + (dtuy - dyut) / 2.0 - (uy * dut + ut * duy) / 2.0 + div(ut * uy * theta, r3);
+    const double stn = div(-un * (2.0 * ut2 + t2 * un2), r2t) + (dtun - div(dnut, rt2)) / 2.0 -
+                       (un * dut + ut * dun) / 2.0 + div(ut * un * theta, r3);
+    const double sxx = -(dxux + ux * dux) + div((1.0 + ux * ux) * theta, r3);
+    const double sxy = -(dxuy + dyux) / 2.0 - (uy * dux + ux * duy) / 2.0 + div(ux * uy * theta, r3);
+    const double sxn = div(-ut * ux * un, rt) - (dxun + div(dnux, rt2)) / 2.0 - (un * dux + ux * dun) / 2.0 + div(ux * un * theta, r3);
+    const double syy = -(dyuy + uy * duy) + div((1.0 + uy * uy) * theta, r3);
+    const double syn = div(-ut * uy * un, rt) - (dyun + div(dnuy, rt2)) / 2.0 - (un * duy + uy * dun) / 2.0 + div(uy * un * theta, r3);
+    const double snn = div(-ut * (1.0 + 2.0 * t2 * un2), rt3) - div(dnun, rt2) - un * dun + div((rt2.r + un2) * theta, r3);
     const double wtx = (-dtux - dxut + ut * dux - ux * dut - t * un2 * ux) / 2.0;
     const double wty = (-dtuy - dyut + ut * duy - uy * dut - t * un2 * uy) / 2.0;
     const double wtn = (-t2 * dtun - 2.0 * t * un - dnut - ut * Dun - t2 * un * dut - t3 * un * un2) / 2.0;
     const double wxy = (dxuy - dyux + ux * duy - uy * dux) / 2.0;
     const double wxn = (t2 * dxun - dnux - ux * Dun - t2 * un * dux) / 2.0;
     const double wyn = (t2 * dyun - dnuy - uy * Dun - t2 * un * duy) / 2.0;
-    const double wxt = wtx, wyt = wty, wnt = wtn / t2, wyx = -wxy, wnx = -wxn / t2, wny = -wyn / t2;
+    const double wxt = wtx, wyt = wty, wnt = div(wtn, rt2), wyx = -wxy, wnx = div(-wxn, rt2), wny = div(-wyn, rt2);
     const double I1tt = 2.0 * ut * (pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun);
     const double I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
                         (pitn * ux + pixn * ut) * Dun;
@@ -317,22 +360,22 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
     const double ux2 = ux * ux, uy2 = uy * uy;
     const double ps = pitt * stt - 2.0 * pitx * stx - 2.0 * pity * sty + pixx * sxx + 2.0 * pixy * sxy + piyy * syy -
                       2.0 * pitn * stn * t2 + 2.0 * pixn * sxn * t2 + 2.0 * piyn * syn * t2 + pinn * snn * t2 * t2;
-    const double I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - (1.0 - ut2) * ps / 3.0;
+    const double I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - div((1.0 - ut2) * ps, r3);
     const double I4tx = (pitt * stx + pitx * stt) / 2.0 - (pitx * sxx + pixx * stx) / 2.0 -
-                        (pity * sxy + pixy * sty) / 2.0 - t2 * (pitn * sxn + pixn * stn) / 2.0 + (ut * ux) * ps / 3.0;
+                        (pity * sxy + pixy * sty) / 2.0 - t2 * (pitn * sxn + pixn * stn) / 2.0 + div((ut * ux) * ps, r3);
     const double I4ty = (pitt * sty + pity * stt) / 2.0 - (pitx * sxy + pixy * stx) / 2.0 -
-                        (pity * syy + piyy * sty) / 2.0 - t2 * (pitn * syn + piyn * stn) / 2.0 + (ut * uy) * ps / 3.0;
+                        (pity * syy + piyy * sty) / 2.0 - t2 * (pitn * syn + piyn * stn) / 2.0 + div((ut * uy) * ps, r3);
     const double I4tn = (pitt * stn + pitn * stt) / 2.0 - (pitx * sxn + pixn * stx) / 2.0 -
-                        (pity * syn + piyn * sty) / 2.0 - t2 * (pitn * snn + pinn * stn) / 2.0 + (ut * un) * ps / 3.0;
-    const double I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + (1.0 + ux2) * ps / 3.0;
+                        (pity * syn + piyn * sty) / 2.0 - t2 * (pitn * snn + pinn * stn) / 2.0 + div((ut * un) * ps, r3);
+    const double I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + div((1.0 + ux2) * ps, r3);
     const double I4xy = (pitx * sty + pity * stx) / 2.0 - (pixx * sxy + pixy * sxx) / 2.0 -
-                        (pixy * syy + piyy * sxy) / 2.0 - t2 * (pixn * syn + piyn * sxn) / 2.0 + (ux * uy) * ps / 3.0;
+                        (pixy * syy + piyy * sxy) / 2.0 - t2 * (pixn * syn + piyn * sxn) / 2.0 + div((ux * uy) * ps, r3);
     const double I4xn = (pitx * stn + pitn * stx) / 2.0 - (pixx * sxn + pixn * sxx) / 2.0 -
-                        (pixy * syn + piyn * sxy) / 2.0 - t2 * (pixn * snn + pinn * sxn) / 2.0 + (ux * un) * ps / 3.0;
-    const double I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + (1.0 + uy2) * ps / 3.0;
+                        (pixy * syn + piyn * sxy) / 2.0 - t2 * (pixn * snn + pinn * sxn) / 2.0 + div((ux * un) * ps, r3);
+    const double I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + div((1.0 + uy2) * ps, r3);
     const double I4yn = (pity * stn + pitn * sty) / 2.0 - (pixy * sxn + pixn * sxy) / 2.0 -
-                        (piyy * syn + piyn * syy) / 2.0 - t2 * (piyn * snn + pinn * syn) / 2.0 + (uy * un) * ps / 3.0;
-    const double I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + (1.0 / t2 + un2) * ps / 3.0;
+                        (piyy * syn + piyn * syy) / 2.0 - t2 * (piyn * snn + pinn * syn) / 2.0 + div((uy * un) * ps, r3);
+    const double I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + div((rt2.r + un2) * ps, r3);
     const double Itt = I1tt + delta_pipi * I2tt - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
     const double Itx = I1tx + delta_pipi * I2tx - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
     const double Ity = I1ty + delta_pipi * I2ty - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
@@ -343,20 +386,20 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
     const double Iyy = I1yy + delta_pipi * I2yy - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
     const double Iyn = I1yn + delta_pipi * I2yn - I3yn + tau_pipi * I4yn - lambda_piPi * Pi * syn;
     const double Inn = I1nn + delta_pipi * I2nn - I3nn + tau_pipi * I4nn - lambda_piPi * Pi * snn;
-    S[4] = (2.0 * beta_pi * stt - pitt * taupiInv - Itt - 2.0 * un * t * pitn) / ut + pitt * dkvk;
-    S[5] = (2.0 * beta_pi * stx - pitx * taupiInv - Itx - un * t * pixn) / ut + pitx * dkvk;
-    S[6] = (2.0 * beta_pi * sty - pity * taupiInv - Ity - un * t * piyn) / ut + pity * dkvk;
-    S[7] = (2.0 * beta_pi * stn - pitn * taupiInv - Itn - un * t * pinn - (ut * pitn + un * pitt) / t) / ut +
+    S[4] = div(2.0 * beta_pi * stt - pitt * taupiInv - Itt - 2.0 * un * t * pitn, rut) + pitt * dkvk;
+    S[5] = div(2.0 * beta_pi * stx - pitx * taupiInv - Itx - un * t * pixn, rut) + pitx * dkvk;
+    S[6] = div(2.0 * beta_pi * sty - pity * taupiInv - Ity - un * t * piyn, rut) + pity * dkvk;
+    S[7] = div(2.0 * beta_pi * stn - pitn * taupiInv - Itn - un * t * pinn - div(ut * pitn + un * pitt, rt), rut) +
            pitn * dkvk;
-    S[8] = (2.0 * beta_pi * sxx - pixx * taupiInv - Ixx) / ut + pixx * dkvk;
-    S[9] = (2.0 * beta_pi * sxy - pixy * taupiInv - Ixy) / ut + pixy * dkvk;
-    S[10] = (2.0 * beta_pi * sxn - pixn * taupiInv - Ixn - (ut * pixn + un * pitx) / t) / ut + pixn * dkvk;
-    S[11] = (2.0 * beta_pi * syy - piyy * taupiInv - Iyy) / ut + piyy * dkvk;
-    S[12] = (2.0 * beta_pi * syn - piyn * taupiInv - Iyn - (ut * piyn + un * pity) / t) / ut + piyn * dkvk;
-    S[13] = (2.0 * beta_pi * snn - pinn * taupiInv - Inn - 2 * (ut * pinn + un * pitn) / t) / ut + pinn * dkvk;
+    S[8] = div(2.0 * beta_pi * sxx - pixx * taupiInv - Ixx, rut) + pixx * dkvk;
+    S[9] = div(2.0 * beta_pi * sxy - pixy * taupiInv - Ixy, rut) + pixy * dkvk;
+    S[10] = div(2.0 * beta_pi * sxn - pixn * taupiInv - Ixn - div(ut * pixn + un * pitx, rt), rut) + pixn * dkvk;
+    S[11] = div(2.0 * beta_pi * syy - piyy * taupiInv - Iyy, rut) + piyy * dkvk;
+    S[12] = div(2.0 * beta_pi * syn - piyn * taupiInv - Iyn - div(ut * piyn + un * pity, rt), rut) + piyn * dkvk;
+    S[13] = div(2.0 * beta_pi * snn - pinn * taupiInv - Inn - div(2 * (ut * pinn + un * pitn), rt), rut) + pinn * dkvk;
     if constexpr (NC == 15) {
         const double dPi = -beta_Pi * theta - Pi * tauPiInv - delta_PiPi * Pi * theta + lambda_Pipi * ps;
-        S[14] = dPi / ut + Pi * dkvk;
+        S[14] = div(dPi, rut) + Pi * dkvk;
     } else {
         (void)beta_Pi; (void)lambda_Pipi; (void)tauPiInv;
     }
@@ -411,58 +454,142 @@ __device__ __forceinline__ double regulate(const StageParams &P, double t, doubl
     return facShear;
 }
 
+template <int NC>
+struct CellOut {
+    double acc[NC];
+    Prim w;
+    double fs;
+};
+
+// Everything one cell computes in a stage, into registers (no stores).
+template <int NC, bool TABLE, bool DIM3, class DV>
+__device__ __forceinline__ void cell_update(const StageParams &P, const EosDev &E, long s, int i, int j, int k,
+                                            double t, double t2pow, double t_new, CellOut<NC> &o, int &nan,
+                                            DV &div) {
+    const double ecell = P.e[s];
+    const Recip rut = make_recip(P.u[0][s]);
+    const Recip rt2pow = make_recip(t2pow);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) o.acc[n] = P.q[n][s];
+    source_init<NC, TABLE>(P, E, s, i, j, k, t, rt2pow, rut, ecell, o.acc, div);
+    __builtin_amdgcn_sched_barrier(0);
+    auto add = [&](int n, double r) { o.acc[n] += r; };
+    dir_contrib<NC, 0, TABLE>(P, E, s, i, P.nx, 1, ecell, t, rt2pow, rut, P.u[1][s], add, nan, div);
+    __builtin_amdgcn_sched_barrier(0);
+    dir_contrib<NC, 1, TABLE>(P, E, s, j, P.ny, P.nx, ecell, t, rt2pow, rut, P.u[2][s], add, nan, div);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (DIM3) {
+        dir_contrib<NC, 2, TABLE>(P, E, s, k, P.nz, (long)P.nx * P.ny, ecell, t, rt2pow, rut, P.u[3][s], add, nan,
+                                  div);
+    } else {
+        // With one rapidity cell every point of the eta line is the cell itself:
+        // both face fluxes coincide and the contribution is exactly +0.
+#pragma unroll
+        for (int n = 0; n < NC; ++n) o.acc[n] += 0.0;
+    }
+    if (P.stage == 1) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            o.acc[n] += P.qn[n][s];
+            o.acc[n] /= 2;
+        }
+    }
+    o.w = recover<TABLE>(E, t_new, o.acc[0], o.acc[1], o.acc[2], o.acc[3], o.acc[4], o.acc[5], o.acc[6], o.acc[7],
+                         NC == 15 ? o.acc[NC - 1] : 0.0, ecell, nan, div);
+    o.fs = regulate<NC>(P, t_new, o.acc, o.w, nan);
+}
+
+template <int NC>
+__device__ __forceinline__ void store_cell(const StageParams &P, long s, const CellOut<NC> &o, int nan) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) P.qo[n][s] = o.acc[n];
+    P.eo[s] = o.w.e;
+    P.po[s] = o.w.p;
+    P.uo[0][s] = o.w.ut;
+    P.uo[1][s] = o.w.ux;
+    P.uo[2][s] = o.w.uy;
+    P.uo[3][s] = o.w.un;
+    if (P.regfac) P.regfac[s] = o.fs;
+    if (nan != NAN_NONE) report_nan(P.err, nan, s);
+}
+
+// Wave-specialised stage kernel.  A block owns 64 consecutive cells and has
+// one wave per role: wave 0 evaluates acc = q + dt*S (the source pass), waves
+// 1..3 the X, Y, eta contributions r_d of the same 64 cells.  The partial
+// results meet in LDS and wave 0 sums them in the reference's order
+// ((acc + r_x) + r_y) + r_eta, applies the convex combination of stage 2,
+// recovers (e, p, u), regulates and stores.
+constexpr int CELLS_PER_BLOCK = 64;
+
 template <int NC, bool TABLE, bool DIM3>
-__global__ __launch_bounds__(256) void stage_kernel(const StageParams P, const EosDev E) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void stage_kernel(const StageParams P,
+                                                                                             const EosDev E) {
+    constexpr int ROLES = DIM3 ? 4 : 3;
+    __shared__ double part[ROLES][NC][CELLS_PER_BLOCK];
+    __shared__ int flags[ROLES][CELLS_PER_BLOCK];  // NaN site of each role
+    const int role = threadIdx.x / CELLS_PER_BLOCK;
+    const int lane = threadIdx.x % CELLS_PER_BLOCK;
     const long N = (long)P.nx * P.ny * P.nz;
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    const int i = (int)(s % P.nx);
-    const long rest = s / P.nx;
+    const long s = (long)blockIdx.x * CELLS_PER_BLOCK + lane;
+    const bool valid = s < N && role < ROLES;
+    const long sc = valid ? s : 0;
+    const int i = (int)(sc % P.nx);
+    const long rest = sc / P.nx;
     const int j = (int)(rest % P.ny);
     const int k = (int)(rest / P.ny);
     const StepTimes st = P.tab ? P.tab[*P.base + P.off] : P.st;
     const double t = P.stage ? st.t2 : st.t1;
     const double t2pow = P.stage ? st.t2sq : st.t1sq;
     const double t_new = st.t2;
+    if (valid) {
+        DivFast div;
+        int nan = NAN_NONE;
+        const double ecell = P.e[s];
+        const Recip rut = make_recip(P.u[0][s]);
+        const Recip rt2pow = make_recip(t2pow);
+        if (role == 0) {
+            double acc[NC];
+#pragma unroll
+            for (int n = 0; n < NC; ++n) acc[n] = P.q[n][s];
+            source_init<NC, TABLE>(P, E, s, i, j, k, t, rt2pow, rut, ecell, acc, div);
+#pragma unroll
+            for (int n = 0; n < NC; ++n) part[0][n][lane] = acc[n];
+        } else {
+            auto put = [&](int n, double r) { part[role][n][lane] = r; };
+            if (role == 1) dir_contrib<NC, 0, TABLE>(P, E, s, i, P.nx, 1, ecell, t, rt2pow, rut, P.u[1][s], put, nan, div);
+            else if (role == 2) dir_contrib<NC, 1, TABLE>(P, E, s, j, P.ny, P.nx, ecell, t, rt2pow, rut, P.u[2][s], put, nan, div);
+            else if constexpr (DIM3) dir_contrib<NC, 2, TABLE>(P, E, s, k, P.nz, (long)P.nx * P.ny, ecell, t, rt2pow, rut, P.u[3][s], put, nan, div);
+        }
+        flags[role][lane] = nan;
+    }
+    __syncthreads();
+    if (role != 0 || !valid) return;
+    // NaN site in the reference's detection order: X, Y, eta fluxes, then the cell
     int nan = NAN_NONE;
-
-    const double ecell = P.e[s];
-    double acc[NC];
+    for (int r = 1; r < ROLES && nan == NAN_NONE; ++r) nan = flags[r][lane];
+    CellOut<NC> o;
 #pragma unroll
-    for (int n = 0; n < NC; ++n) acc[n] = P.q[n][s];
-    source_init<NC, TABLE>(P, E, s, i, j, k, t, t2pow, ecell, acc);
-
-    const double ut_c = P.u[0][s];
-    dir_contrib<NC, 0, TABLE>(P, E, s, i, P.nx, 1, ecell, t, t2pow, ut_c, P.u[1][s], acc, nan);
-    dir_contrib<NC, 1, TABLE>(P, E, s, j, P.ny, P.nx, ecell, t, t2pow, ut_c, P.u[2][s], acc, nan);
-    if constexpr (DIM3) {
-        dir_contrib<NC, 2, TABLE>(P, E, s, k, P.nz, (long)P.nx * P.ny, ecell, t, t2pow, ut_c, P.u[3][s], acc, nan);
-    } else {
-        // With one rapidity cell every point of the eta line is the cell itself:
-        // both face fluxes coincide and the contribution is exactly +0.
-#pragma unroll
-        for (int n = 0; n < NC; ++n) acc[n] += 0.0;
+    for (int n = 0; n < NC; ++n) {
+        double a = part[0][n][lane];
+        a += part[1][n][lane];
+        a += part[2][n][lane];
+        if constexpr (DIM3) a += part[3][n][lane];
+        else a += 0.0;  // one rapidity cell: the eta contribution is exactly +0
+        o.acc[n] = a;
     }
     if (P.stage == 1) {
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            acc[n] += P.qn[n][s];
-            acc[n] /= 2;
+            o.acc[n] += P.qn[n][s];
+            o.acc[n] /= 2;
         }
     }
-    const Prim w = recover<TABLE>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
-                                  NC == 15 ? acc[NC - 1] : 0.0, ecell, nan);
-    const double fs = regulate<NC>(P, t_new, acc, w, nan);
-#pragma unroll
-    for (int n = 0; n < NC; ++n) P.qo[n][s] = acc[n];
-    P.eo[s] = w.e;
-    P.po[s] = w.p;
-    P.uo[0][s] = w.ut;
-    P.uo[1][s] = w.ux;
-    P.uo[2][s] = w.uy;
-    P.uo[3][s] = w.un;
-    if (P.regfac) P.regfac[s] = fs;
-    if (nan != NAN_NONE) report_nan(P.err, nan, s);
+    DivFast div;
+    int nan_cell = NAN_NONE;
+    o.w = recover<TABLE>(E, t_new, o.acc[0], o.acc[1], o.acc[2], o.acc[3], o.acc[4], o.acc[5], o.acc[6], o.acc[7],
+                         NC == 15 ? o.acc[NC - 1] : 0.0, P.e[s], nan_cell, div);
+    o.fs = regulate<NC>(P, t_new, o.acc, o.w, nan_cell);
+    store_cell<NC>(P, s, o, nan != NAN_NONE ? nan : nan_cell);
 }
 
 }  // namespace vh

@@ -132,6 +132,50 @@ double vh_eos_temperature(const vh_handle *h, double e);
  * from HIP events on the handle's stream; 0 if none. */
 double vh_last_kernel_ms(const vh_handle *h);
 
+/* ---- initial conditions (host side; no device needed) ------------------ */
+
+/* Seeded Monte-Carlo Glauber energy density (the reference's IC type 4,
+ * rhic/src/InitialConditions.cpp:1120-1140 with its generator in
+ * rhic/src/MonteCarloGlauberModel.cpp:18-111, here with every nuclear
+ * parameter and a 64-bit seed exposed).  e = e0 * T(x,y) * L(eta) + e_floor,
+ * T a sum of normalised Gaussians (width) over wounded nucleons, L the
+ * reference's plateau + Gaussian tails (InitialConditions.cpp:1046-1064). */
+typedef struct {
+    int A;                 /* nucleons per nucleus */
+    double R, a;           /* Woods-Saxon radius and diffuseness [fm] */
+    double sigma_nn_fm2;   /* inelastic NN cross section [fm^2] */
+    double b;              /* impact parameter [fm]; < 0: uniform in [0, b_max] from the seed */
+    double b_max;
+    double width;          /* participant Gaussian width [fm] */
+    double eta_flat;       /* rapidityMean: full width of the plateau */
+    double eta_variance;   /* rapidityVariance of the tails */
+    double e0;             /* normalisation [fm^-2] */
+    double e_floor;        /* additive floor [fm^-4] (reference: 1e-3) */
+    uint64_t seed;
+} vh_glauber_params;
+
+typedef struct {
+    double b;
+    int n_part, n_coll;
+} vh_glauber_info;
+
+vh_status vh_ic_mc_glauber(const vh_glauber_params *gp, int nx, int ny, int nz, double dx, double dy, double dz,
+                           double *e_out, vh_glauber_info *info);
+
+/* input/Tmunu.dat (IC type 11) reader, values through float as the reference
+ * reads them (rhic/src/InitialConditions.cpp:41-105); Pi may be NULL.  The
+ * writer produces a file that reader (and the reference) accept. */
+vh_status vh_ic_read_tmunu(const char *path, int nx, int ny, int nz, double *e, double *p, double *ut, double *ux,
+                           double *uy, double *un, double *pi10, double *Pi);
+vh_status vh_ic_write_tmunu(const char *path, int nx, int ny, int nz, double dx, double dy, double dz,
+                            const double *e, const double *p, const double *ut, const double *ux, const double *uy,
+                            const double *un, const double *pi10, const double *Pi);
+
+/* Builds (without launching) the HIP graph vh_steps replays for this count,
+ * so a timed vh_steps call measures replay only.  No counterpart in the
+ * reference (which has no graphs). */
+vh_status vh_prepare_steps(vh_handle *h, int count);
+
 const char *vh_status_string(vh_status s);
 const char *vh_build_info(void);
 

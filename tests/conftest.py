@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+# The reference build races in OpenMP loops (unreduced counter in
+# checkForCellsAboveTc, shared ghost-cell indices); goldens and comparisons run
+# it on one thread.  Set before any OpenMP library is loaded.
+os.environ["OMP_NUM_THREADS"] = "1"
+
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 
