@@ -173,13 +173,14 @@ __device__ __forceinline__ double approx_deriv(double x, double y, double z) {
 }
 
 // Newton-Raphson + bisection root solve of the table EoS,
-// rhic/src/EnergyMomentumTensor.cpp:26-111.  One out-of-line body serves all
-// thirteen call sites of a stage; it uses the division operator throughout.
+// rhic/src/EnergyMomentumTensor.cpp:26-111, with the division operator
+// throughout.  Callers instantiate it inside loops so each kernel carries
+// few copies.
 struct RootResult {
     double e;
     int nan;
 };
-__device__ __noinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
+__device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
     DivExact dx;
     double e0 = ePrev;
     for (int j = 0; j < MAX_ITERS_NR; ++j) {

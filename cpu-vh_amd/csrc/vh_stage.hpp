@@ -110,14 +110,25 @@ __device__ __forceinline__ double left_fwd(const Line &l) { return l.q + l.Dc / 
 __device__ __forceinline__ double right_bwd(const Line &l) { return l.q - l.Dc / 2; }
 __device__ __forceinline__ double left_bwd(const Line &l) { return l.qm + l.Dm / 2; }
 
+// Components of pi^{tau mu} and pi^{a mu} (a = x, y, eta) in the conserved
+// ordering, for the pi-gradient terms of loadSourceTermsX/Y/Z
+// (SourceTerms.cpp:316-419): S[mu] = d_a(pi^{tau mu}) v_a - d_a(pi^{a mu}).
+__device__ __forceinline__ int pi_a_mu(int dir, int mu) {
+    // x: pitx pixx pixy pixn | y: pity pixy piyy piyn | eta: pitn pixn piyn pinn
+    const int tab = dir == 0 ? 0x0A090805 : (dir == 1 ? 0x0C0B0906 : 0x0D0C0A07);
+    return (tab >> (8 * mu)) & 0xff;
+}
+
 // One direction's contribution (eulerStepKernelX/Y/Z, :300-634, with
-// loadSourceTermsX/Y/Z, rhic/src/SourceTerms.cpp:316-419), added into acc:
-// r = ((-H_{+1/2} + H_{-1/2}) / d [+ S_d]) * dt, acc += r.  Every face flux
+// loadSourceTermsX/Y/Z, rhic/src/SourceTerms.cpp:316-419), handed to sink
+// variable by variable: r = ((-H_{+1/2} + H_{-1/2}) / d [+ S_d]) * dt.  dir
+// is wave-uniform, so the three directions share one body.  Every face flux
 // uses the owning cell's e as the root solve's previous value, as the
-// reference does.
-template <int NC, int DIR, bool TABLE, class DV, class SINK>
-__device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &E, long s, int coord, int ncoord,
-                                            long stride, double ePrev, double t, const Recip &rt2pow,
+// reference does; the four half-site root solves run in a loop so the solver
+// is instantiated once.
+template <int NC, bool TABLE, class DV, class SINK>
+__device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &E, long s, int dir, int coord,
+                                            int ncoord, long stride, double ePrev, double t, const Recip &rt2pow,
                                             const Recip &rut, double v_c, SINK &&sink, int &nan, DV &div) {
     auto nb = [&](int o) -> long {
         int c = coord + o;
@@ -125,75 +136,53 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
         return s + (long)(c - coord) * stride;
     };
     const long sm2 = nb(-2), sm1 = nb(-1), sp1 = nb(1), sp2 = nb(2);
-    const double d = DIR == 0 ? P.dx : (DIR == 1 ? P.dy : P.dz);
+    const double d = dir == 0 ? P.dx : (dir == 1 ? P.dy : P.dz);
     const Recip rd = make_recip(d);
 
-    // Half-site primitive states of the two faces (four root solves).
-    Side Rf, Lf, Rb, Lb;
-    {
-        double qRf[9], qLf[9], qRb[9], qLb[9];
+    // Half-site values of the recovery inputs (q[0..7], Pi) on both faces.
+    double qRf[9], qLf[9], qRb[9], qLb[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) {
-            const int n = m < 8 ? m : NC - 1;
-            if (m == 8 && NC == 14) { qRf[m] = qLf[m] = qRb[m] = qLb[m] = 0.0; continue; }
-            const Line l = make_line(P.q[n], sm2, sm1, s, sp1, sp2);
-            qRf[m] = right_fwd(l); qLf[m] = left_fwd(l);
-            qRb[m] = right_bwd(l); qLb[m] = left_bwd(l);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        Rf = make_side<DIR>(recover<TABLE>(E, t, qRf[0], qRf[1], qRf[2], qRf[3], qRf[4], qRf[5], qRf[6], qRf[7], qRf[8], ePrev, nan, div));
-        Lf = make_side<DIR>(recover<TABLE>(E, t, qLf[0], qLf[1], qLf[2], qLf[3], qLf[4], qLf[5], qLf[6], qLf[7], qLf[8], ePrev, nan, div));
-        Rb = make_side<DIR>(recover<TABLE>(E, t, qRb[0], qRb[1], qRb[2], qRb[3], qRb[4], qRb[5], qRb[6], qRb[7], qRb[8], ePrev, nan, div));
-        Lb = make_side<DIR>(recover<TABLE>(E, t, qLb[0], qLb[1], qLb[2], qLb[3], qLb[4], qLb[5], qLb[6], qLb[7], qLb[8], ePrev, nan, div));
+    for (int m = 0; m < 9; ++m) {
+        if (m == 8 && NC == 14) { qRf[m] = qLf[m] = qRb[m] = qLb[m] = 0.0; continue; }
+        const int n = m < 8 ? m : NC - 1;
+        const Line l = make_line(P.q[n], sm2, sm1, s, sp1, sp2);
+        qRf[m] = right_fwd(l); qLf[m] = left_fwd(l);
+        qRb[m] = right_bwd(l); qLb[m] = left_bwd(l);
     }
+    // Four root solves: forward-right, forward-left, backward-right, backward-left.
+    Side sd[4];
+#pragma unroll 1
+    for (int kk = 0; kk < 4; ++kk) {
+        double x[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) x[m] = kk == 0 ? qRf[m] : (kk == 1 ? qLf[m] : (kk == 2 ? qRb[m] : qLb[m]));
+        const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], ePrev, nan, div);
+        const Side side{dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un), make_recip(w.ut)};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == kk) sd[q] = side;
+    }
+    const Side Rf = sd[0], Lf = sd[1], Rb = sd[2], Lb = sd[3];
     const double af = face_speed(Rf, Lf, div), ab = face_speed(Rb, Lb, div);
 
     // pi-gradient terms of T^{tau mu} (central differences, fac = 1/d/2)
     const double fac = 1 / d / 2;
     auto dq = [&](int n) { return (P.q[n][sp1] - P.q[n][sm1]) * fac; };
     const double v = div(v_c, rut);
-    double S0, S1, S2, S3;
-    if constexpr (DIR == 0) {
-        double dxpitt = dq(4), dxpitx = dq(5), dxpity = dq(6), dxpitn = dq(7), dxpixx = dq(8), dxpixy = dq(9),
-               dxpixn = dq(10);
-        if constexpr (NC == 14) {
-            S0 = dxpitt * v - dxpitx;
-            S1 = dxpitx * v - dxpixx;
-        } else {
-            double dxPi = dq(14);
-            S0 = dxpitt * v - dxpitx - v * dxPi;
-            S1 = dxpitx * v - dxpixx - dxPi;
-        }
-        S2 = dxpity * v - dxpixy;
-        S3 = dxpitn * v - dxpixn;
-    } else if constexpr (DIR == 1) {
-        double dypitt = dq(4), dypitx = dq(5), dypity = dq(6), dypitn = dq(7), dypixy = dq(9), dypiyy = dq(11),
-               dypiyn = dq(12);
-        if constexpr (NC == 14) {
-            S0 = dypitt * v - dypity;
-            S2 = dypity * v - dypiyy;
-        } else {
-            double dyPi = dq(14);
-            S0 = dypitt * v - dypity - v * dyPi;
-            S2 = dypity * v - dypiyy - dyPi;
-        }
-        S1 = dypitx * v - dypixy;
-        S3 = dypitn * v - dypiyn;
-    } else {
-        double dnpitt = dq(4), dnpitx = dq(5), dnpity = dq(6), dnpitn = dq(7), dnpixn = dq(10), dnpiyn = dq(12),
-               dnpinn = dq(13);
-        if constexpr (NC == 14) {
-            S0 = dnpitt * v - dnpitn;
-            S3 = dnpitn * v - dnpinn;
-        } else {
-            double dnPi = dq(14);
-            S0 = dnpitt * v - dnpitn - v * dnPi;
-            S3 = dnpitn * v - dnpinn - div(dnPi, rt2pow);
-        }
-        S1 = dnpitx * v - dnpixn;
-        S2 = dnpity * v - dnpiyn;
+    double S[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) S[mu] = dq(4 + mu) * v - dq(pi_a_mu(dir, mu));
+    if constexpr (NC == 15) {
+        const double dPi = dq(14);
+        S[0] = S[0] - v * dPi;
+        // S[a] -= dPi, with dPi / pow(t,2) along eta (SourceTerms.cpp:345,380,415)
+        const double dPa = dir == 2 ? div(dPi, rt2pow) : dPi;
+#pragma unroll
+        for (int mu = 1; mu < 4; ++mu)
+            if (mu == dir + 1) S[mu] = S[mu] - dPa;
     }
-    // Per-variable fluxes of both faces, then r and the accumulation.
+
+    // Per-variable fluxes of both faces, then r.
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
         const Line l = make_line(P.q[n], sm2, sm1, s, sp1, sp2);
@@ -202,18 +191,13 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
         double r = -Hf;
         r += Hb;
         r = div(r, rd);
-        if (n == 0) r += S0;
-        if (n == 1) r += S1;
-        if (n == 2) r += S2;
-        if (n == 3) r += S3;
+        if (n < 4) r += S[n];
         r *= P.dt;
         sink(n, r);
         __builtin_amdgcn_sched_barrier(0);  // keep one variable's loads live at a time
     }
 }
 
-// loadSourceTerms2 + setPimunuSourceTerms (rhic/src/SourceTerms.cpp:421-612,
-// :55-313), then acc = q + dt*S (eulerStepKernelSource :233-296).
 // Divisions by the cell's u^tau, by t, t^2, t^3, 2t, pow(t,2), the lattice
 // spacings, dt, 3 and 5 go through correctly rounded reciprocals (divr),
 // which return the division operator's value.
@@ -461,44 +445,6 @@ struct CellOut {
     double fs;
 };
 
-// Everything one cell computes in a stage, into registers (no stores).
-template <int NC, bool TABLE, bool DIM3, class DV>
-__device__ __forceinline__ void cell_update(const StageParams &P, const EosDev &E, long s, int i, int j, int k,
-                                            double t, double t2pow, double t_new, CellOut<NC> &o, int &nan,
-                                            DV &div) {
-    const double ecell = P.e[s];
-    const Recip rut = make_recip(P.u[0][s]);
-    const Recip rt2pow = make_recip(t2pow);
-#pragma unroll
-    for (int n = 0; n < NC; ++n) o.acc[n] = P.q[n][s];
-    source_init<NC, TABLE>(P, E, s, i, j, k, t, rt2pow, rut, ecell, o.acc, div);
-    __builtin_amdgcn_sched_barrier(0);
-    auto add = [&](int n, double r) { o.acc[n] += r; };
-    dir_contrib<NC, 0, TABLE>(P, E, s, i, P.nx, 1, ecell, t, rt2pow, rut, P.u[1][s], add, nan, div);
-    __builtin_amdgcn_sched_barrier(0);
-    dir_contrib<NC, 1, TABLE>(P, E, s, j, P.ny, P.nx, ecell, t, rt2pow, rut, P.u[2][s], add, nan, div);
-    __builtin_amdgcn_sched_barrier(0);
-    if constexpr (DIM3) {
-        dir_contrib<NC, 2, TABLE>(P, E, s, k, P.nz, (long)P.nx * P.ny, ecell, t, rt2pow, rut, P.u[3][s], add, nan,
-                                  div);
-    } else {
-        // With one rapidity cell every point of the eta line is the cell itself:
-        // both face fluxes coincide and the contribution is exactly +0.
-#pragma unroll
-        for (int n = 0; n < NC; ++n) o.acc[n] += 0.0;
-    }
-    if (P.stage == 1) {
-#pragma unroll
-        for (int n = 0; n < NC; ++n) {
-            o.acc[n] += P.qn[n][s];
-            o.acc[n] /= 2;
-        }
-    }
-    o.w = recover<TABLE>(E, t_new, o.acc[0], o.acc[1], o.acc[2], o.acc[3], o.acc[4], o.acc[5], o.acc[6], o.acc[7],
-                         NC == 15 ? o.acc[NC - 1] : 0.0, ecell, nan, div);
-    o.fs = regulate<NC>(P, t_new, o.acc, o.w, nan);
-}
-
 template <int NC>
 __device__ __forceinline__ void store_cell(const StageParams &P, long s, const CellOut<NC> &o, int nan) {
 #pragma unroll
@@ -527,7 +473,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     constexpr int ROLES = DIM3 ? 4 : 3;
     __shared__ double part[ROLES][NC][CELLS_PER_BLOCK];
     __shared__ int flags[ROLES][CELLS_PER_BLOCK];  // NaN site of each role
-    const int role = threadIdx.x / CELLS_PER_BLOCK;
+    const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / CELLS_PER_BLOCK);
     const int lane = threadIdx.x % CELLS_PER_BLOCK;
     const long N = (long)P.nx * P.ny * P.nz;
     const long s = (long)blockIdx.x * CELLS_PER_BLOCK + lane;
@@ -556,9 +502,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
             for (int n = 0; n < NC; ++n) part[0][n][lane] = acc[n];
         } else {
             auto put = [&](int n, double r) { part[role][n][lane] = r; };
-            if (role == 1) dir_contrib<NC, 0, TABLE>(P, E, s, i, P.nx, 1, ecell, t, rt2pow, rut, P.u[1][s], put, nan, div);
-            else if (role == 2) dir_contrib<NC, 1, TABLE>(P, E, s, j, P.ny, P.nx, ecell, t, rt2pow, rut, P.u[2][s], put, nan, div);
-            else if constexpr (DIM3) dir_contrib<NC, 2, TABLE>(P, E, s, k, P.nz, (long)P.nx * P.ny, ecell, t, rt2pow, rut, P.u[3][s], put, nan, div);
+            const int dir = role - 1;
+            const int coord = dir == 0 ? i : (dir == 1 ? j : k);
+            const int ncoord = dir == 0 ? P.nx : (dir == 1 ? P.ny : P.nz);
+            const long stride = dir == 0 ? 1 : (dir == 1 ? (long)P.nx : (long)P.nx * P.ny);
+            dir_contrib<NC, TABLE>(P, E, s, dir, coord, ncoord, stride, ecell, t, rt2pow, rut, P.u[1 + dir][s], put,
+                                   nan, div);
         }
         flags[role][lane] = nan;
     }

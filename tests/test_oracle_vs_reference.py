@@ -72,7 +72,9 @@ def test_root_solve_takes_the_table_branch():
     # the root satisfies the table-EoS energy equation to the solver's tolerance:
     # M0 = (e + p) ut^2 - p with ut^2 = 1 + |M|^2/((e+p)^2 ut^2)  <=>  e = M0 - M/(M0 + p)
     M0, M = q[0], q[1] ** 2
-    p = L.orc_eos_pressure(L.orc_eos_create(table.ctypes.data), e_table)
+    E = L.orc_eos_create(table.ctypes.data)
+    p = L.orc_eos_pressure(E, e_table)
+    L.orc_eos_destroy(E)
     assert abs(e_table - (M0 - M / (M0 + p))) < 2e-3 * e_table
 
 
