@@ -468,7 +468,7 @@ __device__ __forceinline__ void store_cell(const StageParams &P, long s, const C
 constexpr int CELLS_PER_BLOCK = 64;
 
 template <int NC, bool TABLE, bool DIM3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void stage_kernel(const StageParams P,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel(const StageParams P,
                                                                                              const EosDev E) {
     constexpr int ROLES = DIM3 ? 4 : 3;
     __shared__ double part[ROLES][NC][CELLS_PER_BLOCK];
