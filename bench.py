@@ -1,12 +1,18 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X viscous-hydro step (one GPU).
+"""Benchmark of the MI355X viscous-hydro step (one GPU).  Prints one JSON line.
 
-A step is one RK2 step (rungeKutta2 + buffer swap) of the plain FP64 drop-in
-over one event.  Default workload: the 3+1D single event at real size,
-131x131x61 cells (dx=dy=deta=0.1), tau0=0.6 fm/c, dtau=0.02 fm/c, seeded
-MC-Glauber Au+Au initial energy density, HotQCD+HRG table EoS, eta/s(T) with
-minimum 0.08 at 154 MeV, bulk pressure evolved, regulation below 100 MeV.
-Prints one JSON line.
+Headline (value): events/s of the batched mixed-precision form on the
+batching workload -- 256 seeded 2+1D boost-invariant MC-Glauber events with
+square grids drawn from 81^2..201^2 (dx = 0.1 fm, dtau = 0.02 fm/c) and
+Latin-hypercube per-event parameters ((eta/s)_min in [0.04, 0.2], zeta/s peak
+in [0, 0.1], tau0 in [0.3, 1.0] fm/c), all resident in one batch and stepped
+together until each is below T_F = 150 MeV (three consecutive coarse checks,
+every 5 steps, with no cell above e_F; finished events leave the batch).
+Beside it: the plain FP64 drop-in looping over events one at a time (a
+sample of the same events), the same batch in FP64, the 3+1D single event at
+real size (131x131x61) through the plain FP64 path, the mixed form's error
+against FP64 on sample events, the dominant kernel's roofline and the
+reference CPU build timed on this host.
 """
 import argparse
 import json
@@ -88,26 +94,152 @@ def load_traffic(name):
     return d.get("hbm_bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="glauber3d", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(events shard across GPUs with no communication, see DESIGN.md)", file=sys.stderr)
-        return 2
-    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: one GPU only", file=sys.stderr)
-        return 2
+# ------------------------------------------------------- batching workload --
+BATCH = dict(n_events=256, n_min=81, n_max=201, dx=0.1, dt=0.02, seed=20240601, e0=10.0, T_F_GeV=0.150,
+             etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0, T_reg_GeV=0.1, coarse=5, max_steps=3000)
 
+
+def lhs(n, bounds, rng):
+    """Latin-hypercube sample: one point per stratum in every dimension."""
+    out = np.empty((n, len(bounds)))
+    for d, (lo, hi) in enumerate(bounds):
+        strata = (rng.permutation(n) + rng.random(n)) / n
+        out[:, d] = lo + (hi - lo) * strata
+    return out
+
+
+def make_batch_events(table, n_events):
+    """Seeded 2+1D boost-invariant MC-Glauber events of the batching workload."""
     import cpuvh_amd as C
-    w = WORKLOADS[args.workload]
-    table = C.load_eos_table(os.path.join(REPO, "EoS", "hrg_hotqcd_eos_binary.dat"))
+    rng = np.random.default_rng(BATCH["seed"])
+    sides = rng.integers(BATCH["n_min"], BATCH["n_max"] + 1, n_events)
+    params = lhs(n_events, [(0.04, 0.2), (0.0, 0.1), (0.3, 1.0)], rng)
+    events = []
+    for k in range(n_events):
+        n = int(sides[k])
+        e, info = C.mc_glauber(n, n, 1, BATCH["dx"], BATCH["dx"], BATCH["dx"], seed=BATCH["seed"] + 7919 * k,
+                               e0=BATCH["e0"])
+        N = e.size
+        ic = {"e": e, "p": table_pressure(table, e), "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N),
+              "un": np.zeros(N), "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
+        events.append(dict(n=n, N=N, etabar=float(params[k, 0]), zeta_peak=float(params[k, 1]),
+                           t0=float(params[k, 2]), ic=ic, b=info["b"]))
+    return events
+
+
+def add_event(b, ev):
+    d = BATCH["dx"]
+    return b.add(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], ev["ic"],
+                 T_reg_GeV=BATCH["T_reg_GeV"], etas_Tk_GeV=BATCH["etas_Tk_GeV"],
+                 etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
+
+
+def run_batched(events, table, precision, warmup):
+    """All events resident in one batch, stepped in chunks of the coarse
+    freezeout interval until each meets the stop rule; returns timings."""
+    import cpuvh_amd as C
+    cap = sum(ev["N"] for ev in events)
+    kw = dict(max_events=len(events), max_steps=BATCH["max_steps"], bulk=True, eos_table=table,
+              precision=precision)
+    coarse = BATCH["coarse"]
+    tiles = (-1, -1)
+    if warmup > 0:  # throwaway batch: kernels, graphs and caches warm
+        wb = C.Batch(cap, **kw)
+        for ev in events:
+            add_event(wb, ev)
+        tiles = wb.autotune(reps=2)
+        wb.steps(warmup)
+        wb.sync()
+        wb.close()
+    b = C.Batch(cap, **kw)
+    ids = [add_event(b, ev) for ev in events]
+    if precision == C.PREC_MIXED:
+        b.set_tiles(max(tiles[0], 0), max(tiles[1], 0))
+    eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+    strikes = {i: 0 for i in ids}
+    cells = {i: ev["N"] for i, ev in zip(ids, events)}
+    steps_done = {}
+    zone_updates = 0
+    graph_ms = 0.0
+    batched_steps = 0
+    t_start = time.perf_counter()
+    while strikes:
+        b.steps(coarse)
+        counts = b.counts_above(eF)  # synchronising
+        graph_ms += b.last_ms()
+        zone_updates += coarse * sum(cells[i] for i in strikes)
+        batched_steps += coarse
+        for i in list(strikes):
+            strikes[i] = strikes[i] + 1 if counts[i] == 0 else 0
+            if strikes[i] >= 3 or b.info(i)["steps_done"] + coarse > BATCH["max_steps"]:
+                steps_done[i] = b.info(i)["steps_done"]
+                b.remove(i)
+                del strikes[i]
+    wall = time.perf_counter() - t_start
+    b.close()
+    return dict(wall_s=wall, events=len(events), events_per_s=len(events) / wall, batched_steps=batched_steps,
+                zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall, graph_ms=graph_ms,
+                steps_to_freezeout=[steps_done[i] for i in ids], tiles=tiles)
+
+
+def run_plain_loop(events, table):
+    """The plain FP64 drop-in, one event at a time, same stop rule."""
+    import cpuvh_amd as C
+    coarse = BATCH["coarse"]
+    d = BATCH["dx"]
+    zone_updates = 0
+    steps = []
+    t_start = time.perf_counter()
+    for ev in events:
+        h = C.Hydro(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"],
+                    bulk=True, eos_table=table, etas_Tk_GeV=BATCH["etas_Tk_GeV"],
+                    etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
+        ic = ev["ic"]
+        h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic["Pi"])
+        h.init_conserved(ev["t0"])
+        eF = h.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+        n, strikes = 1, 0
+        while strikes < 3 and n + coarse - 1 <= BATCH["max_steps"]:
+            h.steps(n, coarse)
+            n += coarse
+            strikes = strikes + 1 if h.count_above(eF) == 0 else 0
+        steps.append(n - 1)
+        zone_updates += ev["N"] * (n - 1)
+        h.close()
+    wall = time.perf_counter() - t_start
+    return dict(wall_s=wall, events=len(events), events_per_s=len(events) / wall, zone_updates=zone_updates,
+                zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
+
+
+def precision_check(events, table, nsteps):
+    """Mixed form against FP64 on the same events after nsteps steps."""
+    import cpuvh_amd as C
+    cap = sum(ev["N"] for ev in events)
+    res = {}
+    runs = {}
+    for prec in (C.PREC_FP64, C.PREC_MIXED):
+        b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
+        ids = [add_event(b, ev) for ev in events]
+        b.steps(nsteps)
+        b.sync()
+        runs[prec] = [{f: b.get(i, f) for f in ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi")} for i in ids]
+        b.close()
+    for f in ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi"):
+        worst_pt, worst_norm = 0.0, 0.0
+        for r64, rmx in zip(runs[C.PREC_FP64], runs[C.PREC_MIXED]):
+            a, c = r64[f], rmx[f]
+            scale = float(np.max(np.abs(a))) or 1.0
+            worst_norm = max(worst_norm, float(np.max(np.abs(a - c))) / scale)
+            if f in ("e", "ut"):
+                worst_pt = max(worst_pt, float(np.max(np.abs(a - c) / np.abs(a))))
+        res[f] = {"max_abs_over_max": worst_norm}
+        if f in ("e", "ut"):
+            res[f]["max_pointwise_rel"] = worst_pt
+    return {"steps": nsteps, "events": len(events), "fields": res}
+
+
+def run_dropin_3d(w, table, steps, warmup):
+    import cpuvh_amd as C
     ic, info = make_ic(w, table)
     nx, ny, nz, dx = w["nx"], w["ny"], w["nz"], w["dx"]
     N = nx * ny * nz
@@ -116,43 +248,117 @@ def main():
                 etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"])
     h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic["Pi"])
     h.init_conserved(w["t0"])
-    if args.warmup > 0:
-        h.steps(1, args.warmup)
+    if warmup > 0:
+        h.steps(1, warmup)
     h.sync()
-    h.prepare_steps(args.steps)
+    h.prepare_steps(steps)
     t_start = time.perf_counter()
-    h.steps(1 + args.warmup, args.steps)
+    h.steps(1 + warmup, steps)
     h.sync()
     elapsed = time.perf_counter() - t_start
     kernel_ms = h.last_kernel_ms()
     h.close()
-
     nc = 15 if w["bulk"] else 14
     bytes_per_zone = (5 * nc + 32) * 8  # read/write of the fused two-kernel step
     bytes_per_launch = bytes_per_zone / 2 * N
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
-    ms_per_step = elapsed / args.steps * 1e3
-    zone_rate = N * args.steps / elapsed
+    return ic, {
+        "workload": "3+1D single event 131x131x61, dx=deta=0.1, tau0=0.6, dtau=0.02, MC-Glauber Au+Au, "
+                    "HotQCD+HRG table EoS, eta/s(T) min 0.08 @154 MeV, bulk on",
+        "cells": N, "glauber": info, "steps": steps, "warmup": warmup, "ms_per_step": elapsed / steps * 1e3,
+        "zone_upd_per_s": N * steps / elapsed, "events_per_s_at_600_steps": 1.0 / (elapsed / steps * 600),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": load_traffic("stage_kernel_glauber3d_pmc.json"),
+                     "kernel": "stage_kernel (one RK stage, FP64, fused)", "avg_launch_ms": kernel_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600, help="timed steps of the 3+1D FP64 drop-in line")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--events", type=int, default=BATCH["n_events"])
+    ap.add_argument("--plain-sample", type=int, default=32, help="events of the plain FP64 loop")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64-batch", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(events shard across GPUs with no communication)", file=sys.stderr)
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: one GPU only", file=sys.stderr)
+        return 2
+
+    import cpuvh_amd as C
+    table = C.load_eos_table(os.path.join(REPO, "EoS", "hrg_hotqcd_eos_binary.dat"))
+    print("bench: building initial states", file=sys.stderr, flush=True)
+    events = make_batch_events(table, args.events)
+    print("bench: batched mixed", file=sys.stderr, flush=True)
+    mixed = run_batched(events, table, C.PREC_MIXED, args.warmup)
+    fp64b = None
+    if not args.no_fp64_batch:
+        print("bench: batched fp64", file=sys.stderr, flush=True)
+        fp64b = run_batched(events, table, C.PREC_FP64, args.warmup)
+    stride = max(1, len(events) // max(1, args.plain_sample))
+    sample = events[::stride][:args.plain_sample]
+    print("bench: plain fp64 loop", file=sys.stderr, flush=True)
+    plain = run_plain_loop(sample, table)
+    print("bench: precision check", file=sys.stderr, flush=True)
+    prec = precision_check(events[:: max(1, len(events) // 4)][:4], table, 300)
+    print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
+    ic3d, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.steps, args.warmup)
+
+    # dominant kernel: the mixed stage kernel over the batch (two launches per step)
+    nc = 15
+    bytes_per_zone_mx = (5 * nc + 32 + 5 * 4) * 4  # FP32 fields + FP32 low parts of T^{tau mu}
+    achieved = mixed["zone_updates"] * bytes_per_zone_mx / (mixed["graph_ms"] * 1e-3) / 1e9
+    launches = 2 * mixed["batched_steps"]
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(w, ic, threads)
-    traffic = load_traffic(f"stage_kernel_{args.workload}_pmc.json")
+        print("bench: cpu reference", file=sys.stderr, flush=True)
+        med = sorted(events, key=lambda ev: ev["N"])[len(events) // 2]
+        wcpu = dict(nx=med["n"], ny=med["n"], nz=1, dx=BATCH["dx"], dt=BATCH["dt"], t0=med["t0"], lattice=True,
+                    bulk=True, etabar=med["etabar"], T_reg_GeV=BATCH["T_reg_GeV"])
+        cpu = cpu_baseline(wcpu, med["ic"], threads, min_seconds=12.0, max_steps=400)
+        if cpu and "value" in cpu:
+            mean_steps = float(np.mean(plain["steps_to_freezeout"]))
+            mean_cells = float(np.mean([ev["N"] for ev in events]))
+            cpu["events_per_s"] = cpu["value"] / (mean_steps * mean_cells)
+            cpu["sample"] += (f"; median-size event of the batch ({med['n']}^2 cells); events/s = zone-upd/s / "
+                              f"(mean cells {mean_cells:.0f} x mean steps to freezeout {mean_steps:.0f})")
     out = {
-        "metric": METRIC, "value": zone_rate, "unit": "zone-upd/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded MC-Glauber initial state)",
-        "config": {"workload": "3+1D single event 131x131x61, dx=deta=0.1, tau0=0.6, dtau=0.02, 600 steps/event, "
-                               "MC-Glauber Au+Au, HotQCD+HRG table EoS, eta/s(T) min 0.08 @154 MeV, bulk on, "
-                               "FP64 plain drop-in",
-                   "cells": N, "glauber": info, "form": "plain FP64"},
-        "events_per_s": 1.0 / (ms_per_step * 1e-3 * w["steps_per_event"]),
+        "metric": METRIC, "value": mixed["events_per_s"], "unit": "events/s", "n_gpus": 1,
+        "steps": mixed["batched_steps"], "warmup": args.warmup,
+        "ms_per_step": mixed["wall_s"] / mixed["batched_steps"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 storage/flux/source + f64 root solve/EoS/accumulation",
+        "data": "synthetic (seeded MC-Glauber initial states)",
+        "config": {"workload": f"{len(events)} 2+1D MC-Glauber Au+Au events, grids {BATCH['n_min']}^2..{BATCH['n_max']}^2, "
+                               "dx=0.1, dtau=0.02, LHS (eta/s)_min [0.04,0.2], zeta/s peak [0,0.1], tau0 [0.3,1.0], "
+                               "HotQCD+HRG table EoS, bulk on, to freezeout at T=150 MeV (coarse check every 5 steps, "
+                               "3 strikes), one batch, mixed precision",
+                   "cells_total": int(sum(ev["N"] for ev in events)), "form": "batched mixed precision",
+                   "tiles": mixed["tiles"]},
+        "batched_mixed": {k: v for k, v in mixed.items() if k != "steps_to_freezeout"},
+        "plain_fp64_loop": {k: v for k, v in plain.items() if k != "steps_to_freezeout"},
+        "speedup_vs_plain_fp64": mixed["events_per_s"] / plain["events_per_s"],
+        "batched_fp64": None if fp64b is None else {k: v for k, v in fp64b.items() if k != "steps_to_freezeout"},
+        "mixed_vs_fp64_error": prec,
+        "fp64_dropin_3d": dropin,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "kernel": "stage_kernel (one RK stage, fused)", "avg_launch_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("stage_mixed_kernel_batch_pmc.json"),
+                     "kernel": "stage_mixed_kernel (one RK stage, all 2+1D events)",
+                     "avg_launch_ms": mixed["graph_ms"] / launches,
+                     "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},
         "cpu_baseline": cpu,
     }
+    if cpu and "events_per_s" in cpu:
+        out["speedup_vs_cpu_reference"] = mixed["events_per_s"] / cpu["events_per_s"]
     print(json.dumps(out))
     return 0
 

@@ -20,13 +20,18 @@ FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NOMEM, VH_ERR_NAN, VH_ERR_STATE, VH_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
 EOS_CONFORMAL, EOS_TABLE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
 
 # The exported entry points, as declared in include/cpuvh_amd.h.
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
-           "vh_prepare_steps"]
+           "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
+           "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
+           "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
+           "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
+           "vh_batch_set_tiles"]
 
 
 class VhConfig(ctypes.Structure):
@@ -37,6 +42,20 @@ class VhConfig(ctypes.Structure):
                 ("eos_table", ctypes.c_void_p),
                 ("etas_Tk_GeV", ctypes.c_double), ("etas_slope_lo", ctypes.c_double),
                 ("etas_slope_hi", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("capacity_cells", ctypes.c_int64), ("max_events", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("bulk", ctypes.c_int), ("eos_kind", ctypes.c_int),
+                ("eos_table", ctypes.c_void_p), ("precision", ctypes.c_int)]
+
+
+class EventParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("dt", ctypes.c_double), ("t0", ctypes.c_double), ("etabar", ctypes.c_double),
+                ("T_reg_GeV", ctypes.c_double), ("etas_Tk_GeV", ctypes.c_double),
+                ("etas_slope_lo", ctypes.c_double), ("etas_slope_hi", ctypes.c_double)]
 
 
 class GlauberParams(ctypes.Structure):
@@ -92,6 +111,26 @@ def _lib():
         L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
         L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
         L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
+        I, I64 = ctypes.c_int, ctypes.c_int64
+        L.vh_batch_create.argtypes = [ctypes.POINTER(BatchConfig), ctypes.POINTER(P)]
+        L.vh_batch_destroy.argtypes = [P]
+        L.vh_batch_add.argtypes = [P, ctypes.POINTER(EventParams)] + [P] * 8 + [ctypes.POINTER(I)]
+        L.vh_batch_remove.argtypes = [P, I]
+        L.vh_batch_steps.argtypes = [P, I]
+        L.vh_batch_sync.argtypes = [P]
+        L.vh_batch_nan_info.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
+        L.vh_batch_clear_error.argtypes = [P]
+        L.vh_batch_counts_above.argtypes = [P, D, P]
+        L.vh_batch_download.argtypes = [P, I, I, P]
+        L.vh_batch_fo_flag.argtypes = [P, I, D, P, P, I64, ctypes.POINTER(I64)]
+        L.vh_batch_event_info.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
+        L.vh_batch_free_cells.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]
+        L.vh_batch_last_ms.argtypes = [P]
+        L.vh_batch_last_ms.restype = D
+        L.vh_batch_eos_energy_density.argtypes = [P, D]
+        L.vh_batch_eos_energy_density.restype = D
+        L.vh_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.vh_batch_set_tiles.argtypes = [P, I, I]
         _LIB = L
     return _LIB
 
@@ -212,6 +251,123 @@ class Hydro:
 
     def last_kernel_ms(self):
         return _lib().vh_last_kernel_ms(self.h)
+
+
+class Batch:
+    """Many events resident on one GPU (a vh_batch), stepped together."""
+
+    def __init__(self, capacity_cells, max_events, max_steps, bulk=False, eos_table=None, device=0,
+                 precision=PREC_FP64):
+        L = _lib()
+        self.nc = 15 if bulk else 14
+        self.precision = precision
+        self.max_events = max_events
+        self._table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)
+        cfg = BatchConfig(device, capacity_cells, max_events, max_steps, int(bool(bulk)),
+                          EOS_CONFORMAL if self._table is None else EOS_TABLE,
+                          None if self._table is None else self._table.ctypes.data, precision)
+        b = ctypes.c_void_p()
+        Hydro._check(L.vh_batch_create(ctypes.byref(cfg), ctypes.byref(b)), "vh_batch_create")
+        self.b = b
+        self.events = {}
+
+    def close(self):
+        if getattr(self, "b", None):
+            _lib().vh_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, ic, T_reg_GeV=0.0, etas_Tk_GeV=0.154,
+            etas_slope_lo=0.0, etas_slope_hi=0.0):
+        """Admits one event; returns its id, or None when the batch is full
+        (VH_ERR_CAPACITY).  ic: dict of x-fastest arrays e, p, ut, ux, uy, un, pi (10*N), Pi (optional)."""
+        N = nx * ny * nz
+        arrs = [np.ascontiguousarray(ic[k], dtype=np.float64).ravel() for k in ("e", "p", "ut", "ux", "uy", "un")]
+        pi = np.ascontiguousarray(ic["pi"], dtype=np.float64).ravel()
+        if any(a.size != N for a in arrs) or pi.size != 10 * N:
+            raise ValueError("initial-condition arrays do not match the lattice")
+        Pi = ic.get("Pi")
+        Pa = None if Pi is None else np.ascontiguousarray(Pi, dtype=np.float64).ravel()
+        prm = EventParams(nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV, etas_Tk_GeV, etas_slope_lo,
+                          etas_slope_hi)
+        eid = ctypes.c_int(-1)
+        st = _lib().vh_batch_add(self.b, ctypes.byref(prm), *[_ptr(a) for a in arrs], _ptr(pi), _ptr(Pa),
+                                 ctypes.byref(eid))
+        if st == VH_ERR_CAPACITY:
+            return None
+        Hydro._check(st, "vh_batch_add")
+        self.events[eid.value] = (nx, ny, nz)
+        return eid.value
+
+    def remove(self, eid):
+        Hydro._check(_lib().vh_batch_remove(self.b, eid), "vh_batch_remove")
+        self.events.pop(eid, None)
+
+    def steps(self, count):
+        Hydro._check(_lib().vh_batch_steps(self.b, count), "vh_batch_steps")
+
+    def sync(self):
+        Hydro._check(_lib().vh_batch_sync(self.b), "vh_batch_sync")
+
+    def nan_info(self):
+        site, ev, cell = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        Hydro._check(_lib().vh_batch_nan_info(self.b, ctypes.byref(site), ctypes.byref(ev), ctypes.byref(cell)),
+                     "vh_batch_nan_info")
+        return site.value, ev.value, cell.value
+
+    def clear_error(self):
+        Hydro._check(_lib().vh_batch_clear_error(self.b), "vh_batch_clear_error")
+
+    def counts_above(self, eF):
+        out = np.empty(self.max_events, dtype=np.int64)
+        Hydro._check(_lib().vh_batch_counts_above(self.b, eF, _ptr(out)), "vh_batch_counts_above")
+        return out
+
+    def get(self, eid, field):
+        f = FIELD[field] if isinstance(field, str) else field
+        nx, ny, nz = self.events[eid]
+        out = np.empty(nx * ny * nz, dtype=np.float64)
+        Hydro._check(_lib().vh_batch_download(self.b, eid, f, _ptr(out)), "vh_batch_download")
+        return out
+
+    def fo_flag_ids(self, eid, eF):
+        nx, ny, nz = self.events[eid]
+        cap = max(1, (nx - 1) * (ny - 1) * (nz - 1 if nz > 1 else 1))
+        ids = np.empty(cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        Hydro._check(_lib().vh_batch_fo_flag(self.b, eid, eF, _ptr(ids), None, cap, ctypes.byref(n)),
+                     "vh_batch_fo_flag")
+        return ids[:n.value].copy()
+
+    def info(self, eid):
+        a, s, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        Hydro._check(_lib().vh_batch_event_info(self.b, eid, ctypes.byref(a), ctypes.byref(s), ctypes.byref(c)),
+                     "vh_batch_event_info")
+        return {"active": bool(a.value), "steps_done": s.value, "cells": c.value}
+
+    def free_cells(self):
+        f, big = ctypes.c_int64(), ctypes.c_int64()
+        Hydro._check(_lib().vh_batch_free_cells(self.b, ctypes.byref(f), ctypes.byref(big)), "vh_batch_free_cells")
+        return f.value, big.value
+
+    def last_ms(self):
+        return _lib().vh_batch_last_ms(self.b)
+
+    def autotune(self, reps=3):
+        t2, t3 = ctypes.c_int(), ctypes.c_int()
+        Hydro._check(_lib().vh_batch_autotune(self.b, reps, ctypes.byref(t2), ctypes.byref(t3)), "vh_batch_autotune")
+        return t2.value, t3.value
+
+    def set_tiles(self, tile2d, tile3d):
+        Hydro._check(_lib().vh_batch_set_tiles(self.b, tile2d, tile3d), "vh_batch_set_tiles")
+
+    def eos_energy_density(self, T):
+        return _lib().vh_batch_eos_energy_density(self.b, T)
 
 
 def build_info():

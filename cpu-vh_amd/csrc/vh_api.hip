@@ -46,9 +46,7 @@ struct vh_handle {
     // equation of state
     HostEos heos;
     EosDev eos{};
-    double *d_ptb = nullptr, *d_Ttb = nullptr;
-    uint2 *d_exc = nullptr;
-    uint32_t *d_bucket = nullptr;
+    DeviceEos deos;
     double e_reg = 0;
     // per-step times for graph replays
     StepTimes *d_times = nullptr;
@@ -63,6 +61,12 @@ struct vh_handle {
 namespace {
 
 thread_local std::string g_last_error;
+
+}  // namespace
+
+void vh::set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
 
 vh_status hip_fail(hipError_t e, const char *what) {
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
@@ -237,39 +241,9 @@ vh_status vh_create(const vh_config *cfg, vh_handle **out) {
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(VH_ERR_HIP);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(VH_ERR_HIP);
 
-    EosDev &E = h->eos;
-    E.kind = c.eos_kind;
-    if (c.eos_kind == VH_EOS_TABLE) {
-        E.cs2_const = h->heos.cs2_is_one_third() ? 1 : 0;
-        E.e_min = h->heos.e_min;
-        E.e_spacing = h->heos.e_spacing;
-        E.r_spacing = 1.0 / h->heos.e_spacing;
-        E.e_max = h->heos.e_max;
-        const size_t tb = sizeof(double) * h->heos.p_tb.size();
-        if (hipMalloc(&h->d_ptb, tb) != hipSuccess || hipMalloc(&h->d_Ttb, tb) != hipSuccess)
-            return fail(VH_ERR_NOMEM);
-        if (hipMemcpy(h->d_ptb, h->heos.p_tb.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->d_Ttb, h->heos.T_tb.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(VH_ERR_HIP);
-        E.p_tb = h->d_ptb;
-        E.T_tb = h->d_Ttb;
-    } else {
-        const QuarterRootExceptions &X = quarter_root_exceptions();
-        std::vector<uint2> pairs(std::max<size_t>(1, X.key.size()));
-        for (size_t i = 0; i < X.key.size(); ++i) pairs[i] = make_uint2(X.key[i], X.val[i]);
-        if (hipMalloc(&h->d_exc, sizeof(uint2) * pairs.size()) != hipSuccess ||
-            hipMalloc(&h->d_bucket, sizeof(uint32_t) * X.bucket.size()) != hipSuccess)
-            return fail(VH_ERR_NOMEM);
-        if (hipMemcpy(h->d_exc, pairs.data(), sizeof(uint2) * pairs.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(h->d_bucket, X.bucket.data(), sizeof(uint32_t) * X.bucket.size(), hipMemcpyHostToDevice) !=
-                hipSuccess)
-            return fail(VH_ERR_HIP);
-        E.exc = h->d_exc;
-        E.exc_bucket = h->d_bucket;
-        E.exc_lo = X.lo;
-        E.exc_hi = X.hi;
-        E.exc_shift = X.shift;
-    }
+    if (hipError_t r = h->deos.init(h->heos, c.eos_kind); r != hipSuccess)
+        return fail(r == hipErrorOutOfMemory ? VH_ERR_NOMEM : hip_fail(r, "equation of state upload"));
+    h->eos = h->deos.E;
     vh_status s = ensure_times(h, 1);
     if (s != VH_OK) return fail(s);
     *out = h;
@@ -292,10 +266,7 @@ vh_status vh_destroy(vh_handle *h) {
     hipFree(h->fo_prev);
     hipFree(h->fo_cur);
     hipFree(h->fo_ids);
-    hipFree(h->d_ptb);
-    hipFree(h->d_Ttb);
-    hipFree(h->d_exc);
-    hipFree(h->d_bucket);
+    h->deos.release();
     hipFree(h->d_times);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;

@@ -1,0 +1,777 @@
+// Batches of events (include/cpuvh_amd.h, vh_batch_*): many independent
+// lattices advance together, one launch per RK stage for all 2+1D events and
+// one for all 3+1D events.  Each event keeps its own lattice, transport
+// parameters and clock; the reference runs one event per process
+// (rhic/src/HydroPlugin.cpp:74-321).
+//
+// Device state: one pool of 61 fields x capacity cells (the vh_handle layout;
+// FP64, or FP32 plus the low parts of T^{tau mu} in the mixed form), each event
+// owning a contiguous range [off, off + N) of every field; ranges come from a
+// first-fit allocator, so admission is by HBM footprint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cpuvh_amd.h"
+#include "vh_host_eos.hpp"
+#include "vh_kernels.hpp"
+#include "vh_mixed.hpp"
+
+using namespace vh;
+
+namespace {
+
+vh_status bfail(hipError_t e, const char *what) {
+    set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return VH_ERR_HIP;
+}
+
+#define BCHK(call)                                        \
+    do {                                                  \
+        hipError_t _e = (call);                           \
+        if (_e != hipSuccess) return bfail(_e, #call);    \
+    } while (0)
+
+struct Event {
+    bool used = false;
+    vh_event_params prm{};
+    long off = 0, N = 0;
+    int steps_done = 0;
+    double e_reg = 0;
+    double *fo_prev = nullptr, *fo_cur = nullptr;
+    long long *fo_ids = nullptr;
+};
+
+}  // namespace
+
+struct vh_batch {
+    vh_batch_config cfg{};
+    int nc = 14, dev = 0;
+    hipStream_t stream = nullptr;
+    long cap = 0;
+    double *pool = nullptr;
+    double *q[3][15] = {};
+    double *u[3][4] = {};
+    double *e[2] = {}, *p[2] = {};
+    int iq_cur = 0, iq_stage = 1, iq_next = 2;
+    int iu_cur = 0, iu_prev = 1, iu_stage = 2;
+    HostEos heos;
+    DeviceEos deos;
+    int *d_err = nullptr;
+    std::vector<Event> ev;
+    std::map<long, long> free_seg;  // offset -> length
+    EvDesc *d_ev = nullptr;
+    std::vector<EvDesc> h_ev;
+    StepTimes *d_times = nullptr;  // max_events x (max_steps + 1)
+    int *d_blk_ev = nullptr, *d_blk_first = nullptr;
+    int nblk2 = 0, nblk3 = 0;      // block table: [2+1D blocks | 3+1D blocks]
+    // 64-cell chunks of every event (counting kernels of the mixed form)
+    int *d_cnt_ev = nullptr;
+    long *d_cnt_start = nullptr;
+    int ncnt = 0;
+    // mixed form: FP32 fields, tile variants of the 2+1D and 3+1D launches
+    bool mixed = false;
+    float *fpool = nullptr;
+    float *qf[3][15] = {}, *qfl[3][4] = {}, *uf[3][4] = {}, *ef[2] = {}, *pf[2] = {};
+    int tile2 = 0, tile3 = 0;
+    bool layout_dirty = true;
+    int layout_version = 0;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+    unsigned long long *d_counts = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;  // ev0/ev1 bracket a graph launch not yet read
+    double last_ms = 0;
+};
+
+namespace {
+
+void drop_graphs(vh_batch *b) {
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+long alloc_cells(vh_batch *b, long n) {
+    for (auto it = b->free_seg.begin(); it != b->free_seg.end(); ++it) {
+        if (it->second >= n) {
+            const long off = it->first, len = it->second;
+            b->free_seg.erase(it);
+            if (len > n) b->free_seg[off + n] = len - n;
+            return off;
+        }
+    }
+    return -1;
+}
+
+void free_cells(vh_batch *b, long off, long n) {
+    auto it = b->free_seg.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != b->free_seg.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        b->free_seg.erase(nx);
+    }
+    if (it != b->free_seg.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            b->free_seg.erase(it);
+        }
+    }
+}
+
+// Blocks of one event's stage launch: 64-cell chunks (FP64) or tiles (mixed).
+long event_blocks(const vh_batch *b, const Event &E) {
+    if (!b->mixed) return (E.N + CELLS_PER_BLOCK - 1) / CELLS_PER_BLOCK;
+    const int *T = E.prm.nz > 1 ? MIXED_TILES_3D[b->tile3] : MIXED_TILES_2D[b->tile2];
+    return (long)((E.prm.nx + T[0] - 1) / T[0]) * ((E.prm.ny + T[1] - 1) / T[1]) * ((E.prm.nz + T[2] - 1) / T[2]);
+}
+
+vh_status rebuild_layout(vh_batch *b) {
+    if (!b->layout_dirty) return VH_OK;
+    std::vector<int> bev, bfirst, cev;
+    std::vector<long> cstart;
+    int n2 = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int id = 0; id < (int)b->ev.size(); ++id) {
+            const Event &E = b->ev[id];
+            if (!E.used || (E.prm.nz > 1) != (pass == 1)) continue;
+            const long nb = event_blocks(b, E);
+            // 2+1D and 3+1D blocks go to separate launches, each indexing its
+            // part of the table from 0: first is relative to that part.
+            const int first = (int)bev.size() - (pass == 1 ? n2 : 0);
+            for (long k = 0; k < nb; ++k) {
+                bev.push_back(id);
+                bfirst.push_back(first);
+            }
+            if (pass == 0) n2 += (int)nb;
+            for (long c = 0; c < E.N; c += 64) {
+                cev.push_back(id);
+                cstart.push_back(c);
+            }
+        }
+    b->nblk2 = n2;
+    b->nblk3 = (int)bev.size() - n2;
+    // every block must name a tile inside its event: 0 <= k - first < blocks
+    for (size_t k = 0; k < bev.size(); ++k) {
+        const long local = (long)k - (k < (size_t)n2 ? 0 : n2) - bfirst[k];
+        if (local < 0 || local >= event_blocks(b, b->ev[bev[k]])) {
+            set_last_error("batch block table inconsistent");
+            return VH_ERR_STATE;
+        }
+    }
+    b->ncnt = (int)cev.size();
+    (void)hipFree(b->d_blk_ev);
+    (void)hipFree(b->d_blk_first);
+    (void)hipFree(b->d_cnt_ev);
+    (void)hipFree(b->d_cnt_start);
+    b->d_blk_ev = b->d_blk_first = b->d_cnt_ev = nullptr;
+    b->d_cnt_start = nullptr;
+    if (!bev.empty()) {
+        BCHK(hipMalloc(&b->d_blk_ev, sizeof(int) * bev.size()));
+        BCHK(hipMalloc(&b->d_blk_first, sizeof(int) * bev.size()));
+        BCHK(hipMemcpy(b->d_blk_ev, bev.data(), sizeof(int) * bev.size(), hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->d_blk_first, bfirst.data(), sizeof(int) * bev.size(), hipMemcpyHostToDevice));
+        BCHK(hipMalloc(&b->d_cnt_ev, sizeof(int) * cev.size()));
+        BCHK(hipMalloc(&b->d_cnt_start, sizeof(long) * cstart.size()));
+        BCHK(hipMemcpy(b->d_cnt_ev, cev.data(), sizeof(int) * cev.size(), hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->d_cnt_start, cstart.data(), sizeof(long) * cstart.size(), hipMemcpyHostToDevice));
+    }
+    drop_graphs(b);
+    b->layout_dirty = false;
+    ++b->layout_version;
+    return VH_OK;
+}
+
+StageParams batch_params(const vh_batch *b, int stage, int iq_in, int iu_in, int iu_prev_used, int ie_in,
+                         int iq_out, int iu_out, int ie_out) {
+    StageParams P{};
+    for (int n = 0; n < 15; ++n) {
+        P.q[n] = b->q[iq_in][n];
+        P.qn[n] = b->q[b->iq_cur][n];
+        P.qo[n] = b->q[iq_out][n];
+    }
+    for (int c = 0; c < 4; ++c) {
+        P.u[c] = b->u[iu_in][c];
+        P.up[c] = b->u[iu_prev_used][c];
+        P.uo[c] = b->u[iu_out][c];
+    }
+    P.e = b->e[ie_in];
+    P.p = b->p[ie_in];
+    P.eo = b->e[ie_out];
+    P.po = b->p[ie_out];
+    P.stage = stage;
+    P.err = b->d_err;
+    return P;
+}
+
+mx::MxFields mixed_fields(const vh_batch *b, int stage, int iq_in, int iu_in, int iu_prev_used, int ie_in,
+                          int iq_out, int iu_out, int ie_out) {
+    mx::MxFields F{};
+    for (int n = 0; n < 15; ++n) {
+        F.q[n] = b->qf[iq_in][n];
+        F.qn[n] = b->qf[b->iq_cur][n];
+        F.qo[n] = b->qf[iq_out][n];
+    }
+    for (int n = 0; n < 4; ++n) {
+        F.ql[n] = b->qfl[iq_in][n];
+        F.qnl[n] = b->qfl[b->iq_cur][n];
+        F.qol[n] = b->qfl[iq_out][n];
+    }
+    for (int c = 0; c < 4; ++c) {
+        F.u[c] = b->uf[iu_in][c];
+        F.up[c] = b->uf[iu_prev_used][c];
+        F.uo[c] = b->uf[iu_out][c];
+    }
+    F.e = b->ef[ie_in];
+    F.p = b->pf[ie_in];
+    F.eo = b->ef[ie_out];
+    F.po = b->pf[ie_out];
+    F.stage = stage;
+    F.err = b->d_err;
+    return F;
+}
+
+// The launches of one stage (both RK stages: stage = 0, 1) for all events.
+vh_status enqueue_stage(vh_batch *b, int stage, int off) {
+    const int *be2 = b->d_blk_ev, *bf2 = b->d_blk_first;
+    const int *be3 = b->d_blk_ev ? b->d_blk_ev + b->nblk2 : nullptr;
+    const int *bf3 = b->d_blk_first ? b->d_blk_first + b->nblk2 : nullptr;
+    if (b->mixed) {
+        mx::MxFields F = stage == 0 ? mixed_fields(b, 0, b->iq_cur, b->iu_cur, b->iu_prev, 0, b->iq_stage, b->iu_stage, 1)
+                                    : mixed_fields(b, 1, b->iq_stage, b->iu_stage, b->iu_cur, 1, b->iq_next, b->iu_prev, 0);
+        F.off = off;
+        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, false, b->tile2, b->d_ev, be2, bf2, b->nblk2, b->stream));
+        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, true, b->tile3, b->d_ev, be3, bf3, b->nblk3, b->stream));
+        return VH_OK;
+    }
+    StageParams P = stage == 0 ? batch_params(b, 0, b->iq_cur, b->iu_cur, b->iu_prev, 0, b->iq_stage, b->iu_stage, 1)
+                               : batch_params(b, 1, b->iq_stage, b->iu_stage, b->iu_cur, 1, b->iq_next, b->iu_prev, 0);
+    P.off = off;
+    P.tab = b->d_times;
+    BCHK(launch_stage_batch(P, b->deos.E, b->nc, false, b->d_ev, be2, bf2, b->nblk2, b->stream));
+    BCHK(launch_stage_batch(P, b->deos.E, b->nc, true, b->d_ev, be3, bf3, b->nblk3, b->stream));
+    return VH_OK;
+}
+
+vh_status enqueue_batch_step(vh_batch *b, int off) {
+    vh_status s = enqueue_stage(b, 0, off);
+    if (s == VH_OK) s = enqueue_stage(b, 1, off);
+    if (s != VH_OK) return s;
+    std::swap(b->iq_cur, b->iq_next);
+    std::swap(b->iu_cur, b->iu_prev);
+    return VH_OK;
+}
+
+double *field_of(vh_batch *b, int field) {
+    if (field >= 0 && field < b->nc) return b->q[b->iq_cur][field];
+    if (field == 15) return b->e[0];
+    if (field == 16) return b->p[0];
+    if (field >= 17 && field <= 20) return b->u[b->iu_cur][field - 17];
+    if (field >= 21 && field <= 24) return b->u[b->iu_prev][field - 21];
+    return nullptr;
+}
+
+bool field_of_mixed(vh_batch *b, int field, const float **hi, const float **lo) {
+    *lo = nullptr;
+    if (field >= 0 && field < b->nc) {
+        *hi = b->qf[b->iq_cur][field];
+        if (field < 4) *lo = b->qfl[b->iq_cur][field];
+        return true;
+    }
+    if (field == 15) *hi = b->ef[0];
+    else if (field == 16) *hi = b->pf[0];
+    else if (field >= 17 && field <= 20) *hi = b->uf[b->iu_cur][field - 17];
+    else if (field >= 21 && field <= 24) *hi = b->uf[b->iu_prev][field - 21];
+    else return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out) {
+    if (!cfg || !out) return VH_ERR_ARG;
+    *out = nullptr;
+    const vh_batch_config &c = *cfg;
+    if (c.capacity_cells < 1 || c.max_events < 1 || c.max_steps < 1 ||
+        (c.eos_kind != VH_EOS_CONFORMAL && c.eos_kind != VH_EOS_TABLE) || (c.eos_kind == VH_EOS_TABLE && !c.eos_table))
+        return VH_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_last_error("no HIP device");
+        return VH_ERR_HIP;
+    }
+    if (c.device < 0 || c.device >= ndev) return VH_ERR_ARG;
+    vh_batch *b = new vh_batch();
+    auto fail = [&](vh_status s) {
+        vh_batch_destroy(b);
+        return s;
+    };
+    b->cfg = c;
+    b->cfg.eos_table = nullptr;
+    b->dev = c.device;
+    b->nc = c.bulk ? 15 : 14;
+    b->cap = (long)c.capacity_cells;
+    if (hipSetDevice(b->dev) != hipSuccess) return fail(VH_ERR_HIP);
+    if (c.eos_kind == VH_EOS_TABLE) b->heos.load(c.eos_table);
+    if (b->deos.init(b->heos, c.eos_kind) != 0) return fail(VH_ERR_NOMEM);
+    if (c.precision != VH_PREC_FP64 && c.precision != VH_PREC_MIXED) return fail(VH_ERR_ARG);
+    b->mixed = c.precision == VH_PREC_MIXED;
+    const size_t nfield = 3 * 15 + 3 * 4 + 4;
+    if (!b->mixed) {
+        if (hipMalloc(&b->pool, sizeof(double) * nfield * b->cap) != hipSuccess) return fail(VH_ERR_NOMEM);
+        double *ptr = b->pool;
+        for (int s = 0; s < 3; ++s)
+            for (int n = 0; n < 15; ++n, ptr += b->cap) b->q[s][n] = ptr;
+        for (int s = 0; s < 3; ++s)
+            for (int k = 0; k < 4; ++k, ptr += b->cap) b->u[s][k] = ptr;
+        for (int s = 0; s < 2; ++s, ptr += b->cap) b->e[s] = ptr;
+        for (int s = 0; s < 2; ++s, ptr += b->cap) b->p[s] = ptr;
+    } else {
+        if (hipMalloc(&b->fpool, sizeof(float) * (nfield + 3 * 4) * b->cap) != hipSuccess) return fail(VH_ERR_NOMEM);
+        float *ptr = b->fpool;
+        for (int s = 0; s < 3; ++s)
+            for (int n = 0; n < 15; ++n, ptr += b->cap) b->qf[s][n] = ptr;
+        for (int s = 0; s < 3; ++s)
+            for (int n = 0; n < 4; ++n, ptr += b->cap) b->qfl[s][n] = ptr;
+        for (int s = 0; s < 3; ++s)
+            for (int k = 0; k < 4; ++k, ptr += b->cap) b->uf[s][k] = ptr;
+        for (int s = 0; s < 2; ++s, ptr += b->cap) b->ef[s] = ptr;
+        for (int s = 0; s < 2; ++s, ptr += b->cap) b->pf[s] = ptr;
+    }
+    if (hipMalloc(&b->d_err, sizeof(int) * 4) != hipSuccess || hipMemset(b->d_err, 0, sizeof(int) * 4) != hipSuccess)
+        return fail(VH_ERR_NOMEM);
+    if (hipMalloc(&b->d_ev, sizeof(EvDesc) * c.max_events) != hipSuccess) return fail(VH_ERR_NOMEM);
+    if (hipMalloc(&b->d_counts, sizeof(unsigned long long) * c.max_events) != hipSuccess) return fail(VH_ERR_NOMEM);
+    if (hipMalloc(&b->d_times, sizeof(StepTimes) * (size_t)c.max_events * (c.max_steps + 1)) != hipSuccess)
+        return fail(VH_ERR_NOMEM);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(VH_ERR_HIP);
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return fail(VH_ERR_HIP);
+    b->ev.resize(c.max_events);
+    b->h_ev.resize(c.max_events);
+    std::memset(b->h_ev.data(), 0, sizeof(EvDesc) * c.max_events);
+    if (hipMemcpy(b->d_ev, b->h_ev.data(), sizeof(EvDesc) * c.max_events, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(VH_ERR_HIP);
+    b->free_seg[0] = b->cap;
+    *out = b;
+    return VH_OK;
+}
+
+vh_status vh_batch_destroy(vh_batch *b) {
+    if (!b) return VH_ERR_ARG;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (auto &E : b->ev) {
+        (void)hipFree(E.fo_prev);
+        (void)hipFree(E.fo_cur);
+        (void)hipFree(E.fo_ids);
+    }
+    (void)hipFree(b->pool);
+    (void)hipFree(b->fpool);
+    (void)hipFree(b->d_cnt_ev);
+    (void)hipFree(b->d_cnt_start);
+    (void)hipFree(b->d_err);
+    (void)hipFree(b->d_ev);
+    (void)hipFree(b->d_counts);
+    (void)hipFree(b->d_times);
+    (void)hipFree(b->d_blk_ev);
+    (void)hipFree(b->d_blk_first);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->deos.release();
+    delete b;
+    return VH_OK;
+}
+
+vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e, const double *p, const double *ut,
+                       const double *ux, const double *uy, const double *un, const double *pi10, const double *Pi,
+                       int *id_out) {
+    if (!b || !prm || !e || !p || !ut || !ux || !uy || !un || !pi10 || !id_out) return VH_ERR_ARG;
+    const vh_event_params &c = *prm;
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1 || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0) || !(c.dt > 0) ||
+        !(c.t0 > 0) || !(c.etabar > 0))
+        return VH_ERR_ARG;
+    int id = -1;
+    for (int k = 0; k < (int)b->ev.size(); ++k)
+        if (!b->ev[k].used) {
+            id = k;
+            break;
+        }
+    if (id < 0) return VH_ERR_CAPACITY;
+    const long N = (long)c.nx * c.ny * c.nz;
+    const long off = alloc_cells(b, N);
+    if (off < 0) return VH_ERR_CAPACITY;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipStreamSynchronize(b->stream));
+    Event &E = b->ev[id];
+    E = Event();
+    E.used = true;
+    E.prm = c;
+    E.off = off;
+    E.N = N;
+    E.steps_done = 0;
+    E.e_reg = b->heos.energy_density(c.T_reg_GeV / 0.197326938);
+    const size_t bytes = sizeof(double) * N;
+    if (!b->mixed) {
+        BCHK(hipMemcpy(b->e[0] + off, e, bytes, hipMemcpyHostToDevice));
+        BCHK(hipMemcpy(b->p[0] + off, p, bytes, hipMemcpyHostToDevice));
+        const double *uu[4] = {ut, ux, uy, un};
+        for (int k = 0; k < 4; ++k) {
+            BCHK(hipMemcpy(b->u[b->iu_cur][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
+            BCHK(hipMemcpy(b->u[b->iu_prev][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
+        }
+        for (int n = 0; n < 10; ++n)
+            BCHK(hipMemcpy(b->q[b->iq_cur][4 + n] + off, pi10 + (size_t)n * N, bytes, hipMemcpyHostToDevice));
+        if (b->nc == 15) {
+            if (Pi) BCHK(hipMemcpy(b->q[b->iq_cur][14] + off, Pi, bytes, hipMemcpyHostToDevice));
+            else BCHK(hipMemset(b->q[b->iq_cur][14] + off, 0, bytes));
+        }
+        double *qq[15];
+        for (int n = 0; n < 15; ++n) qq[n] = b->q[b->iq_cur][n] + off;
+        const double *uc[4];
+        for (int k = 0; k < 4; ++k) uc[k] = b->u[b->iu_cur][k] + off;
+        BCHK(launch_init_conserved(qq, b->nc == 15 ? qq[14] : nullptr, b->e[0] + off, b->p[0] + off, uc, N,
+                                   b->stream));
+    } else {
+        // FP64 primitives staged on the device; conserved variables formed in FP64
+        double *st = nullptr;
+        BCHK(hipMalloc(&st, bytes * 17));
+        const double *src[8] = {e, p, ut, ux, uy, un, pi10, Pi};
+        const size_t len[8] = {1, 1, 1, 1, 1, 1, 10, 1};
+        size_t at = 0;
+        hipError_t r = hipSuccess;
+        for (int k = 0; k < 8 && r == hipSuccess; ++k) {
+            if (src[k]) r = hipMemcpy(st + at * N, src[k], bytes * len[k], hipMemcpyHostToDevice);
+            else r = hipMemset(st + at * N, 0, bytes * len[k]);
+            at += len[k];
+        }
+        mx::MxFields F{};
+        for (int n = 0; n < 15; ++n) F.qo[n] = b->qf[b->iq_cur][n];
+        for (int n = 0; n < 4; ++n) F.qol[n] = b->qfl[b->iq_cur][n];
+        for (int k = 0; k < 4; ++k) F.uo[k] = b->uf[b->iu_cur][k];
+        F.eo = b->ef[0];
+        F.po = b->pf[0];
+        if (r == hipSuccess) r = launch_init_mixed(F, st, st + N, st + 2 * N, st + 6 * N, st + 16 * N, N, off, b->nc,
+                                                   b->stream);
+        for (int k = 0; k < 4 && r == hipSuccess; ++k)  // previous u := u
+            r = hipMemcpyAsync(b->uf[b->iu_prev][k] + off, b->uf[b->iu_cur][k] + off, sizeof(float) * N,
+                               hipMemcpyDeviceToDevice, b->stream);
+        if (r == hipSuccess) r = hipStreamSynchronize(b->stream);
+        (void)hipFree(st);
+        if (r != hipSuccess) {
+            free_cells(b, off, N);
+            E = Event();
+            return bfail(r, "mixed initial state");
+        }
+    }
+    // the event's clock (HydroPlugin.cpp:226,307) with glibc pow(t,2)
+    std::vector<StepTimes> tab(b->cfg.max_steps + 1);
+    tab[0] = StepTimes{0, 0, 0, 0};
+    for (int n = 1; n <= b->cfg.max_steps; ++n) {
+        const double t = (n == 1) ? c.t0 : c.t0 + (n - 1) * c.dt;
+        const double t2 = t + c.dt;
+        tab[n] = StepTimes{t, pow(t, 2), t2, pow(t2, 2)};
+    }
+    BCHK(hipMemcpy(b->d_times + (size_t)id * (b->cfg.max_steps + 1), tab.data(), sizeof(StepTimes) * tab.size(),
+                   hipMemcpyHostToDevice));
+    EvDesc &D = b->h_ev[id];
+    D.nx = c.nx; D.ny = c.ny; D.nz = c.nz; D.active = 1;
+    D.off = off;
+    D.dx = c.dx; D.dy = c.dy; D.dz = c.dz; D.dt = c.dt;
+    D.e_reg = E.e_reg;
+    D.etas_min = c.etabar;
+    D.etas_Tk_GeV = c.etas_Tk_GeV;
+    D.etas_slope_lo = c.etas_slope_lo;
+    D.etas_slope_hi = c.etas_slope_hi;
+    D.tab = b->d_times + (size_t)id * (b->cfg.max_steps + 1);
+    D.base = 1;
+    BCHK(hipMemcpyAsync(b->d_ev + id, &D, sizeof(EvDesc), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->layout_dirty = true;
+    *id_out = id;
+    return VH_OK;
+}
+
+vh_status vh_batch_remove(vh_batch *b, int id) {
+    if (!b || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipStreamSynchronize(b->stream));
+    Event &E = b->ev[id];
+    free_cells(b, E.off, E.N);
+    (void)hipFree(E.fo_prev);
+    (void)hipFree(E.fo_cur);
+    (void)hipFree(E.fo_ids);
+    E = Event();
+    b->h_ev[id].active = 0;
+    BCHK(hipMemcpy(b->d_ev + id, &b->h_ev[id], sizeof(EvDesc), hipMemcpyHostToDevice));
+    b->layout_dirty = true;
+    return VH_OK;
+}
+
+vh_status vh_batch_steps(vh_batch *b, int count) {
+    if (!b || count < 0) return VH_ERR_ARG;
+    if (count == 0) return VH_OK;
+    BCHK(hipSetDevice(b->dev));
+    for (const Event &E : b->ev)
+        if (E.used && E.steps_done + count > b->cfg.max_steps) return VH_ERR_CAPACITY;
+    vh_status s = rebuild_layout(b);
+    if (s != VH_OK) return s;
+    if (b->nblk2 + b->nblk3 == 0) return VH_OK;
+    const auto key = std::make_tuple(count, b->iq_cur * 3 + b->iu_cur, b->layout_version);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
+        hipGraph_t g = nullptr;
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        vh_status cs = VH_OK;
+        for (int k = 0; k < count && cs == VH_OK; ++k) cs = enqueue_batch_step(b, k);
+        if (cs == VH_OK) {
+            hipError_t ae = launch_advance_clocks(b->d_ev, (int)b->ev.size(), count, b->stream);
+            if (ae != hipSuccess) cs = bfail(ae, "advance clocks");
+        }
+        hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
+        if (cs != VH_OK) return cs;
+        if (ce != hipSuccess) return bfail(ce, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return bfail(ie, "hipGraphInstantiate");
+        it = b->graphs.emplace(key, ge).first;
+    }
+    BCHK(hipEventRecord(b->ev0, b->stream));
+    BCHK(hipGraphLaunch(it->second, b->stream));
+    BCHK(hipEventRecord(b->ev1, b->stream));
+    b->timed = true;
+    for (int k = 0; k < count; ++k) {
+        std::swap(b->iq_cur, b->iq_next);
+        std::swap(b->iu_cur, b->iu_prev);
+    }
+    for (Event &E : b->ev)
+        if (E.used) E.steps_done += count;
+    return VH_OK;
+}
+
+vh_status vh_batch_sync(vh_batch *b) {
+    if (!b) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    int err[4] = {0, 0, 0, 0};
+    BCHK(hipMemcpyAsync(err, b->d_err, sizeof(int) * 3, hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    if (b->timed) {
+        float ms = 0;
+        BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->last_ms = ms;
+        b->timed = false;
+    }
+    return err[0] ? VH_ERR_NAN : VH_OK;
+}
+
+double vh_batch_last_ms(const vh_batch *b) { return b ? b->last_ms : 0.0; }
+
+vh_status vh_batch_counts_above(vh_batch *b, double eF, int64_t *counts) {
+    if (!b || !counts) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    vh_status s = rebuild_layout(b);
+    if (s != VH_OK) return s;
+    BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long) * b->ev.size(), b->stream));
+    if (b->mixed)
+        BCHK(launch_count_above_batch_f(b->ef[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start, b->ncnt, b->d_counts,
+                                        b->stream));
+    else
+        BCHK(launch_count_above_batch(b->e[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start, b->ncnt, b->d_counts,
+                                      b->stream));
+    std::vector<unsigned long long> c(b->ev.size());
+    BCHK(hipMemcpyAsync(c.data(), b->d_counts, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost,
+                        b->stream));
+    s = vh_batch_sync(b);
+    for (size_t id = 0; id < c.size(); ++id) counts[id] = b->ev[id].used ? (int64_t)c[id] : -1;
+    return s;
+}
+
+vh_status vh_batch_download(vh_batch *b, int id, int field, double *out) {
+    if (!b || !out || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    const Event &E = b->ev[id];
+    if (!b->mixed) {
+        double *src = field_of(b, field);
+        if (!src) return VH_ERR_ARG;
+        BCHK(hipMemcpyAsync(out, src + E.off, sizeof(double) * E.N, hipMemcpyDeviceToHost, b->stream));
+        return vh_batch_sync(b);
+    }
+    const float *hi = nullptr, *lo = nullptr;
+    if (!field_of_mixed(b, field, &hi, &lo)) return VH_ERR_ARG;
+    double *tmp = nullptr;
+    BCHK(hipMalloc(&tmp, sizeof(double) * E.N));
+    hipError_t r = launch_widen(hi + E.off, lo ? lo + E.off : nullptr, tmp, E.N, b->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(out, tmp, sizeof(double) * E.N, hipMemcpyDeviceToHost, b->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (r != hipSuccess) return bfail(r, "vh_batch_download");
+    return vh_batch_sync(b);
+}
+
+vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells) {
+    if (!b || id < 0 || id >= (int)b->ev.size()) return VH_ERR_ARG;
+    const Event &E = b->ev[id];
+    if (active) *active = E.used ? 1 : 0;
+    if (steps_done) *steps_done = E.steps_done;
+    if (cells) *cells = E.N;
+    return VH_OK;
+}
+
+vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest) {
+    if (!b) return VH_ERR_ARG;
+    long tot = 0, big = 0;
+    for (auto &kv : b->free_seg) {
+        tot += kv.second;
+        big = std::max(big, kv.second);
+    }
+    if (free_cells) *free_cells = tot;
+    if (largest) *largest = big;
+    return VH_OK;
+}
+
+vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
+                           int64_t *n_flagged) {
+    if (!b || !n_flagged || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used || (cap > 0 && !ids))
+        return VH_ERR_ARG;
+    Event &E = b->ev[id];
+    const vh_event_params &c = E.prm;
+    if (c.nx < 2 || c.ny < 2) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    const size_t bytes = sizeof(double) * E.N;
+    const long M = (long)(c.nx - 1) * (c.ny - 1) * (c.nz > 1 ? c.nz - 1 : 1);
+    if (!E.fo_prev) {
+        BCHK(hipMalloc(&E.fo_prev, bytes));
+        BCHK(hipMalloc(&E.fo_cur, bytes));
+        BCHK(hipMalloc(&E.fo_ids, sizeof(long long) * std::max(1L, M)));
+        BCHK(hipMemsetAsync(E.fo_prev, 0, bytes, b->stream));
+        BCHK(hipMemsetAsync(E.fo_cur, 0, bytes, b->stream));
+    }
+    std::swap(E.fo_prev, E.fo_cur);
+    if (b->mixed) BCHK(launch_widen(b->ef[0] + E.off, nullptr, E.fo_cur, E.N, b->stream));
+    else BCHK(hipMemcpyAsync(E.fo_cur, b->e[0] + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
+    BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
+    BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts, b->stream));
+    unsigned long long n = 0;
+    BCHK(hipMemcpyAsync(&n, b->d_counts, sizeof(n), hipMemcpyDeviceToHost, b->stream));
+    vh_status s = vh_batch_sync(b);
+    if (s != VH_OK) return s;
+    *n_flagged = (int64_t)n;
+    if ((int64_t)n > cap) return VH_ERR_CAPACITY;
+    if (n == 0) return VH_OK;
+    std::vector<long long> tmp(n);
+    BCHK(hipMemcpy(tmp.data(), E.fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    std::sort(tmp.begin(), tmp.end());
+    for (size_t m = 0; m < n; ++m) ids[m] = tmp[m];
+    if (corners) {
+        BCHK(hipMemcpy(E.fo_ids, tmp.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+        const int per = c.nz > 1 ? 16 : 8;
+        double *dc = nullptr;
+        BCHK(hipMalloc(&dc, sizeof(double) * per * n));
+        hipError_t le = launch_fo_corners(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, E.fo_ids, (long long)n, dc, b->stream);
+        if (le == hipSuccess) le = hipMemcpyAsync(corners, dc, sizeof(double) * per * n, hipMemcpyDeviceToHost, b->stream);
+        if (le == hipSuccess) le = hipStreamSynchronize(b->stream);
+        (void)hipFree(dc);
+        if (le != hipSuccess) return bfail(le, "fo corners");
+    }
+    return VH_OK;
+}
+
+double vh_batch_eos_energy_density(const vh_batch *b, double T) { return b ? b->heos.energy_density(T) : NAN; }
+
+vh_status vh_batch_autotune(vh_batch *b, int reps, int *tile2d, int *tile3d) {
+    if (!b || reps < 1) return VH_ERR_ARG;
+    if (tile2d) *tile2d = -1;
+    if (tile3d) *tile3d = -1;
+    if (!b->mixed) return VH_OK;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipStreamSynchronize(b->stream));
+    // Stage-1 launches write only the stage buffers, which the next step
+    // overwrites, so timing them leaves the events' state untouched.
+    int best[2] = {b->tile2, b->tile3};
+    for (int dim = 0; dim < 2; ++dim) {
+        float best_ms = 0;
+        for (int v = 0; v < MIXED_NTILE; ++v) {
+            if (dim == 0) b->tile2 = v;
+            else b->tile3 = v;
+            b->layout_dirty = true;
+            vh_status s = rebuild_layout(b);
+            if (s != VH_OK) return s;
+            if ((dim == 0 ? b->nblk2 : b->nblk3) == 0) break;
+            mx::MxFields F = mixed_fields(b, 0, b->iq_cur, b->iu_cur, b->iu_prev, 0, b->iq_stage, b->iu_stage, 1);
+            const int *be = b->d_blk_ev + (dim ? b->nblk2 : 0), *bf = b->d_blk_first + (dim ? b->nblk2 : 0);
+            const int nb = dim ? b->nblk3 : b->nblk2;
+            const int tile = dim ? b->tile3 : b->tile2;
+            BCHK(launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, b->d_ev, be, bf, nb, b->stream));
+            BCHK(hipEventRecord(b->ev0, b->stream));
+            for (int r = 0; r < reps; ++r)
+                BCHK(launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, b->d_ev, be, bf, nb, b->stream));
+            BCHK(hipEventRecord(b->ev1, b->stream));
+            BCHK(hipStreamSynchronize(b->stream));
+            float ms = 0;
+            BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->timed = false;
+            if (v == 0 || ms < best_ms) {
+                best_ms = ms;
+                best[dim] = v;
+            }
+        }
+        if (dim == 0) b->tile2 = best[0];
+        else b->tile3 = best[1];
+    }
+    b->layout_dirty = true;
+    vh_status s = rebuild_layout(b);
+    if (tile2d) *tile2d = b->tile2;
+    if (tile3d) *tile3d = b->tile3;
+    return s;
+}
+
+vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d) {
+    if (!b || tile2d < 0 || tile2d >= MIXED_NTILE || tile3d < 0 || tile3d >= MIXED_NTILE) return VH_ERR_ARG;
+    if (!b->mixed) return VH_OK;
+    b->tile2 = tile2d;
+    b->tile3 = tile3d;
+    b->layout_dirty = true;
+    return VH_OK;
+}
+
+vh_status vh_batch_nan_info(vh_batch *b, int *site, int *event, int64_t *cell) {
+    if (!b) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    int err[4] = {0, 0, 0, 0};
+    BCHK(hipMemcpyAsync(err, b->d_err, sizeof(int) * 3, hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    const long s = (long)(uint32_t)err[1] | ((long)err[2] << 32);
+    int ev = -1;
+    long c = -1;
+    if (err[0])
+        for (int id = 0; id < (int)b->ev.size(); ++id)
+            if (b->ev[id].used && s >= b->ev[id].off && s < b->ev[id].off + b->ev[id].N) {
+                ev = id;
+                c = s - b->ev[id].off;
+            }
+    if (site) *site = err[0];
+    if (event) *event = ev;
+    if (cell) *cell = c;
+    return VH_OK;
+}
+
+vh_status vh_batch_clear_error(vh_batch *b) {
+    if (!b) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipStreamSynchronize(b->stream));
+    BCHK(hipMemset(b->d_err, 0, sizeof(int) * 4));
+    return VH_OK;
+}
+
+}  // extern "C"

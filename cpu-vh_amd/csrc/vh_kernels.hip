@@ -1,4 +1,7 @@
 // gfx950 kernels of the viscous-hydro step and their launchers.
+#include <algorithm>
+#include <vector>
+
 #include "vh_kernels.hpp"
 
 namespace vh {
@@ -117,6 +120,64 @@ hipError_t launch_stage(const StageParams &P, const EosDev &E, int nc, bool dim3
     return dim3d ? launch_stage_t<15, false, true>(P, E, st) : launch_stage_t<15, false, false>(P, E, st);
 }
 
+template <int NC, bool TABLE, bool DIM3>
+static hipError_t launch_stage_batch_t(const StageParams &P, const EosDev &E, const EvDesc *ev, const int *blk_ev,
+                                       const int *blk_first, int nblocks, hipStream_t st) {
+    const unsigned threads = (DIM3 ? 4 : 3) * CELLS_PER_BLOCK;
+    hipLaunchKernelGGL((stage_kernel_batch<NC, TABLE, DIM3>), dim3(nblocks), dim3(threads), 0, st, P, E, ev, blk_ev,
+                       blk_first);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_batch(const StageParams &P, const EosDev &E, int nc, bool dim3d, const EvDesc *ev,
+                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    const bool tab = E.kind == 1;
+#define VH_SB(NC_, T_, D_) launch_stage_batch_t<NC_, T_, D_>(P, E, ev, blk_ev, blk_first, nblocks, st)
+    if (nc == 14) {
+        if (tab) return dim3d ? VH_SB(14, true, true) : VH_SB(14, true, false);
+        return dim3d ? VH_SB(14, false, true) : VH_SB(14, false, false);
+    }
+    if (tab) return dim3d ? VH_SB(15, true, true) : VH_SB(15, true, false);
+    return dim3d ? VH_SB(15, false, true) : VH_SB(15, false, false);
+#undef VH_SB
+}
+
+// Per-event count of cells with e > eF over a table of 64-cell chunks: chunk b
+// covers cells [start[b], start[b] + 64) of event ev_of[b].
+__global__ __launch_bounds__(64) void count_above_batch_kernel(const double *__restrict__ e, double eF,
+                                                               const EvDesc *__restrict__ ev,
+                                                               const int *__restrict__ ev_of,
+                                                               const long *__restrict__ start,
+                                                               unsigned long long *__restrict__ counts) {
+    const int id = ev_of[blockIdx.x];
+    const EvDesc &D = ev[id];
+    const long N = (long)D.nx * D.ny * D.nz;
+    const long sl = start[blockIdx.x] + threadIdx.x;
+    const bool above = sl < N && e[D.off + sl] > eF;
+    const unsigned long long mask = __ballot(above);
+    if (threadIdx.x == 0 && mask) atomicAdd(&counts[id], (unsigned long long)__popcll(mask));
+}
+
+hipError_t launch_count_above_batch(const double *e, double eF, const EvDesc *ev, const int *ev_of,
+                                    const long *start, int nchunks, unsigned long long *counts, hipStream_t st) {
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_above_batch_kernel, dim3(nchunks), dim3(64), 0, st, e, eF, ev, ev_of, start, counts);
+    return hipGetLastError();
+}
+
+// Every event's clock moves on by count steps (the last node of a batch graph).
+__global__ __launch_bounds__(64) void advance_clocks_kernel(EvDesc *__restrict__ ev, int n, int count) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k < n) ev[k].base += count;
+}
+
+hipError_t launch_advance_clocks(EvDesc *ev, int n, int count, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(advance_clocks_kernel, dim3((n + 63) / 64), dim3(64), 0, st, ev, n, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,
                                  const double *const u[4], long N, hipStream_t st) {
     const unsigned grid = (unsigned)((N + 255) / 256);
@@ -146,6 +207,53 @@ hipError_t launch_fo_corners(const double *eprev, const double *ecur, int nx, in
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(fo_corners_kernel, dim3(grid), dim3(256), 0, st, eprev, ecur, nx, ny, nz, ids, n, corners);
     return hipGetLastError();
+}
+
+hipError_t DeviceEos::init(const HostEos &h, int kind) {
+    E = EosDev{};
+    E.kind = kind;
+    hipError_t r;
+    if (kind == 1) {
+        E.cs2_const = h.cs2_is_one_third() ? 1 : 0;
+        E.e_min = h.e_min;
+        E.e_spacing = h.e_spacing;
+        E.r_spacing = 1.0 / h.e_spacing;
+        E.e_max = h.e_max;
+        const size_t tb = sizeof(double) * h.p_tb.size();
+        if ((r = hipMalloc(&ptb, tb)) != hipSuccess || (r = hipMalloc(&Ttb, tb)) != hipSuccess) return r;
+        if ((r = hipMemcpy(ptb, h.p_tb.data(), tb, hipMemcpyHostToDevice)) != hipSuccess ||
+            (r = hipMemcpy(Ttb, h.T_tb.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+            return r;
+        E.p_tb = ptb;
+        E.T_tb = Ttb;
+    } else {
+        const QuarterRootExceptions &X = quarter_root_exceptions();
+        std::vector<uint2> pairs(std::max<size_t>(1, X.key.size()));
+        for (size_t i = 0; i < X.key.size(); ++i) pairs[i] = make_uint2(X.key[i], X.val[i]);
+        if ((r = hipMalloc(&exc, sizeof(uint2) * pairs.size())) != hipSuccess ||
+            (r = hipMalloc(&bucket, sizeof(uint32_t) * X.bucket.size())) != hipSuccess)
+            return r;
+        if ((r = hipMemcpy(exc, pairs.data(), sizeof(uint2) * pairs.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+            (r = hipMemcpy(bucket, X.bucket.data(), sizeof(uint32_t) * X.bucket.size(), hipMemcpyHostToDevice)) !=
+                hipSuccess)
+            return r;
+        E.exc = exc;
+        E.exc_bucket = bucket;
+        E.exc_lo = X.lo;
+        E.exc_hi = X.hi;
+        E.exc_shift = X.shift;
+    }
+    return hipSuccess;
+}
+
+void DeviceEos::release() {
+    (void)hipFree(ptb);
+    (void)hipFree(Ttb);
+    (void)hipFree(exc);
+    (void)hipFree(bucket);
+    ptb = Ttb = nullptr;
+    exc = nullptr;
+    bucket = nullptr;
 }
 
 }  // namespace vh

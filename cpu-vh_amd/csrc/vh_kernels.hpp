@@ -1,11 +1,33 @@
 // Launch interface between the C-ABI layer and the gfx950 kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <string>
 #include "vh_stage.hpp"
+#include "vh_host_eos.hpp"
 
 namespace vh {
 
+// Message vh_status_string returns for VH_ERR_HIP (per host thread).
+void set_last_error(const std::string &msg);
+
+// Device copy of an equation of state: the table columns (table EoS) or the
+// powf(x, 0.25f) exception list (conformal EoS), and the EosDev view of them.
+struct DeviceEos {
+    EosDev E{};
+    double *ptb = nullptr, *Ttb = nullptr;
+    uint2 *exc = nullptr;
+    uint32_t *bucket = nullptr;
+    hipError_t init(const HostEos &h, int kind);  // on the current device
+    void release();
+};
+
 hipError_t launch_stage(const StageParams &P, const EosDev &E, int nc, bool dim3d, hipStream_t st);
+hipError_t launch_stage_batch(const StageParams &P, const EosDev &E, int nc, bool dim3d, const EvDesc *ev,
+                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st);
+hipError_t launch_count_above_batch(const double *e, double eF, const EvDesc *ev, const int *ev_of,
+                                    const long *start, int nchunks, unsigned long long *counts, hipStream_t st);
+hipError_t launch_advance_clocks(EvDesc *ev, int n, int count, hipStream_t st);
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,
                                  const double *const u[4], long N, hipStream_t st);
 hipError_t launch_count_above(const double *e, long N, double eF, unsigned long long *out, hipStream_t st);
@@ -13,5 +35,20 @@ hipError_t launch_fo_flag(const double *eprev, const double *ecur, int nx, int n
                           long long *ids, long long cap, unsigned long long *count, hipStream_t st);
 hipError_t launch_fo_corners(const double *eprev, const double *ecur, int nx, int ny, int nz, const long long *ids,
                              long long n, double *corners, hipStream_t st);
+
+// Mixed-precision form (FP32 storage; FP64 root solve, EoS and accumulation).
+namespace mx {
+struct MxFields;
+}
+constexpr int MIXED_NTILE = 3;
+extern const int MIXED_TILES_2D[MIXED_NTILE][3];  // (TX, TY, TZ) of each 2+1D tile variant
+extern const int MIXED_TILES_3D[MIXED_NTILE][3];
+hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, const EvDesc *ev,
+                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st);
+hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,
+                             const double *pi10, const double *Pi, long N, long off, int nc, hipStream_t st);
+hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
+                                      const long *start, int nblocks, unsigned long long *counts, hipStream_t st);
+hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, hipStream_t st);
 
 }  // namespace vh

@@ -1,0 +1,136 @@
+// Launchers of the mixed-precision kernels (tile variants for the autotuner).
+#include "vh_kernels.hpp"
+#include "vh_mixed.hpp"
+
+namespace vh {
+
+namespace mx {
+
+// setConservedVariables (rhic/src/DynamicalVariables.cpp:169-218) in FP64 from
+// FP64 primitives, stored as the mixed form's FP32 fields.
+__global__ __launch_bounds__(256) void init_mixed_kernel(const MxFields F, const double *__restrict__ e,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ u4,
+                                                         const double *__restrict__ pi10,
+                                                         const double *__restrict__ Pi, long N, long off, int nc) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const long s = off + c;
+    const double ut = u4[c], ux = u4[N + c], uy = u4[2 * N + c], un = u4[3 * N + c];
+    const double Pb = (nc == 15 && Pi) ? Pi[c] : 0.0;
+    const double P = p[c] + Pb;
+    const double w = e[c] + P;
+    double T4[4];
+    T4[0] = w * ut * ut - P + pi10[c];
+    T4[1] = w * ut * ux + pi10[N + c];
+    T4[2] = w * ut * uy + pi10[2 * N + c];
+    T4[3] = w * ut * un + pi10[3 * N + c];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const float hi = (float)T4[n];
+        F.qo[n][s] = hi;
+        F.qol[n][s] = (float)(T4[n] - (double)hi);
+    }
+    for (int n = 0; n < 10; ++n) F.qo[4 + n][s] = (float)pi10[n * N + c];
+    if (nc == 15) F.qo[14][s] = (float)Pb;
+    F.eo[s] = (float)e[c];
+    F.po[s] = (float)p[c];
+    F.uo[0][s] = (float)ut;
+    F.uo[1][s] = (float)ux;
+    F.uo[2][s] = (float)uy;
+    F.uo[3][s] = (float)un;
+}
+
+}  // namespace mx
+
+namespace {
+
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
+hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, const int *blk_ev,
+                     const int *blk_first, int nblocks, hipStream_t st) {
+    const unsigned threads = (DIM3 ? 4 : 3) * 64;
+    hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ>), dim3(nblocks), dim3(threads), 0, st, F,
+                       E, ev, blk_ev, blk_first);
+    return hipGetLastError();
+}
+
+template <int NC, bool TABLE>
+hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, int tile, const EvDesc *ev,
+                          const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
+    if (!dim3d) {
+        switch (tile) {
+        case 0: return launch_mx<NC, TABLE, false, 8, 7, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 1: return launch_mx<NC, TABLE, false, 16, 3, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 2: return launch_mx<NC, TABLE, false, 32, 1, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        }
+    } else {
+        switch (tile) {
+        case 0: return launch_mx<NC, TABLE, true, 4, 4, 3>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 1: return launch_mx<NC, TABLE, true, 4, 3, 4>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 2: return launch_mx<NC, TABLE, true, 8, 2, 2>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+const int MIXED_TILES_2D[MIXED_NTILE][3] = {{8, 7, 1}, {16, 3, 1}, {32, 1, 1}};
+const int MIXED_TILES_3D[MIXED_NTILE][3] = {{4, 4, 3}, {4, 3, 4}, {8, 2, 2}};
+
+hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, const EvDesc *ev,
+                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    if (tile < 0 || tile >= MIXED_NTILE) return hipErrorInvalidValue;
+    const bool tab = E.kind == 1;
+    if (nc == 14)
+        return tab ? launch_mx_tile<14, true>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st)
+                   : launch_mx_tile<14, false>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st);
+    return tab ? launch_mx_tile<15, true>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st)
+               : launch_mx_tile<15, false>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st);
+}
+
+hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,
+                             const double *pi10, const double *Pi, long N, long off, int nc, hipStream_t st) {
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mx::init_mixed_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, e, p, u4, pi10,
+                       Pi, N, off, nc);
+    return hipGetLastError();
+}
+
+// Per-event count of cells with e > eF (FP32 state, compared in FP64).
+__global__ __launch_bounds__(64) void count_above_batch_f_kernel(const float *__restrict__ e, double eF,
+                                                                 const EvDesc *__restrict__ ev,
+                                                                 const int *__restrict__ ev_of,
+                                                                 const long *__restrict__ start,
+                                                                 unsigned long long *__restrict__ counts) {
+    const int id = ev_of[blockIdx.x];
+    const EvDesc &D = ev[id];
+    const long N = (long)D.nx * D.ny * D.nz;
+    const long sl = start[blockIdx.x] + threadIdx.x;
+    const bool above = sl < N && (double)e[D.off + sl] > eF;
+    const unsigned long long mask = __ballot(above);
+    if (threadIdx.x == 0 && mask) atomicAdd(&counts[id], (unsigned long long)__popcll(mask));
+}
+
+hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
+                                      const long *start, int nblocks, unsigned long long *counts, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_above_batch_f_kernel, dim3(nblocks), dim3(64), 0, st, e, eF, ev, ev_of, start, counts);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void widen_kernel(const float *__restrict__ hi, const float *__restrict__ lo,
+                                                    double *__restrict__ out, long n) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    out[c] = lo ? (double)hi[c] + (double)lo[c] : (double)hi[c];
+}
+
+hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hi, lo, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace vh

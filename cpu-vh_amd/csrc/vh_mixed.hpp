@@ -1,0 +1,444 @@
+// Mixed-precision stage kernel of the viscous-hydro step (gfx950).
+//
+// Same equations as the FP64 path (rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:866-924
+// and the files it calls), re-associated for speed:
+//   * storage is FP32: q (high parts), u, previous u, e, p; the energy-momentum
+//     components T^{tau mu} also keep an FP32 low part, so their stored value
+//     is the FP64 accumulation to ~2^-48 (compensated storage);
+//   * limited slopes, half-site values, face fluxes and the source terms run
+//     in FP32 with fused multiply-adds and reciprocal multiplies;
+//   * the conserved-to-primitive root solve, the EoS table lookups, the
+//     accumulation acc = q + dt*S + r_x + r_y + r_eta (and the stage-2 convex
+//     combination) and the regulator run in FP64;
+//   * each face flux is evaluated once and shared by the two cells it
+//     separates (the reference evaluates it from both sides, with each
+//     cell's own e as the root solve's starting value; here each side of a
+//     face starts from the e of the cell on that side).
+// A block owns a TX x TY x TZ tile of one event.  Wave 0 evaluates the
+// source terms of the tile's cells; waves 1..3 evaluate the x, y, eta faces
+// of the tile (TX+1)*TY*TZ etc., at most 64 per pass).  Fluxes meet in LDS.
+#pragma once
+#include "vh_stage.hpp"
+
+namespace vh {
+namespace mx {
+
+constexpr float delta_pipi = 1.33333f;  // rhic/src/SourceTerms.cpp:50-53
+constexpr float tau_pipi = 1.42857f;
+constexpr float delta_PiPi = 0.666667f;
+constexpr float lambda_piPi = 1.2f;
+constexpr float THETA_F = 1.1f;         // rhic/include/FluxLimiter.h:13
+
+// Device pointers of one stage (pooled state of a batch).
+struct MxFields {
+    const float *q[15];   // stage input, high parts
+    const float *ql[4];   // stage input, low parts of T^{tau mu}
+    const float *qn[15];  // q^n (stage 2)
+    const float *qnl[4];
+    float *qo[15];
+    float *qol[4];
+    const float *u[4];    // stage input velocity
+    const float *up[4];   // velocity of the time derivative
+    const float *e, *p;   // stage input e, p
+    float *uo[4];
+    float *eo, *po;
+    int stage;            // 0: first stage, 1: second stage
+    int off;              // step offset inside a captured graph
+    int *err;
+};
+
+// rhic/src/FluxLimiter.cpp:14-32 in FP32 (sign(0) = +1).
+__device__ __forceinline__ float minmod_f(float x, float y) {
+    const float sx = x < 0.f ? -1.f : 1.f, sy = y < 0.f ? -1.f : 1.f;
+    return (sx + sy) * 0.5f * __builtin_fminf(__builtin_fabsf(x), __builtin_fabsf(y));
+}
+__device__ __forceinline__ float deriv_f(float x, float y, float z) {
+    return minmod_f(THETA_F * (y - x), minmod_f((z - x) * 0.5f, THETA_F * (z - y)));
+}
+
+__device__ __forceinline__ long clampi(long c, long n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+
+template <bool TABLE>
+__device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
+    if constexpr (TABLE) {
+        DivExact dx;
+        return (float)eos_temperature<true>(E, (double)e, dx);
+    } else {
+        return __builtin_sqrtf(__builtin_sqrtf(e * (float)(1.0 / EOS_FACTOR)));
+    }
+}
+
+// Source terms of one cell (loadSourceTerms2 + setPimunuSourceTerms,
+// rhic/src/SourceTerms.cpp:55-313,421-612) in FP32, and the pi-gradient
+// terms of loadSourceTermsX/Y/Z (:316-419) for each direction: Sd[d][mu].
+template <int NC, bool TABLE, bool DIM3>
+__device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, const EosDev &E, long s, int i, int j,
+                                          int k, float t, float ecell, const float (&q)[NC], float (&S)[NC],
+                                          float (&Sd)[3][4]) {
+#pragma clang fp contract(fast)
+    const float rdx = 1.0f / (float)G.dx, rdy = 1.0f / (float)G.dy, rdz = 1.0f / (float)G.dz;
+    const float rt = 1.0f / t, rdt = 1.0f / (float)G.dt;
+    const float r3 = 1.0f / 3.0f, r5 = 1.0f / 5.0f;
+    const float ttt = q[0], ttx = q[1], tty = q[2], ttn = q[3];
+    const float pitt = q[4], pitx = q[5], pity = q[6], pitn = q[7], pixx = q[8], pixy = q[9], pixn = q[10],
+                piyy = q[11], piyn = q[12], pinn = q[13];
+    const float Pi = NC == 15 ? q[NC - 1] : 0.0f;
+    const float *__restrict__ utv = F.u[0];
+    const float *__restrict__ uxv = F.u[1];
+    const float *__restrict__ uyv = F.u[2];
+    const float *__restrict__ unv = F.u[3];
+    const float *__restrict__ pv = F.p;
+    const float p = pv[s];
+    const float ut = utv[s], ux = uxv[s], uy = uyv[s], un = unv[s];
+    const float utp = F.up[0][s], uxp = F.up[1][s], uyp = F.up[2][s], unp = F.up[3][s];
+    const long strY = G.nx, strZ = (long)G.nx * G.ny;
+    const long sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
+    const long sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
+    const long sk = k > 0 ? s - strZ : s, sf = k < G.nz - 1 ? s + strZ : s;
+    const float dxut = deriv_f(utv[sl], ut, utv[sr]) * rdx;
+    const float dxux = deriv_f(uxv[sl], ux, uxv[sr]) * rdx;
+    const float dxuy = deriv_f(uyv[sl], uy, uyv[sr]) * rdx;
+    const float dxun = deriv_f(unv[sl], un, unv[sr]) * rdx;
+    const float dyut = deriv_f(utv[sb], ut, utv[stp]) * rdy;
+    const float dyux = deriv_f(uxv[sb], ux, uxv[stp]) * rdy;
+    const float dyuy = deriv_f(uyv[sb], uy, uyv[stp]) * rdy;
+    const float dyun = deriv_f(unv[sb], un, unv[stp]) * rdy;
+    float dnut = 0.f, dnux = 0.f, dnuy = 0.f, dnun = 0.f, dnp = 0.f;
+    if constexpr (DIM3) {
+        dnut = deriv_f(utv[sk], ut, utv[sf]) * rdz;
+        dnux = deriv_f(uxv[sk], ux, uxv[sf]) * rdz;
+        dnuy = deriv_f(uyv[sk], uy, uyv[sf]) * rdz;
+        dnun = deriv_f(unv[sk], un, unv[sf]) * rdz;
+        dnp = deriv_f(pv[sk], p, pv[sf]) * rdz;
+    }
+    const float dxp = deriv_f(pv[sl], p, pv[sr]) * rdx;
+    const float dyp = deriv_f(pv[sb], p, pv[stp]) * rdy;
+
+    const float t2pow = t * t;
+    const float pP = p + Pi;
+    const float tnn = (ecell + pP) * un * un + pP * rt * rt + pinn;
+    const float rut = 1.0f / ut;
+    const float vx = ux * rut, vy = uy * rut, vn = un * rut;
+    const float dxvx = (dxux - vx * dxut) * rut;
+    const float dyvy = (dyuy - vy * dyut) * rut;
+    const float dnvn = (dnun - vn * dnut) * rut;
+    const float dkvk = dxvx + dyvy + dnvn;
+    S[0] = -(ttt * rt + t * tnn) + dkvk * (pitt - p - Pi) - vx * dxp - vy * dyp - vn * dnp;
+    S[1] = -ttx * rt - dxp + dkvk * pitx;
+    S[2] = -tty * rt - dyp + dkvk * pity;
+    S[3] = -(3 * ttn) * rt - dnp / t2pow + dkvk * pitn;
+
+    // pi-gradient terms of each direction: d_a(pi^{tau mu}) v_a - d_a(pi^{a mu})
+    {
+        const long nbm[3] = {sl, sb, sk}, nbp[3] = {sr, stp, sf};
+        const float fac[3] = {0.5f * rdx, 0.5f * rdy, 0.5f * rdz};
+        const float vd[3] = {vx, vy, vn};
+#pragma unroll
+        for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
+            auto dq = [&](int n) { return (F.q[n][nbp[d]] - F.q[n][nbm[d]]) * fac[d]; };
+#pragma unroll
+            for (int mu = 0; mu < 4; ++mu) Sd[d][mu] = dq(4 + mu) * vd[d] - dq(pi_a_mu(d, mu));
+            if constexpr (NC == 15) {
+                const float dPi = dq(14);
+                Sd[d][0] -= vd[d] * dPi;
+                Sd[d][d + 1] -= d == 2 ? dPi / t2pow : dPi;
+            }
+        }
+        if constexpr (!DIM3) Sd[2][0] = Sd[2][1] = Sd[2][2] = Sd[2][3] = 0.f;
+    }
+
+    const float e = ecell;
+    const float T = temperature_f<TABLE>(E, e);
+    const float d_etabar = (float)etabar_of_T(G, (double)T);
+    const float taupiInv = T * r5 / d_etabar;
+    const float beta_pi = (e + p) * r5;
+    DivExact dx64;
+    const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
+    const float a = 1.0f / 3.0f - cs2;
+    const float a2 = a * a;
+    const float beta_Pi = 15 * a2 * (e + p);
+    const float lambda_Pipi = 8 * a * r5;
+    float tauPiInv = 0.0f;
+    if constexpr (NC == 15) {
+        // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48)
+        if (a2 != 0.0f) {
+            const float x = T / 1.01355f;
+            float zetabar;
+            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) / 0.025f) + 0.25f * __expf(-(x - 1) / 0.13f) + 0.001f;
+            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) / 0.0025f) + 0.22f * __expf((x - 1) / 0.022f) + 0.03f;
+            else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
+            tauPiInv = 15 * a2 * T / zetabar;
+        }
+    }
+    const float ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
+    const float rt2 = 1.0f / t2, rt3 = 1.0f / t3, r2t = 1.0f / (2.0f * t);
+    const float dtut = (ut - utp) * rdt;
+    const float dtux = (ux - uxp) * rdt;
+    const float dtuy = (uy - uyp) * rdt;
+    const float dtun = (un - unp) * rdt;
+    const float Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
+    const float DuxUpper = ut * dtux + ux * dxux + uy * dyux + un * dnux;
+    const float Dux = -DuxUpper;
+    const float DuyUpper = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
+    const float Duy = -DuyUpper;
+    const float DunUpper = ut * dtun + ux * dxun + uy * dyun + un * dnun + (2 * ut * un) * rt;
+    const float Dun = -t2 * DunUpper;
+    const float dut = Dut - t * un * un;
+    const float dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
+    const float duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
+    const float dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
+    const float theta = (ut) * rt + dtut + dxux + dyuy + dnun;
+    const float stt = -t * ut * un2 + (dtut - ut * dut) + ((ut2 - 1) * theta) * r3;
+    const float stx = -(t * un2 * ux) / 2.0f + (dtux - dxut) / 2.0f - (ux * dut + ut * dux) / 2.0f + (ut * ux * theta) * r3;
+    const float sty = -(t * un2 * uy) / 2.0f + (dtuy - dyut) / 2.0f - (uy * dut + ut * duy) / 2.0f + (ut * uy * theta) * r3;
+    const float stn = (-un * (2.0f * ut2 + t2 * un2)) * r2t + (dtun - (dnut) * rt2) / 2.0f -
+                       (un * dut + ut * dun) / 2.0f + (ut * un * theta) * r3;
+    const float sxx = -(dxux + ux * dux) + ((1.0f + ux * ux) * theta) * r3;
+    const float sxy = -(dxuy + dyux) / 2.0f - (uy * dux + ux * duy) / 2.0f + (ux * uy * theta) * r3;
+    const float sxn = (-ut * ux * un) * rt - (dxun + (dnux) * rt2) / 2.0f - (un * dux + ux * dun) / 2.0f + (ux * un * theta) * r3;
+    const float syy = -(dyuy + uy * duy) + ((1.0f + uy * uy) * theta) * r3;
+    const float syn = (-ut * uy * un) * rt - (dyun + (dnuy) * rt2) / 2.0f - (un * duy + uy * dun) / 2.0f + (uy * un * theta) * r3;
+    const float snn = (-ut * (1.0f + 2.0f * t2 * un2)) * rt3 - (dnun) * rt2 - un * dun + ((rt2 + un2) * theta) * r3;
+    const float wtx = (-dtux - dxut + ut * dux - ux * dut - t * un2 * ux) / 2.0f;
+    const float wty = (-dtuy - dyut + ut * duy - uy * dut - t * un2 * uy) / 2.0f;
+    const float wtn = (-t2 * dtun - 2.0f * t * un - dnut - ut * Dun - t2 * un * dut - t3 * un * un2) / 2.0f;
+    const float wxy = (dxuy - dyux + ux * duy - uy * dux) / 2.0f;
+    const float wxn = (t2 * dxun - dnux - ux * Dun - t2 * un * dux) / 2.0f;
+    const float wyn = (t2 * dyun - dnuy - uy * Dun - t2 * un * duy) / 2.0f;
+    const float wxt = wtx, wyt = wty, wnt = (wtn) * rt2, wyx = -wxy, wnx = (-wxn) * rt2, wny = (-wyn) * rt2;
+    const float I1tt = 2.0f * ut * (pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun);
+    const float I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
+                        (pitn * ux + pixn * ut) * Dun;
+    const float I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy +
+                        (pitn * uy + piyn * ut) * Dun;
+    const float I1tn = (pitt * un + pitn * ut) * Dut + (pitx * un + pixn * ut) * Dux + (pity * un + piyn * ut) * Duy +
+                        (pitn * un + pinn * ut) * Dun;
+    const float I1xx = 2.0f * ux * (pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun);
+    const float I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy +
+                        (pixn * uy + piyn * ux) * Dun;
+    const float I1xn = (pitx * un + pitn * ux) * Dut + (pixx * un + pixn * ux) * Dux + (pixy * un + piyn * ux) * Duy +
+                        (pixn * un + pinn * ux) * Dun;
+    const float I1yy = 2.0f * uy * (pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun);
+    const float I1yn = (pity * un + pitn * uy) * Dut + (pixy * un + pixn * uy) * Dux + (piyy * un + piyn * uy) * Duy +
+                        (piyn * un + pinn * uy) * Dun;
+    const float I1nn = 2.0f * un * (pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun);
+    const float I2tt = theta * pitt, I2tx = theta * pitx, I2ty = theta * pity, I2tn = theta * pitn,
+                 I2xx = theta * pixx, I2xy = theta * pixy, I2xn = theta * pixn, I2yy = theta * piyy,
+                 I2yn = theta * piyn, I2nn = theta * pinn;
+    const float I3tt = 2.0f * (pitx * wtx + pity * wty + pitn * wtn);
+    const float I3tx = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
+    const float I3ty = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
+    const float I3tn = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
+    const float I3xx = 2.0f * (pitx * wxt + pixy * wxy + pixn * wxn);
+    const float I3xy = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
+    const float I3xn = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
+    const float I3yy = 2.0f * (pity * wyt + pixy * wyx + piyn * wyn);
+    const float I3yn = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
+    const float I3nn = 2.0f * (pitn * wnt + pixn * wnx + piyn * wny);
+    const float ux2 = ux * ux, uy2 = uy * uy;
+    const float ps = pitt * stt - 2.0f * pitx * stx - 2.0f * pity * sty + pixx * sxx + 2.0f * pixy * sxy + piyy * syy -
+                      2.0f * pitn * stn * t2 + 2.0f * pixn * sxn * t2 + 2.0f * piyn * syn * t2 + pinn * snn * t2 * t2;
+    const float I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - ((1.0f - ut2) * ps) * r3;
+    const float I4tx = (pitt * stx + pitx * stt) / 2.0f - (pitx * sxx + pixx * stx) / 2.0f -
+                        (pity * sxy + pixy * sty) / 2.0f - t2 * (pitn * sxn + pixn * stn) / 2.0f + ((ut * ux) * ps) * r3;
+    const float I4ty = (pitt * sty + pity * stt) / 2.0f - (pitx * sxy + pixy * stx) / 2.0f -
+                        (pity * syy + piyy * sty) / 2.0f - t2 * (pitn * syn + piyn * stn) / 2.0f + ((ut * uy) * ps) * r3;
+    const float I4tn = (pitt * stn + pitn * stt) / 2.0f - (pitx * sxn + pixn * stx) / 2.0f -
+                        (pity * syn + piyn * sty) / 2.0f - t2 * (pitn * snn + pinn * stn) / 2.0f + ((ut * un) * ps) * r3;
+    const float I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + ((1.0f + ux2) * ps) * r3;
+    const float I4xy = (pitx * sty + pity * stx) / 2.0f - (pixx * sxy + pixy * sxx) / 2.0f -
+                        (pixy * syy + piyy * sxy) / 2.0f - t2 * (pixn * syn + piyn * sxn) / 2.0f + ((ux * uy) * ps) * r3;
+    const float I4xn = (pitx * stn + pitn * stx) / 2.0f - (pixx * sxn + pixn * sxx) / 2.0f -
+                        (pixy * syn + piyn * sxy) / 2.0f - t2 * (pixn * snn + pinn * sxn) / 2.0f + ((ux * un) * ps) * r3;
+    const float I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + ((1.0f + uy2) * ps) * r3;
+    const float I4yn = (pity * stn + pitn * sty) / 2.0f - (pixy * sxn + pixn * sxy) / 2.0f -
+                        (piyy * syn + piyn * syy) / 2.0f - t2 * (piyn * snn + pinn * syn) / 2.0f + ((uy * un) * ps) * r3;
+    const float I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + ((rt2 + un2) * ps) * r3;
+    const float Itt = I1tt + delta_pipi * I2tt - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
+    const float Itx = I1tx + delta_pipi * I2tx - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
+    const float Ity = I1ty + delta_pipi * I2ty - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
+    const float Itn = I1tn + delta_pipi * I2tn - I3tn + tau_pipi * I4tn - lambda_piPi * Pi * stn;
+    const float Ixx = I1xx + delta_pipi * I2xx - I3xx + tau_pipi * I4xx - lambda_piPi * Pi * sxx;
+    const float Ixy = I1xy + delta_pipi * I2xy - I3xy + tau_pipi * I4xy - lambda_piPi * Pi * sxy;
+    const float Ixn = I1xn + delta_pipi * I2xn - I3xn + tau_pipi * I4xn - lambda_piPi * Pi * sxn;
+    const float Iyy = I1yy + delta_pipi * I2yy - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
+    const float Iyn = I1yn + delta_pipi * I2yn - I3yn + tau_pipi * I4yn - lambda_piPi * Pi * syn;
+    const float Inn = I1nn + delta_pipi * I2nn - I3nn + tau_pipi * I4nn - lambda_piPi * Pi * snn;
+    S[4] = (2.0f * beta_pi * stt - pitt * taupiInv - Itt - 2.0f * un * t * pitn) * rut + pitt * dkvk;
+    S[5] = (2.0f * beta_pi * stx - pitx * taupiInv - Itx - un * t * pixn) * rut + pitx * dkvk;
+    S[6] = (2.0f * beta_pi * sty - pity * taupiInv - Ity - un * t * piyn) * rut + pity * dkvk;
+    S[7] = (2.0f * beta_pi * stn - pitn * taupiInv - Itn - un * t * pinn - (ut * pitn + un * pitt) * rt) * rut +
+           pitn * dkvk;
+    S[8] = (2.0f * beta_pi * sxx - pixx * taupiInv - Ixx) * rut + pixx * dkvk;
+    S[9] = (2.0f * beta_pi * sxy - pixy * taupiInv - Ixy) * rut + pixy * dkvk;
+    S[10] = (2.0f * beta_pi * sxn - pixn * taupiInv - Ixn - (ut * pixn + un * pitx) * rt) * rut + pixn * dkvk;
+    S[11] = (2.0f * beta_pi * syy - piyy * taupiInv - Iyy) * rut + piyy * dkvk;
+    S[12] = (2.0f * beta_pi * syn - piyn * taupiInv - Iyn - (ut * piyn + un * pity) * rt) * rut + piyn * dkvk;
+    S[13] = (2.0f * beta_pi * snn - pinn * taupiInv - Inn - (2 * (ut * pinn + un * pitn)) * rt) * rut + pinn * dkvk;
+    if constexpr (NC == 15) {
+        const float dPi = -beta_Pi * theta - Pi * tauPiInv - delta_PiPi * Pi * theta + lambda_Pipi * ps;
+        S[14] = (dPi) * rut + Pi * dkvk;
+    } else {
+        (void)beta_Pi; (void)lambda_Pipi; (void)tauPiInv;
+    }
+
+}
+
+// Face fluxes of one direction for a tile: the faces between cells a = b-1
+// and b along dir, for every b of the tile plus one more layer.  Each face:
+// limited slopes of the two cells from the 4-point line (clamped at the
+// lattice edges, which equals the reference's ghost cells), half-site
+// values qL = q_a + D_a/2 and qR = q_b - D_b/2
+// (HalfSiteExtrapolation.cpp:12-26), an FP64 root solve on each side, the
+// local speed max(|v_L|, |v_R|) and H = (v_R qR + v_L qL - a (qR - qL)) / 2
+// (SemiDiscreteKurganovTadmorScheme.cpp:14-62).
+template <int NC, bool TABLE, int TX, int TY, int TZ, int NF>
+__device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i0, int j0,
+                                         int k0, double t, float (&H)[NC][NF], int lane) {
+#pragma clang fp contract(fast)
+    const int ex = TX + (dir == 0), ey = TY + (dir == 1), ez = TZ + (dir == 2);
+    const int nf = ex * ey * ez;
+    const long nx = G.nx, ny = G.ny, nz = G.nz;
+    const long ndir = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    const long stride = dir == 0 ? 1 : (dir == 1 ? nx : nx * ny);
+    for (int f = lane; f < nf; f += 64) {
+        const int fx = f % ex, fr = f / ex, fy = fr % ey, fz = fr / ey;
+        const long i = i0 + fx, j = j0 + fy, k = k0 + fz;  // cell b (after the face)
+        const long cb = dir == 0 ? i : (dir == 1 ? j : k);
+        const bool need = (dir == 0 ? i - 1 < nx : i < nx) && (dir == 1 ? j - 1 < ny : j < ny) &&
+                          (dir == 2 ? k - 1 < nz : k < nz);
+        if (!need) continue;
+        const long s0 = G.off + (dir == 0 ? 0 : i) + nx * ((dir == 1 ? 0 : j) + ny * (dir == 2 ? 0 : k));
+        const long p0 = s0 + clampi(cb - 2, ndir) * stride, p1 = s0 + clampi(cb - 1, ndir) * stride;
+        const long p2 = s0 + clampi(cb, ndir) * stride, p3 = s0 + clampi(cb + 1, ndir) * stride;
+        float qL[NC], qR[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            const float *__restrict__ fld = F.q[n];
+            const float x0 = fld[p0], x1 = fld[p1], x2 = fld[p2], x3 = fld[p3];
+            qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
+            qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
+        }
+        const double eL = F.e[p1], eR = F.e[p2];
+        float v[2];
+        int nan = NAN_NONE;
+        DivFast div;
+#pragma unroll 1
+        for (int side = 0; side < 2; ++side) {
+            double x[9];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const int n = m < 8 ? m : NC - 1;
+                x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
+            }
+            const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
+                                          side == 0 ? eL : eR, nan, div);
+            const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
+            if (side == 0) v[0] = vs;
+            else v[1] = vs;
+        }
+        if (nan != NAN_NONE) report_nan(F.err, nan, p2);
+        const float a = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
+#pragma unroll
+        for (int n = 0; n < NC; ++n) H[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - a * (qR[n] - qL[n]));
+    }
+}
+
+template <int TX, int TY, int TZ>
+struct Tile {
+    static constexpr int NCELL = TX * TY * TZ;
+    static constexpr int NFX = (TX + 1) * TY * TZ, NFY = TX * (TY + 1) * TZ, NFZ = TX * TY * (TZ + 1);
+    static constexpr int NF = NFX > NFY ? (NFX > NFZ ? NFX : NFZ) : (NFY > NFZ ? NFY : NFZ);
+    static_assert(NCELL <= 64, "a tile holds at most one wave of cells");
+};
+
+// One RK stage for every event of a batch: block b is tile (b - blk_first[b])
+// of event blk_ev[b].
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, const EosDev E,
+                                                          const EvDesc *__restrict__ ev,
+                                                          const int *__restrict__ blk_ev,
+                                                          const int *__restrict__ blk_first) {
+#pragma clang fp contract(fast)
+    using T = Tile<TX, TY, TZ>;
+    constexpr int ROLES = DIM3 ? 4 : 3;
+    __shared__ float H[ROLES - 1][NC][T::NF];
+    const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blockIdx.x]);
+    const int tile = (int)blockIdx.x - __builtin_amdgcn_readfirstlane(blk_first[blockIdx.x]);
+    const EvDesc &G = ev[eid];
+    const int ntx = (G.nx + TX - 1) / TX, nty = (G.ny + TY - 1) / TY, ntz = (G.nz + TZ - 1) / TZ;
+    if (tile < 0 || tile >= ntx * nty * ntz) return;  // block-uniform: no barrier is skipped by part of a block
+    const int i0 = (tile % ntx) * TX, j0 = ((tile / ntx) % nty) * TY, k0 = (tile / (ntx * nty)) * TZ;
+    const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    const StepTimes st = G.tab[G.base + F.off];
+    const double t = F.stage ? st.t2 : st.t1;
+    const double t_new = st.t2;
+    if (role > 0) {
+        faces_mx<NC, TABLE, TX, TY, TZ, T::NF>(F, G, E, role - 1, i0, j0, k0, t, H[role - 1], lane);
+        __syncthreads();
+        return;
+    }
+    const int lx = lane % TX, ly = (lane / TX) % TY, lz = lane / (TX * TY);
+    const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
+    const bool valid = lane < T::NCELL && i < G.nx && j < G.ny && k < G.nz;
+    const long s = G.off + (valid ? i + (long)G.nx * (j + (long)G.ny * k) : 0);
+    double acc[NC];
+    float Sd[3][4];
+    if (valid) {
+        float q[NC], S[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n) q[n] = F.q[n][s];
+        source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, F.e[s], q, S, Sd);
+        const float dt = (float)G.dt;
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            const double qn = n < 4 ? (double)q[n] + (double)F.ql[n][s] : (double)q[n];
+            acc[n] = qn + (double)(dt * S[n]);
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const float dt = (float)G.dt;
+    const float rd[3] = {1.0f / (float)G.dx, 1.0f / (float)G.dy, 1.0f / (float)G.dz};
+#pragma unroll
+    for (int d = 0; d < ROLES - 1; ++d) {
+        const int ex = TX + (d == 0), ey = TY + (d == 1);
+        const int fb = lx + ex * (ly + ey * lz);  // face before the cell along d
+        const int fa = fb + (d == 0 ? 1 : (d == 1 ? ex : ex * ey));
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            float r = (H[d][n][fb] - H[d][n][fa]) * rd[d];
+            if (n < 4) r += Sd[d][n];
+            acc[n] += (double)(r * dt);
+        }
+    }
+    if (F.stage == 1) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            const double qn = n < 4 ? (double)F.qn[n][s] + (double)F.qnl[n][s] : (double)F.qn[n][s];
+            acc[n] = (acc[n] + qn) * 0.5;
+        }
+    }
+    int nan = NAN_NONE;
+    DivFast div;
+    const Prim w = recover<TABLE>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
+                                  NC == 15 ? acc[NC - 1] : 0.0, (double)F.e[s], nan, div);
+    regulate<NC>(G, t_new, acc, w, nan);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+        const float hi = (float)acc[n];
+        F.qo[n][s] = hi;
+        if (n < 4) F.qol[n][s] = (float)(acc[n] - (double)hi);
+    }
+    F.eo[s] = (float)w.e;
+    F.po[s] = (float)w.p;
+    F.uo[0][s] = (float)w.ut;
+    F.uo[1][s] = (float)w.ux;
+    F.uo[2][s] = (float)w.uy;
+    F.uo[3][s] = (float)w.un;
+    if (nan != NAN_NONE) report_nan(F.err, nan, s);
+}
+
+}  // namespace mx
+}  // namespace vh

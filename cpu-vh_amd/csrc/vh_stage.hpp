@@ -47,6 +47,20 @@ struct StageParams {
     int *err;              // err[0] site, err[1..2] cell (64-bit)
 };
 
+// One event's lattice, transport parameters and clock.  A single-event
+// launch builds it from StageParams; in a batch, event k's copy sits in a
+// device array and its cells occupy [off, off + nx*ny*nz) of every pooled
+// field.  Step n of the event runs at tab[n] (host-computed times).
+struct EvDesc {
+    int nx, ny, nz, active;
+    long off;
+    double dx, dy, dz, dt, e_reg;
+    double etas_min, etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    const StepTimes *tab;
+    int base;  // step index of graph node offset 0
+    int pad;
+};
+
 __device__ __forceinline__ void report_nan(int *err, int site, long s) {
     if (atomicCAS(err, 0, site) == 0) {
         err[1] = (int)(s & 0xffffffff);
@@ -54,11 +68,12 @@ __device__ __forceinline__ void report_nan(int *err, int site, long s) {
     }
 }
 
-__device__ __forceinline__ double etabar_of_T(const StageParams &P, double T) {
-    if (P.etas_slope_lo == 0.0 && P.etas_slope_hi == 0.0) return P.etas_min;
+template <class GEO>
+__device__ __forceinline__ double etabar_of_T(const GEO &G, double T) {
+    if (G.etas_slope_lo == 0.0 && G.etas_slope_hi == 0.0) return G.etas_min;
     double TGeV = T * HBARC;
-    if (TGeV > P.etas_Tk_GeV) return P.etas_min + P.etas_slope_hi * (TGeV - P.etas_Tk_GeV);
-    return P.etas_min + P.etas_slope_lo * (P.etas_Tk_GeV - TGeV);
+    if (TGeV > G.etas_Tk_GeV) return G.etas_min + G.etas_slope_hi * (TGeV - G.etas_Tk_GeV);
+    return G.etas_min + G.etas_slope_lo * (G.etas_Tk_GeV - TGeV);
 }
 
 // Primitive state on one side of a face, with the reciprocal of u^tau used by
@@ -126,8 +141,8 @@ __device__ __forceinline__ int pi_a_mu(int dir, int mu) {
 // uses the owning cell's e as the root solve's previous value, as the
 // reference does; the four half-site root solves run in a loop so the solver
 // is instantiated once.
-template <int NC, bool TABLE, class DV, class SINK>
-__device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &E, long s, int dir, int coord,
+template <int NC, bool TABLE, class DV, class SINK, class GEO>
+__device__ __forceinline__ void dir_contrib(const StageParams &P, const GEO &G, const EosDev &E, long s, int dir, int coord,
                                             int ncoord, long stride, double ePrev, double t, const Recip &rt2pow,
                                             const Recip &rut, double v_c, SINK &&sink, int &nan, DV &div) {
     auto nb = [&](int o) -> long {
@@ -136,7 +151,7 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
         return s + (long)(c - coord) * stride;
     };
     const long sm2 = nb(-2), sm1 = nb(-1), sp1 = nb(1), sp2 = nb(2);
-    const double d = dir == 0 ? P.dx : (dir == 1 ? P.dy : P.dz);
+    const double d = dir == 0 ? G.dx : (dir == 1 ? G.dy : G.dz);
     const Recip rd = make_recip(d);
 
     // Half-site values of the recovery inputs (q[0..7], Pi) on both faces.
@@ -192,7 +207,7 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
         r += Hb;
         r = div(r, rd);
         if (n < 4) r += S[n];
-        r *= P.dt;
+        r *= G.dt;
         sink(n, r);
         __builtin_amdgcn_sched_barrier(0);  // keep one variable's loads live at a time
     }
@@ -201,12 +216,12 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const EosDev &
 // Divisions by the cell's u^tau, by t, t^2, t^3, 2t, pow(t,2), the lattice
 // spacings, dt, 3 and 5 go through correctly rounded reciprocals (divr),
 // which return the division operator's value.
-template <int NC, bool TABLE, class DV>
-__device__ __forceinline__ void source_init(const StageParams &P, const EosDev &E, long s, int i, int j, int k,
+template <int NC, bool TABLE, class DV, class GEO>
+__device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, const EosDev &E, long s, int i, int j, int k,
                                             double t, const Recip &rt2pow, const Recip &rut, double ecell,
                                             double (&acc)[NC], DV &div) {
-    const Recip rdx = make_recip(P.dx), rdy = make_recip(P.dy), rdz = make_recip(P.dz);
-    const Recip rt = make_recip(t), rdt = make_recip(P.dt);
+    const Recip rdx = make_recip(G.dx), rdy = make_recip(G.dy), rdz = make_recip(G.dz);
+    const Recip rt = make_recip(t), rdt = make_recip(G.dt);
     const Recip r3{3.0, 1.0 / 3.0, true}, r5{5.0, 1.0 / 5.0, true};
     const double ttt = acc[0], ttx = acc[1], tty = acc[2], ttn = acc[3];
     const double pitt = acc[4], pitx = acc[5], pity = acc[6], pitn = acc[7], pixx = acc[8], pixy = acc[9],
@@ -220,10 +235,10 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
     const double p = pv[s];
     const double ut = utv[s], ux = uxv[s], uy = uyv[s], un = unv[s];
     const double utp = P.up[0][s], uxp = P.up[1][s], uyp = P.up[2][s], unp = P.up[3][s];
-    const long strY = P.nx, strZ = (long)P.nx * P.ny;
-    const long sl = i > 0 ? s - 1 : s, sr = i < P.nx - 1 ? s + 1 : s;
-    const long sb = j > 0 ? s - strY : s, stp = j < P.ny - 1 ? s + strY : s;
-    const long sk = k > 0 ? s - strZ : s, sf = k < P.nz - 1 ? s + strZ : s;
+    const long strY = G.nx, strZ = (long)G.nx * G.ny;
+    const long sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
+    const long sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
+    const long sk = k > 0 ? s - strZ : s, sf = k < G.nz - 1 ? s + strZ : s;
     const double dxut = div(approx_deriv(utv[sl], ut, utv[sr]), rdx);
     const double dxux = div(approx_deriv(uxv[sl], ux, uxv[sr]), rdx);
     const double dxuy = div(approx_deriv(uyv[sl], uy, uyv[sr]), rdx);
@@ -256,7 +271,7 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
 
     const double e = ecell;
     const double T = eos_temperature<TABLE>(E, e, div);
-    const double d_etabar = etabar_of_T(P, T);
+    const double d_etabar = etabar_of_T(G, T);
     const double taupiInv = div(T, r5) / d_etabar;
     const double beta_pi = div(e + p, r5);
     const double cs2 = eos_cs2<TABLE>(E, e, div);
@@ -388,12 +403,12 @@ __device__ __forceinline__ void source_init(const StageParams &P, const EosDev &
         (void)beta_Pi; (void)lambda_Pipi; (void)tauPiInv;
     }
 #pragma unroll
-    for (int n = 0; n < NC; ++n) acc[n] = acc[n] + P.dt * S[n];
+    for (int n = 0; n < NC; ++n) acc[n] = acc[n] + G.dt * S[n];
 }
 
 // REG_SCHEME_1, rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:689-860, one cell.
-template <int NC>
-__device__ __forceinline__ double regulate(const StageParams &P, double t, double (&acc)[NC], const Prim &w,
+template <int NC, class GEO>
+__device__ __forceinline__ double regulate(const GEO &G, double t, double (&acc)[NC], const Prim &w,
                                            int &nan) {
     const double pitt = acc[4], pitx = acc[5], pity = acc[6], pitn = acc[7], pixx = acc[8], pixy = acc[9],
                  pixn = acc[10], piyy = acc[11], piyn = acc[12], pinn = acc[13];
@@ -401,7 +416,7 @@ __device__ __forceinline__ double regulate(const StageParams &P, double t, doubl
     const double ut = w.ut, ux = w.ux, uy = w.uy, un = w.un;
     const double t2 = t * t;
     double facShear = 1.0, facBulk = 1.0;
-    if (!(w.e > P.e_reg)) {  // the factors are overridden to 1 above e_reg (:835)
+    if (!(w.e > G.e_reg)) {  // the factors are overridden to 1 above e_reg (:835)
         const double xi0 = 0.1, rhomax = 1.0;
         const double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy +
                             piyy * piyy - 2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 +
@@ -467,23 +482,35 @@ __device__ __forceinline__ void store_cell(const StageParams &P, long s, const C
 // recovers (e, p, u), regulates and stores.
 constexpr int CELLS_PER_BLOCK = 64;
 
-template <int NC, bool TABLE, bool DIM3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel(const StageParams P,
-                                                                                             const EosDev E) {
+// Where an event's cells start in the field arrays, and the times of the
+// current step: a single-event launch reads them from its parameters, a batch
+// from the event's descriptor.
+__device__ __forceinline__ long cell_base(const StageParams &) { return 0; }
+__device__ __forceinline__ long cell_base(const EvDesc &G) { return G.off; }
+__device__ __forceinline__ StepTimes step_times(const StageParams &P, const StageParams &) {
+    return P.tab ? P.tab[*P.base + P.off] : P.st;
+}
+__device__ __forceinline__ StepTimes step_times(const StageParams &P, const EvDesc &G) {
+    return G.tab[G.base + P.off];
+}
+
+template <int NC, bool TABLE, bool DIM3, class GEO>
+__device__ __forceinline__ void stage_body(const StageParams &P, const EosDev &E, const GEO &G, long blk) {
     constexpr int ROLES = DIM3 ? 4 : 3;
     __shared__ double part[ROLES][NC][CELLS_PER_BLOCK];
     __shared__ int flags[ROLES][CELLS_PER_BLOCK];  // NaN site of each role
     const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / CELLS_PER_BLOCK);
     const int lane = threadIdx.x % CELLS_PER_BLOCK;
-    const long N = (long)P.nx * P.ny * P.nz;
-    const long s = (long)blockIdx.x * CELLS_PER_BLOCK + lane;
-    const bool valid = s < N && role < ROLES;
-    const long sc = valid ? s : 0;
-    const int i = (int)(sc % P.nx);
-    const long rest = sc / P.nx;
-    const int j = (int)(rest % P.ny);
-    const int k = (int)(rest / P.ny);
-    const StepTimes st = P.tab ? P.tab[*P.base + P.off] : P.st;
+    const long N = (long)G.nx * G.ny * G.nz;
+    const long sl = blk * CELLS_PER_BLOCK + lane;
+    const bool valid = sl >= 0 && sl < N && role < ROLES;
+    const long sc = valid ? sl : 0;
+    const long s = cell_base(G) + sc;  // index into the (pooled) field arrays
+    const int i = (int)(sc % G.nx);
+    const long rest = sc / G.nx;
+    const int j = (int)(rest % G.ny);
+    const int k = (int)(rest / G.ny);
+    const StepTimes st = step_times(P, G);
     const double t = P.stage ? st.t2 : st.t1;
     const double t2pow = P.stage ? st.t2sq : st.t1sq;
     const double t_new = st.t2;
@@ -497,16 +524,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
             double acc[NC];
 #pragma unroll
             for (int n = 0; n < NC; ++n) acc[n] = P.q[n][s];
-            source_init<NC, TABLE>(P, E, s, i, j, k, t, rt2pow, rut, ecell, acc, div);
+            source_init<NC, TABLE>(P, G, E, s, i, j, k, t, rt2pow, rut, ecell, acc, div);
 #pragma unroll
             for (int n = 0; n < NC; ++n) part[0][n][lane] = acc[n];
         } else {
             auto put = [&](int n, double r) { part[role][n][lane] = r; };
             const int dir = role - 1;
             const int coord = dir == 0 ? i : (dir == 1 ? j : k);
-            const int ncoord = dir == 0 ? P.nx : (dir == 1 ? P.ny : P.nz);
-            const long stride = dir == 0 ? 1 : (dir == 1 ? (long)P.nx : (long)P.nx * P.ny);
-            dir_contrib<NC, TABLE>(P, E, s, dir, coord, ncoord, stride, ecell, t, rt2pow, rut, P.u[1 + dir][s], put,
+            const int ncoord = dir == 0 ? G.nx : (dir == 1 ? G.ny : G.nz);
+            const long stride = dir == 0 ? 1 : (dir == 1 ? (long)G.nx : (long)G.nx * G.ny);
+            dir_contrib<NC, TABLE>(P, G, E, s, dir, coord, ncoord, stride, ecell, t, rt2pow, rut, P.u[1 + dir][s], put,
                                    nan, div);
         }
         flags[role][lane] = nan;
@@ -537,8 +564,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     int nan_cell = NAN_NONE;
     o.w = recover<TABLE>(E, t_new, o.acc[0], o.acc[1], o.acc[2], o.acc[3], o.acc[4], o.acc[5], o.acc[6], o.acc[7],
                          NC == 15 ? o.acc[NC - 1] : 0.0, P.e[s], nan_cell, div);
-    o.fs = regulate<NC>(P, t_new, o.acc, o.w, nan_cell);
+    o.fs = regulate<NC>(G, t_new, o.acc, o.w, nan_cell);
     store_cell<NC>(P, s, o, nan != NAN_NONE ? nan : nan_cell);
+}
+
+template <int NC, bool TABLE, bool DIM3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel(const StageParams P,
+                                                                                             const EosDev E) {
+    stage_body<NC, TABLE, DIM3>(P, E, P, blockIdx.x);
+}
+
+// Many events in one launch: block b belongs to event blk_ev[b] and is that
+// event's (b - blk_first[b])-th block of 64 cells.  The per-event fields of P
+// are ignored; its field pointers address the pooled state.
+template <int NC, bool TABLE, bool DIM3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel_batch(
+    const StageParams P, const EosDev E, const EvDesc *__restrict__ ev, const int *__restrict__ blk_ev,
+    const int *__restrict__ blk_first) {
+    const int e = __builtin_amdgcn_readfirstlane(blk_ev[blockIdx.x]);
+    const int b0 = __builtin_amdgcn_readfirstlane(blk_first[blockIdx.x]);
+    const EvDesc &G = ev[e];
+    stage_body<NC, TABLE, DIM3>(P, E, G, (long)blockIdx.x - b0);
 }
 
 }  // namespace vh

@@ -176,6 +176,77 @@ vh_status vh_ic_write_tmunu(const char *path, int nx, int ny, int nz, double dx,
  * reference (which has no graphs). */
 vh_status vh_prepare_steps(vh_handle *h, int count);
 
+/* ---- batches of events ---------------------------------------------------
+ * No counterpart in the reference, which runs one event per process
+ * (rhic/src/HydroPlugin.cpp:74-321).  A batch holds many independent events
+ * in one pooled device state; each event keeps its own lattice, spacings,
+ * clock, eta/s and T_reg, and vh_batch_steps advances every resident event by
+ * the same number of steps (two launches per RK stage: all 2+1D events, then
+ * all 3+1D events).  Results of an event are bit-identical to running it
+ * alone through vh_handle.  bulk and the EoS are per batch. */
+/* Arithmetic of a batch.  VH_PREC_FP64: the drop-in path (each event
+ * bit-identical to vh_handle).  VH_PREC_MIXED: FP32 storage (with an FP32
+ * low part kept for T^{tau mu}) and FP32 slopes, face fluxes and source
+ * terms; FP64 root solve, EoS table lookups, conserved-variable accumulation
+ * and regulator; each face flux evaluated once for the two cells it
+ * separates.  No counterpart in the reference (FP64 only). */
+typedef enum { VH_PREC_FP64 = 0, VH_PREC_MIXED = 1 } vh_precision;
+
+typedef struct {
+    int device;
+    int64_t capacity_cells;    /* cells of all resident events together (61 doubles, or 73 floats, each) */
+    int max_events;
+    int max_steps;             /* steps an event may take before it must be removed */
+    int bulk;
+    int eos_kind;
+    const double *eos_table;   /* as vh_config.eos_table */
+    int precision;             /* vh_precision */
+} vh_batch_config;
+
+/* One event's parameters: vh_config's per-event fields. */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt, t0;
+    double etabar, T_reg_GeV;
+    double etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+} vh_event_params;
+
+typedef struct vh_batch vh_batch;
+
+vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out);
+vh_status vh_batch_destroy(vh_batch *b);
+/* Admits one event (vh_upload_ic + vh_init_conserved at t0 for it): first-fit
+ * placement of its nx*ny*nz cells; VH_ERR_CAPACITY when no free range or
+ * event slot is large enough.  Synchronising. */
+vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e, const double *p, const double *ut,
+                       const double *ux, const double *uy, const double *un, const double *pi10, const double *Pi,
+                       int *id_out);
+/* Releases an event's cells and slot.  Synchronising. */
+vh_status vh_batch_remove(vh_batch *b, int id);
+/* Every resident event takes count steps (event steps n..n+count-1 with n its
+ * own next step), replayed from a captured HIP graph.  Asynchronous. */
+vh_status vh_batch_steps(vh_batch *b, int count);
+vh_status vh_batch_sync(vh_batch *b);
+/* NaN report of the batch: site (vh_nan_site), event id and cell within it. */
+vh_status vh_batch_nan_info(vh_batch *b, int *site, int *event, int64_t *cell);
+vh_status vh_batch_clear_error(vh_batch *b);
+/* counts[id] = cells of event id with e > eF (-1 for free slots); max_events entries. */
+vh_status vh_batch_counts_above(vh_batch *b, double eF, int64_t *counts);
+/* As vh_download / vh_fo_flag for one event. */
+vh_status vh_batch_download(vh_batch *b, int id, int field, double *out);
+vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
+                           int64_t *n_flagged);
+vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells);
+vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest_range);
+/* Mixed form: times stage launches of each tile variant (reps launches each)
+ * on the resident events and keeps the fastest 2+1D and 3+1D tiles (the
+ * state is not advanced).  FP64 batches: no variants, *tile = -1. */
+vh_status vh_batch_autotune(vh_batch *b, int reps, int *tile2d, int *tile3d);
+vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d);
+/* Duration (ms) of the last vh_batch_steps graph, from HIP events; 0 if none. */
+double vh_batch_last_ms(const vh_batch *b);
+double vh_batch_eos_energy_density(const vh_batch *b, double T);
+
 const char *vh_status_string(vh_status s);
 const char *vh_build_info(void);
 

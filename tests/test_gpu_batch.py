@@ -1,0 +1,153 @@
+"""Batches of events (vh_batch_*): every event of a mixed batch evolves bit
+for bit as it does alone (same kernels, own lattice, spacings, clock and
+transport parameters), including events admitted and removed mid-run; the
+per-event counts and freezeout flags match the single-event calls."""
+import numpy as np
+import pytest
+
+import vhtest as V
+from cpuvh_amd import Batch, Hydro
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy",
+          "piyn", "pinn", "e", "p", "ut", "ux", "uy", "un", "up_t", "up_x", "up_y", "up_n"]
+
+# (nx, ny, nz, dx, dt, t0, etabar, T_reg, eta/s slopes, seed): 2+1D and 3+1D
+# events of different sizes, spacings, clocks and transport parameters.
+EVENTS = [
+    (21, 17, 1, 0.10, 0.020, 0.60, 0.20, 0.00, (0.0, 0.0), 3),
+    (13, 11, 9, 0.12, 0.015, 0.50, 0.08, 0.00, (0.0, 0.0), 4),
+    (33, 29, 1, 0.08, 0.010, 1.00, 0.12, 0.25, (0.0, 0.0), 5),
+    (9, 7, 5, 0.10, 0.020, 0.60, 0.08, 0.10, (0.5, 1.0), 6),
+    (70, 3, 1, 0.05, 0.010, 0.40, 0.16, 0.00, (0.3, 0.7), 7),
+]
+
+
+def _ic(ev, lattice, bulk):
+    nx, ny, nz, dx, dt, t0, etabar, treg, slopes, seed = ev
+    return V.random_ic(nx, ny, nz, dx, t0, seed=seed, lattice=lattice, bulk=bulk)
+
+
+def _solo(ev, ic, lattice, bulk, nsteps):
+    nx, ny, nz, dx, dt, t0, etabar, treg, (slo, shi), seed = ev
+    h = Hydro(nx, ny, nz, dx, dx, dx, dt, t0, etabar, T_reg_GeV=treg, bulk=bulk,
+              eos_table=V.eos_table() if lattice else None, etas_slope_lo=slo, etas_slope_hi=shi)
+    h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic.get("Pi"))
+    h.init_conserved(t0)
+    if nsteps:
+        h.steps(1, nsteps)
+    h.sync()
+    return h
+
+
+def _add(b, ev, ic):
+    nx, ny, nz, dx, dt, t0, etabar, treg, (slo, shi), seed = ev
+    return b.add(nx, ny, nz, dx, dx, dx, dt, t0, etabar, ic, T_reg_GeV=treg, etas_slope_lo=slo,
+                 etas_slope_hi=shi)
+
+
+def _fields(bulk):
+    return FIELDS + (["Pi"] if bulk else [])
+
+
+def _same(b, eid, h, bulk):
+    for f in _fields(bulk):
+        a, c = h.get(f), b.get(eid, f)
+        assert V.bits_equal(a, c), f"event {eid} {f}: max rel {V.max_rel(a, c):.3e}"
+
+
+@pytest.mark.parametrize("lattice,bulk", [(False, False), (True, True), (False, True), (True, False)])
+def test_mixed_batch_equals_solo(lattice, bulk):
+    print(f"case batch lattice={lattice} bulk={bulk}", flush=True)
+    cap = sum(e[0] * e[1] * e[2] for e in EVENTS) + 100
+    b = Batch(cap, max_events=8, max_steps=64, bulk=bulk, eos_table=V.eos_table() if lattice else None)
+    ics = [_ic(ev, lattice, bulk) for ev in EVENTS]
+    ids = [_add(b, ev, ic) for ev, ic in zip(EVENTS, ics)]
+    assert None not in ids
+    b.steps(3)
+    b.steps(4)
+    b.sync()
+    for ev, ic, eid in zip(EVENTS, ics, ids):
+        assert b.info(eid)["steps_done"] == 7
+        _same(b, eid, _solo(ev, ic, lattice, bulk, 7), bulk)
+
+
+def test_batch_matches_oracle():
+    """One event of a batch, placed after a neighbour in the pool, against the
+    oracle directly (unregulated, table EoS, bulk)."""
+    ev = EVENTS[1]
+    nx, ny, nz, dx, dt, t0, etabar, treg, slopes, seed = ev
+    ic = _ic(ev, True, True)
+    n0 = EVENTS[0][0] * EVENTS[0][1] * EVENTS[0][2]
+    b = Batch(n0 + nx * ny * nz, max_events=3, max_steps=16, bulk=True, eos_table=V.eos_table())
+    _add(b, EVENTS[0], _ic(EVENTS[0], True, True))
+    eid = _add(b, ev, ic)
+    o = V.Oracle(nx, ny, nz, dx, dx, dx, dt, t0, etabar, T_reg_GeV=treg, bulk=True, table=V.eos_table())
+    o.set_ic(ic)
+    o.init_conserved(t0)
+    for n in range(1, 6):
+        assert o.step(t0 + (n - 1) * dt) == 0
+    b.steps(5)
+    b.sync()
+    for f in _fields(True):
+        a, c = o.get(f), b.get(eid, f)
+        assert V.bits_equal(a, c), f"{f}: max rel {V.max_rel(a, c):.3e}"
+
+
+def test_admission_and_removal_mid_run():
+    """Events join and leave between step chunks; each keeps its own clock and
+    the freed cells are reused (first fit)."""
+    lattice, bulk = False, False
+    A, B, C = EVENTS[0], EVENTS[1], EVENTS[3]
+    nA, nB, nC = (e[0] * e[1] * e[2] for e in (A, B, C))
+    b = Batch(nA + nB, max_events=4, max_steps=32)
+    icA, icB, icC = (_ic(e, lattice, bulk) for e in (A, B, C))
+    a = _add(b, A, icA)
+    b.steps(3)
+    bb = _add(b, B, icB)
+    b.steps(4)
+    assert _add(b, EVENTS[2], _ic(EVENTS[2], lattice, bulk)) is None  # no room: refused
+    b.remove(a)
+    free, largest = b.free_cells()
+    assert free == nA and largest == nA
+    c = _add(b, C, icC)  # nC <= nA: fits into A's old range
+    assert c is not None
+    b.steps(2)
+    b.sync()
+    assert b.info(bb)["steps_done"] == 6 and b.info(c)["steps_done"] == 2
+    _same(b, bb, _solo(B, icB, lattice, bulk, 6), bulk)
+    _same(b, c, _solo(C, icC, lattice, bulk, 2), bulk)
+
+
+def test_counts_and_freezeout_flags():
+    lattice, bulk = True, False
+    evs = [EVENTS[0], EVENTS[1], EVENTS[4]]
+    ics = [_ic(e, lattice, bulk) for e in evs]
+    b = Batch(sum(e[0] * e[1] * e[2] for e in evs), max_events=3, max_steps=16, eos_table=V.eos_table())
+    ids = [_add(b, e, ic) for e, ic in zip(evs, ics)]
+    solos = [_solo(e, ic, lattice, bulk, 0) for e, ic in zip(evs, ics)]
+    b.steps(2)
+    b.steps(3)
+    for h in solos:
+        h.steps(1, 5)
+        h.sync()
+    eF = 1.5
+    counts = b.counts_above(eF)
+    for eid, h in zip(ids, solos):
+        assert counts[eid] == h.count_above(eF)
+        assert np.array_equal(b.fo_flag_ids(eid, eF), h.fo_flag_ids(eF))
+        assert np.array_equal(b.fo_flag_ids(eid, eF), h.fo_flag_ids(eF))
+
+
+def test_step_budget_and_bad_arguments():
+    b = Batch(1000, max_events=2, max_steps=4)
+    ev = EVENTS[4]
+    eid = _add(b, ev, _ic(ev, False, False))
+    b.steps(4)
+    with pytest.raises(Exception):
+        b.steps(1)  # past max_steps
+    with pytest.raises(Exception):
+        b.remove(eid + 1)
+    b.remove(eid)
+    assert b.free_cells() == (1000, 1000)
