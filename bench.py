@@ -95,7 +95,7 @@ def load_traffic(name):
 
 
 # ------------------------------------------------------- batching workload --
-BATCH = dict(n_events=256, n_min=81, n_max=201, dx=0.1, dt=0.02, seed=20240601, e0=10.0, T_F_GeV=0.150,
+BATCH = dict(n_events=256, n_min=81, n_max=201, dx=0.1, dt=0.02, seed=20240601, e0=10.0, e_floor=0.01, T_F_GeV=0.150,
              etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0, T_reg_GeV=0.1, coarse=5, max_steps=3000)
 
 
@@ -118,7 +118,7 @@ def make_batch_events(table, n_events):
     for k in range(n_events):
         n = int(sides[k])
         e, info = C.mc_glauber(n, n, 1, BATCH["dx"], BATCH["dx"], BATCH["dx"], seed=BATCH["seed"] + 7919 * k,
-                               e0=BATCH["e0"])
+                               e0=BATCH["e0"], e_floor=BATCH["e_floor"])
         N = e.size
         ic = {"e": e, "p": table_pressure(table, e), "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N),
               "un": np.zeros(N), "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
@@ -134,7 +134,7 @@ def add_event(b, ev):
                  etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
 
 
-def run_batched(events, table, precision, warmup):
+def run_batched(events, table, precision, warmup, max_batched_steps=None):
     """All events resident in one batch, stepped in chunks of the coarse
     freezeout interval until each meets the stop rule; returns timings."""
     import cpuvh_amd as C
@@ -163,9 +163,22 @@ def run_batched(events, table, precision, warmup):
     graph_ms = 0.0
     batched_steps = 0
     t_start = time.perf_counter()
-    while strikes:
+    failed = []
+    while strikes and (max_batched_steps is None or batched_steps < max_batched_steps):
         b.steps(coarse)
-        counts = b.counts_above(eF)  # synchronising
+        while True:
+            try:
+                counts = b.counts_above(eF)  # synchronising
+                break
+            except C.VhError as ex:  # an event hit a NaN: it leaves the batch, the others go on
+                if ex.status != C.VH_ERR_NAN:
+                    raise
+                site, bad, cell = b.nan_info()
+                b.clear_error()
+                if bad in strikes:
+                    failed.append(bad)
+                    b.remove(bad)
+                    del strikes[bad]
         graph_ms += b.last_ms()
         zone_updates += coarse * sum(cells[i] for i in strikes)
         batched_steps += coarse
@@ -177,9 +190,11 @@ def run_batched(events, table, precision, warmup):
                 del strikes[i]
     wall = time.perf_counter() - t_start
     b.close()
-    return dict(wall_s=wall, events=len(events), events_per_s=len(events) / wall, batched_steps=batched_steps,
-                zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall, graph_ms=graph_ms,
-                steps_to_freezeout=[steps_done[i] for i in ids], tiles=tiles)
+    done = len(events) - len(failed)
+    return dict(wall_s=wall, events=done, events_failed=len(failed), events_per_s=done / wall,
+                batched_steps=batched_steps, zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall,
+                graph_ms=graph_ms, steps_to_freezeout=[steps_done[i] for i in ids if i in steps_done],
+                failed_ids=sorted(failed), tiles=tiles)
 
 
 def run_plain_loop(events, table):
@@ -189,6 +204,7 @@ def run_plain_loop(events, table):
     d = BATCH["dx"]
     zone_updates = 0
     steps = []
+    failed = 0
     t_start = time.perf_counter()
     for ev in events:
         h = C.Hydro(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"],
@@ -199,16 +215,22 @@ def run_plain_loop(events, table):
         h.init_conserved(ev["t0"])
         eF = h.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
         n, strikes = 1, 0
-        while strikes < 3 and n + coarse - 1 <= BATCH["max_steps"]:
-            h.steps(n, coarse)
-            n += coarse
-            strikes = strikes + 1 if h.count_above(eF) == 0 else 0
-        steps.append(n - 1)
+        try:
+            while strikes < 3 and n + coarse - 1 <= BATCH["max_steps"]:
+                h.steps(n, coarse)
+                n += coarse
+                strikes = strikes + 1 if h.count_above(eF) == 0 else 0
+            steps.append(n - 1)
+        except C.VhError as ex:
+            if ex.status != C.VH_ERR_NAN:
+                raise
+            failed += 1
         zone_updates += ev["N"] * (n - 1)
         h.close()
     wall = time.perf_counter() - t_start
-    return dict(wall_s=wall, events=len(events), events_per_s=len(events) / wall, zone_updates=zone_updates,
-                zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
+    done = len(events) - failed
+    return dict(wall_s=wall, events=done, events_failed=failed, events_per_s=done / wall,
+                zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
 
 
 def precision_check(events, table, nsteps):
@@ -285,7 +307,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-dropin", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--max-batch-steps", type=int, default=None,
+                    help="stop the batched runs after this many steps (profiling; events/s then not to freezeout)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 512 MiB device to device (scale of the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 8, "copy_probe_bytes_written": dst.numel() * 8}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(events shard across GPUs with no communication)", file=sys.stderr)
@@ -299,19 +337,25 @@ def main():
     print("bench: building initial states", file=sys.stderr, flush=True)
     events = make_batch_events(table, args.events)
     print("bench: batched mixed", file=sys.stderr, flush=True)
-    mixed = run_batched(events, table, C.PREC_MIXED, args.warmup)
+    mixed = run_batched(events, table, C.PREC_MIXED, args.warmup, args.max_batch_steps)
     fp64b = None
     if not args.no_fp64_batch:
         print("bench: batched fp64", file=sys.stderr, flush=True)
-        fp64b = run_batched(events, table, C.PREC_FP64, args.warmup)
+        fp64b = run_batched(events, table, C.PREC_FP64, args.warmup, args.max_batch_steps)
     stride = max(1, len(events) // max(1, args.plain_sample))
     sample = events[::stride][:args.plain_sample]
-    print("bench: plain fp64 loop", file=sys.stderr, flush=True)
-    plain = run_plain_loop(sample, table)
-    print("bench: precision check", file=sys.stderr, flush=True)
-    prec = precision_check(events[:: max(1, len(events) // 4)][:4], table, 300)
-    print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
-    ic3d, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.steps, args.warmup)
+    plain = None
+    if not args.skip_plain:
+        print("bench: plain fp64 loop", file=sys.stderr, flush=True)
+        plain = run_plain_loop(sample, table)
+    prec = None
+    if not args.skip_precision:
+        print("bench: precision check", file=sys.stderr, flush=True)
+        prec = precision_check(events[:: max(1, len(events) // 4)][:4], table, 300)
+    dropin = None
+    if not args.skip_dropin:
+        print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
+        _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.steps, args.warmup)
 
     # dominant kernel: the mixed stage kernel over the batch (two launches per step)
     nc = 15
@@ -327,7 +371,7 @@ def main():
                     bulk=True, etabar=med["etabar"], T_reg_GeV=BATCH["T_reg_GeV"])
         cpu = cpu_baseline(wcpu, med["ic"], threads, min_seconds=12.0, max_steps=400)
         if cpu and "value" in cpu:
-            mean_steps = float(np.mean(plain["steps_to_freezeout"]))
+            mean_steps = float(np.mean(mixed["steps_to_freezeout"] or [BATCH["max_steps"]]))
             mean_cells = float(np.mean([ev["N"] for ev in events]))
             cpu["events_per_s"] = cpu["value"] / (mean_steps * mean_cells)
             cpu["sample"] += (f"; median-size event of the batch ({med['n']}^2 cells); events/s = zone-upd/s / "
@@ -345,8 +389,8 @@ def main():
                    "cells_total": int(sum(ev["N"] for ev in events)), "form": "batched mixed precision",
                    "tiles": mixed["tiles"]},
         "batched_mixed": {k: v for k, v in mixed.items() if k != "steps_to_freezeout"},
-        "plain_fp64_loop": {k: v for k, v in plain.items() if k != "steps_to_freezeout"},
-        "speedup_vs_plain_fp64": mixed["events_per_s"] / plain["events_per_s"],
+        "plain_fp64_loop": None if plain is None else {k: v for k, v in plain.items() if k != "steps_to_freezeout"},
+        "speedup_vs_plain_fp64": None if plain is None else mixed["events_per_s"] / plain["events_per_s"],
         "batched_fp64": None if fp64b is None else {k: v for k, v in fp64b.items() if k != "steps_to_freezeout"},
         "mixed_vs_fp64_error": prec,
         "fp64_dropin_3d": dropin,

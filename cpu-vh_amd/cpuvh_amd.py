@@ -31,7 +31,7 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
            "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
-           "vh_batch_set_tiles"]
+           "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run"]
 
 
 class VhConfig(ctypes.Structure):
@@ -56,6 +56,33 @@ class EventParams(ctypes.Structure):
                 ("dt", ctypes.c_double), ("t0", ctypes.c_double), ("etabar", ctypes.c_double),
                 ("T_reg_GeV", ctypes.c_double), ("etas_Tk_GeV", ctypes.c_double),
                 ("etas_slope_lo", ctypes.c_double), ("etas_slope_hi", ctypes.c_double)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("rate_per_s", ctypes.c_double), ("n_requests", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("frac_3d", ctypes.c_double), ("side2_min", ctypes.c_int), ("side2_max", ctypes.c_int),
+                ("side3_min", ctypes.c_int), ("side3_max", ctypes.c_int), ("nz3_min", ctypes.c_int),
+                ("nz3_max", ctypes.c_int), ("dx", ctypes.c_double), ("dt", ctypes.c_double), ("t0", ctypes.c_double),
+                ("etabar", ctypes.c_double), ("etas_Tk_GeV", ctypes.c_double), ("etas_slope_lo", ctypes.c_double),
+                ("etas_slope_hi", ctypes.c_double), ("T_reg_GeV", ctypes.c_double), ("T_F_GeV", ctypes.c_double),
+                ("coarse", ctypes.c_int), ("e0", ctypes.c_double), ("e_floor", ctypes.c_double),
+                ("hbm_budget_bytes", ctypes.c_int64), ("precision", ctypes.c_int), ("bulk", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("max_resident", ctypes.c_int), ("collect_surface", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("steps", ctypes.c_int),
+                ("status", ctypes.c_int), ("glauber_seed", ctypes.c_uint64), ("fo_cubes", ctypes.c_int64),
+                ("e_sum", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_int), ("failed", ctypes.c_int), ("wall_s", ctypes.c_double),
+                ("events_per_s", ctypes.c_double), ("zone_upd_per_s", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("occupancy_mean", ctypes.c_double),
+                ("occupancy_peak", ctypes.c_double)]
 
 
 class GlauberParams(ctypes.Structure):
@@ -131,6 +158,9 @@ def _lib():
         L.vh_batch_eos_energy_density.restype = D
         L.vh_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.vh_batch_set_tiles.argtypes = [P, I, I]
+        L.vh_batch_set_face_mode.argtypes = [P, I]
+        L.vh_serve_run.argtypes = [ctypes.POINTER(ServeConfig), P, ctypes.POINTER(ServeStats),
+                                   ctypes.POINTER(ServeRequest)]
         _LIB = L
     return _LIB
 
@@ -363,11 +393,40 @@ class Batch:
         Hydro._check(_lib().vh_batch_autotune(self.b, reps, ctypes.byref(t2), ctypes.byref(t3)), "vh_batch_autotune")
         return t2.value, t3.value
 
+    def set_face_mode(self, mode):
+        Hydro._check(_lib().vh_batch_set_face_mode(self.b, mode), "vh_batch_set_face_mode")
+
     def set_tiles(self, tile2d, tile3d):
         Hydro._check(_lib().vh_batch_set_tiles(self.b, tile2d, tile3d), "vh_batch_set_tiles")
 
     def eos_energy_density(self, T):
         return _lib().vh_batch_eos_energy_density(self.b, T)
+
+
+SERVE_DEFAULTS = dict(rate_per_s=10.0, n_requests=50, seed=1, frac_3d=0.2, side2_min=101, side2_max=201,
+                      side3_min=101, side3_max=161, nz3_min=41, nz3_max=81, dx=0.1, dt=0.02, t0=0.6, etabar=0.08,
+                      etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0, T_reg_GeV=0.1, T_F_GeV=0.150,
+                      coarse=5, e0=10.0, e_floor=0.01, hbm_budget_bytes=64 << 30, precision=PREC_MIXED, bulk=1,
+                      max_steps=3000, max_resident=256, collect_surface=1, device=0)
+
+
+def serve(eos_table=None, **kw):
+    """Runs the serving loop (vh_serve_run); returns (stats dict, list of request dicts)."""
+    cfg = dict(SERVE_DEFAULTS)
+    unknown = set(kw) - set(cfg)
+    if unknown:
+        raise TypeError(f"unknown serving options {sorted(unknown)}")
+    cfg.update(kw)
+    c = ServeConfig(**cfg)
+    st = ServeStats()
+    reqs = (ServeRequest * cfg["n_requests"])()
+    table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)
+    r = _lib().vh_serve_run(ctypes.byref(c), None if table is None else table.ctypes.data, ctypes.byref(st), reqs)
+    if r != VH_OK:
+        raise VhError(r, "vh_serve_run")
+    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+    out = [{f: getattr(q, f) for f, _ in ServeRequest._fields_} for q in reqs]
+    return stats, out
 
 
 def build_info():

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -80,6 +81,7 @@ struct vh_batch {
     float *fpool = nullptr;
     float *qf[3][15] = {}, *qfl[3][4] = {}, *uf[3][4] = {}, *ef[2] = {}, *pf[2] = {};
     int tile2 = 0, tile3 = 0;
+    int face_mode = 2;
     bool layout_dirty = true;
     int layout_version = 0;
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
@@ -232,6 +234,7 @@ mx::MxFields mixed_fields(const vh_batch *b, int stage, int iq_in, int iu_in, in
     F.eo = b->ef[ie_out];
     F.po = b->pf[ie_out];
     F.stage = stage;
+    F.face_mode = b->face_mode;
     F.err = b->d_err;
     return F;
 }
@@ -742,6 +745,15 @@ vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d) {
     b->tile2 = tile2d;
     b->tile3 = tile3d;
     b->layout_dirty = true;
+    return VH_OK;
+}
+
+vh_status vh_batch_set_face_mode(vh_batch *b, int mode) {
+    if (!b || mode < 0 || mode > 2) return VH_ERR_ARG;
+    if (b->face_mode != mode) {
+        b->face_mode = mode;
+        drop_graphs(b);  // the mode is baked into captured launches
+    }
     return VH_OK;
 }
 

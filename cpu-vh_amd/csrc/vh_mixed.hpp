@@ -44,6 +44,7 @@ struct MxFields {
     float *eo, *po;
     int stage;            // 0: first stage, 1: second stage
     int off;              // step offset inside a captured graph
+    int face_mode;        // see faces_mx
     int *err;
 };
 
@@ -292,15 +293,25 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 // (HalfSiteExtrapolation.cpp:12-26), an FP64 root solve on each side, the
 // local speed max(|v_L|, |v_R|) and H = (v_R qR + v_L qL - a (qR - qL)) / 2
 // (SemiDiscreteKurganovTadmorScheme.cpp:14-62).
+//
+// The reference evaluates a face once per cell: cell a (forward face) and
+// cell b (backward face) each start both root solves from their own e
+// (EnergyMomentumTensor.cpp:188-191: e <= 0.1 selects the closed-form
+// branch, otherwise it is the Newton start).  HA is cell a's view, HB cell
+// b's.  face_mode 0: one evaluation, each side started from the e of the
+// cell on that side (HA = HB); 1: both views always; 2: cell a's view, plus
+// cell b's view only where the two cells fall on different sides of the
+// e <= 0.1 branch.
 template <int NC, bool TABLE, int TX, int TY, int TZ, int NF>
 __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i0, int j0,
-                                         int k0, double t, float (&H)[NC][NF], int lane) {
+                                         int k0, double t, float (&HA)[NC][NF], float (&HB)[NC][NF], int lane) {
 #pragma clang fp contract(fast)
     const int ex = TX + (dir == 0), ey = TY + (dir == 1), ez = TZ + (dir == 2);
     const int nf = ex * ey * ez;
     const long nx = G.nx, ny = G.ny, nz = G.nz;
     const long ndir = dir == 0 ? nx : (dir == 1 ? ny : nz);
     const long stride = dir == 0 ? 1 : (dir == 1 ? nx : nx * ny);
+    const int mode = F.face_mode;
     for (int f = lane; f < nf; f += 64) {
         const int fx = f % ex, fr = f / ex, fy = fr % ey, fz = fr / ey;
         const long i = i0 + fx, j = j0 + fy, k = k0 + fz;  // cell b (after the face)
@@ -319,28 +330,40 @@ __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, con
             qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
             qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
         }
-        const double eL = F.e[p1], eR = F.e[p2];
-        float v[2];
+        const double ea = F.e[p1], eb = F.e[p2];
+        const bool viewB = mode == 1 || (mode == 2 && ((ea <= 0.1) != (eb <= 0.1)));
+        const int nsolve = viewB ? 4 : 2;
+        float v[4];
         int nan = NAN_NONE;
         DivFast div;
 #pragma unroll 1
-        for (int side = 0; side < 2; ++side) {
+        for (int kk = 0; kk < nsolve; ++kk) {
+            const int side = kk & 1;  // 0: left state, 1: right state
+            const double guess = kk >= 2 ? eb : (mode == 0 && side == 1 ? eb : ea);
             double x[9];
 #pragma unroll
             for (int m = 0; m < 9; ++m) {
                 const int n = m < 8 ? m : NC - 1;
                 x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
             }
-            const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
-                                          side == 0 ? eL : eR, nan, div);
+            const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], guess, nan, div);
             const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
-            if (side == 0) v[0] = vs;
-            else v[1] = vs;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q == kk) v[q] = vs;
+        }
+        if (!viewB) {
+            v[2] = v[0];
+            v[3] = v[1];
         }
         if (nan != NAN_NONE) report_nan(F.err, nan, p2);
-        const float a = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
+        const float aA = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
+        const float aB = __builtin_fmaxf(__builtin_fabsf(v[2]), __builtin_fabsf(v[3]));
 #pragma unroll
-        for (int n = 0; n < NC; ++n) H[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - a * (qR[n] - qL[n]));
+        for (int n = 0; n < NC; ++n) {
+            HA[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - aA * (qR[n] - qL[n]));
+            HB[n][f] = 0.5f * (v[3] * qR[n] + v[2] * qL[n] - aB * (qR[n] - qL[n]));
+        }
     }
 }
 
@@ -362,9 +385,10 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
 #pragma clang fp contract(fast)
     using T = Tile<TX, TY, TZ>;
     constexpr int ROLES = DIM3 ? 4 : 3;
-    __shared__ float H[ROLES - 1][NC][T::NF];
-    const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blockIdx.x]);
-    const int tile = (int)blockIdx.x - __builtin_amdgcn_readfirstlane(blk_first[blockIdx.x]);
+    __shared__ float H[2][ROLES - 1][NC][T::NF];  // [cell a's view, cell b's view]
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blk]);
+    const int tile = blk - __builtin_amdgcn_readfirstlane(blk_first[blk]);
     const EvDesc &G = ev[eid];
     const int ntx = (G.nx + TX - 1) / TX, nty = (G.ny + TY - 1) / TY, ntz = (G.nz + TZ - 1) / TZ;
     if (tile < 0 || tile >= ntx * nty * ntz) return;  // block-uniform: no barrier is skipped by part of a block
@@ -375,7 +399,8 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
     const double t = F.stage ? st.t2 : st.t1;
     const double t_new = st.t2;
     if (role > 0) {
-        faces_mx<NC, TABLE, TX, TY, TZ, T::NF>(F, G, E, role - 1, i0, j0, k0, t, H[role - 1], lane);
+        faces_mx<NC, TABLE, TX, TY, TZ, T::NF>(F, G, E, role - 1, i0, j0, k0, t, H[0][role - 1], H[1][role - 1],
+                                               lane);
         __syncthreads();
         return;
     }
@@ -408,7 +433,8 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
         const int fa = fb + (d == 0 ? 1 : (d == 1 ? ex : ex * ey));
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            float r = (H[d][n][fb] - H[d][n][fa]) * rd[d];
+            // the face before the cell is seen from its b side, the face after from its a side
+            float r = (H[1][d][n][fb] - H[0][d][n][fa]) * rd[d];
             if (n < 4) r += Sd[d][n];
             acc[n] += (double)(r * dt);
         }

@@ -1,0 +1,239 @@
+// Serving loop of the batch engine (include/cpuvh_amd.h, vh_serve_run): an
+// open-loop request stream admitted by HBM footprint into one vh_batch,
+// stepped in rounds of `coarse` steps until each request meets the
+// reference's stop rule (rhic/src/HydroPlugin.cpp:266-281: three consecutive
+// freezeout checks with no cell above e_F).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "../../include/cpuvh_amd.h"
+#include "vh_host_eos.hpp"
+
+namespace {
+
+struct Rng {  // SplitMix64
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+    int uniform_int(int lo, int hi) { return lo + (int)(uniform() * (hi - lo + 1)); }
+};
+
+struct Payload {
+    std::vector<double> e, p;
+};
+
+double quantile(std::vector<double> v, double q) {  // nearest rank
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    size_t k = (size_t)std::ceil(q * v.size());
+    k = std::min(std::max<size_t>(k, 1), v.size());
+    return v[k - 1];
+}
+
+}  // namespace
+
+extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_table, vh_serve_stats *stats,
+                                  vh_serve_request *req) {
+    if (!cfg || !stats || !req) return VH_ERR_ARG;
+    const vh_serve_config &c = *cfg;
+    if (c.n_requests < 1 || !(c.rate_per_s > 0) || c.coarse < 1 || c.max_resident < 1 || c.max_steps < c.coarse ||
+        c.side2_min < 2 || c.side2_max < c.side2_min || c.side3_min < 2 || c.side3_max < c.side3_min ||
+        c.nz3_min < 2 || c.nz3_max < c.nz3_min || !(c.dx > 0) || !(c.dt > 0) || !(c.t0 > 0) ||
+        c.hbm_budget_bytes <= 0)
+        return VH_ERR_ARG;
+    const int n = c.n_requests;
+    const bool mixed = c.precision == VH_PREC_MIXED;
+    const int64_t bytes_per_cell = mixed ? (3 * 15 + 3 * 4 + 3 * 4 + 4) * 4 : (3 * 15 + 3 * 4 + 4) * 8;
+
+    // requests and their payload (generated before the clock starts)
+    Rng rng{c.seed};
+    double t_arr = 0;
+    std::vector<Payload> pay(n);
+    vh::HostEos heos;
+    if (eos_table) heos.load(eos_table);
+    size_t nmax = 1;
+    for (int k = 0; k < n; ++k) {
+        t_arr += -std::log(1.0 - rng.uniform()) / c.rate_per_s;
+        vh_serve_request &r = req[k];
+        r = vh_serve_request{};
+        r.arrival_s = t_arr;
+        const bool is3d = rng.uniform() < c.frac_3d;
+        if (is3d) {
+            r.nx = r.ny = rng.uniform_int(c.side3_min, c.side3_max);
+            r.nz = rng.uniform_int(c.nz3_min, c.nz3_max);
+        } else {
+            r.nx = r.ny = rng.uniform_int(c.side2_min, c.side2_max);
+            r.nz = 1;
+        }
+        r.glauber_seed = c.seed ^ (0x5DEECE66Dull * (uint64_t)(k + 1));
+        const size_t N = (size_t)r.nx * r.ny * r.nz;
+        nmax = std::max(nmax, N);
+        vh_glauber_params gp{197, 6.38, 0.535, 4.2, -1.0, 10.0, 0.5, 4.0, 1.96, c.e0, c.e_floor, r.glauber_seed};
+        pay[k].e.resize(N);
+        vh_status s = vh_ic_mc_glauber(&gp, r.nx, r.ny, r.nz, c.dx, c.dx, c.dx, pay[k].e.data(), nullptr);
+        if (s != VH_OK) return s;
+        pay[k].p.resize(N);
+        for (size_t m = 0; m < N; ++m) pay[k].p[m] = heos.pressure(pay[k].e[m]);
+    }
+    std::vector<double> ones(nmax, 1.0), zeros(10 * nmax, 0.0);
+
+    vh_batch_config bc{};
+    bc.device = c.device;
+    bc.capacity_cells = c.hbm_budget_bytes / bytes_per_cell;
+    bc.max_events = c.max_resident;
+    bc.max_steps = c.max_steps;
+    bc.bulk = c.bulk;
+    bc.eos_kind = eos_table ? VH_EOS_TABLE : VH_EOS_CONFORMAL;
+    bc.eos_table = eos_table;
+    bc.precision = c.precision;
+    vh_batch *b = nullptr;
+    vh_status st = vh_batch_create(&bc, &b);
+    if (st != VH_OK) return st;
+    const double eF = vh_batch_eos_energy_density(b, c.T_F_GeV / 0.197326938);
+
+    std::vector<int> owner(c.max_resident, -1), strikes(c.max_resident, 0);
+    std::vector<int64_t> counts(c.max_resident);
+    std::vector<double> ebuf(nmax);
+    std::deque<int> queue;
+    int next = 0, finished = 0, resident = 0;
+    int64_t resident_cells = 0;
+    double occ_int = 0, occ_peak = 0, zone_updates = 0;
+    const auto clock0 = std::chrono::steady_clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - clock0).count(); };
+    double t_prev = 0;
+    auto finish = [&](int id, int status) {
+        const int k = owner[id];
+        vh_serve_request &r = req[k];
+        int active = 0, steps = 0;
+        int64_t cells = 0;
+        vh_batch_event_info(b, id, &active, &steps, &cells);
+        r.steps = steps;
+        r.status = status;
+        if (status == VH_OK && vh_batch_download(b, id, 15, ebuf.data()) == VH_OK) {
+            double sum = 0;
+            for (int64_t m = 0; m < cells; ++m) sum += ebuf[m];
+            r.e_sum = sum;
+        }
+        vh_batch_remove(b, id);
+        r.done_s = now_s();
+        owner[id] = -1;
+        --resident;
+        resident_cells -= cells;
+        ++finished;
+    };
+    while (finished < n && st == VH_OK) {
+        const double now = now_s();
+        while (next < n && req[next].arrival_s <= now) queue.push_back(next++);
+        // admission: first fit in arrival order; a request that cannot fit an
+        // empty batch is refused outright
+        for (auto it = queue.begin(); it != queue.end();) {
+            const int k = *it;
+            vh_serve_request &r = req[k];
+            const int64_t N = (int64_t)r.nx * r.ny * r.nz;
+            if (N > bc.capacity_cells) {
+                r.status = VH_ERR_CAPACITY;
+                r.admit_s = r.done_s = now;
+                ++finished;
+                it = queue.erase(it);
+                continue;
+            }
+            vh_event_params ep{r.nx, r.ny, r.nz, c.dx, c.dx, c.dx, c.dt, c.t0, c.etabar, c.T_reg_GeV,
+                               c.etas_Tk_GeV, c.etas_slope_lo, c.etas_slope_hi};
+            int id = -1;
+            vh_status s = vh_batch_add(b, &ep, pay[k].e.data(), pay[k].p.data(), ones.data(), zeros.data(),
+                                       zeros.data(), zeros.data(), zeros.data(), c.bulk ? zeros.data() : nullptr, &id);
+            if (s == VH_ERR_CAPACITY) {
+                ++it;
+                continue;
+            }
+            if (s != VH_OK) {
+                st = s;
+                break;
+            }
+            owner[id] = k;
+            strikes[id] = 0;
+            r.admit_s = now_s();
+            ++resident;
+            resident_cells += N;
+            it = queue.erase(it);
+        }
+        if (st != VH_OK) break;
+        if (resident == 0) {
+            if (next < n) {
+                const double wait = req[next].arrival_s - now_s();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            }
+            continue;
+        }
+        st = vh_batch_steps(b, c.coarse);
+        if (st != VH_OK) break;
+        zone_updates += (double)resident_cells * c.coarse;
+        // the check; a request whose state hit a NaN leaves as failed
+        for (;;) {
+            st = vh_batch_counts_above(b, eF, counts.data());
+            if (st != VH_ERR_NAN) break;
+            int site = 0, bad = -1;
+            int64_t cell = 0;
+            vh_batch_nan_info(b, &site, &bad, &cell);
+            vh_batch_clear_error(b);
+            if (bad < 0 || owner[bad] < 0) {
+                st = VH_ERR_NAN;
+                break;
+            }
+            finish(bad, VH_ERR_NAN);
+        }
+        if (st != VH_OK) break;
+        for (int id = 0; id < c.max_resident; ++id) {
+            if (owner[id] < 0) continue;
+            if (c.collect_surface) {
+                int64_t nflag = 0;
+                vh_status fs = vh_batch_fo_flag(b, id, eF, nullptr, nullptr, 0, &nflag);
+                if (fs != VH_OK && fs != VH_ERR_CAPACITY) {
+                    st = fs;
+                    break;
+                }
+                req[owner[id]].fo_cubes += nflag;
+            }
+            strikes[id] = counts[id] == 0 ? strikes[id] + 1 : 0;
+            int active = 0, steps = 0;
+            vh_batch_event_info(b, id, &active, &steps, nullptr);
+            if (strikes[id] >= 3 || steps + c.coarse > c.max_steps) finish(id, VH_OK);
+        }
+        const double t = now_s();
+        const double occ = (double)(resident_cells * bytes_per_cell) / (double)c.hbm_budget_bytes;
+        occ_int += occ * (t - t_prev);
+        occ_peak = std::max(occ_peak, occ);
+        t_prev = t;
+    }
+    const double wall = now_s();
+    vh_batch_destroy(b);
+    if (st != VH_OK) return st;
+    std::vector<double> lat;
+    int failed = 0;
+    for (int k = 0; k < n; ++k) {
+        if (req[k].status == VH_OK) lat.push_back(req[k].done_s - req[k].arrival_s);
+        else ++failed;
+    }
+    stats->completed = (int)lat.size();
+    stats->failed = failed;
+    stats->wall_s = wall;
+    stats->events_per_s = lat.size() / wall;
+    stats->zone_upd_per_s = zone_updates / wall;
+    stats->p50_s = quantile(lat, 0.50);
+    stats->p99_s = quantile(lat, 0.99);
+    double m = 0;
+    for (double x : lat) m += x;
+    stats->mean_s = lat.empty() ? 0.0 : m / lat.size();
+    stats->occupancy_mean = wall > 0 ? occ_int / wall : 0.0;
+    stats->occupancy_peak = occ_peak;
+    return VH_OK;
+}

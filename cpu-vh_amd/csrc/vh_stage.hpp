@@ -61,6 +61,18 @@ struct EvDesc {
     int pad;
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  Renumbering so that XCD k works through the
+// k-th contiguous eighth of the blocks keeps a block's stencil neighbours
+// (the blocks just before and after it) in the same L2.  A bijection on
+// [0, nblocks).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
+    const int k = b % NUM_XCD, j = b / NUM_XCD;
+    return k < r ? k * (q + 1) + j : r * (q + 1) + (k - r) * q + j;
+}
+
 __device__ __forceinline__ void report_nan(int *err, int site, long s) {
     if (atomicCAS(err, 0, site) == 0) {
         err[1] = (int)(s & 0xffffffff);
@@ -571,7 +583,7 @@ __device__ __forceinline__ void stage_body(const StageParams &P, const EosDev &E
 template <int NC, bool TABLE, bool DIM3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel(const StageParams P,
                                                                                              const EosDev E) {
-    stage_body<NC, TABLE, DIM3>(P, E, P, blockIdx.x);
+    stage_body<NC, TABLE, DIM3>(P, E, P, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Many events in one launch: block b belongs to event blk_ev[b] and is that
@@ -581,10 +593,11 @@ template <int NC, bool TABLE, bool DIM3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void stage_kernel_batch(
     const StageParams P, const EosDev E, const EvDesc *__restrict__ ev, const int *__restrict__ blk_ev,
     const int *__restrict__ blk_first) {
-    const int e = __builtin_amdgcn_readfirstlane(blk_ev[blockIdx.x]);
-    const int b0 = __builtin_amdgcn_readfirstlane(blk_first[blockIdx.x]);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int e = __builtin_amdgcn_readfirstlane(blk_ev[b]);
+    const int b0 = __builtin_amdgcn_readfirstlane(blk_first[b]);
     const EvDesc &G = ev[e];
-    stage_body<NC, TABLE, DIM3>(P, E, G, (long)blockIdx.x - b0);
+    stage_body<NC, TABLE, DIM3>(P, E, G, (long)b - b0);
 }
 
 }  // namespace vh

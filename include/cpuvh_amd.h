@@ -243,9 +243,70 @@ vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest
  * state is not advanced).  FP64 batches: no variants, *tile = -1. */
 vh_status vh_batch_autotune(vh_batch *b, int reps, int *tile2d, int *tile3d);
 vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d);
+/* Mixed form: how face fluxes are shared.  The reference evaluates each face
+ * twice, once per neighbouring cell, each time starting both root solves
+ * from that cell's e (EnergyMomentumTensor.cpp:188-191 switches formula at
+ * e <= 0.1; otherwise e is the Newton start).  0: one evaluation per face,
+ * each side started from its own cell's e; 1: both evaluations, exactly as
+ * the reference; 2 (default): one evaluation, from the left cell's e, plus
+ * the right cell's evaluation where the two cells sit on different sides of
+ * e = 0.1. */
+vh_status vh_batch_set_face_mode(vh_batch *b, int mode);
 /* Duration (ms) of the last vh_batch_steps graph, from HIP events; 0 if none. */
 double vh_batch_last_ms(const vh_batch *b);
 double vh_batch_eos_energy_density(const vh_batch *b, double T);
+
+/* ---- serving -------------------------------------------------------------
+ * No counterpart in the reference.  An open-loop request loop on one device:
+ * requests (seeded MC-Glauber events, a share of them 3+1D) arrive as a
+ * Poisson process in wall-clock time; each is admitted into one vh_batch as
+ * soon as its state fits the HBM budget (first fit, later requests may pass
+ * one that does not fit yet), stays resident while every resident request
+ * advances by `coarse` steps per round, and leaves when no cell is above
+ * e(T_F) at three consecutive checks (HydroPlugin.cpp:266-281).  Initial
+ * states are generated before the clock starts (the requests' payload). */
+typedef struct {
+    double rate_per_s;         /* mean arrival rate */
+    int n_requests;
+    uint64_t seed;
+    double frac_3d;            /* share of 3+1D requests */
+    int side2_min, side2_max;  /* 2+1D: nx = ny uniform in [min, max], nz = 1 */
+    int side3_min, side3_max;  /* 3+1D: nx = ny uniform in [min, max] */
+    int nz3_min, nz3_max;      /* 3+1D: nz uniform in [min, max] */
+    double dx, dt, t0;         /* dy = deta = dx */
+    double etabar, etas_Tk_GeV, etas_slope_lo, etas_slope_hi, T_reg_GeV;
+    double T_F_GeV;
+    int coarse;                /* steps per round (the freezeout check interval) */
+    double e0, e_floor;        /* MC-Glauber normalisation and floor (Au+Au, b uniform in [0, 10] fm) */
+    int64_t hbm_budget_bytes;  /* device bytes for resident state */
+    int precision;             /* vh_precision */
+    int bulk;
+    int max_steps;             /* per request */
+    int max_resident;          /* event slots */
+    int collect_surface;       /* 1: flag freezeout cubes of every resident request at every check */
+    int device;
+} vh_serve_config;
+
+typedef struct {
+    double arrival_s, admit_s, done_s;  /* seconds from the start of the run */
+    int nx, ny, nz;
+    int steps;                          /* steps taken */
+    int status;                         /* VH_OK, VH_ERR_NAN (blew up), VH_ERR_CAPACITY (never fits) */
+    uint64_t glauber_seed;
+    int64_t fo_cubes;                   /* freezeout cubes flagged, summed over checks */
+    double e_sum;                       /* sum of e over the final state (identity checks) */
+} vh_serve_request;
+
+typedef struct {
+    int completed, failed;
+    double wall_s, events_per_s, zone_upd_per_s;
+    double p50_s, p99_s, mean_s;        /* latency: done - arrival, completed requests */
+    double occupancy_mean, occupancy_peak;  /* resident state bytes / budget */
+} vh_serve_stats;
+
+/* eos_table: NULL for the conformal EoS; requests[] gets n_requests entries. */
+vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_table, vh_serve_stats *stats,
+                       vh_serve_request *requests);
 
 const char *vh_status_string(vh_status s);
 const char *vh_build_info(void);

@@ -48,7 +48,7 @@ def _ic(ev, lattice, bulk):
 
 def _batch(evs, ics, lattice, bulk, precision, nsteps, tiles=None):
     cap = sum(e[0] * e[1] * e[2] for e in evs)
-    b = Batch(cap, max_events=len(evs), max_steps=max(nsteps, 1), bulk=bulk,
+    b = Batch(cap, max_events=len(evs), max_steps=max(nsteps, 16), bulk=bulk,
               eos_table=V.eos_table() if lattice else None, precision=precision)
     if tiles is not None:
         b.set_tiles(*tiles)
