@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace vh {
 
 constexpr double HBARC = 0.197326938;   // rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:886
@@ -71,6 +73,27 @@ struct DivFast {
 };
 struct DivExact {
     __device__ __forceinline__ double operator()(double a, const Recip &R) { return a / R.b; }
+};
+
+// Quotient policies of the root solve and the primitive recovery.  QuotExact
+// is the division operator (the FP64 path).  QuotFast (mixed form only) is a
+// hardware reciprocal refined by two Newton steps and one residual
+// correction: within an ulp of the quotient, not correctly rounded.  With
+// DivRecip the table lookups multiply by the precomputed reciprocal spacing.
+struct QuotExact {
+    static __device__ __forceinline__ double q(double a, double b) { return a / b; }
+};
+struct QuotFast {
+    static __device__ __forceinline__ double q(double a, double b) {
+        double r = __builtin_amdgcn_rcp(b);
+        r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+        r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+        const double q0 = a * r;
+        return __builtin_fma(__builtin_fma(-b, q0, a), r, q0);
+    }
+};
+struct DivRecip {
+    __device__ __forceinline__ double operator()(double a, const Recip &R) { return a * R.r; }
 };
 
 struct EosDev {
@@ -180,21 +203,23 @@ struct RootResult {
     double e;
     int nan;
 };
+template <class QD = QuotExact, class DV = DivExact>
 __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
-    DivExact dx;
+    DV dx;
     double e0 = ePrev;
     for (int j = 0; j < MAX_ITERS_NR; ++j) {
         double p = eos_pressure<true>(E, e0, dx);
         double cs2 = eos_cs2<true>(E, e0, dx);
-        double cst2 = p / e0;
+        double cst2 = QD::q(p, e0);
         double A = M0 * (1.0 - cst2) + Pi;
         double B = M0 * (M0 + Pi) - M;
         double H = (double)f32_sqrt(f32_fabs((float)(A * A + 4 * cst2 * B)));
-        double D = (A - H) / (2 * cst2);
+        double D = QD::q(A - H, 2 * cst2);
         double f = e0 + D;
-        double fp = 1.0 - ((cs2 - cst2) * (B + D * H - ((cs2 - cst2) * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
-        if (__builtin_isnan(f / fp)) return RootResult{ePrev, NAN_ROOT};
-        double e = e0 - f / fp;
+        double fp = 1.0 - QD::q((cs2 - cst2) * (B + D * H - QD::q((cs2 - cst2) * cst2 * D * M0, e0)), cst2 * e0 * H);
+        const double step = QD::q(f, fp);
+        if (__builtin_isnan(step)) return RootResult{ePrev, NAN_ROOT};
+        double e = e0 - step;
         if ((double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e)) return RootResult{e, NAN_NONE};
         e0 = e;
     }
@@ -203,7 +228,7 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
     for (int j = 0; j < MAX_ITERS_BI; ++j) {
         double p_lo = eos_pressure<true>(E, e_lo, dx), p_mid = eos_pressure<true>(E, e_mid, dx),
                p_hi = eos_pressure<true>(E, e_hi, dx);
-        double cst2_lo = p_lo / e_lo, cst2_mid = p_mid / e_mid, cst2_hi = p_hi / e_hi;
+        double cst2_lo = QD::q(p_lo, e_lo), cst2_mid = QD::q(p_mid, e_mid), cst2_hi = QD::q(p_hi, e_hi);
         double A_lo = M0 * (1.0 - cst2_lo) + Pi;
         double A_mid = M0 * (1.0 - cst2_mid) + Pi;
         double A_hi = M0 * (1.0 - cst2_hi) + Pi;
@@ -211,9 +236,9 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
         double H_lo = (double)f32_sqrt(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
         double H_mid = (double)f32_sqrt(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
         double H_hi = (double)f32_sqrt(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
-        double D_lo = (A_lo - H_lo) / (2 * cst2_lo);
-        double D_mid = (A_mid - H_mid) / (2 * cst2_mid);
-        double D_hi = (A_hi - H_hi) / (2 * cst2_hi);
+        double D_lo = QD::q(A_lo - H_lo, 2 * cst2_lo);
+        double D_mid = QD::q(A_mid - H_mid, 2 * cst2_mid);
+        double D_hi = QD::q(A_hi - H_hi, 2 * cst2_hi);
         double f_lo = e_lo + D_lo, f_mid = e_mid + D_mid, f_hi = e_hi + D_hi;
         if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return RootResult{e_mid, NAN_NONE};
         if (f_lo * f_mid < 0.0) {
@@ -228,14 +253,14 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
 }
 
 // energyDensityFromConservedVariables, rhic/src/EnergyMomentumTensor.cpp:22-115
-template <bool TABLE>
+template <bool TABLE, class QD = QuotExact, class DV = DivExact>
 __device__ __forceinline__ double energy_from_conserved(const EosDev &E, double ePrev, double M0, double M, double Pi,
                                                         int &nan) {
     if constexpr (!TABLE) {
         (void)E; (void)ePrev; (void)Pi; (void)nan;
         return (double)f32_fabs((float)((double)f32_sqrt(f32_fabs((float)(4 * M0 * M0 - 3 * M))) - M0));
     } else {
-        const RootResult r = table_root(E, ePrev, M0, M, Pi);
+        const RootResult r = table_root<QD, DV>(E, ePrev, M0, M, Pi);
         if (r.nan) nan = r.nan;
         return r.e;
     }
@@ -247,7 +272,7 @@ struct Prim {
 
 // getInferredVariables, rhic/src/EnergyMomentumTensor.cpp:117-233.
 // ttt..pitn are q[0..7]; Pi = q[14] when bulk is evolved.
-template <bool TABLE, class DV>
+template <bool TABLE, class DV, class QD = QuotExact>
 __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
                                         double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
                                         int &nan, DV &div) {
@@ -257,8 +282,10 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
     double M3 = ttn - pitn;
     double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
     Prim r;
-    if (ePrev <= 0.1) r.e = M0 - M / M0;
-    else r.e = energy_from_conserved<TABLE>(E, ePrev, M0, M, Pi, nan);
+    if (ePrev <= 0.1) r.e = M0 - QD::q(M, M0);
+    else  // the FP64 path keeps the division operator inside the root solve's table lookups
+        r.e = energy_from_conserved<TABLE, QD, std::conditional_t<std::is_same_v<QD, QuotFast>, DivRecip, DivExact>>(
+            E, ePrev, M0, M, Pi, nan);
     if (__builtin_isnan(r.e)) nan = NAN_ENERGY;
     r.p = eos_pressure<TABLE>(E, r.e, div);
     if (r.e < 1.e-7) {
@@ -266,9 +293,9 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
         r.p = 1.e-7;
     }
     double P = r.p + Pi;
-    double Ei = 1.0 / (r.e + P);
+    double Ei = QD::q(1.0, r.e + P);
     r.ut = (double)f32_sqrt(f32_fabs((float)((M0 + P) * Ei)));
-    double E2 = Ei / r.ut;
+    double E2 = QD::q(Ei, r.ut);
     r.ux = M1 * E2;
     r.uy = M2 * E2;
     r.un = M3 * E2;

@@ -7,9 +7,9 @@
 //     is the FP64 accumulation to ~2^-48 (compensated storage);
 //   * limited slopes, half-site values, face fluxes and the source terms run
 //     in FP32 with fused multiply-adds and reciprocal multiplies;
-//   * the conserved-to-primitive root solve, the EoS table lookups, the
+//   * the conserved-to-primitive root solve, the EoS table lookups and the
 //     accumulation acc = q + dt*S + r_x + r_y + r_eta (and the stage-2 convex
-//     combination) and the regulator run in FP64;
+//     combination) run in FP64; the regulator's factors are FP32;
 //   * each face flux is evaluated once and shared by the two cells it
 //     separates (the reference evaluates it from both sides, with each
 //     cell's own e as the root solve's starting value; here each side of a
@@ -62,7 +62,7 @@ __device__ __forceinline__ long clampi(long c, long n) { return c < 0 ? 0 : (c >
 template <bool TABLE>
 __device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
     if constexpr (TABLE) {
-        DivExact dx;
+        DivRecip dx;
         return (float)eos_temperature<true>(E, (double)e, dx);
     } else {
         return __builtin_sqrtf(__builtin_sqrtf(e * (float)(1.0 / EOS_FACTOR)));
@@ -153,7 +153,7 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float d_etabar = (float)etabar_of_T(G, (double)T);
     const float taupiInv = T * r5 / d_etabar;
     const float beta_pi = (e + p) * r5;
-    DivExact dx64;
+    DivRecip dx64;
     const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
     const float a = 1.0f / 3.0f - cs2;
     const float a2 = a * a;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, con
         const int nsolve = viewB ? 4 : 2;
         float v[4];
         int nan = NAN_NONE;
-        DivFast div;
+        DivRecip div;
 #pragma unroll 1
         for (int kk = 0; kk < nsolve; ++kk) {
             const int side = kk & 1;  // 0: left state, 1: right state
@@ -346,7 +346,7 @@ __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, con
                 const int n = m < 8 ? m : NC - 1;
                 x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
             }
-            const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], guess, nan, div);
+            const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], guess, nan, div);
             const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -364,6 +364,48 @@ __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, con
             HA[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - aA * (qR[n] - qL[n]));
             HB[n][f] = 0.5f * (v[3] * qR[n] + v[2] * qL[n] - aB * (qR[n] - qL[n]));
         }
+    }
+}
+
+// REG_SCHEME_1 (rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:689-860) in
+// FP32 for the mixed form: the same factors (rho_max = 1, xi0 = 0.1, applied
+// only at e <= e_reg), computed from the FP64 accumulators rounded to FP32,
+// scale pi (and Pi) in acc.
+template <int NC>
+__device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, double (&acc)[NC], const Prim &w,
+                                            int &nan) {
+#pragma clang fp contract(fast)
+    if (w.e > G.e_reg) return;
+    const float t = (float)t64, t2 = t * t;
+    const float pitt = (float)acc[4], pitx = (float)acc[5], pity = (float)acc[6], pitn = (float)acc[7],
+                pixx = (float)acc[8], pixy = (float)acc[9], pixn = (float)acc[10], piyy = (float)acc[11],
+                piyn = (float)acc[12], pinn = (float)acc[13];
+    const float ut = (float)w.ut, ux = (float)w.ux, uy = (float)w.uy, un = (float)w.un;
+    const float e = (float)w.e, p = (float)w.p;
+    const float pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy +
+                       piyy * piyy - 2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 +
+                       pinn * pinn * t2 * t2;
+    const float spipi = __builtin_sqrtf(__builtin_fabsf(pipi));
+    const float pimumu = pitt - pixx - piyy - pinn * t2;
+    const float piu0 = -(pitn * t2 * un) + pitt * ut - pitx * ux - pity * uy;
+    const float piu1 = -(pixn * t2 * un) + pitx * ut - pixx * ux - pixy * uy;
+    const float piu2 = -(piyn * t2 * un) + pity * ut - pixy * ux - piyy * uy;
+    const float piu3 = -(pinn * t2 * un) + pitn * ut - pixn * ux - piyn * uy;
+    const float a1 = spipi / __builtin_sqrtf(e * e + 3 * p * p);
+    const float inv = 1.0f / (0.1f * spipi);  // 0 / 0 gives NaN, which fmax passes over, as in FP64
+    const float a2 = pimumu * inv, a3 = piu0 * inv, a4 = piu1 * inv, a5 = piu2 * inv, a6 = piu3 * inv;
+    const float rho = __builtin_fmaxf(__builtin_fmaxf(a1, a2), __builtin_fmaxf(__builtin_fmaxf(a3, a4),
+                                                                                 __builtin_fmaxf(a5, a6)));
+    float facShear = __builtin_fabsf(rho) < 1.e-7f ? 1.0f : tanhf(rho) / rho;
+#pragma unroll
+    for (int n = 4; n < 14; ++n) acc[n] *= (double)facShear;
+    if (__builtin_isnan(facShear)) nan = NAN_REG_SHEAR;
+    if constexpr (NC == 15) {
+        const float Pi = (float)acc[NC - 1];
+        const float rb = __builtin_fabsf(Pi) / p;
+        const float facBulk = __builtin_fabsf(rb) < 1.e-7f ? 1.0f : tanhf(rb) / rb;
+        acc[NC - 1] *= (double)facBulk;
+        if (__builtin_isnan(facBulk)) nan = NAN_REG_BULK;
     }
 }
 
@@ -447,10 +489,10 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
         }
     }
     int nan = NAN_NONE;
-    DivFast div;
-    const Prim w = recover<TABLE>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
+    DivRecip div;
+    const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
                                   NC == 15 ? acc[NC - 1] : 0.0, (double)F.e[s], nan, div);
-    regulate<NC>(G, t_new, acc, w, nan);
+    regulate_mx<NC>(G, t_new, acc, w, nan);
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
         const float hi = (float)acc[n];

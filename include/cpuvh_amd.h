@@ -246,9 +246,9 @@ vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d);
 /* Mixed form: how face fluxes are shared.  The reference evaluates each face
  * twice, once per neighbouring cell, each time starting both root solves
  * from that cell's e (EnergyMomentumTensor.cpp:188-191 switches formula at
- * e <= 0.1; otherwise e is the Newton start).  0: one evaluation per face,
- * each side started from its own cell's e; 1: both evaluations, exactly as
- * the reference; 2 (default): one evaluation, from the left cell's e, plus
+ * e <= 0.1; otherwise e is the Newton start).  0 (default): one evaluation
+ * per face, each side started from its own cell's e; 1: both evaluations,
+ * exactly as the reference; 2: one evaluation, from the left cell's e, plus
  * the right cell's evaluation where the two cells sit on different sides of
  * e = 0.1. */
 vh_status vh_batch_set_face_mode(vh_batch *b, int mode);

@@ -105,3 +105,43 @@ def test_count_and_freezeout_flags_match_oracle():
         t = t0 + (n - 1) * dt
         g.step(t)
         o.step(t)
+
+
+def test_gubser_configuration():
+    """The Gubser configuration (201 x 201, dx = 0.05 fm, dtau = 0.0125 fm/c,
+    conformal EoS, eta/s = 0.2, regulation below 100 MeV) on the GPU for 40
+    steps: against the semi-analytic solution with the oracle's tolerances
+    (tests/test_gubser.py), and against the oracle in the unregulated core.
+    (Over hundreds of regulated steps the regulator's tanh, the one libm call
+    without a bit-exact device counterpart, lets the dilute corner regions
+    drift apart.)"""
+    import gubser as Gb
+    n, dx, dt, nsteps, etas = 201, 0.05, 0.0125, 40, 0.2
+    print(f"case gubser {n}x{n} steps={nsteps}", flush=True)
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    Y, X = np.meshgrid(x, x, indexing="ij")
+    sol = Gb.solve(etas)
+    e, u, pi = Gb.fields(sol, 1.0, X, Y)
+    ic = {"e": e.ravel(), "p": (e / 3).ravel(), "ut": u[0].ravel(), "ux": u[1].ravel(), "uy": u[2].ravel(),
+          "un": u[3].ravel(), "pi": np.concatenate([p.ravel() for p in pi])}
+    g = Hydro(n, n, 1, dx, dx, dx, dt, 1.0, etas, T_reg_GeV=0.1)
+    o = V.Oracle(n, n, 1, dx, dx, dx, dt, 1.0, etas, T_reg_GeV=0.1)
+    g.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"])
+    o.set_ic(ic)
+    g.init_conserved(1.0)
+    o.init_conserved(1.0)
+    g.steps(1, nsteps)
+    for s in range(1, nsteps + 1):
+        assert o.step(1.0 + (s - 1) * dt) == 0
+    g.sync()
+    tau = 1.0 + nsteps * dt
+    e2, u2, pi2 = Gb.fields(sol, tau, X, Y)
+    core = (X * X + Y * Y < 1.0).ravel()
+    ge, gux = g.get("e"), g.get("ux")
+    assert np.max(np.abs(ge - e2.ravel())[core] / e2.ravel()[core]) < 2.5e-2
+    assert np.max(np.abs(gux - u2[1].ravel())[core]) < 1e-2
+    c = (n // 2) * n + n // 2
+    assert abs(g.get("pinn")[c] / pi2[9].ravel()[c] - 1) < 4e-2
+    for f in ("e", "ux", "uy", "pitt", "pixx", "pinn"):
+        a, b = o.get(f)[core], g.get(f)[core]
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a)), f
