@@ -85,13 +85,15 @@ def cpu_baseline(w, ic, threads, min_seconds=12.0, max_steps=40):
                       f"constant eta/s={w['etabar']} (the reference has no eta/s(T))"}
 
 
-def load_traffic(name):
+def load_traffic(name, key="hbm_bytes_per_launch"):
+    """HBM bytes measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by the
+    copy calibration), committed under profiles/."""
     path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+    return d.get(key)
 
 
 # ------------------------------------------------------- batching workload --
@@ -260,6 +262,91 @@ def precision_check(events, table, nsteps):
     return {"steps": nsteps, "events": len(events), "fields": res}
 
 
+PRECISION_3D = dict(nx=201, ny=201, nz=101, dx=0.08, dt=0.02, t0=0.6, steps=600, etabar=0.08, etas_Tk_GeV=0.154,
+                    etas_slope_lo=0.5, etas_slope_hi=1.0, T_reg_GeV=0.1, seed=777, e0=10.0)
+
+
+def run_precision_3d(table, steps, checkpoints=(100, 300)):
+    """The precision form's configuration: one 3+1D Glauber event on
+    201 x 201 x 101 cells (dx = deta = 0.08) through the FP64 and the mixed
+    form; per-field max and RMS relative errors and the freezeout cube sets
+    at each checkpoint and at the end."""
+    import cpuvh_amd as C
+    w = PRECISION_3D
+    nx, ny, nz, dx = w["nx"], w["ny"], w["nz"], w["dx"]
+    e, info = C.mc_glauber(nx, ny, nz, dx, dx, dx, seed=w["seed"], e0=w["e0"])
+    N = e.size
+    ic = {"e": e, "p": table_pressure(table, e), "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N),
+          "un": np.zeros(N), "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
+    fields = ("e", "ut", "ux", "uy", "un", "pitt", "pixx", "pixy", "piyy", "pinn", "Pi")
+    batches, wall = {}, {}
+    for prec in (C.PREC_FP64, C.PREC_MIXED):
+        b = C.Batch(N, max_events=1, max_steps=steps, bulk=True, eos_table=table, precision=prec)
+        eid = b.add(nx, ny, nz, dx, dx, dx, w["dt"], w["t0"], w["etabar"], ic, T_reg_GeV=w["T_reg_GeV"],
+                    etas_Tk_GeV=w["etas_Tk_GeV"], etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"])
+        batches[prec] = (b, eid)
+        wall[prec] = 0.0
+    eF = batches[C.PREC_FP64][0].eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+    report, done = [], 0
+    for stop in [c for c in checkpoints if c < steps] + [steps]:
+        runs, cubes = {}, {}
+        for prec, (b, eid) in batches.items():
+            t = time.perf_counter()
+            b.steps(stop - done)
+            b.sync()
+            wall[prec] += time.perf_counter() - t
+            runs[prec] = {f: b.get(eid, f) for f in fields}
+            cubes[prec] = set(b.fo_flag_ids(eid, eF).tolist())  # cubes cut by e_F (previous slice: this state)
+        done = stop
+        hot = runs[C.PREC_FP64]["e"] > eF
+        out = {}
+        for f in fields:
+            a, c = runs[C.PREC_FP64][f], runs[C.PREC_MIXED][f]
+            scale = float(np.max(np.abs(a))) or 1.0
+            norm = float(np.sqrt(np.mean(a * a))) or 1.0
+            r = {"max_abs_over_max": float(np.max(np.abs(a - c))) / scale,
+                 "rms_rel": float(np.sqrt(np.mean((a - c) ** 2))) / norm}
+            if f in ("e", "ut"):
+                rel = np.abs(a - c) / np.abs(a)
+                r["max_pointwise_rel"] = float(np.max(rel))
+                r["max_pointwise_rel_above_eF"] = float(np.max(rel[hot])) if hot.any() else None
+            out[f] = r
+        sym = cubes[C.PREC_FP64] ^ cubes[C.PREC_MIXED]
+        report.append({"step": stop, "cells_above_eF": int(hot.sum()), "fields": out,
+                       "fo_cubes_fp64": len(cubes[C.PREC_FP64]), "fo_cubes_mixed": len(cubes[C.PREC_MIXED]),
+                       "fo_cubes_differing": len(sym)})
+    for b, _ in batches.values():
+        b.close()
+    return {"workload": "3+1D MC-Glauber event 201x201x101, dx=deta=0.08, tau0=0.6, dtau=0.02, table EoS, bulk, "
+                        "eta/s(T) min 0.08", "steps": steps, "cells": N, "glauber": info,
+            "fp64_s": wall[C.PREC_FP64], "mixed_s": wall[C.PREC_MIXED], "checkpoints": report}
+
+
+SERVE = dict(frac_3d=0.2, side2_min=101, side2_max=201, side3_min=101, side3_max=161, nz3_min=41, nz3_max=81,
+             dx=0.1, dt=0.02, t0=0.6, etabar=0.08, etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0,
+             T_reg_GeV=0.1, T_F_GeV=0.150, coarse=5, e0=10.0, e_floor=0.01, hbm_budget_bytes=8 << 30,
+             bulk=1, max_steps=3000, max_resident=256, collect_surface=1)
+
+
+def run_serving(table, rates, n_requests):
+    """Open-loop serving at each arrival rate (vh_serve_run)."""
+    import cpuvh_amd as C
+    out = []
+    for k, rate in enumerate(rates):
+        stats, reqs = C.serve(table, rate_per_s=float(rate), n_requests=n_requests, seed=1000 + k,
+                              precision=C.PREC_MIXED, **SERVE)
+        done = [r for r in reqs if r["status"] == C.VH_OK]
+        out.append({"rate_per_s": rate, "requests": n_requests, "completed": stats["completed"],
+                    "failed": stats["failed"], "events_per_s": stats["events_per_s"],
+                    "zone_upd_per_s": stats["zone_upd_per_s"], "p50_s": stats["p50_s"], "p99_s": stats["p99_s"],
+                    "mean_s": stats["mean_s"], "hbm_occupancy_mean": stats["occupancy_mean"],
+                    "hbm_occupancy_peak": stats["occupancy_peak"],
+                    "mean_wait_for_admission_s": float(np.mean([r["admit_s"] - r["arrival_s"] for r in done]))
+                    if done else None,
+                    "share_3d": float(np.mean([r["nz"] > 1 for r in reqs]))})
+    return out
+
+
 def run_dropin_3d(w, table, steps, warmup):
     import cpuvh_amd as C
     ic, info = make_ic(w, table)
@@ -312,6 +399,9 @@ def main():
     ap.add_argument("--skip-precision", action="store_true")
     ap.add_argument("--max-batch-steps", type=int, default=None,
                     help="stop the batched runs after this many steps (profiling; events/s then not to freezeout)")
+    ap.add_argument("--serve-rates", default="4,16,64", help="arrival rates of the serving sweep (events/s)")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -352,6 +442,15 @@ def main():
     if not args.skip_precision:
         print("bench: precision check", file=sys.stderr, flush=True)
         prec = precision_check(events[:: max(1, len(events) // 4)][:4], table, 300)
+    serving = None
+    if not args.skip_serve:
+        print("bench: serving sweep", file=sys.stderr, flush=True)
+        serving = run_serving(table, [float(r) for r in args.serve_rates.split(",") if r],
+                              args.serve_requests)
+    precision3d = None
+    if not args.skip_precision:
+        print("bench: precision form 3+1D", file=sys.stderr, flush=True)
+        precision3d = run_precision_3d(table, PRECISION_3D["steps"])
     dropin = None
     if not args.skip_dropin:
         print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
@@ -362,6 +461,8 @@ def main():
     bytes_per_zone_mx = (5 * nc + 32 + 5 * 4) * 4  # FP32 fields + FP32 low parts of T^{tau mu}
     achieved = mixed["zone_updates"] * bytes_per_zone_mx / (mixed["graph_ms"] * 1e-3) / 1e9
     launches = 2 * mixed["batched_steps"]
+    per_cell = load_traffic("stage_mixed_kernel_batch_pmc.json", "hbm_bytes_per_cell_stage")
+    traffic_mx = per_cell * mixed["zone_updates"] / mixed["batched_steps"] if per_cell else None
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -393,9 +494,17 @@ def main():
         "speedup_vs_plain_fp64": None if plain is None else mixed["events_per_s"] / plain["events_per_s"],
         "batched_fp64": None if fp64b is None else {k: v for k, v in fp64b.items() if k != "steps_to_freezeout"},
         "mixed_vs_fp64_error": prec,
+        "precision_form_3d": precision3d,
+        "serving": None if serving is None else {
+            "workload": "Poisson arrivals of seeded MC-Glauber Au+Au events, 80% 2+1D (101^2-201^2), 20% 3+1D "
+                        "(101x101x41 - 161x161x81), dx=0.1, table EoS, bulk, mixed precision, admitted into one "
+                        "batch by an 8 GiB HBM budget, resident until no cell is above T=150 MeV, freezeout "
+                        "cubes flagged per request at every check", "sweep": serving},
         "fp64_dropin_3d": dropin,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("stage_mixed_kernel_batch_pmc.json"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_mx,
+                     "traffic_note": "HBM bytes per average launch: bytes per cell-stage measured with rocprofv3 "
+                                     "(profiles/stage_mixed_kernel__batch256_2d.md) x the run's mean cells per launch",
                      "kernel": "stage_mixed_kernel (one RK stage, all 2+1D events)",
                      "avg_launch_ms": mixed["graph_ms"] / launches,
                      "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},

@@ -8,7 +8,9 @@ import pytest
 # it on one thread.  Set before any OpenMP library is loaded.
 os.environ["OMP_NUM_THREADS"] = "1"
 
-sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, _HERE)
+sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "cpu-vh_amd"))
 
 
 def pytest_configure(config):
