@@ -326,6 +326,7 @@ vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out) {
     if (b->deos.init(b->heos, c.eos_kind) != 0) return fail(VH_ERR_NOMEM);
     if (c.precision != VH_PREC_FP64 && c.precision != VH_PREC_MIXED) return fail(VH_ERR_ARG);
     b->mixed = c.precision == VH_PREC_MIXED;
+    if (b->mixed && b->cap >= (1L << 30)) return fail(VH_ERR_ARG);  // 32-bit byte offsets per field
     const size_t nfield = 3 * 15 + 3 * 4 + 4;
     if (!b->mixed) {
         if (hipMalloc(&b->pool, sizeof(double) * nfield * b->cap) != hipSuccess) return fail(VH_ERR_NOMEM);

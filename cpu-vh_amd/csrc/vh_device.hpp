@@ -82,6 +82,8 @@ struct DivExact {
 // DivRecip the table lookups multiply by the precomputed reciprocal spacing.
 struct QuotExact {
     static __device__ __forceinline__ double q(double a, double b) { return a / b; }
+    // the reference's sqrtf of a double (a float island): correctly rounded
+    static __device__ __forceinline__ float sqrt_island(float x) { return (float)__builtin_sqrt((double)x); }
 };
 struct QuotFast {
     static __device__ __forceinline__ double q(double a, double b) {
@@ -91,6 +93,8 @@ struct QuotFast {
         const double q0 = a * r;
         return __builtin_fma(__builtin_fma(-b, q0, a), r, q0);
     }
+    // the hardware's single-precision square root (within an ulp)
+    static __device__ __forceinline__ float sqrt_island(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
 struct DivRecip {
     __device__ __forceinline__ double operator()(double a, const Recip &R) { return a * R.r; }
@@ -213,7 +217,7 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
         double cst2 = QD::q(p, e0);
         double A = M0 * (1.0 - cst2) + Pi;
         double B = M0 * (M0 + Pi) - M;
-        double H = (double)f32_sqrt(f32_fabs((float)(A * A + 4 * cst2 * B)));
+        double H = (double)QD::sqrt_island(f32_fabs((float)(A * A + 4 * cst2 * B)));
         double D = QD::q(A - H, 2 * cst2);
         double f = e0 + D;
         double fp = 1.0 - QD::q((cs2 - cst2) * (B + D * H - QD::q((cs2 - cst2) * cst2 * D * M0, e0)), cst2 * e0 * H);
@@ -233,9 +237,9 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
         double A_mid = M0 * (1.0 - cst2_mid) + Pi;
         double A_hi = M0 * (1.0 - cst2_hi) + Pi;
         double B = M0 * (M0 + Pi) - M;
-        double H_lo = (double)f32_sqrt(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
-        double H_mid = (double)f32_sqrt(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
-        double H_hi = (double)f32_sqrt(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
+        double H_lo = (double)QD::sqrt_island(f32_fabs((float)(A_lo * A_lo + 4 * cst2_lo * B)));
+        double H_mid = (double)QD::sqrt_island(f32_fabs((float)(A_mid * A_mid + 4 * cst2_mid * B)));
+        double H_hi = (double)QD::sqrt_island(f32_fabs((float)(A_hi * A_hi + 4 * cst2_hi * B)));
         double D_lo = QD::q(A_lo - H_lo, 2 * cst2_lo);
         double D_mid = QD::q(A_mid - H_mid, 2 * cst2_mid);
         double D_hi = QD::q(A_hi - H_hi, 2 * cst2_hi);
@@ -258,7 +262,7 @@ __device__ __forceinline__ double energy_from_conserved(const EosDev &E, double 
                                                         int &nan) {
     if constexpr (!TABLE) {
         (void)E; (void)ePrev; (void)Pi; (void)nan;
-        return (double)f32_fabs((float)((double)f32_sqrt(f32_fabs((float)(4 * M0 * M0 - 3 * M))) - M0));
+        return (double)f32_fabs((float)((double)QD::sqrt_island(f32_fabs((float)(4 * M0 * M0 - 3 * M))) - M0));
     } else {
         const RootResult r = table_root<QD, DV>(E, ePrev, M0, M, Pi);
         if (r.nan) nan = r.nan;
@@ -294,7 +298,7 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
     }
     double P = r.p + Pi;
     double Ei = QD::q(1.0, r.e + P);
-    r.ut = (double)f32_sqrt(f32_fabs((float)((M0 + P) * Ei)));
+    r.ut = (double)QD::sqrt_island(f32_fabs((float)((M0 + P) * Ei)));
     double E2 = QD::q(Ei, r.ut);
     r.ux = M1 * E2;
     r.uy = M2 * E2;

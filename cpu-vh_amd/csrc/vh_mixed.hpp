@@ -48,16 +48,30 @@ struct MxFields {
     int *err;
 };
 
-// rhic/src/FluxLimiter.cpp:14-32 in FP32 (sign(0) = +1).
-__device__ __forceinline__ float minmod_f(float x, float y) {
-    const float sx = x < 0.f ? -1.f : 1.f, sy = y < 0.f ? -1.f : 1.f;
-    return (sx + sy) * 0.5f * __builtin_fminf(__builtin_fabsf(x), __builtin_fabsf(y));
-}
+// rhic/src/FluxLimiter.cpp:14-32 in FP32: minmod(a, minmod(b, c)) of
+// a = theta (y - x), b = (z - x) / 2, c = theta (z - y) is min(|a|,|b|,|c|)
+// with the common sign when all three share one, else zero (up to the sign
+// of a zero result).  Three-operand min/max instructions make it a few
+// operations.
 __device__ __forceinline__ float deriv_f(float x, float y, float z) {
-    return minmod_f(THETA_F * (y - x), minmod_f((z - x) * 0.5f, THETA_F * (z - y)));
+    const float a = THETA_F * (y - x), b = (z - x) * 0.5f, c = THETA_F * (z - y);
+    const float lo = __builtin_fminf(a, __builtin_fminf(b, c)), hi = __builtin_fmaxf(a, __builtin_fmaxf(b, c));
+    const float m = __builtin_fminf(__builtin_fabsf(a), __builtin_fminf(__builtin_fabsf(b), __builtin_fabsf(c)));
+    return lo > 0.f ? m : (hi < 0.f ? -m : 0.f);
 }
 
-__device__ __forceinline__ long clampi(long c, long n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+__device__ __forceinline__ int clampi(int c, int n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+
+// Field access by a 32-bit cell index: a scalar base plus a 32-bit byte
+// offset, which the hardware adds itself (global loads with a scalar base),
+// instead of per-lane 64-bit address arithmetic.  The pool holds fewer than
+// 2^30 cells (checked when a mixed batch is created).
+__device__ __forceinline__ float ldf(const float *__restrict__ base, unsigned idx) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (idx << 2));
+}
+__device__ __forceinline__ void stf(float *__restrict__ base, unsigned idx, float v) {
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (idx << 2)) = v;
+}
 
 template <bool TABLE>
 __device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
@@ -73,7 +87,7 @@ __device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
 // rhic/src/SourceTerms.cpp:55-313,421-612) in FP32, and the pi-gradient
 // terms of loadSourceTermsX/Y/Z (:316-419) for each direction: Sd[d][mu].
 template <int NC, bool TABLE, bool DIM3>
-__device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, const EosDev &E, long s, int i, int j,
+__device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, const EosDev &E, unsigned s, int i, int j,
                                           int k, float t, float ecell, const float (&q)[NC], float (&S)[NC],
                                           float (&Sd)[3][4]) {
 #pragma clang fp contract(fast)
@@ -89,31 +103,31 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float *__restrict__ uyv = F.u[2];
     const float *__restrict__ unv = F.u[3];
     const float *__restrict__ pv = F.p;
-    const float p = pv[s];
-    const float ut = utv[s], ux = uxv[s], uy = uyv[s], un = unv[s];
-    const float utp = F.up[0][s], uxp = F.up[1][s], uyp = F.up[2][s], unp = F.up[3][s];
-    const long strY = G.nx, strZ = (long)G.nx * G.ny;
-    const long sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
-    const long sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
-    const long sk = k > 0 ? s - strZ : s, sf = k < G.nz - 1 ? s + strZ : s;
-    const float dxut = deriv_f(utv[sl], ut, utv[sr]) * rdx;
-    const float dxux = deriv_f(uxv[sl], ux, uxv[sr]) * rdx;
-    const float dxuy = deriv_f(uyv[sl], uy, uyv[sr]) * rdx;
-    const float dxun = deriv_f(unv[sl], un, unv[sr]) * rdx;
-    const float dyut = deriv_f(utv[sb], ut, utv[stp]) * rdy;
-    const float dyux = deriv_f(uxv[sb], ux, uxv[stp]) * rdy;
-    const float dyuy = deriv_f(uyv[sb], uy, uyv[stp]) * rdy;
-    const float dyun = deriv_f(unv[sb], un, unv[stp]) * rdy;
+    const float p = ldf(pv, s);
+    const float ut = ldf(utv, s), ux = ldf(uxv, s), uy = ldf(uyv, s), un = ldf(unv, s);
+    const float utp = ldf(F.up[0], s), uxp = ldf(F.up[1], s), uyp = ldf(F.up[2], s), unp = ldf(F.up[3], s);
+    const unsigned strY = G.nx, strZ = (unsigned)G.nx * G.ny;
+    const unsigned sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
+    const unsigned sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
+    const unsigned sk = k > 0 ? s - strZ : s, sf = k < G.nz - 1 ? s + strZ : s;
+    const float dxut = deriv_f(ldf(utv, sl), ut, ldf(utv, sr)) * rdx;
+    const float dxux = deriv_f(ldf(uxv, sl), ux, ldf(uxv, sr)) * rdx;
+    const float dxuy = deriv_f(ldf(uyv, sl), uy, ldf(uyv, sr)) * rdx;
+    const float dxun = deriv_f(ldf(unv, sl), un, ldf(unv, sr)) * rdx;
+    const float dyut = deriv_f(ldf(utv, sb), ut, ldf(utv, stp)) * rdy;
+    const float dyux = deriv_f(ldf(uxv, sb), ux, ldf(uxv, stp)) * rdy;
+    const float dyuy = deriv_f(ldf(uyv, sb), uy, ldf(uyv, stp)) * rdy;
+    const float dyun = deriv_f(ldf(unv, sb), un, ldf(unv, stp)) * rdy;
     float dnut = 0.f, dnux = 0.f, dnuy = 0.f, dnun = 0.f, dnp = 0.f;
     if constexpr (DIM3) {
-        dnut = deriv_f(utv[sk], ut, utv[sf]) * rdz;
-        dnux = deriv_f(uxv[sk], ux, uxv[sf]) * rdz;
-        dnuy = deriv_f(uyv[sk], uy, uyv[sf]) * rdz;
-        dnun = deriv_f(unv[sk], un, unv[sf]) * rdz;
-        dnp = deriv_f(pv[sk], p, pv[sf]) * rdz;
+        dnut = deriv_f(ldf(utv, sk), ut, ldf(utv, sf)) * rdz;
+        dnux = deriv_f(ldf(uxv, sk), ux, ldf(uxv, sf)) * rdz;
+        dnuy = deriv_f(ldf(uyv, sk), uy, ldf(uyv, sf)) * rdz;
+        dnun = deriv_f(ldf(unv, sk), un, ldf(unv, sf)) * rdz;
+        dnp = deriv_f(ldf(pv, sk), p, ldf(pv, sf)) * rdz;
     }
-    const float dxp = deriv_f(pv[sl], p, pv[sr]) * rdx;
-    const float dyp = deriv_f(pv[sb], p, pv[stp]) * rdy;
+    const float dxp = deriv_f(ldf(pv, sl), p, ldf(pv, sr)) * rdx;
+    const float dyp = deriv_f(ldf(pv, sb), p, ldf(pv, stp)) * rdy;
 
     const float t2pow = t * t;
     const float pP = p + Pi;
@@ -131,12 +145,12 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 
     // pi-gradient terms of each direction: d_a(pi^{tau mu}) v_a - d_a(pi^{a mu})
     {
-        const long nbm[3] = {sl, sb, sk}, nbp[3] = {sr, stp, sf};
+        const unsigned nbm[3] = {sl, sb, sk}, nbp[3] = {sr, stp, sf};
         const float fac[3] = {0.5f * rdx, 0.5f * rdy, 0.5f * rdz};
         const float vd[3] = {vx, vy, vn};
 #pragma unroll
         for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
-            auto dq = [&](int n) { return (F.q[n][nbp[d]] - F.q[n][nbm[d]]) * fac[d]; };
+            auto dq = [&](int n) { return (ldf(F.q[n], nbp[d]) - ldf(F.q[n], nbm[d])) * fac[d]; };
 #pragma unroll
             for (int mu = 0; mu < 4; ++mu) Sd[d][mu] = dq(4 + mu) * vd[d] - dq(pi_a_mu(d, mu));
             if constexpr (NC == 15) {
@@ -308,29 +322,30 @@ __device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, con
 #pragma clang fp contract(fast)
     const int ex = TX + (dir == 0), ey = TY + (dir == 1), ez = TZ + (dir == 2);
     const int nf = ex * ey * ez;
-    const long nx = G.nx, ny = G.ny, nz = G.nz;
-    const long ndir = dir == 0 ? nx : (dir == 1 ? ny : nz);
-    const long stride = dir == 0 ? 1 : (dir == 1 ? nx : nx * ny);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int ndir = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    const unsigned stride = dir == 0 ? 1u : (dir == 1 ? (unsigned)nx : (unsigned)nx * ny);
     const int mode = F.face_mode;
     for (int f = lane; f < nf; f += 64) {
         const int fx = f % ex, fr = f / ex, fy = fr % ey, fz = fr / ey;
-        const long i = i0 + fx, j = j0 + fy, k = k0 + fz;  // cell b (after the face)
-        const long cb = dir == 0 ? i : (dir == 1 ? j : k);
+        const int i = i0 + fx, j = j0 + fy, k = k0 + fz;  // cell b (after the face)
+        const int cb = dir == 0 ? i : (dir == 1 ? j : k);
         const bool need = (dir == 0 ? i - 1 < nx : i < nx) && (dir == 1 ? j - 1 < ny : j < ny) &&
                           (dir == 2 ? k - 1 < nz : k < nz);
         if (!need) continue;
-        const long s0 = G.off + (dir == 0 ? 0 : i) + nx * ((dir == 1 ? 0 : j) + ny * (dir == 2 ? 0 : k));
-        const long p0 = s0 + clampi(cb - 2, ndir) * stride, p1 = s0 + clampi(cb - 1, ndir) * stride;
-        const long p2 = s0 + clampi(cb, ndir) * stride, p3 = s0 + clampi(cb + 1, ndir) * stride;
+        const unsigned s0 = (unsigned)G.off + (unsigned)((dir == 0 ? 0 : i) +
+                                                         nx * ((dir == 1 ? 0 : j) + ny * (dir == 2 ? 0 : k)));
+        const unsigned p0 = s0 + clampi(cb - 2, ndir) * stride, p1 = s0 + clampi(cb - 1, ndir) * stride;
+        const unsigned p2 = s0 + clampi(cb, ndir) * stride, p3 = s0 + clampi(cb + 1, ndir) * stride;
         float qL[NC], qR[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
             const float *__restrict__ fld = F.q[n];
-            const float x0 = fld[p0], x1 = fld[p1], x2 = fld[p2], x3 = fld[p3];
+            const float x0 = ldf(fld, p0), x1 = ldf(fld, p1), x2 = ldf(fld, p2), x3 = ldf(fld, p3);
             qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
             qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
         }
-        const double ea = F.e[p1], eb = F.e[p2];
+        const double ea = ldf(F.e, p1), eb = ldf(F.e, p2);
         const bool viewB = mode == 1 || (mode == 2 && ((ea <= 0.1) != (eb <= 0.1)));
         const int nsolve = viewB ? 4 : 2;
         float v[4];
@@ -449,18 +464,18 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
     const int lx = lane % TX, ly = (lane / TX) % TY, lz = lane / (TX * TY);
     const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
     const bool valid = lane < T::NCELL && i < G.nx && j < G.ny && k < G.nz;
-    const long s = G.off + (valid ? i + (long)G.nx * (j + (long)G.ny * k) : 0);
+    const unsigned s = (unsigned)G.off + (valid ? (unsigned)(i + G.nx * (j + G.ny * k)) : 0u);
     double acc[NC];
     float Sd[3][4];
     if (valid) {
         float q[NC], S[NC];
 #pragma unroll
-        for (int n = 0; n < NC; ++n) q[n] = F.q[n][s];
-        source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, F.e[s], q, S, Sd);
+        for (int n = 0; n < NC; ++n) q[n] = ldf(F.q[n], s);
+        source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, ldf(F.e, s), q, S, Sd);
         const float dt = (float)G.dt;
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            const double qn = n < 4 ? (double)q[n] + (double)F.ql[n][s] : (double)q[n];
+            const double qn = n < 4 ? (double)q[n] + (double)ldf(F.ql[n], s) : (double)q[n];
             acc[n] = qn + (double)(dt * S[n]);
         }
     }
@@ -484,27 +499,27 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
     if (F.stage == 1) {
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            const double qn = n < 4 ? (double)F.qn[n][s] + (double)F.qnl[n][s] : (double)F.qn[n][s];
+            const double qn = n < 4 ? (double)ldf(F.qn[n], s) + (double)ldf(F.qnl[n], s) : (double)ldf(F.qn[n], s);
             acc[n] = (acc[n] + qn) * 0.5;
         }
     }
     int nan = NAN_NONE;
     DivRecip div;
     const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
-                                  NC == 15 ? acc[NC - 1] : 0.0, (double)F.e[s], nan, div);
+                                  NC == 15 ? acc[NC - 1] : 0.0, (double)ldf(F.e, s), nan, div);
     regulate_mx<NC>(G, t_new, acc, w, nan);
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
         const float hi = (float)acc[n];
-        F.qo[n][s] = hi;
-        if (n < 4) F.qol[n][s] = (float)(acc[n] - (double)hi);
+        stf(F.qo[n], s, hi);
+        if (n < 4) stf(F.qol[n], s, (float)(acc[n] - (double)hi));
     }
-    F.eo[s] = (float)w.e;
-    F.po[s] = (float)w.p;
-    F.uo[0][s] = (float)w.ut;
-    F.uo[1][s] = (float)w.ux;
-    F.uo[2][s] = (float)w.uy;
-    F.uo[3][s] = (float)w.un;
+    stf(F.eo, s, (float)w.e);
+    stf(F.po, s, (float)w.p);
+    stf(F.uo[0], s, (float)w.ut);
+    stf(F.uo[1], s, (float)w.ux);
+    stf(F.uo[2], s, (float)w.uy);
+    stf(F.uo[3], s, (float)w.un);
     if (nan != NAN_NONE) report_nan(F.err, nan, s);
 }
 

@@ -186,10 +186,10 @@ vh_status vh_prepare_steps(vh_handle *h, int count);
  * alone through vh_handle.  bulk and the EoS are per batch. */
 /* Arithmetic of a batch.  VH_PREC_FP64: the drop-in path (each event
  * bit-identical to vh_handle).  VH_PREC_MIXED: FP32 storage (with an FP32
- * low part kept for T^{tau mu}) and FP32 slopes, face fluxes and source
- * terms; FP64 root solve, EoS table lookups, conserved-variable accumulation
- * and regulator; each face flux evaluated once for the two cells it
- * separates.  No counterpart in the reference (FP64 only). */
+ * low part kept for T^{tau mu}) and FP32 slopes, face fluxes, source terms
+ * and regulator factors; FP64 root solve, EoS table lookups and
+ * conserved-variable accumulation; each face flux evaluated once for the two
+ * cells it separates.  No counterpart in the reference (FP64 only). */
 typedef enum { VH_PREC_FP64 = 0, VH_PREC_MIXED = 1 } vh_precision;
 
 typedef struct {
