@@ -48,26 +48,31 @@ namespace {
 template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
 hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, const int *blk_ev,
                      const int *blk_first, int nblocks, hipStream_t st) {
-    const unsigned threads = (DIM3 ? 4 : 3) * 64;
-    hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ>), dim3(nblocks), dim3(threads), 0, st, F,
-                       E, ev, blk_ev, blk_first);
+    if (F.face_mode == 0)
+        hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, false>), dim3(nblocks),
+                           dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+    else
+        hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, true>), dim3(nblocks),
+                           dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
     return hipGetLastError();
 }
 
+// Tile variants: the face list of a tile fills whole passes of 256 threads
+// closely (16x15: 511 faces, 240 cells; 6x6x6: 756 faces, 216 cells).
 template <int NC, bool TABLE>
 hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, int tile, const EvDesc *ev,
                           const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
     if (!dim3d) {
         switch (tile) {
-        case 0: return launch_mx<NC, TABLE, false, 8, 7, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
-        case 1: return launch_mx<NC, TABLE, false, 16, 3, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
-        case 2: return launch_mx<NC, TABLE, false, 32, 1, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 0: return launch_mx<NC, TABLE, false, 16, 15, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 1: return launch_mx<NC, TABLE, false, 32, 7, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 2: return launch_mx<NC, TABLE, false, 64, 3, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
         }
     } else {
         switch (tile) {
-        case 0: return launch_mx<NC, TABLE, true, 4, 4, 3>(F, E, ev, blk_ev, blk_first, nblocks, st);
-        case 1: return launch_mx<NC, TABLE, true, 4, 3, 4>(F, E, ev, blk_ev, blk_first, nblocks, st);
-        case 2: return launch_mx<NC, TABLE, true, 8, 2, 2>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 0: return launch_mx<NC, TABLE, true, 6, 6, 6>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 1: return launch_mx<NC, TABLE, true, 8, 8, 3>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 2: return launch_mx<NC, TABLE, true, 8, 4, 6>(F, E, ev, blk_ev, blk_first, nblocks, st);
         }
     }
     return hipErrorInvalidValue;
@@ -75,8 +80,8 @@ hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, in
 
 }  // namespace
 
-const int MIXED_TILES_2D[MIXED_NTILE][3] = {{8, 7, 1}, {16, 3, 1}, {32, 1, 1}};
-const int MIXED_TILES_3D[MIXED_NTILE][3] = {{4, 4, 3}, {4, 3, 4}, {8, 2, 2}};
+const int MIXED_TILES_2D[MIXED_NTILE][3] = {{16, 15, 1}, {32, 7, 1}, {64, 3, 1}};
+const int MIXED_TILES_3D[MIXED_NTILE][3] = {{6, 6, 6}, {8, 8, 3}, {8, 4, 6}};
 
 hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, const EvDesc *ev,
                               const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {

@@ -14,9 +14,13 @@
 //     separates (the reference evaluates it from both sides, with each
 //     cell's own e as the root solve's starting value; here each side of a
 //     face starts from the e of the cell on that side).
-// A block owns a TX x TY x TZ tile of one event.  Wave 0 evaluates the
-// source terms of the tile's cells; waves 1..3 evaluate the x, y, eta faces
-// of the tile (TX+1)*TY*TZ etc., at most 64 per pass).  Fluxes meet in LDS.
+// A block of 256 threads owns a TX x TY x TZ tile of one event.  All its
+// threads first evaluate the tile's faces (the x, y and eta faces in one
+// list, (TX+1)*TY*TZ + ... of them) into LDS, then each thread finishes one
+// cell: source terms, the combination with its faces' fluxes, the root
+// solve, the regulator and the stores.  Every wave does both kinds of work,
+// so no wave idles at the barrier and no single wave carries the
+// per-cell tail.
 #pragma once
 #include "vh_stage.hpp"
 
@@ -299,13 +303,12 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 
 }
 
-// Face fluxes of one direction for a tile: the faces between cells a = b-1
-// and b along dir, for every b of the tile plus one more layer.  Each face:
-// limited slopes of the two cells from the 4-point line (clamped at the
-// lattice edges, which equals the reference's ghost cells), half-site
-// values qL = q_a + D_a/2 and qR = q_b - D_b/2
-// (HalfSiteExtrapolation.cpp:12-26), an FP64 root solve on each side, the
-// local speed max(|v_L|, |v_R|) and H = (v_R qR + v_L qL - a (qR - qL)) / 2
+// One face flux between cells a = b-1 and b along dir, b = (i, j, k).  The
+// limited slopes of the two cells come from the 4-point line (clamped at the
+// lattice edges, which equals the reference's ghost cells), the half-site
+// values are qL = q_a + D_a/2 and qR = q_b - D_b/2
+// (HalfSiteExtrapolation.cpp:12-26), each side gets an FP64 root solve, and
+// H = (v_R qR + v_L qL - a (qR - qL)) / 2 with a = max(|v_L|, |v_R|)
 // (SemiDiscreteKurganovTadmorScheme.cpp:14-62).
 //
 // The reference evaluates a face once per cell: cell a (forward face) and
@@ -313,72 +316,66 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 // (EnergyMomentumTensor.cpp:188-191: e <= 0.1 selects the closed-form
 // branch, otherwise it is the Newton start).  HA is cell a's view, HB cell
 // b's.  face_mode 0: one evaluation, each side started from the e of the
-// cell on that side (HA = HB); 1: both views always; 2: cell a's view, plus
-// cell b's view only where the two cells fall on different sides of the
-// e <= 0.1 branch.
-template <int NC, bool TABLE, int TX, int TY, int TZ, int NF>
-__device__ __forceinline__ void faces_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i0, int j0,
-                                         int k0, double t, float (&HA)[NC][NF], float (&HB)[NC][NF], int lane) {
+// cell on that side (HA = HB, only HA stored); 1: both views always; 2: cell
+// a's view, plus cell b's view only where the two cells fall on different
+// sides of the e <= 0.1 branch.
+template <int NC, bool TABLE, bool TWO, int NF>
+__device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i, int j,
+                                        int k, double t, float (&HA)[NC][NF], float (&HB)[TWO ? NC : 1][TWO ? NF : 1],
+                                        int f) {
 #pragma clang fp contract(fast)
-    const int ex = TX + (dir == 0), ey = TY + (dir == 1), ez = TZ + (dir == 2);
-    const int nf = ex * ey * ez;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int ndir = dir == 0 ? nx : (dir == 1 ? ny : nz);
     const unsigned stride = dir == 0 ? 1u : (dir == 1 ? (unsigned)nx : (unsigned)nx * ny);
+    const int cb = dir == 0 ? i : (dir == 1 ? j : k);
+    const unsigned s0 = (unsigned)G.off + (unsigned)((dir == 0 ? 0 : i) +
+                                                     nx * ((dir == 1 ? 0 : j) + ny * (dir == 2 ? 0 : k)));
+    const unsigned p0 = s0 + clampi(cb - 2, ndir) * stride, p1 = s0 + clampi(cb - 1, ndir) * stride;
+    const unsigned p2 = s0 + clampi(cb, ndir) * stride, p3 = s0 + clampi(cb + 1, ndir) * stride;
+    float qL[NC], qR[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+        const float *__restrict__ fld = F.q[n];
+        const float x0 = ldf(fld, p0), x1 = ldf(fld, p1), x2 = ldf(fld, p2), x3 = ldf(fld, p3);
+        qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
+        qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
+    }
+    const double ea = ldf(F.e, p1), eb = ldf(F.e, p2);
     const int mode = F.face_mode;
-    for (int f = lane; f < nf; f += 64) {
-        const int fx = f % ex, fr = f / ex, fy = fr % ey, fz = fr / ey;
-        const int i = i0 + fx, j = j0 + fy, k = k0 + fz;  // cell b (after the face)
-        const int cb = dir == 0 ? i : (dir == 1 ? j : k);
-        const bool need = (dir == 0 ? i - 1 < nx : i < nx) && (dir == 1 ? j - 1 < ny : j < ny) &&
-                          (dir == 2 ? k - 1 < nz : k < nz);
-        if (!need) continue;
-        const unsigned s0 = (unsigned)G.off + (unsigned)((dir == 0 ? 0 : i) +
-                                                         nx * ((dir == 1 ? 0 : j) + ny * (dir == 2 ? 0 : k)));
-        const unsigned p0 = s0 + clampi(cb - 2, ndir) * stride, p1 = s0 + clampi(cb - 1, ndir) * stride;
-        const unsigned p2 = s0 + clampi(cb, ndir) * stride, p3 = s0 + clampi(cb + 1, ndir) * stride;
-        float qL[NC], qR[NC];
-#pragma unroll
-        for (int n = 0; n < NC; ++n) {
-            const float *__restrict__ fld = F.q[n];
-            const float x0 = ldf(fld, p0), x1 = ldf(fld, p1), x2 = ldf(fld, p2), x3 = ldf(fld, p3);
-            qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
-            qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
-        }
-        const double ea = ldf(F.e, p1), eb = ldf(F.e, p2);
-        const bool viewB = mode == 1 || (mode == 2 && ((ea <= 0.1) != (eb <= 0.1)));
-        const int nsolve = viewB ? 4 : 2;
-        float v[4];
-        int nan = NAN_NONE;
-        DivRecip div;
+    const bool viewB = TWO && (mode == 1 || (mode == 2 && ((ea <= 0.1) != (eb <= 0.1))));
+    const int nsolve = viewB ? 4 : 2;
+    float v[4];
+    int nan = NAN_NONE;
+    DivRecip div;
 #pragma unroll 1
-        for (int kk = 0; kk < nsolve; ++kk) {
-            const int side = kk & 1;  // 0: left state, 1: right state
-            const double guess = kk >= 2 ? eb : (mode == 0 && side == 1 ? eb : ea);
-            double x[9];
+    for (int kk = 0; kk < nsolve; ++kk) {
+        const int side = kk & 1;  // 0: left state, 1: right state
+        const double guess = kk >= 2 ? eb : ((!TWO || mode == 0) && side == 1 ? eb : ea);
+        double x[9];
 #pragma unroll
-            for (int m = 0; m < 9; ++m) {
-                const int n = m < 8 ? m : NC - 1;
-                x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
-            }
-            const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], guess, nan, div);
-            const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q == kk) v[q] = vs;
+        for (int m = 0; m < 9; ++m) {
+            const int n = m < 8 ? m : NC - 1;
+            x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
         }
+        const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
+                                                          guess, nan, div);
+        const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == kk) v[q] = vs;
+    }
+    if (nan != NAN_NONE) report_nan(F.err, nan, p2);
+    const float aA = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
+#pragma unroll
+    for (int n = 0; n < NC; ++n) HA[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - aA * (qR[n] - qL[n]));
+    if constexpr (TWO) {
         if (!viewB) {
             v[2] = v[0];
             v[3] = v[1];
         }
-        if (nan != NAN_NONE) report_nan(F.err, nan, p2);
-        const float aA = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
         const float aB = __builtin_fmaxf(__builtin_fabsf(v[2]), __builtin_fabsf(v[3]));
 #pragma unroll
-        for (int n = 0; n < NC; ++n) {
-            HA[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - aA * (qR[n] - qL[n]));
-            HB[n][f] = 0.5f * (v[3] * qR[n] + v[2] * qL[n] - aB * (qR[n] - qL[n]));
-        }
+        for (int n = 0; n < NC; ++n) HB[n][f] = 0.5f * (v[3] * qR[n] + v[2] * qL[n] - aB * (qR[n] - qL[n]));
     }
 }
 
@@ -424,25 +421,28 @@ __device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, double 
     }
 }
 
-template <int TX, int TY, int TZ>
+constexpr int MX_THREADS = 256;
+
+template <int TX, int TY, int TZ, bool DIM3>
 struct Tile {
     static constexpr int NCELL = TX * TY * TZ;
-    static constexpr int NFX = (TX + 1) * TY * TZ, NFY = TX * (TY + 1) * TZ, NFZ = TX * TY * (TZ + 1);
-    static constexpr int NF = NFX > NFY ? (NFX > NFZ ? NFX : NFZ) : (NFY > NFZ ? NFY : NFZ);
-    static_assert(NCELL <= 64, "a tile holds at most one wave of cells");
+    static constexpr int FX = (TX + 1) * TY * TZ, FY = TX * (TY + 1) * TZ, FZ = DIM3 ? TX * TY * (TZ + 1) : 0;
+    static constexpr int NF = FX + FY + FZ;
+    static_assert(NCELL <= MX_THREADS, "one cell per thread");
+    static_assert(DIM3 || TZ == 1, "2+1D tiles are one cell thick");
 };
 
 // One RK stage for every event of a batch: block b is tile (b - blk_first[b])
-// of event blk_ev[b].
-template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, const EosDev E,
-                                                          const EvDesc *__restrict__ ev,
-                                                          const int *__restrict__ blk_ev,
-                                                          const int *__restrict__ blk_first) {
+// of event blk_ev[b].  TWO: both views of a face are stored (face modes 1, 2).
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
+__global__ __launch_bounds__(MX_THREADS) void stage_mixed_kernel(const MxFields F, const EosDev E,
+                                                                 const EvDesc *__restrict__ ev,
+                                                                 const int *__restrict__ blk_ev,
+                                                                 const int *__restrict__ blk_first) {
 #pragma clang fp contract(fast)
-    using T = Tile<TX, TY, TZ>;
-    constexpr int ROLES = DIM3 ? 4 : 3;
-    __shared__ float H[2][ROLES - 1][NC][T::NF];  // [cell a's view, cell b's view]
+    using T = Tile<TX, TY, TZ, DIM3>;
+    __shared__ float HA[NC][T::NF];
+    __shared__ float HB[TWO ? NC : 1][TWO ? T::NF : 1];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blk]);
     const int tile = blk - __builtin_amdgcn_readfirstlane(blk_first[blk]);
@@ -450,24 +450,37 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
     const int ntx = (G.nx + TX - 1) / TX, nty = (G.ny + TY - 1) / TY, ntz = (G.nz + TZ - 1) / TZ;
     if (tile < 0 || tile >= ntx * nty * ntz) return;  // block-uniform: no barrier is skipped by part of a block
     const int i0 = (tile % ntx) * TX, j0 = ((tile / ntx) % nty) * TY, k0 = (tile / (ntx * nty)) * TZ;
-    const int role = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const int lane = threadIdx.x % 64;
     const StepTimes st = G.tab[G.base + F.off];
     const double t = F.stage ? st.t2 : st.t1;
     const double t_new = st.t2;
-    if (role > 0) {
-        faces_mx<NC, TABLE, TX, TY, TZ, T::NF>(F, G, E, role - 1, i0, j0, k0, t, H[0][role - 1], H[1][role - 1],
-                                               lane);
-        __syncthreads();
-        return;
+
+    // faces: x faces, then y faces, then eta faces; face (fx, fy, fz) of
+    // direction d lies between cells b - e_d and b = (i0 + fx, j0 + fy, k0 + fz)
+    for (int f = threadIdx.x; f < T::NF; f += MX_THREADS) {
+        int dir, fl, ex, ey;
+        if (f < T::FX) {
+            dir = 0; fl = f; ex = TX + 1; ey = TY;
+        } else if (f < T::FX + T::FY) {
+            dir = 1; fl = f - T::FX; ex = TX; ey = TY + 1;
+        } else {
+            dir = 2; fl = f - T::FX - T::FY; ex = TX; ey = TY;
+        }
+        const int fx = fl % ex, fy = (fl / ex) % ey, fz = fl / (ex * ey);
+        const int i = i0 + fx, j = j0 + fy, k = k0 + fz;
+        const bool need = (dir == 0 ? i - 1 < G.nx : i < G.nx) && (dir == 1 ? j - 1 < G.ny : j < G.ny) &&
+                          (dir == 2 ? k - 1 < G.nz : k < G.nz);
+        if (need) face_mx<NC, TABLE, TWO, T::NF>(F, G, E, dir, i, j, k, t, HA, HB, f);
     }
-    const int lx = lane % TX, ly = (lane / TX) % TY, lz = lane / (TX * TY);
+    __syncthreads();
+
+    const int c = threadIdx.x;
+    const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
     const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
-    const bool valid = lane < T::NCELL && i < G.nx && j < G.ny && k < G.nz;
-    const unsigned s = (unsigned)G.off + (valid ? (unsigned)(i + G.nx * (j + G.ny * k)) : 0u);
+    if (c >= T::NCELL || i >= G.nx || j >= G.ny || k >= G.nz) return;
+    const unsigned s = (unsigned)G.off + (unsigned)(i + G.nx * (j + G.ny * k));
     double acc[NC];
     float Sd[3][4];
-    if (valid) {
+    {
         float q[NC], S[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n) q[n] = ldf(F.q[n], s);
@@ -479,19 +492,19 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
             acc[n] = qn + (double)(dt * S[n]);
         }
     }
-    __syncthreads();
-    if (!valid) return;
     const float dt = (float)G.dt;
     const float rd[3] = {1.0f / (float)G.dx, 1.0f / (float)G.dy, 1.0f / (float)G.dz};
+    // faces before/after the cell: x, y, eta blocks of the face list
+    const int fb[3] = {lx + (TX + 1) * (ly + TY * lz), T::FX + lx + TX * (ly + (TY + 1) * lz),
+                       T::FX + T::FY + lx + TX * (ly + TY * lz)};
+    const int fstep[3] = {1, TX, TX * TY};
 #pragma unroll
-    for (int d = 0; d < ROLES - 1; ++d) {
-        const int ex = TX + (d == 0), ey = TY + (d == 1);
-        const int fb = lx + ex * (ly + ey * lz);  // face before the cell along d
-        const int fa = fb + (d == 0 ? 1 : (d == 1 ? ex : ex * ey));
+    for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
             // the face before the cell is seen from its b side, the face after from its a side
-            float r = (H[1][d][n][fb] - H[0][d][n][fa]) * rd[d];
+            const float hb = TWO ? HB[TWO ? n : 0][TWO ? fb[d] : 0] : HA[n][fb[d]];
+            float r = (hb - HA[n][fb[d] + fstep[d]]) * rd[d];
             if (n < 4) r += Sd[d][n];
             acc[n] += (double)(r * dt);
         }
@@ -505,8 +518,9 @@ __global__ __launch_bounds__(256) void stage_mixed_kernel(const MxFields F, cons
     }
     int nan = NAN_NONE;
     DivRecip div;
-    const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7],
-                                  NC == 15 ? acc[NC - 1] : 0.0, (double)ldf(F.e, s), nan, div);
+    const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
+                                                      acc[7], NC == 15 ? acc[NC - 1] : 0.0, (double)ldf(F.e, s), nan,
+                                                      div);
     regulate_mx<NC>(G, t_new, acc, w, nan);
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
