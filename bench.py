@@ -6,8 +6,9 @@ batching workload -- 256 seeded 2+1D boost-invariant MC-Glauber events with
 square grids drawn from 81^2..201^2 (dx = 0.1 fm, dtau = 0.02 fm/c) and
 Latin-hypercube per-event parameters ((eta/s)_min in [0.04, 0.2], zeta/s peak
 in [0, 0.1], tau0 in [0.3, 1.0] fm/c), all resident in one batch and stepped
-together until each is below T_F = 150 MeV (three consecutive coarse checks,
-every 5 steps, with no cell above e_F; finished events leave the batch).
+together until each is below T_F = 150 MeV (the third coarse check, every 5
+steps, with no cell above e_F -- the reference never resets that count;
+finished events leave the batch).
 Beside it: the plain FP64 drop-in looping over events one at a time (a
 sample of the same events), the same batch in FP64, the 3+1D single event at
 real size (131x131x61) through the plain FP64 path, the mixed form's error
@@ -185,7 +186,7 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
         zone_updates += coarse * sum(cells[i] for i in strikes)
         batched_steps += coarse
         for i in list(strikes):
-            strikes[i] = strikes[i] + 1 if counts[i] == 0 else 0
+            strikes[i] += counts[i] == 0
             if strikes[i] >= 3 or b.info(i)["steps_done"] + coarse > BATCH["max_steps"]:
                 steps_done[i] = b.info(i)["steps_done"]
                 b.remove(i)
@@ -221,7 +222,7 @@ def run_plain_loop(events, table):
             while strikes < 3 and n + coarse - 1 <= BATCH["max_steps"]:
                 h.steps(n, coarse)
                 n += coarse
-                strikes = strikes + 1 if h.count_above(eF) == 0 else 0
+                strikes += h.count_above(eF) == 0
             steps.append(n - 1)
         except C.VhError as ex:
             if ex.status != C.VH_ERR_NAN:
@@ -328,13 +329,13 @@ SERVE = dict(frac_3d=0.2, side2_min=101, side2_max=201, side3_min=101, side3_max
              bulk=1, max_steps=3000, max_resident=256, collect_surface=1)
 
 
-def run_serving(table, rates, n_requests):
+def run_serving(table, rates, n_requests, share_3d):
     """Open-loop serving at each arrival rate (vh_serve_run)."""
     import cpuvh_amd as C
     out = []
     for k, rate in enumerate(rates):
         stats, reqs = C.serve(table, rate_per_s=float(rate), n_requests=n_requests, seed=1000 + k,
-                              precision=C.PREC_MIXED, **SERVE)
+                              precision=C.PREC_MIXED, **dict(SERVE, frac_3d=share_3d))
         done = [r for r in reqs if r["status"] == C.VH_OK]
         out.append({"rate_per_s": rate, "requests": n_requests, "completed": stats["completed"],
                     "failed": stats["failed"], "events_per_s": stats["events_per_s"],
@@ -402,6 +403,10 @@ def main():
     ap.add_argument("--serve-rates", default="4,16,64", help="arrival rates of the serving sweep (events/s)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-3d-share", type=float, default=0.0,
+                    help="share of 3+1D requests in the serving sweep (default 0: 2+1D requests only)")
+    ap.add_argument("--precision-3d", action="store_true",
+                    help="also run the 201x201x101 precision-form comparison (off by default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -446,9 +451,9 @@ def main():
     if not args.skip_serve:
         print("bench: serving sweep", file=sys.stderr, flush=True)
         serving = run_serving(table, [float(r) for r in args.serve_rates.split(",") if r],
-                              args.serve_requests)
+                              args.serve_requests, args.serve_3d_share)
     precision3d = None
-    if not args.skip_precision:
+    if args.precision_3d:
         print("bench: precision form 3+1D", file=sys.stderr, flush=True)
         precision3d = run_precision_3d(table, PRECISION_3D["steps"])
     dropin = None
@@ -496,10 +501,11 @@ def main():
         "mixed_vs_fp64_error": prec,
         "precision_form_3d": precision3d,
         "serving": None if serving is None else {
-            "workload": "Poisson arrivals of seeded MC-Glauber Au+Au events, 80% 2+1D (101^2-201^2), 20% 3+1D "
-                        "(101x101x41 - 161x161x81), dx=0.1, table EoS, bulk, mixed precision, admitted into one "
-                        "batch by an 8 GiB HBM budget, resident until no cell is above T=150 MeV, freezeout "
-                        "cubes flagged per request at every check", "sweep": serving},
+            "workload": f"Poisson arrivals of seeded MC-Glauber Au+Au events, {100 * (1 - args.serve_3d_share):.0f}% "
+                        f"2+1D (101^2-201^2), {100 * args.serve_3d_share:.0f}% 3+1D (101x101x41 - 161x161x81), "
+                        "dx=0.1, table EoS, bulk, mixed precision, admitted into one batch by an 8 GiB HBM budget, "
+                        "resident until no cell is above T=150 MeV, freezeout cubes flagged per request at every "
+                        "check", "sweep": serving},
         "fp64_dropin_3d": dropin,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_mx,

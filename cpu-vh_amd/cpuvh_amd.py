@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libcpuvh_amd.so")
 
 FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn",
                "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
-               "up_t", "up_x", "up_y", "up_n", "regfac"]
+               "up_t", "up_x", "up_y", "up_n", "regfac", "regfac_bulk"]
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NOMEM, VH_ERR_NAN, VH_ERR_STATE, VH_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
@@ -31,7 +31,7 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
            "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
-           "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run"]
+           "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic"]
 
 
 class VhConfig(ctypes.Structure):
@@ -135,6 +135,8 @@ def _lib():
         L.vh_build_info.restype = ctypes.c_char_p
         L.vh_ic_mc_glauber.argtypes = [ctypes.POINTER(GlauberParams), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        D, D, D, P, ctypes.POINTER(GlauberInfo)]
+        L.vh_ic_mc_glauber_classic.argtypes = [ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               D, D, D, P, ctypes.POINTER(ctypes.c_int)]
         L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
         L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
         L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
@@ -443,6 +445,18 @@ def mc_glauber(nx, ny, nz, dx, dy, dz, seed, A=197, R=6.38, a=0.535, sigma_nn_fm
     if st != VH_OK:
         raise VhError(st, "vh_ic_mc_glauber")
     return e, {"b": info.b, "n_part": info.n_part, "n_coll": info.n_coll}
+
+
+def mc_glauber_classic(nx, ny, nz, dx, dy, dz, A=197, b=0.0, sigma_nn_mb=42.0, e0=1.0, eta_flat=4.0,
+                       eta_variance=1.96):
+    """The reference's own IC type 4 (vh_ic_mc_glauber_classic); returns (e, wounded nucleons)."""
+    e = np.empty(nx * ny * nz, dtype=np.float64)
+    nw = ctypes.c_int(0)
+    st = _lib().vh_ic_mc_glauber_classic(A, b, sigma_nn_mb, e0, eta_flat, eta_variance, nx, ny, nz, dx, dy, dz,
+                                         _ptr(e), ctypes.byref(nw))
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_mc_glauber_classic")
+    return e, nw.value
 
 
 def read_tmunu(path, nx, ny, nz):

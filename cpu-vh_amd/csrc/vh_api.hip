@@ -37,7 +37,8 @@ struct vh_handle {
     int iq_cur = 0, iq_stage = 1, iq_next = 2;
     int iu_cur = 0, iu_prev = 1, iu_stage = 2;
     bool conserved_ready = false;
-    double *regfac = nullptr;
+    double *regfac = nullptr;   // regFacShear output, allocated on first request
+    double *regfacb = nullptr;  // regFacBulk output (bulk builds), likewise
     int *d_err = nullptr;
     unsigned long long *d_count = nullptr;
     // freezeout slices
@@ -128,6 +129,7 @@ StageParams stage_params(const vh_handle *h, int stage, int iq_in, int iu_in, in
     P.eo = h->e[ie_out];
     P.po = h->p[ie_out];
     P.regfac = h->regfac;
+    P.regfacb = h->regfacb;
     P.stage = stage;
     P.dt = h->cfg.dt;
     P.dx = h->cfg.dx;
@@ -259,6 +261,7 @@ vh_status vh_destroy(vh_handle *h) {
     if (h->ev1) hipEventDestroy(h->ev1);
     hipFree(h->pool);
     hipFree(h->regfac);
+    hipFree(h->regfacb);
     hipFree(h->d_err);
     hipFree(h->d_count);
     hipFree(h->d_base);
@@ -406,19 +409,22 @@ static double *field_ptr(vh_handle *h, int field) {
     if (field >= 17 && field <= 20) return h->u[h->iu_cur][field - 17];
     if (field >= 21 && field <= 24) return h->u[h->iu_prev][field - 21];
     if (field == 25) return h->regfac;
+    if (field == 26) return h->nc == 15 ? h->regfacb : nullptr;
     return nullptr;
 }
 
 vh_status vh_download(vh_handle *h, int field, double *out) {
     if (!h || !out) return VH_ERR_ARG;
     HIPCHK(hipSetDevice(h->dev));
-    if (field == 25 && !h->regfac) {
+    if (field == 26 && h->nc != 15) return VH_ERR_ARG;
+    double **rf = field == 25 ? &h->regfac : field == 26 ? &h->regfacb : nullptr;
+    if (rf && !*rf) {
         // enable regulation-factor output from the next step on
         HIPCHK(hipStreamSynchronize(h->stream));
-        HIPCHK(hipMalloc(&h->regfac, sizeof(double) * h->N));
+        HIPCHK(hipMalloc(rf, sizeof(double) * h->N));
         for (long s = 0; s < h->N; ++s) out[s] = 1.0;
         std::vector<double> ones(h->N, 1.0);
-        HIPCHK(hipMemcpy(h->regfac, ones.data(), sizeof(double) * h->N, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(*rf, ones.data(), sizeof(double) * h->N, hipMemcpyHostToDevice));
         for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
         h->graphs.clear();
         return VH_OK;
@@ -433,7 +439,7 @@ vh_status vh_upload_field(vh_handle *h, int field, const double *in) {
     if (!h || !in) return VH_ERR_ARG;
     HIPCHK(hipSetDevice(h->dev));
     double *dst = field_ptr(h, field);
-    if (!dst || field == 25) return VH_ERR_ARG;
+    if (!dst || field == 25 || field == 26) return VH_ERR_ARG;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(dst, in, sizeof(double) * h->N, hipMemcpyHostToDevice));
     return VH_OK;

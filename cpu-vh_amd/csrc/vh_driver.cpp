@@ -1,0 +1,462 @@
+// cpuvh_run — the command-line hydro driver on top of the C ABI.
+//
+// Usage (the reference's options, rhic/src/CommandLineArguments.cpp:17-23):
+//   cpuvh_run --hydro --config <dir> --output <dir>       (short: -h -c -o; -t is accepted)
+//
+// Reads the reference's rhic-conf files with the reference's reader semantics
+// (rhic/src/LatticeParameters.cpp:68-157, rhic/src/InitialConditionParameters.cpp:75-150,
+// rhic/src/HydroParameters.cpp:52-100): values are taken by position, one per
+// line, keys ignored, whitespace removed, the text after '=' converted with
+// atoi/atof; a missing file leaves its parameters at zero.  Options the
+// reference fixes at compile time (bulk viscosity, the EoS) come from an
+// optional, keyed <config>/gpu.properties, so the rhic-conf files stay
+// unchanged:
+//   bulk=0|1                 evolve Pi (the reference's -DPI build)           [0]
+//   eos=conformal|table      the EoS (CONFORMAL_EOS / HOTQCDHRG_EOS)          [conformal]
+//   eosFile=<path>           table file, 4 x 100000 doubles                   [EoS/hrg_hotqcd_eos_binary.dat]
+//   device=<n>               HIP device                                       [0]
+//   snapshotEvery=<n>        steps between snapshots (reference: 100)          [100]
+//
+// The time loop is the reference's run() (rhic/src/HydroPlugin.cpp:74-321),
+// in its order: for n = 1 .. nt+1 (nt+1 steps), snapshot when (n-1) % 100 == 0,
+// then on every n % coarse == 0 the freezeout cube flags (the previous
+// slice starts at zero) and the count of cells above e_F — the run stops once
+// that count has been zero three times (the count of zeros is never reset) —
+// then one RK2 step at t, then t = t0 + n dt.  Consecutive steps with no host
+// work between them are replayed from a HIP graph (vh_steps).
+//
+// Initial conditions: types 0 (constant energy density, :953-981), 4 (the
+// reference's MC-Glauber, bit for bit) and 11 (input/Tmunu.dat under the
+// working directory, :41-105), with u = (1,0,0,0) and pi = 0 or its
+// Navier-Stokes value (:492-646) for 0 and 4.  Other types exit as the
+// reference does for an undefined type.
+//
+// Output: the snapshot files of rhic/src/HydroPlugin.cpp:40-72 in the format
+// of rhic/src/FileIO.cpp:14-59 (<name>_<t %.3f>.dat, "x y z value" lines, x
+// fastest).  In place of Cornelius's surface.dat the driver writes
+// surface_cubes.dat: one line per hypercube that Cornelius would cut
+// (tau x y eta of the cube's low corner as FreezeOut.cpp:228-231 places it, then ix
+// iy iz).
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <getopt.h>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+#include "../../include/cpuvh_amd.h"
+
+namespace {
+
+constexpr double HBARC = 0.197326938;  // rhic/src/HydroPlugin.cpp:112
+
+// One positional value per line: whitespace removed, the text after the
+// first '=' (the whole line when there is none).  When the file has run out,
+// std::getline leaves the previous line in place, so the previous value
+// repeats — as in the reference's readers.
+class PositionalReader {
+  public:
+    explicit PositionalReader(const std::string &path) : in_(path) {}
+    bool ok() const { return in_.is_open(); }
+    std::string next() {
+        std::getline(in_, line_);
+        line_.erase(std::remove_if(line_.begin(), line_.end(), [](unsigned char c) { return std::isspace(c); }),
+                    line_.end());
+        const size_t eq = line_.find('=');
+        line_ = line_.substr(eq == std::string::npos ? 0 : eq + 1);
+        return line_;
+    }
+    int next_int() { return std::atoi(next().c_str()); }
+    double next_double() { return std::atof(next().c_str()); }
+
+  private:
+    std::ifstream in_;
+    std::string line_;
+};
+
+struct Lattice {  // rhic/include/LatticeParameters.h:17-34
+    int nx = 0, ny = 0, nz = 0, nt = 0;
+    double dx = 0, dy = 0, dz = 0, dt = 0;
+    int coarse = 0;
+};
+struct InitCond {  // rhic/include/InitialConditionParameters.h
+    int type = 0, n_source_files = 0, A = 0;
+    double e0 = 0, sigma_nn_mb = 0, b = 0, f_binary = 0, eta_variance = 0, eta_flat = 0;
+    int source_type = 0;
+};
+struct HydroPar {  // rhic/include/HydroParameters.h:15-23
+    double t0 = 0, etabar = 0, T_F_GeV = 0;
+    int pi_navier_stokes = 0;
+    double T_reg_GeV = 0;
+    int do_freezeout = 0;
+};
+struct Extra {
+    int bulk = 0;
+    int eos_table = 0;
+    std::string eos_file = "EoS/hrg_hotqcd_eos_binary.dat";
+    int device = 0;
+    int snapshot_every = 100;
+};
+
+void read_config(const std::string &dir, Lattice &L, InitCond &I, HydroPar &H, Extra &X) {
+    {
+        PositionalReader r(dir + "/lattice.properties");
+        if (r.ok()) {
+            L.nx = r.next_int(); L.ny = r.next_int(); L.nz = r.next_int(); L.nt = r.next_int();
+            L.dx = r.next_double(); L.dy = r.next_double(); L.dz = r.next_double(); L.dt = r.next_double();
+            L.coarse = r.next_int();
+        } else {
+            std::fprintf(stderr, "No configuration file found for lattice parameters\n");
+        }
+    }
+    {
+        PositionalReader r(dir + "/ic.properties");
+        if (r.ok()) {
+            I.type = r.next_int(); I.n_source_files = r.next_int(); I.A = r.next_int();
+            I.e0 = r.next_double(); I.sigma_nn_mb = r.next_double(); I.b = r.next_double();
+            I.f_binary = r.next_double(); I.eta_variance = r.next_double(); I.eta_flat = r.next_double();
+            I.source_type = r.next_int();
+        } else {
+            std::fprintf(stderr, "No configuration file found for initial condition parameters\n");
+        }
+    }
+    {
+        PositionalReader r(dir + "/hydro.properties");
+        if (r.ok()) {
+            H.t0 = r.next_double(); H.etabar = r.next_double(); H.T_F_GeV = r.next_double();
+            H.pi_navier_stokes = r.next_int(); H.T_reg_GeV = r.next_double(); H.do_freezeout = r.next_int();
+        } else {
+            std::fprintf(stderr, "No configuration file found for hydro parameters\n");
+        }
+    }
+    std::ifstream in(dir + "/gpu.properties");
+    std::string line;
+    while (std::getline(in, line)) {
+        line.erase(std::remove_if(line.begin(), line.end(), [](unsigned char c) { return std::isspace(c); }),
+                   line.end());
+        if (line.empty() || line[0] == '#') continue;
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string key = line.substr(0, eq), val = line.substr(eq + 1);
+        if (key == "bulk") X.bulk = std::atoi(val.c_str());
+        else if (key == "eos") X.eos_table = (val == "table" || val == "1");
+        else if (key == "eosFile") X.eos_file = val;
+        else if (key == "device") X.device = std::atoi(val.c_str());
+        else if (key == "snapshotEvery") X.snapshot_every = std::atoi(val.c_str());
+        else std::fprintf(stderr, "gpu.properties: unknown key '%s' ignored\n", key.c_str());
+    }
+}
+
+[[noreturn]] void die(const char *what, vh_status st) {
+    std::printf("%s failed: %s\n", what, vh_status_string(st));
+    std::exit(-1);
+}
+
+void check(const char *what, vh_status st) {
+    if (st != VH_OK) die(what, st);
+}
+
+// The reference stops the process on the first NaN with the message of the
+// site that saw it (EnergyMomentumTensor.cpp:42,197-203;
+// FullyDiscreteKurganovTadmorScheme.cpp:849,854).
+void check_sync(vh_handle *h, vh_status st) {
+    if (st == VH_ERR_NAN) {
+        int site = 0;
+        int64_t cell = -1;
+        vh_nan_info(h, &site, &cell);
+        static const char *msg[] = {"NaN", "Found e nan inside getInferredVariables", "f/fp is nan",
+                                    "Found facShear nan", "Found facBulk nan"};
+        std::printf("%s \n", msg[site >= 0 && site <= 4 ? site : 0]);
+        std::exit(-1);
+    }
+    check("step", st);
+}
+
+// pi^{mu nu} (and Pi) at their Navier-Stokes values for a fluid at rest in
+// Bjorken expansion, rhic/src/InitialConditions.cpp:530-596.
+void navier_stokes_pi(vh_handle *h, const HydroPar &H, int bulk, size_t N, const std::vector<double> &e,
+                      const std::vector<double> &p, std::vector<double> &pi, std::vector<double> &Pi) {
+    const double t = H.t0, etabar = H.etabar;
+    for (size_t s = 0; s < N; ++s) {
+        double T = vh_eos_temperature(h, e[s]);
+        if (T == 0) T = 1.e-3;
+        const double pinn = -4.0 / (3.0 * t * t * t) * etabar * (e[s] + p[s]) / T;
+        for (int c = 0; c < 10; ++c) pi[c * N + s] = 0;
+        pi[4 * N + s] = -t * t * pinn / 2;  // pixx
+        pi[7 * N + s] = -t * t * pinn / 2;  // piyy
+        pi[9 * N + s] = pinn;
+        if (bulk) {
+            const double x = T / 1.01355;
+            double zetabar = -13.77 * x * x + 27.55 * x - 13.45;
+            if (x > 1.05) zetabar = 0.9 * std::exp(-(x - 1) / 0.025) + 0.25 * std::exp(-(x - 1) / 0.13) + 0.001;
+            else if (x < 0.995) zetabar = 0.9 * std::exp((x - 1) / 0.0025) + 0.22 * std::exp((x - 1) / 0.022) + 0.03;
+            Pi[s] = -zetabar * (e[s] + p[s]) / T / t;
+        }
+    }
+}
+
+// Initial state on the host, interior-only, x fastest.
+void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const HydroPar &H, int bulk,
+                   const std::string &root) {
+    const size_t N = (size_t)L.nx * L.ny * L.nz;
+    std::vector<double> e(N), p(N), ut(N, 1.0), ux(N, 0.0), uy(N, 0.0), un(N, 0.0), pi(10 * N, 0.0), Pi(N, 0.0);
+    std::printf("Setting initial conditions: ");
+    switch (I.type) {
+    case 0: {
+        std::printf("constant energy density.\n");
+        const double ed = vh_eos_energy_density(h, 3.05);  // InitialConditions.cpp:958-959
+        for (size_t s = 0; s < N; ++s) { e[s] = ed; p[s] = vh_eos_pressure(h, ed); }
+        break;
+    }
+    case 4: {
+        std::printf("Monte carlo Glauber.\n");
+        int nw = 0;
+        check("vh_ic_mc_glauber_classic",
+              vh_ic_mc_glauber_classic(I.A, I.b, I.sigma_nn_mb, I.e0, I.eta_flat, I.eta_variance, L.nx, L.ny, L.nz,
+                                       L.dx, L.dy, L.dz, e.data(), &nw));
+        std::printf("==> Found %d wounded nucleons.\n", nw);
+        for (size_t s = 0; s < N; ++s) p[s] = vh_eos_pressure(h, e[s]);
+        break;
+    }
+    case 11: {
+        std::printf("Reading initial T ^mu nu from /input/Tmunu.dat \n");
+        const std::string path = root + "/input/Tmunu.dat";
+        if (vh_ic_read_tmunu(path.c_str(), L.nx, L.ny, L.nz, e.data(), p.data(), ut.data(), ux.data(), uy.data(),
+                             un.data(), pi.data(), Pi.data()) != VH_OK) {
+            std::printf("Could not read %s\n", path.c_str());
+            std::exit(-1);
+        }
+        if (!bulk) std::fill(Pi.begin(), Pi.end(), 0.0);
+        break;
+    }
+    default:
+        std::printf("Initial condition type %d is not available in this build. Exiting ...\n", I.type);
+        std::exit(-1);
+    }
+    if (I.type == 0 || I.type == 4) {  // setFluidVelocityInitialCondition + setPimunuInitialCondition
+        if (H.pi_navier_stokes == 1) {
+            std::printf("Initialize \\pi^\\mu\\nu to its asymptotic Navier-Stokes value.\n");
+            if (bulk) std::printf("Initialize \\Pi to its asymptotic Navier-Stokes value.\n");
+            navier_stokes_pi(h, H, bulk, N, e, p, pi, Pi);
+        } else {
+            std::printf("Initialize \\pi^\\mu\\nu to zero.\n");
+        }
+    }
+    check("vh_upload_ic", vh_upload_ic(h, e.data(), p.data(), ut.data(), ux.data(), uy.data(), un.data(), pi.data(),
+                                       bulk ? Pi.data() : nullptr));
+}
+
+void write_snapshot(const std::vector<double> &v, double t, const std::string &dir, const char *name,
+                    const Lattice &L) {
+    char fname[4096];
+    std::snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", dir.c_str(), name, t);
+    FILE *fp = std::fopen(fname, "w");
+    if (!fp) {
+        std::printf("cannot write %s\n", fname);
+        std::exit(-1);
+    }
+    size_t s = 0;
+    for (int k = 0; k < L.nz; ++k) {
+        const double z = (k - (L.nz - 1) / 2.) * L.dz;
+        for (int j = 0; j < L.ny; ++j) {
+            const double y = (j - (L.ny - 1) / 2.) * L.dy;
+            for (int i = 0; i < L.nx; ++i, ++s) {
+                const double x = (i - (L.nx - 1) / 2.) * L.dx;
+                std::fprintf(fp, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z, v[s]);
+            }
+        }
+    }
+    std::fclose(fp);
+}
+
+// outputDynamicalQuantities, rhic/src/HydroPlugin.cpp:40-72.  Before the
+// first step the regulation factors are the reference's calloc'd zeros.
+void snapshot(vh_handle *h, double t, const std::string &dir, const Lattice &L, int bulk, bool before_first_step,
+              std::vector<double> &buf) {
+    struct F { const char *name; int id; };
+    const F shear[] = {{"e", 15}, {"ux", 18}, {"uy", 19}, {"ut", 17}, {"pixx", 8}, {"pixy", 9}, {"pinn", 13},
+                       {"regFacShear", 25}};
+    const F bulkf[] = {{"Pi", 14}, {"regFacBulk", 26}};
+    auto one = [&](const F &f) {
+        if (before_first_step && (f.id == 25 || f.id == 26)) std::fill(buf.begin(), buf.end(), 0.0);
+        else check_sync(h, vh_download(h, f.id, buf.data()));
+        write_snapshot(buf, t, dir, f.name, L);
+    };
+    for (const F &f : shear) one(f);
+    if (bulk)
+        for (const F &f : bulkf) one(f);
+}
+
+void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X, const std::string &root,
+         const std::string &out) {
+    std::vector<double> table;
+    if (X.eos_table) {
+        table.resize(4 * 100000);
+        FILE *f = std::fopen(X.eos_file.c_str(), "rb");
+        if (!f || std::fread(table.data(), sizeof(double), table.size(), f) != table.size()) {
+            std::printf("Could not read the EoS table %s\n", X.eos_file.c_str());
+            std::exit(-1);
+        }
+        std::fclose(f);
+    }
+    vh_config cfg{};
+    cfg.nx = L.nx; cfg.ny = L.ny; cfg.nz = L.nz;
+    cfg.dx = L.dx; cfg.dy = L.dy; cfg.dz = L.dz; cfg.dt = L.dt; cfg.t0 = H.t0;
+    cfg.etabar = H.etabar;
+    cfg.T_reg_GeV = H.T_reg_GeV;
+    cfg.bulk = X.bulk;
+    cfg.eos_kind = X.eos_table ? VH_EOS_TABLE : VH_EOS_CONFORMAL;
+    cfg.eos_table = X.eos_table ? table.data() : nullptr;
+    cfg.etas_Tk_GeV = 0.154;
+    cfg.device = X.device;
+    vh_handle *h = nullptr;
+    check("vh_create", vh_create(&cfg, &h));
+
+    const int nt = L.nt, nx = L.nx, ny = L.ny, nz = L.nz, coarse = L.coarse;
+    const double t0 = H.t0, dt = L.dt;
+    const double TF = H.T_F_GeV / HBARC;
+    const double eF = vh_eos_energy_density(h, TF);
+    std::printf("Grid size = %d x %d x %d\n", nx, ny, nz);
+    std::printf("spatial resolution = (%.3f, %.3f, %.3f)\n", L.dx, L.dy, L.dz);
+    std::printf("freezeout temperature = %.3f [fm^-1] (eF = %.3f [fm^-4])\n", TF, eF);
+    std::printf("Coarse graining FO surface by factor %d in proper time\n", coarse);
+    std::printf("Regulating viscous currents using : Regulation Scheme 1\n");
+    if (coarse < 1) {
+        std::printf("coarseProperTimeFOFactor must be positive\n");
+        std::exit(-1);
+    }
+
+    int dim = 0;
+    if (nx > 1 && ny > 1 && nz > 1) dim = 4;
+    else if (nx > 1 && ny > 1 && nz < 2) dim = 3;
+    else std::printf("simulation is not in 3+1D or 2+1D; freezeout finder will not work!\n");
+    FILE *cubes = std::fopen((out + "/surface_cubes.dat").c_str(), "w");
+    if (!cubes) {
+        std::printf("cannot write %s/surface_cubes.dat\n", out.c_str());
+        std::exit(-1);
+    }
+
+    initial_state(h, L, I, H, X.bulk, root);
+    check("vh_init_conserved", vh_init_conserved(h, t0));
+
+    const size_t N = (size_t)nx * ny * nz;
+    std::vector<double> buf(N);
+    check("vh_download", vh_download(h, 25, buf.data()));  // record regFacShear from the first step on
+    if (X.bulk) check("vh_download", vh_download(h, 26, buf.data()));
+    const size_t ncubes = (size_t)std::max(nx - 1, 1) * std::max(ny - 1, 1) * std::max(nz - 1, 1);
+    std::vector<int64_t> ids(ncubes);
+    const int sc = nx / 2, sj = ny / 2, sk = nz / 2;  // the reference's centre cell (HydroPlugin.cpp:211-214)
+    const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
+
+    double t = t0, total_ms = 0;
+    int nsteps = 0, zero_counts = 0, pend_first = 0, pend_count = 0;
+    using clk = std::chrono::steady_clock;
+    // queued steps run as one graph replay; their wall time counts as step time
+    auto flush = [&]() {
+        if (!pend_count) return;
+        const auto a = clk::now();
+        check_sync(h, vh_steps(h, pend_first, pend_count));
+        check_sync(h, vh_sync(h));
+        total_ms += std::chrono::duration<double, std::milli>(clk::now() - a).count();
+        pend_count = 0;
+    };
+    const int wf = X.snapshot_every > 0 ? X.snapshot_every : 100;
+    for (int n = 1; n <= nt + 1; ++n) {
+        const bool snap = (n - 1) % wf == 0, fo = n % coarse == 0;
+        if (snap || fo) flush();
+        if (snap) {
+            double ec = 0;
+            check_sync(h, vh_download(h, 15, buf.data()));
+            ec = buf[sctr];
+            check_sync(h, vh_download(h, 16, buf.data()));
+            const double pc = buf[sctr];
+            std::printf("n = %d:%d (t = %.3f),\t (e, p) = (%.3f, %.3f) [fm^-4],\t (T = %.3f [GeV]),\t", n - 1, nt, t,
+                        ec, pc, vh_eos_temperature(h, ec) * HBARC);
+            snapshot(h, t, out, L, X.bulk, n == 1, buf);
+        }
+        if (fo) {
+            if (H.do_freezeout && dim) {
+                int64_t nf = 0;
+                vh_status st = vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf);
+                check_sync(h, st);
+                const double tau = t0 + (double)n * dt;  // FreezeOut.cpp:228
+                for (int64_t m = 0; m < nf; ++m) {
+                    const int64_t c = ids[m];
+                    const int ix = (int)(c % (nx - 1)), iy = (int)((c / (nx - 1)) % (ny - 1));
+                    const int iz = dim == 4 ? (int)(c / ((int64_t)(nx - 1) * (ny - 1))) : 0;
+                    const double x = (double)ix * L.dx - ((double)(nx - 1)) / 2.0 * L.dx;
+                    const double y = (double)iy * L.dy - ((double)(ny - 1)) / 2.0 * L.dy;
+                    const double z = dim == 4 ? (double)iz * L.dz - ((double)(nz - 1)) / 2.0 * L.dz : 0.0;
+                    std::fprintf(cubes, "%g %g %g %g %d %d %d\n", tau, x, y, z, ix, iy, iz);
+                }
+            }
+            int64_t above = 0;
+            check_sync(h, vh_count_above(h, eF, &above));
+            if (above == 0) zero_counts += 1;
+            if (zero_counts >= 3) {
+                std::printf("\nAll cells have dropped below freezeout energy density\n");
+                break;
+            }
+        }
+        if (!pend_count) pend_first = n;
+        ++pend_count;
+        if (snap) {
+            const double before = total_ms;
+            flush();
+            std::printf("(Elapsed time: %.3f ms)\n", total_ms - before);
+        }
+        ++nsteps;
+        t = t0 + n * dt;
+    }
+    flush();
+    std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
+    std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
+    std::fclose(cubes);
+    check("vh_destroy", vh_destroy(h));
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    bool run_hydro = false, run_test = false;
+    const char *config = nullptr, *output = nullptr;
+    static const option opts[] = {{"test", optional_argument, nullptr, 't'},
+                                  {"hydro", optional_argument, nullptr, 'h'},
+                                  {"output", required_argument, nullptr, 'o'},
+                                  {"config", required_argument, nullptr, 'c'},
+                                  {nullptr, 0, nullptr, 0}};
+    int c;
+    while ((c = getopt_long(argc, argv, "t::h::o:c:", opts, nullptr)) != -1) {
+        if (c == 't') run_test = true;
+        else if (c == 'h') run_hydro = true;
+        else if (c == 'o') output = optarg;
+        else if (c == 'c') config = optarg;
+        else return 64;
+    }
+    char cwd[4096];
+    const std::string root = getcwd(cwd, sizeof cwd) ? cwd : ".";
+    std::printf("configDirectory = %s\n", config ? config : "(null)");
+    std::printf("outputDirectory = %s\n", output ? output : "(null)");
+    std::printf("runHydro = %s\n", run_hydro ? "True" : "False");
+    std::printf("runTest = %s\n", run_test ? "True" : "False");
+    std::printf("%s\n", vh_build_info());
+    Lattice L;
+    InitCond I;
+    HydroPar H;
+    Extra X;
+    read_config(config ? config : ".", L, I, H, X);
+    const auto a = std::chrono::steady_clock::now();
+    if (run_hydro) {
+        run(L, I, H, X, root, output ? output : ".");
+        std::printf("Done hydro.\n");
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    std::printf("Hydro took %f seconds\n", sec);
+    return 0;
+}

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -123,6 +124,94 @@ extern "C" vh_status vh_ic_mc_glauber(const vh_glauber_params *gp, int nx, int n
         info->n_part = (int)px.size();
         info->n_coll = ncoll;
     }
+    return VH_OK;
+}
+
+// The reference's own type-4 generator, reproduced value for value: the C
+// library's rand() seeded with the reference's fixed seed, Woods-Saxon
+// nucleons with its hard-coded radius, diffuseness and acceptance bound,
+// wounded nucleons from the black-disk cut sqrt(0.1 sigma_NN[mb] / pi),
+// participants listed in the reference's interleaved order (projectile i,
+// then target i), unnormalised Gaussians of width 0.96 fm summed in that
+// order, times e0 and the longitudinal profile, plus the 1e-3 floor
+// (rhic/src/MonteCarloGlauberModel.cpp:18-111, rhic/src/GlauberModel.cpp:19-24,
+// rhic/src/InitialConditions.cpp:1046-1064, :1107-1140).  Products are
+// written as the reference's g++ -O3 build evaluates them (pow(d, 2) is d*d).
+namespace {
+
+double classic_uniform() { return (double)std::rand() / (double)RAND_MAX; }
+
+void classic_nucleus(int A, double *x, double *y) {
+    const double n0 = 0.17, thick = 0.54;  // GlauberModel.cpp:20-22
+    const double radius = 1.12 * std::pow((double)A, 1. / 3.) - 0.86 * std::pow((double)A, -1. / 3.);
+    const double bound = 1.01 * 4.5310551374155095, rmax = 20;  // MonteCarloGlauberModel.cpp:19-20
+    int m = 0;
+    while (m < A) {
+        const double r = rmax * (double)std::rand() / (double)RAND_MAX;
+        const double u = classic_uniform();
+        if (u < r * r * (n0 / (1 + std::exp((r - radius) / thick))) / bound) {
+            const double v = 2 * (classic_uniform() - 0.5);
+            const double f = 2 * M_PI * (double)std::rand() / (double)RAND_MAX;
+            x[m] = r * std::sqrt(1 - v * v) * std::cos(f);
+            y[m] = r * std::sqrt(1 - v * v) * std::sin(f);
+            ++m;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" vh_status vh_ic_mc_glauber_classic(int A, double b, double sigma_nn_mb, double e0, double eta_flat,
+                                              double eta_variance, int nx, int ny, int nz, double dx, double dy,
+                                              double dz, double *e_out, int *n_wounded) {
+    if (!e_out || nx < 1 || ny < 1 || nz < 1 || A < 1) return VH_ERR_ARG;
+    std::srand(1328398221u);  // MonteCarloGlauberModel.cpp:92
+    std::vector<double> x1(A), y1(A), x2(A), y2(A);
+    classic_nucleus(A, x1.data(), y1.data());
+    classic_nucleus(A, x2.data(), y2.data());
+    for (int i = 0; i < A; ++i) {
+        x1[i] -= b / 2;
+        x2[i] += b / 2;
+    }
+    const double dn = std::sqrt(0.1 * sigma_nn_mb / M_PI);
+    std::vector<int> hit1(A, 0), hit2(A, 0);
+    for (int i = 0; i < A; ++i)
+        for (int j = 0; j < A; ++j) {
+            const double ddx = x1[i] - x2[j], ddy = y1[i] - y2[j];
+            double d = ddx * ddx;
+            d += ddy * ddy;
+            if (std::sqrt(d) < dn) {
+                ++hit1[i];
+                ++hit2[j];
+            }
+        }
+    std::vector<double> px, py;
+    for (int i = 0; i < A; ++i) {
+        if (hit1[i] > 0) { px.push_back(x1[i]); py.push_back(y1[i]); }
+        if (hit2[i] > 0) { px.push_back(x2[i]); py.push_back(y2[i]); }
+    }
+    const double sig = 0.96;  // MonteCarloGlauberModel.cpp:90
+    std::vector<double> eT((size_t)nx * ny, 0.0);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (size_t n = 0; n < px.size(); ++n) {
+                const double x = (i - ((double)nx - 1.) / 2.) * dx - px[n];
+                const double y = (j - ((double)ny - 1.) / 2.) * dy - py[n];
+                eT[(size_t)i + (size_t)nx * j] += std::exp(-x * x / 2 / sig / sig - y * y / 2 / sig / sig);
+            }
+    std::vector<double> eL(nz);
+    for (int k = 0; k < nz; ++k) {
+        const double eta = (k - (nz - 1) / 2) * dz;
+        const double es = std::fabs(eta) - eta_flat / 2;
+        eL[k] = std::exp(-es * es / eta_variance / 2 * (es < 0 ? 0.0 : 1.0));
+    }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double eTr = e0 * eT[(size_t)i + (size_t)nx * j];
+                e_out[(size_t)i + (size_t)nx * (j + (size_t)ny * k)] = (eTr * eL[k]) + 1.e-3;
+            }
+    if (n_wounded) *n_wounded = (int)px.size();
     return VH_OK;
 }
 

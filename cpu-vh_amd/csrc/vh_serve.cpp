@@ -1,8 +1,9 @@
 // Serving loop of the batch engine (include/cpuvh_amd.h, vh_serve_run): an
 // open-loop request stream admitted by HBM footprint into one vh_batch,
 // stepped in rounds of `coarse` steps until each request meets the
-// reference's stop rule (rhic/src/HydroPlugin.cpp:266-281: three consecutive
-// freezeout checks with no cell above e_F).
+// reference's stop rule (rhic/src/HydroPlugin.cpp:266-281: the third
+// freezeout check with no cell above e_F; the count of such checks is never
+// reset).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -203,7 +204,7 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
                 }
                 req[owner[id]].fo_cubes += nflag;
             }
-            strikes[id] = counts[id] == 0 ? strikes[id] + 1 : 0;
+            if (counts[id] == 0) ++strikes[id];
             int active = 0, steps = 0;
             vh_batch_event_info(b, id, &active, &steps, nullptr);
             if (strikes[id] >= 3 || steps + c.coarse > c.max_steps) finish(id, VH_OK);

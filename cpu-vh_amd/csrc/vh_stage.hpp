@@ -35,7 +35,8 @@ struct StageParams {
     double *uo[4];
     double *eo;
     double *po;
-    double *regfac;        // optional
+    double *regfac;        // optional: regFacShear of the last stage
+    double *regfacb;       // optional: regFacBulk of the last stage (bulk builds)
     const StepTimes *tab;  // per-step times (graph replays), or null: use st
     const int *base;       // step index of graph node offset 0
     int off;
@@ -421,7 +422,7 @@ __device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, 
 // REG_SCHEME_1, rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:689-860, one cell.
 template <int NC, class GEO>
 __device__ __forceinline__ double regulate(const GEO &G, double t, double (&acc)[NC], const Prim &w,
-                                           int &nan) {
+                                           int &nan, double &fb_out) {
     const double pitt = acc[4], pitx = acc[5], pity = acc[6], pitn = acc[7], pixx = acc[8], pixy = acc[9],
                  pixn = acc[10], piyy = acc[11], piyn = acc[12], pinn = acc[13];
     const double Pi = NC == 15 ? acc[NC - 1] : 0.0;
@@ -462,6 +463,7 @@ __device__ __forceinline__ double regulate(const GEO &G, double t, double (&acc)
         acc[NC - 1] *= facBulk;
         if (__builtin_isnan(facBulk)) nan = NAN_REG_BULK;
     }
+    fb_out = facBulk;
     return facShear;
 }
 
@@ -469,7 +471,7 @@ template <int NC>
 struct CellOut {
     double acc[NC];
     Prim w;
-    double fs;
+    double fs, fb;
 };
 
 template <int NC>
@@ -483,6 +485,7 @@ __device__ __forceinline__ void store_cell(const StageParams &P, long s, const C
     P.uo[2][s] = o.w.uy;
     P.uo[3][s] = o.w.un;
     if (P.regfac) P.regfac[s] = o.fs;
+    if (NC == 15 && P.regfacb) P.regfacb[s] = o.fb;
     if (nan != NAN_NONE) report_nan(P.err, nan, s);
 }
 
@@ -576,7 +579,7 @@ __device__ __forceinline__ void stage_body(const StageParams &P, const EosDev &E
     int nan_cell = NAN_NONE;
     o.w = recover<TABLE>(E, t_new, o.acc[0], o.acc[1], o.acc[2], o.acc[3], o.acc[4], o.acc[5], o.acc[6], o.acc[7],
                          NC == 15 ? o.acc[NC - 1] : 0.0, P.e[s], nan_cell, div);
-    o.fs = regulate<NC>(G, t_new, o.acc, o.w, nan_cell);
+    o.fs = regulate<NC>(G, t_new, o.acc, o.w, nan_cell, o.fb);
     store_cell<NC>(P, s, o, nan != NAN_NONE ? nan : nan_cell);
 }
 

@@ -14,6 +14,11 @@
  *   14     Pi (bulk builds only)
  *   15 e   16 p   17..20 ut ux uy un   21..24 previous-step ut ux uy un
  *   25     regulation factor of the last stage (regFacShear)
+ *   26     bulk regulation factor of the last stage (regFacBulk; bulk builds only)
+ * Fields 25 and 26 are recorded only once requested: the first vh_download of
+ * either returns ones and turns its recording on from the next step on
+ * (the reference writes both in regulateDissipativeCurrents,
+ * FullyDiscreteKurganovTadmorScheme.cpp:848,853).
  *
  * All entry points return a vh_status; none aborts the process.
  */
@@ -161,6 +166,18 @@ typedef struct {
 
 vh_status vh_ic_mc_glauber(const vh_glauber_params *gp, int nx, int ny, int nz, double dx, double dy, double dz,
                            double *e_out, vh_glauber_info *info);
+
+/* The reference's own IC type 4 (rhic/src/InitialConditions.cpp:1107-1140,
+ * rhic/src/MonteCarloGlauberModel.cpp:18-111, rhic/src/GlauberModel.cpp:19-24):
+ * the C library's rand() seeded with its fixed seed 1328398221, so the same
+ * ic.properties give the same profile as the reference on this C library.
+ * A = numberOfNucleonsPerNuclei, b = impactParameter [fm], sigma_nn_mb =
+ * scatteringCrossSectionNN, e0 = initialEnergyDensity, eta_flat/eta_variance =
+ * rapidityMean/rapidityVariance; e includes the reference's 1e-3 floor.
+ * Reseeds the process-wide rand() state, as the reference does. */
+vh_status vh_ic_mc_glauber_classic(int A, double b, double sigma_nn_mb, double e0, double eta_flat,
+                                   double eta_variance, int nx, int ny, int nz, double dx, double dy, double dz,
+                                   double *e_out, int *n_wounded);
 
 /* input/Tmunu.dat (IC type 11) reader, values through float as the reference
  * reads them (rhic/src/InitialConditions.cpp:41-105); Pi may be NULL.  The

@@ -27,7 +27,7 @@
 //
 // Field numbering: 0..13 ttt ttx tty ttn pitt pitx pity pitn pixx pixy pixn
 // piyy piyn pinn, 14 Pi, 15 e, 16 p, 17..20 ut ux uy un, 21..24 previous u,
-// 25 regFacShear.  Arrays are x-fastest over the interior: i + nx*(j + ny*k).
+// 25 regFacShear, 26 regFacBulk.  Arrays are x-fastest over the interior: i + nx*(j + ny*k).
 // =============================================================================
 #include <cmath>
 #include <cstdio>
@@ -171,7 +171,7 @@ struct Orc {
     std::vector<double> q[15], qS[15], Q[15];
     std::vector<double> e, p, eS, pS;
     std::vector<double> u[4], up[4], uS[4];
-    std::vector<double> regfac;
+    std::vector<double> regfac, regfacb;
     int err = 0;
     long err_cell = -1;
     double e_reg = 0;
@@ -593,6 +593,7 @@ struct Orc {
         if (std::isnan(facShear)) flag(ERR_NAN_REG_SHEAR, s);
         if (P.bulk) {
             Qc[14] *= facBulk;
+            regfacb[s] = facBulk;  // :853
             if (std::isnan(facBulk)) flag(ERR_NAN_REG_BULK, s);
         }
     }
@@ -674,6 +675,7 @@ ORC_API void *orc_create(const orc_params *cp, const double *eos_raw) {
     for (int c = 0; c < 4; ++c) { o->u[c].assign(o->N, 0.0); o->up[c].assign(o->N, 0.0); o->uS[c].assign(o->N, 0.0); }
     o->e.assign(o->N, 0.0); o->p.assign(o->N, 0.0); o->eS.assign(o->N, 0.0); o->pS.assign(o->N, 0.0);
     o->regfac.assign(o->N, 1.0);
+    o->regfacb.assign(o->N, 1.0);
     // rungeKutta2 :884-888 (recomputed per call there; constant here)
     o->e_reg = o->eos.energy_density(P.T_reg_GeV / HBARC);
     return o;
@@ -729,6 +731,7 @@ ORC_API int orc_get(void *h, int field, double *out) {
     else if (field >= 17 && field <= 20) src = &o->u[field - 17];
     else if (field >= 21 && field <= 24) src = &o->up[field - 21];
     else if (field == 25) src = &o->regfac;
+    else if (field == 26 && o->nc == 15) src = &o->regfacb;
     if (!src) return -1;
     std::memcpy(out, src->data(), sizeof(double) * o->N);
     return 0;

@@ -82,3 +82,42 @@ def test_tmunu_round_trip_through_float(tmp_path):
     rows = np.loadtxt(path)
     assert rows.shape == (nx * ny * nz, 20)
     assert rows[1, 2] > rows[0, 2] and rows[nz, 1] > rows[0, 1]
+
+
+def test_classic_glauber_sum_rule_and_participants():
+    """The reference's type-4 profile: a fixed-seed event (the same call gives
+    the same bits), a central Au+Au participant count in the physical range,
+    and the transverse integral of e - 1e-3 equal to e0 N_w 2 pi sigma^2 with
+    the reference's sigma = 0.96 fm (a Gaussian sum sampled at dx << sigma
+    integrates exactly); the longitudinal plateau leaves mid-rapidity at 1 and
+    the tails follow exp(-(|eta| - eta_flat/2)^2 / (2 var))."""
+    n, dx = 161, 0.2
+    e, nw = C.mc_glauber_classic(n, n, 1, dx, dx, 0.1, e0=2.0)
+    e2, nw2 = C.mc_glauber_classic(n, n, 1, dx, dx, 0.1, e0=2.0)
+    assert nw == nw2 and V.bits_equal(e, e2)
+    assert 330 <= nw <= 394, nw
+    integral = float(np.sum(e - 1e-3)) * dx * dx
+    assert abs(integral / (2.0 * nw * 2 * np.pi * 0.96 ** 2) - 1) < 1e-6
+    # no overlap, no participants: the floor only
+    e0, nw0 = C.mc_glauber_classic(21, 21, 1, dx, dx, 0.1, b=40.0)
+    assert nw0 == 0 and np.all(e0 == 1e-3)
+    # longitudinal profile on an odd grid: eta = (k - (nz-1)/2) deta
+    nz, deta = 41, 0.25
+    e3, _ = C.mc_glauber_classic(31, 31, nz, dx, dx, deta, e0=1.0, eta_flat=4.0, eta_variance=1.96)
+    col = e3.reshape(nz, 31, 31)[:, 15, 15] - 1e-3
+    eta = (np.arange(nz) - (nz - 1) // 2) * deta
+    ratio = col / col[(nz - 1) // 2]
+    tail = np.abs(eta) > 2.0
+    assert np.allclose(ratio[~tail], 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(ratio[tail], np.exp(-(np.abs(eta[tail]) - 2.0) ** 2 / (2 * 1.96)), rtol=1e-12)
+
+
+def test_driver_binary_parses_and_exits_without_hydro(tmp_path):
+    import subprocess
+    exe = os.path.join(os.path.dirname(C.LIB_PATH), "cpuvh_run")
+    (tmp_path / "lattice.properties").write_text("numLatticePointsX=5\n")
+    r = subprocess.run([exe, "--config", str(tmp_path), "-o", str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "runHydro = False" in r.stdout and "Hydro took" in r.stdout
+    assert "No configuration file found for initial condition parameters" in r.stderr

@@ -36,7 +36,7 @@ def _alone(r, cfg):
         except C.VhError as ex:  # the state blew up: the loop reports the request as failed
             assert ex.status == C.VH_ERR_NAN
             return C.VH_ERR_NAN, steps, None
-        strikes = strikes + 1 if counts[eid] == 0 else 0
+        strikes += counts[eid] == 0  # never reset, as HydroPlugin.cpp:268 counts
         if strikes >= 3 or steps + cfg["coarse"] > cfg["max_steps"]:
             break
     return C.VH_OK, steps, float(np.cumsum(b.get(eid, "e"))[-1])
