@@ -111,6 +111,10 @@ def lhs(n, bounds, rng):
     return out
 
 
+TILES_OVERRIDE = None  # (2+1D, 3+1D) tile variants of the mixed kernel, when set from the command line
+FACE_MODE = 0  # face semantics of the mixed kernel (vh_batch_set_face_mode)
+
+
 def make_batch_events(table, n_events):
     """Seeded 2+1D boost-invariant MC-Glauber events of the batching workload."""
     import cpuvh_amd as C
@@ -157,7 +161,10 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
     b = C.Batch(cap, **kw)
     ids = [add_event(b, ev) for ev in events]
     if precision == C.PREC_MIXED:
+        if TILES_OVERRIDE is not None:
+            tiles = TILES_OVERRIDE
         b.set_tiles(max(tiles[0], 0), max(tiles[1], 0))
+        b.set_face_mode(FACE_MODE)
     eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
     strikes = {i: 0 for i in ids}
     cells = {i: ev["N"] for i, ev in zip(ids, events)}
@@ -236,31 +243,57 @@ def run_plain_loop(events, table):
                 zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
 
 
-def precision_check(events, table, nsteps):
-    """Mixed form against FP64 on the same events after nsteps steps."""
+def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
+    """Mixed form against FP64 on the same events, at checkpoints and after
+    nsteps steps; beside it the FP64 path's own sensitivity: FP64 started from
+    the initial state rounded to FP32 (the mixed form's stored state at t0)
+    against FP64 from the exact one.  The reference's root solver stops at a
+    relative change of 1e-3 and switches formula at e = 0.1, so differences
+    at the rounding level grow by orders of magnitude over hundreds of steps
+    in any arithmetic that is not bit-identical."""
     import cpuvh_amd as C
+    fields = ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi")
     cap = sum(ev["N"] for ev in events)
-    res = {}
-    runs = {}
-    for prec in (C.PREC_FP64, C.PREC_MIXED):
+    stops = [c for c in checkpoints if c < nsteps] + [nsteps]
+    variants = {"fp64": (C.PREC_FP64, False), "mixed": (C.PREC_MIXED, False), "fp64_f32_ic": (C.PREC_FP64, True)}
+    snaps = {}
+    for name, (prec, round_ic) in variants.items():
         b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
-        ids = [add_event(b, ev) for ev in events]
-        b.steps(nsteps)
-        b.sync()
-        runs[prec] = [{f: b.get(i, f) for f in ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi")} for i in ids]
+        if prec == C.PREC_MIXED:
+            b.set_face_mode(FACE_MODE)
+        ids = []
+        for ev in events:
+            if round_ic:
+                ev = dict(ev, ic={k: v.astype(np.float32).astype(np.float64) for k, v in ev["ic"].items()})
+            ids.append(add_event(b, ev))
+        done = 0
+        for stop in stops:
+            b.steps(stop - done)
+            b.sync()
+            done = stop
+            snaps[(name, stop)] = [{f: b.get(i, f) for f in fields} for i in ids]
         b.close()
-    for f in ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi"):
-        worst_pt, worst_norm = 0.0, 0.0
-        for r64, rmx in zip(runs[C.PREC_FP64], runs[C.PREC_MIXED]):
-            a, c = r64[f], rmx[f]
-            scale = float(np.max(np.abs(a))) or 1.0
-            worst_norm = max(worst_norm, float(np.max(np.abs(a - c))) / scale)
+
+    def errors(ref, got):
+        res = {}
+        for f in fields:
+            worst_pt, worst_norm = 0.0, 0.0
+            for r, g in zip(ref, got):
+                a, c = r[f], g[f]
+                scale = float(np.max(np.abs(a))) or 1.0
+                worst_norm = max(worst_norm, float(np.max(np.abs(a - c))) / scale)
+                if f in ("e", "ut"):
+                    worst_pt = max(worst_pt, float(np.max(np.abs(a - c) / np.abs(a))))
+            res[f] = {"max_abs_over_max": worst_norm}
             if f in ("e", "ut"):
-                worst_pt = max(worst_pt, float(np.max(np.abs(a - c) / np.abs(a))))
-        res[f] = {"max_abs_over_max": worst_norm}
-        if f in ("e", "ut"):
-            res[f]["max_pointwise_rel"] = worst_pt
-    return {"steps": nsteps, "events": len(events), "fields": res}
+                res[f]["max_pointwise_rel"] = worst_pt
+        return res
+    report = []
+    for stop in stops:
+        report.append({"step": stop, "mixed_vs_fp64": errors(snaps[("fp64", stop)], snaps[("mixed", stop)]),
+                       "fp64_sensitivity": errors(snaps[("fp64", stop)], snaps[("fp64_f32_ic", stop)])})
+    return {"steps": nsteps, "events": len(events), "checkpoints": report,
+            "fields": report[-1]["mixed_vs_fp64"]}
 
 
 PRECISION_3D = dict(nx=201, ny=201, nz=101, dx=0.08, dt=0.02, t0=0.6, steps=600, etabar=0.08, etas_Tk_GeV=0.154,
@@ -405,6 +438,9 @@ def main():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-3d-share", type=float, default=0.0,
                     help="share of 3+1D requests in the serving sweep (default 0: 2+1D requests only)")
+    ap.add_argument("--tiles", default=None,
+                    help="mixed-kernel tile variants 'T2,T3' for the timed batch (counter runs without the autotuner)")
+    ap.add_argument("--face-mode", type=int, default=None, help="face semantics of the mixed kernel (0, 1, 2)")
     ap.add_argument("--precision-3d", action="store_true",
                     help="also run the 201x201x101 precision-form comparison (off by default)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -428,6 +464,11 @@ def main():
         return 2
 
     import cpuvh_amd as C
+    global TILES_OVERRIDE, FACE_MODE
+    if args.face_mode is not None:
+        FACE_MODE = args.face_mode
+    if args.tiles:
+        TILES_OVERRIDE = tuple(int(x) for x in args.tiles.split(","))
     table = C.load_eos_table(os.path.join(REPO, "EoS", "hrg_hotqcd_eos_binary.dat"))
     print("bench: building initial states", file=sys.stderr, flush=True)
     events = make_batch_events(table, args.events)

@@ -81,7 +81,7 @@ struct vh_batch {
     float *fpool = nullptr;
     float *qf[3][15] = {}, *qfl[3][4] = {}, *uf[3][4] = {}, *ef[2] = {}, *pf[2] = {};
     int tile2 = 0, tile3 = 0;
-    int face_mode = 0;
+    int face_mode = 1;  // the reference's face semantics (vh_batch_set_face_mode)
     bool layout_dirty = true;
     int layout_version = 0;
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;

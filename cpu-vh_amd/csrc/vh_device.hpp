@@ -207,27 +207,35 @@ struct RootResult {
     double e;
     int nan;
 };
-template <class QD = QuotExact, class DV = DivExact>
-__device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
+// One Newton step of energyDensityFromConservedVariables
+// (rhic/src/EnergyMomentumTensor.cpp:26-47) from e0; bad is set when the
+// step is NaN (the reference's "f/fp is nan").
+template <class QD, class DV>
+__device__ __forceinline__ double nr_step(const EosDev &E, double e0, double M0, double M, double Pi, DV &dx,
+                                          bool &bad) {
+    double p = eos_pressure<true>(E, e0, dx);
+    double cs2 = eos_cs2<true>(E, e0, dx);
+    double cst2 = QD::q(p, e0);
+    double A = M0 * (1.0 - cst2) + Pi;
+    double B = M0 * (M0 + Pi) - M;
+    double H = (double)QD::sqrt_island(f32_fabs((float)(A * A + 4 * cst2 * B)));
+    double D = QD::q(A - H, 2 * cst2);
+    double f = e0 + D;
+    double fp = 1.0 - QD::q((cs2 - cst2) * (B + D * H - QD::q((cs2 - cst2) * cst2 * D * M0, e0)), cst2 * e0 * H);
+    const double step = QD::q(f, fp);
+    bad = __builtin_isnan(step);
+    return e0 - step;
+}
+
+// the reference's stop test, evaluated in float (:44)
+__device__ __forceinline__ bool nr_converged(double e, double e0) {
+    return (double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e);
+}
+
+// bisection fallback after MAX_ITERS_NR Newton steps (:48-111)
+template <class QD, class DV>
+__device__ __forceinline__ double table_bisect(const EosDev &E, double ePrev, double M0, double M, double Pi) {
     DV dx;
-    double e0 = ePrev;
-    for (int j = 0; j < MAX_ITERS_NR; ++j) {
-        double p = eos_pressure<true>(E, e0, dx);
-        double cs2 = eos_cs2<true>(E, e0, dx);
-        double cst2 = QD::q(p, e0);
-        double A = M0 * (1.0 - cst2) + Pi;
-        double B = M0 * (M0 + Pi) - M;
-        double H = (double)QD::sqrt_island(f32_fabs((float)(A * A + 4 * cst2 * B)));
-        double D = QD::q(A - H, 2 * cst2);
-        double f = e0 + D;
-        double fp = 1.0 - QD::q((cs2 - cst2) * (B + D * H - QD::q((cs2 - cst2) * cst2 * D * M0, e0)), cst2 * e0 * H);
-        const double step = QD::q(f, fp);
-        if (__builtin_isnan(step)) return RootResult{ePrev, NAN_ROOT};
-        double e = e0 - step;
-        if ((double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e)) return RootResult{e, NAN_NONE};
-        e0 = e;
-    }
-    // bisection fallback (:48-111)
     double e_lo = 0.0, e_mid = ePrev, e_hi = 2.0 * (ePrev + 1.0);
     for (int j = 0; j < MAX_ITERS_BI; ++j) {
         double p_lo = eos_pressure<true>(E, e_lo, dx), p_mid = eos_pressure<true>(E, e_mid, dx),
@@ -244,7 +252,7 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
         double D_mid = QD::q(A_mid - H_mid, 2 * cst2_mid);
         double D_hi = QD::q(A_hi - H_hi, 2 * cst2_hi);
         double f_lo = e_lo + D_lo, f_mid = e_mid + D_mid, f_hi = e_hi + D_hi;
-        if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return RootResult{e_mid, NAN_NONE};
+        if ((e_hi - e_lo) < (ACC * e_hi) / 2.0) return e_mid;
         if (f_lo * f_mid < 0.0) {
             e_hi = e_mid;
             e_mid = (e_lo + e_hi) / 2.0;
@@ -253,7 +261,23 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
             e_mid = (e_lo + e_hi) / 2.0;
         }
     }
-    return RootResult{ePrev, NAN_NONE};
+    return ePrev;
+}
+
+// Table-EoS root solve (rhic/src/EnergyMomentumTensor.cpp:22-111): Newton
+// from ePrev, then bisection.
+template <class QD = QuotExact, class DV = DivExact>
+__device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, double M0, double M, double Pi) {
+    DV dx;
+    double e0 = ePrev;
+    for (int j = 0; j < MAX_ITERS_NR; ++j) {
+        bool bad;
+        const double e = nr_step<QD, DV>(E, e0, M0, M, Pi, dx, bad);
+        if (bad) return RootResult{ePrev, NAN_ROOT};
+        if (nr_converged(e, e0)) return RootResult{e, NAN_NONE};
+        e0 = e;
+    }
+    return RootResult{table_bisect<QD, DV>(E, ePrev, M0, M, Pi), NAN_NONE};
 }
 
 // energyDensityFromConservedVariables, rhic/src/EnergyMomentumTensor.cpp:22-115
@@ -274,22 +298,30 @@ struct Prim {
     double e, p, ut, ux, uy, un;
 };
 
-// getInferredVariables, rhic/src/EnergyMomentumTensor.cpp:117-233.
-// ttt..pitn are q[0..7]; Pi = q[14] when bulk is evolved.
-template <bool TABLE, class DV, class QD = QuotExact>
-__device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
-                                        double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
-                                        int &nan, DV &div) {
-    double M0 = ttt - pitt;
-    double M1 = ttx - pitx;
-    double M2 = tty - pity;
-    double M3 = ttn - pitn;
-    double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+// getInferredVariables, rhic/src/EnergyMomentumTensor.cpp:117-233, in two
+// parts: the energy density from the conserved variables (the closed form
+// below ePrev = 0.1, else the root solve), then pressure, floors and
+// velocity.  ttt..pitn are q[0..7]; Pi = q[14] when bulk is evolved.
+struct Moments {
+    double M0, M1, M2, M3, M;
+};
+
+__device__ __forceinline__ Moments moments(double t, double ttt, double ttx, double tty, double ttn, double pitt,
+                                           double pitx, double pity, double pitn) {
+    Moments m;
+    m.M0 = ttt - pitt;
+    m.M1 = ttx - pitx;
+    m.M2 = tty - pity;
+    m.M3 = ttn - pitn;
+    m.M = m.M1 * m.M1 + m.M2 * m.M2 + t * t * m.M3 * m.M3;
+    return m;
+}
+
+template <bool TABLE, class DV, class QD>
+__device__ __forceinline__ Prim recover_finish(const EosDev &E, const Moments &m, double e, double Pi, int &nan,
+                                               DV &div) {
     Prim r;
-    if (ePrev <= 0.1) r.e = M0 - QD::q(M, M0);
-    else  // the FP64 path keeps the division operator inside the root solve's table lookups
-        r.e = energy_from_conserved<TABLE, QD, std::conditional_t<std::is_same_v<QD, QuotFast>, DivRecip, DivExact>>(
-            E, ePrev, M0, M, Pi, nan);
+    r.e = e;
     if (__builtin_isnan(r.e)) nan = NAN_ENERGY;
     r.p = eos_pressure<TABLE>(E, r.e, div);
     if (r.e < 1.e-7) {
@@ -298,11 +330,11 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
     }
     double P = r.p + Pi;
     double Ei = QD::q(1.0, r.e + P);
-    r.ut = (double)QD::sqrt_island(f32_fabs((float)((M0 + P) * Ei)));
+    r.ut = (double)QD::sqrt_island(f32_fabs((float)((m.M0 + P) * Ei)));
     double E2 = QD::q(Ei, r.ut);
-    r.ux = M1 * E2;
-    r.uy = M2 * E2;
-    r.un = M3 * E2;
+    r.ux = m.M1 * E2;
+    r.uy = m.M2 * E2;
+    r.un = m.M3 * E2;
     if (r.ut > 1.0e3) {
         r.ut = 1.0;
         r.ux = 0.0;
@@ -310,6 +342,19 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
         r.un = 0.0;
     }
     return r;
+}
+
+template <bool TABLE, class DV, class QD = QuotExact>
+__device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
+                                        double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
+                                        int &nan, DV &div) {
+    const Moments m = moments(t, ttt, ttx, tty, ttn, pitt, pitx, pity, pitn);
+    double e;
+    if (ePrev <= 0.1) e = m.M0 - QD::q(m.M, m.M0);
+    else  // the FP64 path keeps the division operator inside the root solve's table lookups
+        e = energy_from_conserved<TABLE, QD, std::conditional_t<std::is_same_v<QD, QuotFast>, DivRecip, DivExact>>(
+            E, ePrev, m.M0, m.M, Pi, nan);
+    return recover_finish<TABLE, DV, QD>(E, m, e, Pi, nan, div);
 }
 
 }  // namespace vh

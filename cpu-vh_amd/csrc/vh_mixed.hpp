@@ -316,9 +316,10 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 // (EnergyMomentumTensor.cpp:188-191: e <= 0.1 selects the closed-form
 // branch, otherwise it is the Newton start).  HA is cell a's view, HB cell
 // b's.  face_mode 0: one evaluation, each side started from the e of the
-// cell on that side (HA = HB, only HA stored); 1: both views always; 2: cell
-// a's view, plus cell b's view only where the two cells fall on different
-// sides of the e <= 0.1 branch.
+// cell on that side (HA = HB, only HA stored); 1: both views, the
+// reference's semantics (cell b's view is evaluated only where it can
+// differ); 2: cell a's view, plus cell b's view only where the two cells fall
+// on different sides of the e <= 0.1 branch.
 template <int NC, bool TABLE, bool TWO, int NF>
 __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i, int j,
                                         int k, double t, float (&HA)[NC][NF], float (&HB)[TWO ? NC : 1][TWO ? NF : 1],
@@ -342,7 +343,11 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
     }
     const double ea = ldf(F.e, p1), eb = ldf(F.e, p2);
     const int mode = F.face_mode;
-    const bool viewB = TWO && (mode == 1 || (mode == 2 && ((ea <= 0.1) != (eb <= 0.1))));
+    // cell b's view differs from cell a's only where the two start values
+    // differ and at least one of them leads to the Newton solve (at e <= 0.1
+    // both states take the closed form whatever the start)
+    const bool lowA = ea <= 0.1, lowB = eb <= 0.1;
+    const bool viewB = TWO && (mode == 1 ? (ea != eb && !(lowA && lowB)) : (mode == 2 && lowA != lowB));
     const int nsolve = viewB ? 4 : 2;
     float v[4];
     int nan = NAN_NONE;
@@ -381,17 +386,15 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
 
 // REG_SCHEME_1 (rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:689-860) in
 // FP32 for the mixed form: the same factors (rho_max = 1, xi0 = 0.1, applied
-// only at e <= e_reg), computed from the FP64 accumulators rounded to FP32,
-// scale pi (and Pi) in acc.
+// only at e <= e_reg) scale pi (pv[0..9]) and Pi (pv[10]).
 template <int NC>
-__device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, double (&acc)[NC], const Prim &w,
+__device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, float (&pv)[NC - 4], const Prim &w,
                                             int &nan) {
 #pragma clang fp contract(fast)
     if (w.e > G.e_reg) return;
     const float t = (float)t64, t2 = t * t;
-    const float pitt = (float)acc[4], pitx = (float)acc[5], pity = (float)acc[6], pitn = (float)acc[7],
-                pixx = (float)acc[8], pixy = (float)acc[9], pixn = (float)acc[10], piyy = (float)acc[11],
-                piyn = (float)acc[12], pinn = (float)acc[13];
+    const float pitt = pv[0], pitx = pv[1], pity = pv[2], pitn = pv[3], pixx = pv[4], pixy = pv[5], pixn = pv[6],
+                piyy = pv[7], piyn = pv[8], pinn = pv[9];
     const float ut = (float)w.ut, ux = (float)w.ux, uy = (float)w.uy, un = (float)w.un;
     const float e = (float)w.e, p = (float)w.p;
     const float pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy +
@@ -410,13 +413,13 @@ __device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, double 
                                                                                  __builtin_fmaxf(a5, a6)));
     float facShear = __builtin_fabsf(rho) < 1.e-7f ? 1.0f : tanhf(rho) / rho;
 #pragma unroll
-    for (int n = 4; n < 14; ++n) acc[n] *= (double)facShear;
+    for (int n = 0; n < 10; ++n) pv[n] *= facShear;
     if (__builtin_isnan(facShear)) nan = NAN_REG_SHEAR;
     if constexpr (NC == 15) {
-        const float Pi = (float)acc[NC - 1];
+        const float Pi = pv[10];
         const float rb = __builtin_fabsf(Pi) / p;
         const float facBulk = __builtin_fabsf(rb) < 1.e-7f ? 1.0f : tanhf(rb) / rb;
-        acc[NC - 1] *= (double)facBulk;
+        pv[10] *= facBulk;
         if (__builtin_isnan(facBulk)) nan = NAN_REG_BULK;
     }
 }
@@ -427,22 +430,28 @@ template <int TX, int TY, int TZ, bool DIM3>
 struct Tile {
     static constexpr int NCELL = TX * TY * TZ;
     static constexpr int FX = (TX + 1) * TY * TZ, FY = TX * (TY + 1) * TZ, FZ = DIM3 ? TX * TY * (TZ + 1) : 0;
-    static constexpr int NF = FX + FY + FZ;
+    // faces of the largest direction: one LDS face buffer serves every direction in turn
+    static constexpr int FMAX = FX > FY ? (FX > FZ ? FX : FZ) : (FY > FZ ? FY : FZ);
     static_assert(NCELL <= MX_THREADS, "one cell per thread");
     static_assert(DIM3 || TZ == 1, "2+1D tiles are one cell thick");
 };
 
 // One RK stage for every event of a batch: block b is tile (b - blk_first[b])
-// of event blk_ev[b].  TWO: both views of a face are stored (face modes 1, 2).
+// of event blk_ev[b], one cell per thread.  Direction by direction (x, y,
+// eta) the block evaluates every face of the tile in that direction into one
+// LDS face buffer and each cell adds its (H_before - H_after)/dx; then each
+// cell forms q + dt (S + divergence + S_x + S_y + S_eta) with the FP64
+// accumulation, recovers (e, p, u), regulates and stores.  TWO: both views
+// of a face are stored (face modes 1, 2).
 template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
-__global__ __launch_bounds__(MX_THREADS) void stage_mixed_kernel(const MxFields F, const EosDev E,
+__global__ __launch_bounds__(MX_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void stage_mixed_kernel(const MxFields F, const EosDev E,
                                                                  const EvDesc *__restrict__ ev,
                                                                  const int *__restrict__ blk_ev,
                                                                  const int *__restrict__ blk_first) {
 #pragma clang fp contract(fast)
     using T = Tile<TX, TY, TZ, DIM3>;
-    __shared__ float HA[NC][T::NF];
-    __shared__ float HB[TWO ? NC : 1][TWO ? T::NF : 1];
+    __shared__ float HA[NC][T::FMAX];
+    __shared__ float HB[TWO ? NC : 1][TWO ? T::FMAX : 1];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blk]);
     const int tile = blk - __builtin_amdgcn_readfirstlane(blk_first[blk]);
@@ -454,80 +463,89 @@ __global__ __launch_bounds__(MX_THREADS) void stage_mixed_kernel(const MxFields 
     const double t = F.stage ? st.t2 : st.t1;
     const double t_new = st.t2;
 
-    // faces: x faces, then y faces, then eta faces; face (fx, fy, fz) of
-    // direction d lies between cells b - e_d and b = (i0 + fx, j0 + fy, k0 + fz)
-    for (int f = threadIdx.x; f < T::NF; f += MX_THREADS) {
-        int dir, fl, ex, ey;
-        if (f < T::FX) {
-            dir = 0; fl = f; ex = TX + 1; ey = TY;
-        } else if (f < T::FX + T::FY) {
-            dir = 1; fl = f - T::FX; ex = TX; ey = TY + 1;
-        } else {
-            dir = 2; fl = f - T::FX - T::FY; ex = TX; ey = TY;
-        }
-        const int fx = fl % ex, fy = (fl / ex) % ey, fz = fl / (ex * ey);
-        const int i = i0 + fx, j = j0 + fy, k = k0 + fz;
-        const bool need = (dir == 0 ? i - 1 < G.nx : i < G.nx) && (dir == 1 ? j - 1 < G.ny : j < G.ny) &&
-                          (dir == 2 ? k - 1 < G.nz : k < G.nz);
-        if (need) face_mx<NC, TABLE, TWO, T::NF>(F, G, E, dir, i, j, k, t, HA, HB, f);
-    }
-    __syncthreads();
-
     const int c = threadIdx.x;
     const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
     const int i = i0 + lx, j = j0 + ly, k = k0 + lz;
-    if (c >= T::NCELL || i >= G.nx || j >= G.ny || k >= G.nz) return;
+    const bool live = c < T::NCELL && i < G.nx && j < G.ny && k < G.nz;
+    // the flux divergence of every direction, summed in FP32 in the
+    // reference's order x, y, eta; the only per-cell state live across the
+    // face passes
+    float fdiv[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) fdiv[n] = 0.0f;
+#pragma unroll 1
+    for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
+        // faces of direction d: (fx, fy, fz) lies between cells b - e_d and
+        // b = (i0 + fx, j0 + fy, k0 + fz)
+        const int ex = TX + (d == 0), ey = TY + (d == 1), ez = TZ + (d == 2);
+        for (int f = threadIdx.x; f < ex * ey * ez; f += MX_THREADS) {
+            const int fx = f % ex, fy = (f / ex) % ey, fz = f / (ex * ey);
+            const int fi = i0 + fx, fj = j0 + fy, fk = k0 + fz;
+            const bool need = (d == 0 ? fi - 1 < G.nx : fi < G.nx) && (d == 1 ? fj - 1 < G.ny : fj < G.ny) &&
+                              (d == 2 ? fk - 1 < G.nz : fk < G.nz);
+            if (need) face_mx<NC, TABLE, TWO, T::FMAX>(F, G, E, d, fi, fj, fk, t, HA, HB, f);
+        }
+        __syncthreads();
+        if (live) {
+            const float rd = 1.0f / (float)(d == 0 ? G.dx : (d == 1 ? G.dy : G.dz));
+            const int fb = lx + ex * (ly + ey * lz);  // the face before the cell; the one after is fb + fstep
+            const int fstep = d == 0 ? 1 : (d == 1 ? ex : ex * ey);
+#pragma unroll
+            for (int n = 0; n < NC; ++n) {
+                // the face before the cell is seen from its b side, the face after from its a side
+                const float hb = TWO ? HB[TWO ? n : 0][TWO ? fb : 0] : HA[n][fb];
+                fdiv[n] += (hb - HA[n][fb + fstep]) * rd;
+            }
+        }
+        if (d + 1 < (DIM3 ? 3 : 2)) __syncthreads();  // the next direction reuses the face buffer
+    }
+    if (!live) return;
     const unsigned s = (unsigned)G.off + (unsigned)(i + G.nx * (j + G.ny * k));
-    double acc[NC];
-    float Sd[3][4];
+    // acc = q + dt (S + sum_d (divergence_d + S_d)), formed in FP64; T^{tau mu}
+    // stays FP64 (it is stored as FP32 plus an FP32 low part), pi and Pi are
+    // kept as the FP32 values they are stored as
+    double acc[4];
+    float pv[NC - 4];
     {
-        float q[NC], S[NC];
+        float q[NC], S[NC], Sd[3][4];
 #pragma unroll
         for (int n = 0; n < NC; ++n) q[n] = ldf(F.q[n], s);
         source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, ldf(F.e, s), q, S, Sd);
         const float dt = (float)G.dt;
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            const double qn = n < 4 ? (double)q[n] + (double)ldf(F.ql[n], s) : (double)q[n];
-            acc[n] = qn + (double)(dt * S[n]);
-        }
-    }
-    const float dt = (float)G.dt;
-    const float rd[3] = {1.0f / (float)G.dx, 1.0f / (float)G.dy, 1.0f / (float)G.dz};
-    // faces before/after the cell: x, y, eta blocks of the face list
-    const int fb[3] = {lx + (TX + 1) * (ly + TY * lz), T::FX + lx + TX * (ly + (TY + 1) * lz),
-                       T::FX + T::FY + lx + TX * (ly + TY * lz)};
-    const int fstep[3] = {1, TX, TX * TY};
-#pragma unroll
-    for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
-#pragma unroll
-        for (int n = 0; n < NC; ++n) {
-            // the face before the cell is seen from its b side, the face after from its a side
-            const float hb = TWO ? HB[TWO ? n : 0][TWO ? fb[d] : 0] : HA[n][fb[d]];
-            float r = (hb - HA[n][fb[d] + fstep[d]]) * rd[d];
-            if (n < 4) r += Sd[d][n];
-            acc[n] += (double)(r * dt);
+            float r = fdiv[n];
+            if (n < 4) {
+                r += DIM3 ? (Sd[0][n] + Sd[1][n]) + Sd[2][n] : Sd[0][n] + Sd[1][n];
+                const double qn = (double)q[n] + (double)ldf(F.ql[n], s);
+                acc[n] = (qn + (double)(dt * S[n])) + (double)(dt * r);
+            } else {
+                pv[n - 4] = (float)(((double)q[n] + (double)(dt * S[n])) + (double)(dt * r));
+            }
         }
     }
     if (F.stage == 1) {
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            const double qn = n < 4 ? (double)ldf(F.qn[n], s) + (double)ldf(F.qnl[n], s) : (double)ldf(F.qn[n], s);
-            acc[n] = (acc[n] + qn) * 0.5;
+            if (n < 4) acc[n] = (acc[n] + ((double)ldf(F.qn[n], s) + (double)ldf(F.qnl[n], s))) * 0.5;
+            else pv[n - 4] = (pv[n - 4] + ldf(F.qn[n], s)) * 0.5f;
         }
     }
     int nan = NAN_NONE;
     DivRecip div;
-    const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
-                                                      acc[7], NC == 15 ? acc[NC - 1] : 0.0, (double)ldf(F.e, s), nan,
+    const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t_new, acc[0], acc[1], acc[2], acc[3], (double)pv[0],
+                                                      (double)pv[1], (double)pv[2], (double)pv[3],
+                                                      NC == 15 ? (double)pv[NC - 5] : 0.0, (double)ldf(F.e, s), nan,
                                                       div);
-    regulate_mx<NC>(G, t_new, acc, w, nan);
+    regulate_mx<NC>(G, t_new, pv, w, nan);
 #pragma unroll
-    for (int n = 0; n < NC; ++n) {
+    for (int n = 0; n < 4; ++n) {
         const float hi = (float)acc[n];
         stf(F.qo[n], s, hi);
-        if (n < 4) stf(F.qol[n], s, (float)(acc[n] - (double)hi));
+        stf(F.qol[n], s, (float)(acc[n] - (double)hi));
     }
+#pragma unroll
+    for (int n = 4; n < NC; ++n) stf(F.qo[n], s, pv[n - 4]);
     stf(F.eo, s, (float)w.e);
     stf(F.po, s, (float)w.p);
     stf(F.uo[0], s, (float)w.ut);

@@ -263,11 +263,12 @@ vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d);
 /* Mixed form: how face fluxes are shared.  The reference evaluates each face
  * twice, once per neighbouring cell, each time starting both root solves
  * from that cell's e (EnergyMomentumTensor.cpp:188-191 switches formula at
- * e <= 0.1; otherwise e is the Newton start).  0 (default): one evaluation
- * per face, each side started from its own cell's e; 1: both evaluations,
- * exactly as the reference; 2: one evaluation, from the left cell's e, plus
- * the right cell's evaluation where the two cells sit on different sides of
- * e = 0.1. */
+ * e <= 0.1; otherwise e is the Newton start).  1 (default): both
+ * evaluations, as the reference (the second only where it can differ);
+ * 0: one evaluation per face, each side started from its own cell's e
+ * (faster, but it departs from the reference where the two starts differ);
+ * 2: one evaluation, from the left cell's e, plus the right cell's
+ * evaluation where the two cells sit on different sides of e = 0.1. */
 vh_status vh_batch_set_face_mode(vh_batch *b, int mode);
 /* Duration (ms) of the last vh_batch_steps graph, from HIP events; 0 if none. */
 double vh_batch_last_ms(const vh_batch *b);
