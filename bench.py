@@ -112,7 +112,7 @@ def lhs(n, bounds, rng):
 
 
 TILES_OVERRIDE = None  # (2+1D, 3+1D) tile variants of the mixed kernel, when set from the command line
-FACE_MODE = 0  # face semantics of the mixed kernel (vh_batch_set_face_mode)
+FACE_MODE = None  # face semantics of the mixed kernel when set (vh_batch_set_face_mode); default: the library's
 
 
 def make_batch_events(table, n_events):
@@ -154,6 +154,8 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
         wb = C.Batch(cap, **kw)
         for ev in events:
             add_event(wb, ev)
+        if precision == C.PREC_MIXED and FACE_MODE is not None:
+            wb.set_face_mode(FACE_MODE)
         tiles = wb.autotune(reps=2)
         wb.steps(warmup)
         wb.sync()
@@ -164,7 +166,8 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
         if TILES_OVERRIDE is not None:
             tiles = TILES_OVERRIDE
         b.set_tiles(max(tiles[0], 0), max(tiles[1], 0))
-        b.set_face_mode(FACE_MODE)
+        if FACE_MODE is not None:
+            b.set_face_mode(FACE_MODE)
     eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
     strikes = {i: 0 for i in ids}
     cells = {i: ev["N"] for i, ev in zip(ids, events)}
@@ -256,10 +259,10 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
     cap = sum(ev["N"] for ev in events)
     stops = [c for c in checkpoints if c < nsteps] + [nsteps]
     variants = {"fp64": (C.PREC_FP64, False), "mixed": (C.PREC_MIXED, False), "fp64_f32_ic": (C.PREC_FP64, True)}
-    snaps = {}
+    snaps, cubes = {}, {}
     for name, (prec, round_ic) in variants.items():
         b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
-        if prec == C.PREC_MIXED:
+        if prec == C.PREC_MIXED and FACE_MODE is not None:
             b.set_face_mode(FACE_MODE)
         ids = []
         for ev in events:
@@ -272,6 +275,8 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
             b.sync()
             done = stop
             snaps[(name, stop)] = [{f: b.get(i, f) for f in fields} for i in ids]
+            eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+            cubes[(name, stop)] = [set(b.fo_flag_ids(i, eF).tolist()) for i in ids]
         b.close()
 
     def errors(ref, got):
@@ -290,8 +295,13 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
         return res
     report = []
     for stop in stops:
+        def cube_diff(other):
+            return sum(len(a ^ c) for a, c in zip(cubes[("fp64", stop)], cubes[(other, stop)]))
         report.append({"step": stop, "mixed_vs_fp64": errors(snaps[("fp64", stop)], snaps[("mixed", stop)]),
-                       "fp64_sensitivity": errors(snaps[("fp64", stop)], snaps[("fp64_f32_ic", stop)])})
+                       "fp64_sensitivity": errors(snaps[("fp64", stop)], snaps[("fp64_f32_ic", stop)]),
+                       "fo_cubes_fp64": sum(len(a) for a in cubes[("fp64", stop)]),
+                       "fo_cubes_differing_mixed": cube_diff("mixed"),
+                       "fo_cubes_differing_fp64_f32_ic": cube_diff("fp64_f32_ic")})
     return {"steps": nsteps, "events": len(events), "checkpoints": report,
             "fields": report[-1]["mixed_vs_fp64"]}
 

@@ -259,6 +259,11 @@ __device__ __forceinline__ double table_bisect(const EosDev &E, double ePrev, do
         } else if (f_mid * f_hi < 0.0) {
             e_lo = e_mid;
             e_mid = (e_lo + e_hi) / 2.0;
+        } else {
+            // no sign change (or NaN): the bracket never moves again, so the
+            // reference's remaining iterations all repeat this one and its
+            // loop ends with ePrev -- return that now
+            return ePrev;
         }
     }
     return ePrev;
