@@ -86,6 +86,11 @@ struct QuotExact {
     static __device__ __forceinline__ float sqrt_island(float x) { return (float)__builtin_sqrt((double)x); }
 };
 struct QuotFast {
+    static __device__ __forceinline__ double rcp(double b) {
+        double r = __builtin_amdgcn_rcp(b);
+        r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+        return __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    }
     static __device__ __forceinline__ double q(double a, double b) {
         double r = __builtin_amdgcn_rcp(b);
         r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
@@ -213,6 +218,24 @@ struct RootResult {
 template <class QD, class DV>
 __device__ __forceinline__ double nr_step(const EosDev &E, double e0, double M0, double M, double Pi, DV &dx,
                                           bool &bad) {
+    if constexpr (std::is_same_v<QD, QuotFast>) {
+        // mixed form: the same iteration with four refined reciprocals in
+        // place of five quotients (1/e0, 1/p, 1/H, 1/fp; cst2 e0 H = p H)
+        const double p = eos_pressure<true>(E, e0, dx);
+        const double cs2 = eos_cs2<true>(E, e0, dx);
+        const double r0 = QuotFast::rcp(e0), rp = QuotFast::rcp(p);
+        const double cst2 = p * r0;
+        const double A = M0 * (1.0 - cst2) + Pi;
+        const double B = M0 * (M0 + Pi) - M;
+        const double H = (double)QuotFast::sqrt_island(f32_fabs((float)(A * A + 4 * cst2 * B)));
+        const double D = (A - H) * (0.5 * e0 * rp);
+        const double f = e0 + D;
+        const double c = cs2 - cst2;
+        const double fp = 1.0 - c * (B + D * H - c * cst2 * D * M0 * r0) * (rp * QuotFast::rcp(H));
+        const double step = f * QuotFast::rcp(fp);
+        bad = __builtin_isnan(step);
+        return e0 - step;
+    }
     double p = eos_pressure<true>(E, e0, dx);
     double cs2 = eos_cs2<true>(E, e0, dx);
     double cst2 = QD::q(p, e0);
