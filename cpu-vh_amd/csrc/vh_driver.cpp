@@ -2,6 +2,7 @@
 //
 // Usage (the reference's options, rhic/src/CommandLineArguments.cpp:17-23):
 //   cpuvh_run --hydro --config <dir> --output <dir>       (short: -h -c -o; -t is accepted)
+//   cpuvh_run --config <dir> --print-config              (the parsed parameters, one per line)
 //
 // Reads the reference's rhic-conf files with the reference's reader semantics
 // (rhic/src/LatticeParameters.cpp:68-157, rhic/src/InitialConditionParameters.cpp:75-150,
@@ -424,12 +425,13 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
 }  // namespace
 
 int main(int argc, char **argv) {
-    bool run_hydro = false, run_test = false;
+    bool run_hydro = false, run_test = false, print_config = false;
     const char *config = nullptr, *output = nullptr;
     static const option opts[] = {{"test", optional_argument, nullptr, 't'},
                                   {"hydro", optional_argument, nullptr, 'h'},
                                   {"output", required_argument, nullptr, 'o'},
                                   {"config", required_argument, nullptr, 'c'},
+                                  {"print-config", no_argument, nullptr, 'p'},
                                   {nullptr, 0, nullptr, 0}};
     int c;
     while ((c = getopt_long(argc, argv, "t::h::o:c:", opts, nullptr)) != -1) {
@@ -437,6 +439,7 @@ int main(int argc, char **argv) {
         else if (c == 'h') run_hydro = true;
         else if (c == 'o') output = optarg;
         else if (c == 'c') config = optarg;
+        else if (c == 'p') print_config = true;
         else return 64;
     }
     char cwd[4096];
@@ -451,6 +454,15 @@ int main(int argc, char **argv) {
     HydroPar H;
     Extra X;
     read_config(config ? config : ".", L, I, H, X);
+    if (print_config) {
+        std::printf("lattice %d %d %d %d %.17g %.17g %.17g %.17g %d\n", L.nx, L.ny, L.nz, L.nt, L.dx, L.dy, L.dz, L.dt,
+                    L.coarse);
+        std::printf("ic %d %d %d %.17g %.17g %.17g %.17g %.17g %.17g %d\n", I.type, I.n_source_files, I.A, I.e0,
+                    I.sigma_nn_mb, I.b, I.f_binary, I.eta_variance, I.eta_flat, I.source_type);
+        std::printf("hydro %.17g %.17g %.17g %d %.17g %d\n", H.t0, H.etabar, H.T_F_GeV, H.pi_navier_stokes, H.T_reg_GeV,
+                    H.do_freezeout);
+        std::printf("gpu %d %d %s %d %d\n", X.bulk, X.eos_table, X.eos_file.c_str(), X.device, X.snapshot_every);
+    }
     const auto a = std::chrono::steady_clock::now();
     if (run_hydro) {
         run(L, I, H, X, root, output ? output : ".");

@@ -121,3 +121,40 @@ def test_driver_binary_parses_and_exits_without_hydro(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "runHydro = False" in r.stdout and "Hydro took" in r.stdout
     assert "No configuration file found for initial condition parameters" in r.stderr
+
+
+def _print_config(conf_dir):
+    import subprocess
+    exe = os.path.join(os.path.dirname(C.LIB_PATH), "cpuvh_run")
+    r = subprocess.run([exe, "--config", str(conf_dir), "--print-config"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.split()[:1] in (["lattice"], ["ic"], ["hydro"], ["gpu"])}
+
+
+def test_driver_reads_reference_rhic_conf():
+    """The reference's own rhic-conf files (tests/golden/rhic-conf, copied as
+    data): values by position, keys ignored -- the misspelt
+    coarseProperTimeFOFactorFO key still gives the FO factor."""
+    cfg = _print_config(os.path.join(V.REPO, "tests", "golden", "rhic-conf"))
+    lat = cfg["lattice"]
+    assert [int(x) for x in lat[:4]] == [101, 101, 1, 10000]
+    assert [float(x) for x in lat[4:8]] == [0.2, 0.2, 0.1, 0.05] and int(lat[8]) == 2
+    ic = cfg["ic"]
+    assert [int(ic[0]), int(ic[1]), int(ic[2])] == [2, 0, 208]
+    assert [float(x) for x in ic[3:9]] == [70.0, 62.0, 0.0, 1.0, 0.16, 5.9] and int(ic[9]) == 0
+    hy = cfg["hydro"]
+    assert [float(hy[0]), float(hy[1]), float(hy[2]), int(hy[3]), float(hy[4]), int(hy[5])] == [0.5, 0.2, 0.155, 0,
+                                                                                                  0.1, 1]
+    assert cfg["gpu"] == ["0", "0", "EoS/hrg_hotqcd_eos_binary.dat", "0", "100"]
+
+
+def test_driver_reader_short_file_and_spacing(tmp_path):
+    """Whitespace anywhere is dropped; once a file runs out, the last line's
+    value repeats (std::getline leaves the line buffer as it was); a missing
+    file leaves zeros; gpu.properties is keyed."""
+    (tmp_path / "lattice.properties").write_text("numLatticePointsX=7\nnumLatticePointsY = 9\n numLatticePointsRapidity=3")
+    (tmp_path / "gpu.properties").write_text("# compile-time options\nbulk = 1\neos=table\nsnapshotEvery=25\n")
+    cfg = _print_config(tmp_path)
+    assert cfg["lattice"] == ["7", "9", "3", "3", "3", "3", "3", "3", "3"]
+    assert [float(x) for x in cfg["hydro"]] == [0.0] * 6
+    assert cfg["gpu"][:2] == ["1", "1"] and cfg["gpu"][4] == "25"
