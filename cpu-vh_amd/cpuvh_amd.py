@@ -31,7 +31,8 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
            "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
-           "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic"]
+           "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
+           "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube"]
 
 
 class VhConfig(ctypes.Structure):
@@ -137,6 +138,9 @@ def _lib():
                                        D, D, D, P, ctypes.POINTER(GlauberInfo)]
         L.vh_ic_mc_glauber_classic.argtypes = [ctypes.c_int, D, D, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                D, D, D, P, ctypes.POINTER(ctypes.c_int)]
+        L.vh_surface_square.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.vh_surface_cube.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.vh_surface_hypercube.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
         L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
         L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
@@ -457,6 +461,21 @@ def mc_glauber_classic(nx, ny, nz, dx, dy, dz, A=197, b=0.0, sigma_nn_mb=42.0, e
     if st != VH_OK:
         raise VhError(st, "vh_ic_mc_glauber_classic")
     return e, nw.value
+
+
+def surface_elements(corners, E0, d):
+    """Freezeout surface elements of one square (4 corner values), cube (8) or
+    hypercube (16): an array of rows (centroid..., normal...)."""
+    corners = np.ascontiguousarray(corners, dtype=np.float64).ravel()
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    dim = {4: 2, 8: 3, 16: 4}[corners.size]
+    out = np.empty(16 * 2 * dim)
+    n = ctypes.c_int(0)
+    name = {2: "vh_surface_square", 3: "vh_surface_cube", 4: "vh_surface_hypercube"}[dim]
+    st = getattr(_lib(), name)(_ptr(corners), E0, _ptr(d), _ptr(out), 16, ctypes.byref(n))
+    if st != VH_OK:
+        raise VhError(st, name)
+    return out[:2 * dim * n.value].reshape(n.value, 2 * dim)
 
 
 def read_tmunu(path, nx, ny, nz):

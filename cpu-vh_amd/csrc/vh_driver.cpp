@@ -34,10 +34,14 @@
 //
 // Output: the snapshot files of rhic/src/HydroPlugin.cpp:40-72 in the format
 // of rhic/src/FileIO.cpp:14-59 (<name>_<t %.3f>.dat, "x y z value" lines, x
-// fastest).  In place of Cornelius's surface.dat the driver writes
-// surface_cubes.dat: one line per hypercube that Cornelius would cut
-// (tau x y eta of the cube's low corner as FreezeOut.cpp:228-231 places it, then ix
-// iy iz).
+// fastest), and surface.dat as freezeout/FreezeOut.cpp:200-361 writes it: for
+// every cube (2+1D) or hypercube (3+1D) between the previous and the current
+// coarse step that the surface cuts, in the reference's cube order (x, then
+// y, then eta), each surface element's position, dsigma_mu = t * normal, and
+// u^x u^y u^eta e T P pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi linearly
+// interpolated at its centroid — 20 columns, default stream formatting.  The
+// GPU flags the cut cubes (vh_fo_flag); the elements come from
+// vh_surface_cube / vh_surface_hypercube.
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -46,6 +50,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <getopt.h>
 #include <string>
 #include <unistd.h>
@@ -294,6 +299,105 @@ void snapshot(vh_handle *h, double t, const std::string &dir, const Lattice &L, 
         for (const F &f : bulkf) one(f);
 }
 
+// FreezeOut.cpp:22-56: multilinear interpolation inside the cell, the
+// terms summed in the reference's order.  v[t][x][y][z] corner values.
+double interp3(double x0, double x1, double x2, const double (&a)[2][2][2]) {
+    double r = 0;
+    r = ((1 - x0) * (1 - x1) * (1 - x2) * a[0][0][0]) + ((x0) * (1 - x1) * (1 - x2) * a[1][0][0]) +
+        ((1 - x0) * (x1) * (1 - x2) * a[0][1][0]) + ((1 - x0) * (1 - x1) * (x2) * a[0][0][1]) +
+        ((x0) * (x1) * (1 - x2) * a[1][1][0]) + ((x0) * (1 - x1) * (x2) * a[1][0][1]) +
+        ((1 - x0) * (x1) * (x2) * a[0][1][1]) + ((x0) * (x1) * (x2) * a[1][1][1]);
+    return r;
+}
+double interp4(double x0, double x1, double x2, double x3, const double (&a)[2][2][2][2]) {
+    double r = 0;
+    r = ((1 - x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[0][0][0][0]) +
+        ((x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[1][0][0][0]) +
+        ((1 - x0) * (x1) * (1 - x2) * (1 - x3) * a[0][1][0][0]) +
+        ((1 - x0) * (1 - x1) * (x2) * (1 - x3) * a[0][0][1][0]) +
+        ((1 - x0) * (1 - x1) * (1 - x2) * (x3) * a[0][0][0][1]) + ((x0) * (x1) * (1 - x2) * (1 - x3) * a[1][1][0][0]) +
+        ((x0) * (1 - x1) * (x2) * (1 - x3) * a[1][0][1][0]) + ((x0) * (1 - x1) * (1 - x2) * (x3) * a[1][0][0][1]) +
+        ((1 - x0) * (x1) * (x2) * (1 - x3) * a[0][1][1][0]) + ((1 - x0) * (x1) * (1 - x2) * (x3) * a[0][1][0][1]) +
+        ((1 - x0) * (1 - x1) * (x2) * (x3) * a[0][0][1][1]) + ((x0) * (x1) * (x2) * (1 - x3) * a[1][1][1][0]) +
+        ((x0) * (x1) * (1 - x2) * (x3) * a[1][1][0][1]) + ((x0) * (1 - x1) * (x2) * (x3) * a[1][0][1][1]) +
+        ((1 - x0) * (x1) * (x2) * (x3) * a[0][1][1][1]) + ((x0) * (x1) * (x2) * (x3) * a[1][1][1][1]);
+    return r;
+}
+
+// The hydro variables surface.dat interpolates, in the reference's order
+// (FreezeOut.cpp:57-97): u^x u^y u^eta e pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi.
+constexpr int FO_VARS = 10;
+constexpr int FO_FIELD[FO_VARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
+
+// callFOFinder2p1D / callFOFinder3p1D (FreezeOut.cpp:200-361) for the cut
+// cubes ids (x-fastest cube index, increasing): visited in the reference's
+// order (x outer, eta inner), each surface element written as one line.
+void write_surface(std::ofstream &os, std::vector<int64_t> &ids, int64_t nf, const std::vector<double> (&sl)[2][FO_VARS],
+                   int dim, const Lattice &L, const double *spacing, double cell_tau, double t, double eF,
+                   const vh_handle *h) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int64_t cx = nx - 1, cy = ny - 1;
+    auto key = [&](int64_t c) {  // (ix, iy, iz) lexicographic
+        const int64_t ix = c % cx, iy = (c / cx) % cy, iz = c / (cx * cy);
+        return (ix * cy + iy) * (int64_t)std::max(nz - 1, 1) + iz;
+    };
+    std::sort(ids.begin(), ids.begin() + nf, [&](int64_t a, int64_t b) { return key(a) < key(b); });
+    const size_t sxy = (size_t)nx * ny;
+    double el[16 * 8];
+    for (int64_t m = 0; m < nf; ++m) {
+        const int64_t c = ids[m];
+        const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
+        auto at = [&](int x, int y, int z) { return (size_t)(ix + x) + (size_t)nx * (iy + y) + sxy * (iz + z); };
+        double corners[16];
+        const int nzc = dim == 4 ? 2 : 1;
+        for (int it = 0; it < 2; ++it)
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y)
+                    for (int z = 0; z < nzc; ++z)
+                        corners[((it * 2 + x) * 2 + y) * nzc + z] = sl[it][3][at(x, y, z)];
+        int ne = 0;
+        const vh_status st = dim == 4 ? vh_surface_hypercube(corners, eF, spacing, el, 16, &ne)
+                                      : vh_surface_cube(corners, eF, spacing, el, 16, &ne);
+        if (st != VH_OK) {
+            std::printf("freezeout surface: %s at cube (%d, %d, %d)\n", vh_status_string(st), ix, iy, iz);
+            std::exit(-1);
+        }
+        const double cell_x = (double)ix * L.dx - (((double)(nx - 1)) / 2.0 * L.dx);
+        const double cell_y = (double)iy * L.dy - (((double)(ny - 1)) / 2.0 * L.dy);
+        const double cell_z = dim == 4 ? (double)iz * L.dz - (((double)(nz - 1)) / 2.0 * L.dz) : 0.0;
+        for (int e = 0; e < ne; ++e) {
+            const double *cen = el + 2 * dim * e, *nor = cen + dim;
+            double frac[4] = {0, 0, 0, 0}, var[FO_VARS];
+            for (int k = 0; k < dim; ++k) frac[k] = cen[k] / spacing[k];
+            for (int v = 0; v < FO_VARS; ++v) {
+                if (dim == 4) {
+                    double a[2][2][2][2];
+                    for (int it = 0; it < 2; ++it)
+                        for (int x = 0; x < 2; ++x)
+                            for (int y = 0; y < 2; ++y)
+                                for (int z = 0; z < 2; ++z) a[it][x][y][z] = sl[it][v][at(x, y, z)];
+                    var[v] = interp4(frac[0], frac[1], frac[2], frac[3], a);
+                } else {
+                    double a[2][2][2];
+                    for (int it = 0; it < 2; ++it)
+                        for (int x = 0; x < 2; ++x)
+                            for (int y = 0; y < 2; ++y) a[it][x][y] = sl[it][v][at(x, y, 0)];
+                    var[v] = interp3(frac[0], frac[1], frac[2], a);
+                }
+            }
+            const double tau = cen[0] + cell_tau, x = cen[1] + cell_x, y = cen[2] + cell_y;
+            const double eta = dim == 4 ? cen[3] + cell_z : 0.0;
+            const double ds0 = t * nor[0], ds1 = t * nor[1], ds2 = t * nor[2], ds3 = dim == 4 ? t * nor[3] : 0.0;
+            const double eps = var[3];
+            const double T = vh_eos_temperature(h, eps), P = vh_eos_pressure(h, eps);
+            os << tau << " " << x << " " << y << " " << eta << " " << ds0 << " " << ds1 << " " << ds2 << " " << ds3
+               << " " << var[0] << " " << var[1] << " " << var[2] << " " << eps << " " << T << " " << P << " "
+               << var[4] << " " << var[5] << " " << var[6] << " " << var[7] << " " << var[8] << " " << var[9]
+               << std::endl;
+        }
+    }
+}
+
 void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X, const std::string &root,
          const std::string &out) {
     std::vector<double> table;
@@ -337,9 +441,9 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     if (nx > 1 && ny > 1 && nz > 1) dim = 4;
     else if (nx > 1 && ny > 1 && nz < 2) dim = 3;
     else std::printf("simulation is not in 3+1D or 2+1D; freezeout finder will not work!\n");
-    FILE *cubes = std::fopen((out + "/surface_cubes.dat").c_str(), "w");
-    if (!cubes) {
-        std::printf("cannot write %s/surface_cubes.dat\n", out.c_str());
+    std::ofstream surface(out + "/surface.dat");
+    if (!surface) {
+        std::printf("cannot write %s/surface.dat\n", out.c_str());
         std::exit(-1);
     }
 
@@ -352,6 +456,13 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     if (X.bulk) check("vh_download", vh_download(h, 26, buf.data()));
     const size_t ncubes = (size_t)std::max(nx - 1, 1) * std::max(ny - 1, 1) * std::max(nz - 1, 1);
     std::vector<int64_t> ids(ncubes);
+    // the previous and current coarse slices of the interpolated variables
+    // (the previous one starts at zero, HydroPlugin.cpp:159-170)
+    std::vector<double> slice[2][FO_VARS];
+    if (H.do_freezeout && dim)
+        for (int s2 = 0; s2 < 2; ++s2)
+            for (int v = 0; v < FO_VARS; ++v) slice[s2][v].assign(N, 0.0);
+    const double spacing[4] = {dt * coarse, L.dx, L.dy, L.dz};  // HydroPlugin.cpp:130-145
     const int sc = nx / 2, sj = ny / 2, sk = nz / 2;  // the reference's centre cell (HydroPlugin.cpp:211-214)
     const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
 
@@ -383,19 +494,15 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
         }
         if (fo) {
             if (H.do_freezeout && dim) {
-                int64_t nf = 0;
-                vh_status st = vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf);
-                check_sync(h, st);
-                const double tau = t0 + (double)n * dt;  // FreezeOut.cpp:228
-                for (int64_t m = 0; m < nf; ++m) {
-                    const int64_t c = ids[m];
-                    const int ix = (int)(c % (nx - 1)), iy = (int)((c / (nx - 1)) % (ny - 1));
-                    const int iz = dim == 4 ? (int)(c / ((int64_t)(nx - 1) * (ny - 1))) : 0;
-                    const double x = (double)ix * L.dx - ((double)(nx - 1)) / 2.0 * L.dx;
-                    const double y = (double)iy * L.dy - ((double)(ny - 1)) / 2.0 * L.dy;
-                    const double z = dim == 4 ? (double)iz * L.dz - ((double)(nz - 1)) / 2.0 * L.dz : 0.0;
-                    std::fprintf(cubes, "%g %g %g %g %d %d %d\n", tau, x, y, z, ix, iy, iz);
+                // swapAndSetHydroVariables (FreezeOut.cpp:57-97), then the cut cubes
+                for (int v = 0; v < FO_VARS; ++v) {
+                    std::swap(slice[0][v], slice[1][v]);
+                    if (FO_FIELD[v] == 14 && !X.bulk) std::fill(slice[1][v].begin(), slice[1][v].end(), 0.0);
+                    else check_sync(h, vh_download(h, FO_FIELD[v], slice[1][v].data()));
                 }
+                int64_t nf = 0;
+                check_sync(h, vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf));
+                write_surface(surface, ids, nf, slice, dim, L, spacing, t0 + (double)n * dt, t, eF, h);
             }
             int64_t above = 0;
             check_sync(h, vh_count_above(h, eF, &above));
@@ -418,7 +525,7 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     flush();
     std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
     std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
-    std::fclose(cubes);
+    surface.close();
     check("vh_destroy", vh_destroy(h));
 }
 

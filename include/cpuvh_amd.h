@@ -167,6 +167,25 @@ typedef struct {
 vh_status vh_ic_mc_glauber(const vh_glauber_params *gp, int nx, int ny, int nz, double dx, double dy, double dz,
                            double *e_out, vh_glauber_info *info);
 
+/* ---- freezeout surface elements (host side; no device needed) ----------
+ * The constant-value surface inside one lattice cell, built as the
+ * reference's surface finder Cornelius 1.3 builds it
+ * (freezeout/cornelius-c++-1.3/cornelius.cpp: Cornelius::find_surface_2d
+ * :2127-2151, find_surface_3d :2161-2246, find_surface_4d :2248-2287).
+ * corners: 4, 8 or 16 values, the corner at (i d0, j d1, k d2, l d3) at index
+ * ((i*2 + j)*2 + k)*2 + l (as many indices as dimensions); E0 the level; d
+ * the cell's edge lengths.  Each element is its centroid (relative to corner
+ * 0) followed by its normal vector, which points to the side below E0 and has
+ * the element's length, area or volume as its norm: 4, 6 or 8 doubles per
+ * element.  VH_ERR_CAPACITY if more than cap elements (*n_elements still
+ * set). */
+vh_status vh_surface_square(const double *corners, double E0, const double *d, double *elements, int cap,
+                            int *n_elements);
+vh_status vh_surface_cube(const double *corners, double E0, const double *d, double *elements, int cap,
+                          int *n_elements);
+vh_status vh_surface_hypercube(const double *corners, double E0, const double *d, double *elements, int cap,
+                               int *n_elements);
+
 /* The reference's own IC type 4 (rhic/src/InitialConditions.cpp:1107-1140,
  * rhic/src/MonteCarloGlauberModel.cpp:18-111, rhic/src/GlauberModel.cpp:19-24):
  * the C library's rand() seeded with its fixed seed 1328398221, so the same

@@ -1,9 +1,12 @@
 """The command-line driver (cpu-vh_amd/cpuvh_run) against the oracle stepped
 through the reference's run() loop (rhic/src/HydroPlugin.cpp:74-321): same
 initial state from the same rhic-conf files, the same snapshot files byte for
-byte (rhic/src/FileIO.cpp:14-59 format), the same freezeout cubes at every
-coarse step (the previous slice starts at zero) and the same stopping step
-(nt+1 steps at most; the third check with no cell above e_F ends the run)."""
+byte (rhic/src/FileIO.cpp:14-59 format), the same surface.dat (every coarse
+step; the previous slice starts at zero; freezeout/FreezeOut.cpp:200-361: the
+cut cubes in the reference's order, elements from the surface finder pinned
+by the Cornelius fixtures in test_surface.py, the ten hydro variables
+interpolated at each centroid) and the same stopping step (nt+1 steps at
+most; the third check with no cell above e_F ends the run)."""
 import os
 import re
 import subprocess
@@ -78,7 +81,74 @@ def _snapshot_text(v, nx, ny, nz, dx):
     return "".join(lines)
 
 
-def _oracle_loop(lat, hyd, bulk, table, ic, out):
+def _interp(fr, a):
+    """FreezeOut.cpp:22-56, the terms in the reference's order; a[t][x][y][z]."""
+    if len(fr) == 3:
+        x0, x1, x2 = fr
+        return (((1 - x0) * (1 - x1) * (1 - x2) * a[0][0][0]) + ((x0) * (1 - x1) * (1 - x2) * a[1][0][0])
+                + ((1 - x0) * (x1) * (1 - x2) * a[0][1][0]) + ((1 - x0) * (1 - x1) * (x2) * a[0][0][1])
+                + ((x0) * (x1) * (1 - x2) * a[1][1][0]) + ((x0) * (1 - x1) * (x2) * a[1][0][1])
+                + ((1 - x0) * (x1) * (x2) * a[0][1][1]) + ((x0) * (x1) * (x2) * a[1][1][1]))
+    x0, x1, x2, x3 = fr
+    terms = [((1 - x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[0][0][0][0]),
+             ((x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[1][0][0][0]),
+             ((1 - x0) * (x1) * (1 - x2) * (1 - x3) * a[0][1][0][0]),
+             ((1 - x0) * (1 - x1) * (x2) * (1 - x3) * a[0][0][1][0]),
+             ((1 - x0) * (1 - x1) * (1 - x2) * (x3) * a[0][0][0][1]),
+             ((x0) * (x1) * (1 - x2) * (1 - x3) * a[1][1][0][0]),
+             ((x0) * (1 - x1) * (x2) * (1 - x3) * a[1][0][1][0]),
+             ((x0) * (1 - x1) * (1 - x2) * (x3) * a[1][0][0][1]),
+             ((1 - x0) * (x1) * (x2) * (1 - x3) * a[0][1][1][0]),
+             ((1 - x0) * (x1) * (1 - x2) * (x3) * a[0][1][0][1]),
+             ((1 - x0) * (1 - x1) * (x2) * (x3) * a[0][0][1][1]),
+             ((x0) * (x1) * (x2) * (1 - x3) * a[1][1][1][0]),
+             ((x0) * (x1) * (1 - x2) * (x3) * a[1][1][0][1]),
+             ((x0) * (1 - x1) * (x2) * (x3) * a[1][0][1][1]),
+             ((1 - x0) * (x1) * (x2) * (x3) * a[0][1][1][1]),
+             ((x0) * (x1) * (x2) * (x3) * a[1][1][1][1])]
+    r = terms[0]
+    for v in terms[1:]:
+        r = r + v
+    return r
+
+
+FO_FIELDS = ["ux", "uy", "un", "e", "pixx", "pixy", "pixn", "piyy", "piyn", "Pi"]
+
+
+def _surface_lines(ids, slices, dim, nx, ny, nz, dx, spacing, cell_tau, t, eF, L, eos):
+    """callFOFinder2p1D/3p1D (FreezeOut.cpp:200-361) for the cut cubes, in the
+    reference's order (x outer, eta inner)."""
+    cx, cy = nx - 1, ny - 1
+    cubes = sorted((c % cx, (c // cx) % cy, c // (cx * cy) if dim == 4 else 0) for c in ids.tolist())
+    nzc = 2 if dim == 4 else 1
+    lines = []
+    for ix, iy, iz in cubes:
+        def at(x, y, z):
+            return (ix + x) + nx * ((iy + y) + ny * (iz + z))
+        corners = [slices[it][3][at(x, y, z)] for it in range(2) for x in range(2) for y in range(2)
+                   for z in range(nzc)]
+        cell = [ix * dx - ((nx - 1) / 2.0 * dx), iy * dx - ((ny - 1) / 2.0 * dx),
+                iz * dx - ((nz - 1) / 2.0 * dx) if dim == 4 else 0.0]
+        for row in C.surface_elements(np.array(corners), eF, spacing):
+            cen, nor = row[:dim], row[dim:]
+            fr = [cen[k] / spacing[k] for k in range(dim)]
+            var = []
+            for v in range(10):
+                if dim == 4:
+                    a = [[[[slices[it][v][at(x, y, z)] for z in range(2)] for y in range(2)] for x in range(2)]
+                         for it in range(2)]
+                else:
+                    a = [[[slices[it][v][at(x, y, 0)] for y in range(2)] for x in range(2)] for it in range(2)]
+                var.append(_interp(fr, a))
+            pos = [cen[0] + cell_tau, cen[1] + cell[0], cen[2] + cell[1], cen[3] + cell[2] if dim == 4 else 0.0]
+            ds = [t * nor[k] for k in range(dim)] + ([0.0] if dim == 3 else [])
+            eps = var[3]
+            vals = pos + ds + var[0:3] + [eps, L.orc_eos_temperature(eos, eps), L.orc_eos_pressure(eos, eps)] + var[4:]
+            lines.append(" ".join("%g" % x for x in vals) + "\n")
+    return lines
+
+
+def _oracle_loop(lat, hyd, bulk, table, ic):
     """The reference's run() loop with the oracle as the step."""
     nx, ny, nz, nt, dx, dt, coarse = (lat[k] for k in ("nx", "ny", "nz", "nt", "dx", "dt", "coarse"))
     t0 = hyd["t0"]
@@ -88,29 +158,23 @@ def _oracle_loop(lat, hyd, bulk, table, ic, out):
     L = V.oracle_lib()
     eos = L.orc_eos_create(None if table is None else table.ctypes.data)
     eF = L.orc_eos_energy_density(eos, hyd["TF"] / 0.197326938)
-    L.orc_eos_destroy(eos)
     dim = 4 if (nx > 1 and ny > 1 and nz > 1) else (3 if (nx > 1 and ny > 1) else 0)
-    cur = np.zeros(nx * ny * nz)
-    cubes, files = [], {}
+    N = nx * ny * nz
+    slices = [[np.zeros(N) for _ in FO_FIELDS], [np.zeros(N) for _ in FO_FIELDS]]
+    spacing = np.array([dt * coarse, dx, dx, dx][:dim])
+    surface, files = [], {}
     t, zero, nsteps = t0, 0, 0
     for n in range(1, nt + 2):
         if (n - 1) % lat["snap"] == 0:
             for name, f in SHEAR + (BULK if bulk else []):
-                v = np.zeros(nx * ny * nz) if (n == 1 and f in (25, 26)) else o.get(f)
+                v = np.zeros(N) if (n == 1 and f in (25, 26)) else o.get(f)
                 files["%s_%.3f.dat" % (name, t)] = _snapshot_text(v, nx, ny, nz, dx)
         if n % coarse == 0:
             e = o.get("e")
             if dim:
-                ids = V.oracle_fo_flag(cur, e, nx, ny, nz, eF)
-                cur = e
-                tau = t0 + n * dt
-                for c in ids.tolist():
-                    ix, iy = c % (nx - 1), (c // (nx - 1)) % (ny - 1)
-                    iz = c // ((nx - 1) * (ny - 1)) if dim == 4 else 0
-                    x = ix * dx - (nx - 1) / 2.0 * dx
-                    y = iy * dx - (ny - 1) / 2.0 * dx
-                    z = iz * dx - (nz - 1) / 2.0 * dx if dim == 4 else 0.0
-                    cubes.append("%g %g %g %g %d %d %d\n" % (tau, x, y, z, ix, iy, iz))
+                slices = [slices[1], [o.get(f) if (f != "Pi" or bulk) else np.zeros(N) for f in FO_FIELDS]]
+                ids = V.oracle_fo_flag(slices[0][3], e, nx, ny, nz, eF)
+                surface.extend(_surface_lines(ids, slices, dim, nx, ny, nz, dx, spacing, t0 + n * dt, t, eF, L, eos))
             if int(np.count_nonzero(e > eF)) == 0:
                 zero += 1
             if zero >= 3:
@@ -118,7 +182,8 @@ def _oracle_loop(lat, hyd, bulk, table, ic, out):
         assert o.step(t) == 0
         nsteps += 1
         t = t0 + n * dt
-    files["surface_cubes.dat"] = "".join(cubes)
+    L.orc_eos_destroy(eos)
+    files["surface.dat"] = "".join(surface)
     return files, nsteps
 
 
@@ -137,7 +202,7 @@ def _run_case(tmp_path, lat, hyd, gpu, ic_fields, bulk, table):
     ic = _read_tmunu_f32(str(tmp_path / "input" / "Tmunu.dat"), nx, ny, nz)
     if not bulk:
         ic.pop("Pi")
-    want, want_steps = _oracle_loop(lat, hyd, bulk, table, ic, out)
+    want, want_steps = _oracle_loop(lat, hyd, bulk, table, ic)
     assert steps == want_steps, (steps, want_steps)
     got = sorted(os.listdir(out))
     assert got == sorted(want), (got, sorted(want))
