@@ -48,12 +48,21 @@ namespace {
 template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
 hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, const int *blk_ev,
                      const int *blk_first, int nblocks, hipStream_t st) {
-    if (F.face_mode == 0)
-        hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, false>), dim3(nblocks),
-                           dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
-    else
-        hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, true>), dim3(nblocks),
-                           dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+    if constexpr (DIM3) {
+        if (F.face_mode == 0)
+            hipLaunchKernelGGL((mx::stage_mixed_kernel_3d<NC, TABLE, DIM3, TX, TY, TZ, false>), dim3(nblocks),
+                               dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+        else
+            hipLaunchKernelGGL((mx::stage_mixed_kernel_3d<NC, TABLE, DIM3, TX, TY, TZ, true>), dim3(nblocks),
+                               dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+    } else {
+        if (F.face_mode == 0)
+            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, false>), dim3(nblocks),
+                               dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+        else
+            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, true>), dim3(nblocks),
+                               dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
+    }
     return hipGetLastError();
 }
 

@@ -444,10 +444,8 @@ struct Tile {
 // accumulation, recovers (e, p, u), regulates and stores.  TWO: both views
 // of a face are stored (face modes 1, 2).
 template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
-__global__ __launch_bounds__(MX_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void stage_mixed_kernel(const MxFields F, const EosDev E,
-                                                                 const EvDesc *__restrict__ ev,
-                                                                 const int *__restrict__ blk_ev,
-                                                                 const int *__restrict__ blk_first) {
+__device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev &E, const EvDesc *__restrict__ ev,
+                                                 const int *__restrict__ blk_ev, const int *__restrict__ blk_first) {
 #pragma clang fp contract(fast)
     using T = Tile<TX, TY, TZ, DIM3>;
     __shared__ float HA[NC][T::FMAX];
@@ -553,6 +551,23 @@ __global__ __launch_bounds__(MX_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     stf(F.uo[2], s, (float)w.uy);
     stf(F.uo[3], s, (float)w.un);
     if (nan != NAN_NONE) report_nan(F.err, nan, s);
+}
+
+// 2+1D: 128 VGPRs (four waves per SIMD) with a 12-byte spill measured
+// faster than 3 waves without; 3+1D keeps the register budget the compiler
+// picks (it would spill ~110 bytes at four waves).
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
+__global__ __launch_bounds__(MX_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void stage_mixed_kernel(
+    const MxFields F, const EosDev E, const EvDesc *__restrict__ ev, const int *__restrict__ blk_ev,
+    const int *__restrict__ blk_first) {
+    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO>(F, E, ev, blk_ev, blk_first);
+}
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
+__global__ __launch_bounds__(MX_THREADS) void stage_mixed_kernel_3d(const MxFields F, const EosDev E,
+                                                                    const EvDesc *__restrict__ ev,
+                                                                    const int *__restrict__ blk_ev,
+                                                                    const int *__restrict__ blk_first) {
+    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO>(F, E, ev, blk_ev, blk_first);
 }
 
 }  // namespace mx
