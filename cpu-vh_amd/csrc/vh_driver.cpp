@@ -51,6 +51,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <getopt.h>
 #include <string>
 #include <unistd.h>
@@ -332,7 +333,7 @@ constexpr int FO_FIELD[FO_VARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
 // callFOFinder2p1D / callFOFinder3p1D (FreezeOut.cpp:200-361) for the cut
 // cubes ids (x-fastest cube index, increasing): visited in the reference's
 // order (x outer, eta inner), each surface element written as one line.
-void write_surface(std::ofstream &os, std::vector<int64_t> &ids, int64_t nf, const std::vector<double> (&sl)[2][FO_VARS],
+void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, const std::vector<double> (&sl)[2][FO_VARS],
                    int dim, const Lattice &L, const double *spacing, double cell_tau, double t, double eF,
                    const vh_handle *h) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -343,8 +344,14 @@ void write_surface(std::ofstream &os, std::vector<int64_t> &ids, int64_t nf, con
     };
     std::sort(ids.begin(), ids.begin() + nf, [&](int64_t a, int64_t b) { return key(a) < key(b); });
     const size_t sxy = (size_t)nx * ny;
-    double el[16 * 8];
+    // cubes are independent: each thread formats its cubes' lines, written
+    // afterwards in the reference's order
+    std::vector<std::string> text(nf);
+    int failed = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : failed)
     for (int64_t m = 0; m < nf; ++m) {
+        double el[16 * 8];
+        std::ostringstream os;
         const int64_t c = ids[m];
         const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
         auto at = [&](int x, int y, int z) { return (size_t)(ix + x) + (size_t)nx * (iy + y) + sxy * (iz + z); };
@@ -360,7 +367,8 @@ void write_surface(std::ofstream &os, std::vector<int64_t> &ids, int64_t nf, con
                                       : vh_surface_cube(corners, eF, spacing, el, 16, &ne);
         if (st != VH_OK) {
             std::printf("freezeout surface: %s at cube (%d, %d, %d)\n", vh_status_string(st), ix, iy, iz);
-            std::exit(-1);
+            failed = 1;
+            continue;
         }
         const double cell_x = (double)ix * L.dx - (((double)(nx - 1)) / 2.0 * L.dx);
         const double cell_y = (double)iy * L.dy - (((double)(ny - 1)) / 2.0 * L.dy);
@@ -393,9 +401,13 @@ void write_surface(std::ofstream &os, std::vector<int64_t> &ids, int64_t nf, con
             os << tau << " " << x << " " << y << " " << eta << " " << ds0 << " " << ds1 << " " << ds2 << " " << ds3
                << " " << var[0] << " " << var[1] << " " << var[2] << " " << eps << " " << T << " " << P << " "
                << var[4] << " " << var[5] << " " << var[6] << " " << var[7] << " " << var[8] << " " << var[9]
-               << std::endl;
+               << '\n';
         }
+        text[m] = os.str();
     }
+    if (failed) std::exit(-1);
+    for (const std::string &line : text) out << line;
+    out.flush();
 }
 
 void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X, const std::string &root,
