@@ -32,7 +32,8 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
            "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
-           "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube"]
+           "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube",
+           "vh_ic_optical_glauber"]
 
 
 class VhConfig(ctypes.Structure):
@@ -141,6 +142,8 @@ def _lib():
         L.vh_surface_square.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.vh_surface_cube.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.vh_surface_hypercube.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.vh_ic_optical_glauber.argtypes = [ctypes.c_int, D, D, D, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            D, D, D, P]
         L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
         L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
         L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
@@ -476,6 +479,17 @@ def surface_elements(corners, E0, d):
     if st != VH_OK:
         raise VhError(st, name)
     return out[:2 * dim * n.value].reshape(n.value, 2 * dim)
+
+
+def optical_glauber(nx, ny, nz, dx, dy, dz, A=197, b=0.0, sigma_nn_mb=42.0, alpha=1.0, e0=1.0, eta_flat=4.0,
+                    eta_variance=1.96):
+    """The reference's IC type 2 (vh_ic_optical_glauber)."""
+    e = np.empty(nx * ny * nz, dtype=np.float64)
+    st = _lib().vh_ic_optical_glauber(A, b, sigma_nn_mb, alpha, e0, eta_flat, eta_variance, nx, ny, nz, dx, dy, dz,
+                                      _ptr(e))
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_optical_glauber")
+    return e
 
 
 def read_tmunu(path, nx, ny, nz):

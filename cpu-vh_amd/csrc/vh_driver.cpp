@@ -26,10 +26,10 @@
 // then one RK2 step at t, then t = t0 + n dt.  Consecutive steps with no host
 // work between them are replayed from a HIP graph (vh_steps).
 //
-// Initial conditions: types 0 (constant energy density, :953-981), 4 (the
-// reference's MC-Glauber, bit for bit) and 11 (input/Tmunu.dat under the
+// Initial conditions: types 0 (constant energy density, :953-981), 2 (the
+// optical Glauber, :1069-1105), 4 (the reference's MC-Glauber, bit for bit) and 11 (input/Tmunu.dat under the
 // working directory, :41-105), with u = (1,0,0,0) and pi = 0 or its
-// Navier-Stokes value (:492-646) for 0 and 4.  Other types exit as the
+// Navier-Stokes value (:492-646) for 0, 2 and 4.  Other types exit as the
 // reference does for an undefined type.
 //
 // Output: the snapshot files of rhic/src/HydroPlugin.cpp:40-72 in the format
@@ -221,6 +221,14 @@ void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const Hydr
         for (size_t s = 0; s < N; ++s) { e[s] = ed; p[s] = vh_eos_pressure(h, ed); }
         break;
     }
+    case 2: {
+        std::printf("Continous optical Glauber.\n");
+        check("vh_ic_optical_glauber",
+              vh_ic_optical_glauber(I.A, I.b, I.sigma_nn_mb, I.f_binary, I.e0, I.eta_flat, I.eta_variance, L.nx, L.ny,
+                                    L.nz, L.dx, L.dy, L.dz, e.data()));
+        for (size_t s = 0; s < N; ++s) p[s] = vh_eos_pressure(h, e[s]);
+        break;
+    }
     case 4: {
         std::printf("Monte carlo Glauber.\n");
         int nw = 0;
@@ -246,7 +254,7 @@ void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const Hydr
         std::printf("Initial condition type %d is not available in this build. Exiting ...\n", I.type);
         std::exit(-1);
     }
-    if (I.type == 0 || I.type == 4) {  // setFluidVelocityInitialCondition + setPimunuInitialCondition
+    if (I.type == 0 || I.type == 2 || I.type == 4) {  // setFluidVelocityInitialCondition + setPimunuInitialCondition
         if (H.pi_navier_stokes == 1) {
             std::printf("Initialize \\pi^\\mu\\nu to its asymptotic Navier-Stokes value.\n");
             if (bulk) std::printf("Initialize \\Pi to its asymptotic Navier-Stokes value.\n");

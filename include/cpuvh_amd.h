@@ -198,6 +198,17 @@ vh_status vh_ic_mc_glauber_classic(int A, double b, double sigma_nn_mb, double e
                                    double eta_variance, int nx, int ny, int nz, double dx, double dy, double dz,
                                    double *e_out, int *n_wounded);
 
+/* The reference's IC type 2, the optical Glauber profile
+ * (rhic/src/InitialConditions.cpp:1069-1105, rhic/src/GlauberModel.cpp:19-108):
+ * alpha (fractionOfBinaryCollisions) * binary collisions + (1 - alpha) *
+ * wounded nucleons from the Woods-Saxon thickness, normalised to 1 at the
+ * centre of a central collision, times e0 and the longitudinal profile, plus
+ * the 1e-3 floor.  The thickness integral is accurate to 1e-13 (the reference
+ * asks GSL for 1e-8). */
+vh_status vh_ic_optical_glauber(int A, double b, double sigma_nn_mb, double alpha, double e0, double eta_flat,
+                                double eta_variance, int nx, int ny, int nz, double dx, double dy, double dz,
+                                double *e_out);
+
 /* input/Tmunu.dat (IC type 11) reader, values through float as the reference
  * reads them (rhic/src/InitialConditions.cpp:41-105); Pi may be NULL.  The
  * writer produces a file that reader (and the reference) accept. */

@@ -158,3 +158,27 @@ def test_driver_reader_short_file_and_spacing(tmp_path):
     assert cfg["lattice"] == ["7", "9", "3", "3", "3", "3", "3", "3", "3"]
     assert [float(x) for x in cfg["hydro"]] == [0.0] * 6
     assert cfg["gpu"][:2] == ["1", "1"] and cfg["gpu"][4] == "25"
+
+
+def test_optical_glauber_closed_forms():
+    """The reference's IC type 2: normalised to e0 (+ the 1e-3 floor) at the
+    centre of a central collision for any binary fraction; for wounded
+    nucleons only (alpha = 0) the profile is T(x, y)/T(0, 0), whose integral
+    is the Fermi integral of the Woods-Saxon density, (4 pi/3) n0 R^3
+    (1 + (pi d/R)^2) - 8 pi n0 d^3 Li3(-e^(-R/d)), over T(0, 0) =
+    2 n0 d ln(1 + e^(R/d))."""
+    for alpha in (0.0, 0.3, 1.0):
+        e = C.optical_glauber(41, 41, 1, 0.4, 0.4, 0.1, alpha=alpha, e0=5.0)
+        assert abs(e.reshape(41, 41)[20, 20] - 5.001) < 1e-12
+    n, dx = 221, 0.2  # +-22 fm: the density is down by e^-29 at the edge
+    e = C.optical_glauber(n, n, 1, dx, dx, 0.1, alpha=0.0, e0=1.0)
+    A, n0, d = 197.0, 0.17, 0.54
+    R = 1.12 * A ** (1 / 3) - 0.86 * A ** (-1 / 3)
+    li3 = sum((-np.exp(-R / d)) ** k / k ** 3 for k in range(1, 8))
+    volume = (4 * np.pi / 3) * n0 * R ** 3 * (1 + (np.pi * d / R) ** 2) - 8 * np.pi * n0 * d ** 3 * li3
+    want = volume / (2 * n0 * d * np.log1p(np.exp(R / d)))
+    got = float(np.sum(e - 1e-3)) * dx * dx
+    assert abs(got / want - 1) < 1e-8, (got, want)
+    # an off-centre collision is symmetric under x -> -x and y -> -y
+    e = C.optical_glauber(31, 29, 1, 0.5, 0.5, 0.1, b=6.0, alpha=0.5, e0=1.0).reshape(29, 31)
+    assert np.allclose(e, e[::-1, :], rtol=0, atol=1e-14) and np.allclose(e, e[:, ::-1], rtol=0, atol=1e-14)

@@ -229,3 +229,28 @@ def test_driver_3d_table_bulk_runs_nt_plus_one_steps(tmp_path):
     steps, _ = _run_case(tmp_path, lat, hyd, {"bulk": 1, "eos": "table", "eosFile": eos_file, "snapshotEvery": 5},
                          ic, True, table)
     assert steps == lat["nt"] + 1
+
+
+def test_driver_runs_reference_default_configuration(tmp_path):
+    """The reference's own rhic-conf directory (copied as data): IC type 2
+    (optical Glauber Pb+Pb, e0 = 70), 101 x 101, conformal EoS, regulation
+    below 100 MeV.  The run ends on the third cold check well before nt, the
+    first e snapshot is the optical Glauber profile, and surface.dat holds
+    20-column elements whose d sigma_tau is negative (the surface closes in
+    time) somewhere."""
+    conf = os.path.join(V.REPO, "tests", "golden", "rhic-conf")
+    out = tmp_path / "out"
+    out.mkdir()
+    print("case driver reference rhic-conf", flush=True)
+    r = subprocess.run([DRIVER, "--hydro", "--config", conf, "--output", str(out)], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    steps = int(re.search(r"Steps taken: (\d+)", r.stdout).group(1))
+    assert "All cells have dropped below freezeout energy density" in r.stdout and steps < 10001
+    e0 = np.loadtxt(out / "e_0.500.dat")[:, 3]
+    want = C.optical_glauber(101, 101, 1, 0.2, 0.2, 0.1, A=208, b=0.0, sigma_nn_mb=62.0, alpha=1.0, e0=70.0,
+                             eta_flat=5.9, eta_variance=0.16)
+    assert np.array_equal(np.round(want, 8), e0)
+    surf = np.loadtxt(out / "surface.dat")
+    assert surf.ndim == 2 and surf.shape[1] == 20 and surf.shape[0] > 100
+    assert (surf[:, 4] < 0).any() and np.all(np.isfinite(surf))
