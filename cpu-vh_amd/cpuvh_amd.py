@@ -33,7 +33,7 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
            "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
            "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube",
-           "vh_ic_optical_glauber"]
+           "vh_ic_optical_glauber", "vh_ic_test_problem", "vh_ic_read_fields", "vh_ic_read_gubser"]
 
 
 class VhConfig(ctypes.Structure):
@@ -144,6 +144,9 @@ def _lib():
         L.vh_surface_hypercube.argtypes = [P, D, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.vh_ic_optical_glauber.argtypes = [ctypes.c_int, D, D, D, D, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             D, D, D, P]
+        L.vh_ic_test_problem.argtypes = [ctypes.c_int, D, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D] + [P] * 6
+        L.vh_ic_read_fields.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D] + [P] * 6
+        L.vh_ic_read_gubser.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 7
         L.vh_ic_read_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [P] * 8
         L.vh_ic_write_tmunu.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D, D] + [P] * 8
         L.vh_prepare_steps.argtypes = [P, ctypes.c_int]
@@ -490,6 +493,41 @@ def optical_glauber(nx, ny, nz, dx, dy, dz, A=197, b=0.0, sigma_nn_mb=42.0, alph
     if st != VH_OK:
         raise VhError(st, "vh_ic_optical_glauber")
     return e
+
+
+def analytic_ic(type_, nx, ny, nz, dx, dy, e0=1.0):
+    """The reference's analytic initial states, types 3 and 5-9 (vh_ic_test_problem): dict of x-fastest arrays."""
+    N = nx * ny * nz
+    out = {k: np.empty(N) for k in ("e", "p", "ut", "ux", "uy", "un")}
+    st = _lib().vh_ic_test_problem(type_, e0, nx, ny, nz, dx, dy,
+                                   *[_ptr(out[k]) for k in ("e", "p", "ut", "ux", "uy", "un")])
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_test_problem")
+    return out
+
+
+def read_fields(root, nx, ny, nz, t0):
+    """IC type 10 reader (vh_ic_read_fields): root/input/{e,ux,uy,un,pitt..pinn}.dat."""
+    N = nx * ny * nz
+    out = {k: np.empty(N) for k in ("e", "ut", "ux", "uy", "un")}
+    out["pi"] = np.empty(10 * N)
+    st = _lib().vh_ic_read_fields(root.encode(), nx, ny, nz, t0,
+                                  *[_ptr(out[k]) for k in ("e", "ut", "ux", "uy", "un", "pi")])
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_read_fields")
+    return out
+
+
+def read_gubser(path, nx, ny, nz):
+    """IC type 1 reader (vh_ic_read_gubser)."""
+    N = nx * ny * nz
+    out = {k: np.empty(N) for k in ("e", "p", "ut", "ux", "uy", "un")}
+    out["pi"] = np.empty(10 * N)
+    st = _lib().vh_ic_read_gubser(path.encode(), nx, ny, nz,
+                                  *[_ptr(out[k]) for k in ("e", "p", "ut", "ux", "uy", "un", "pi")])
+    if st != VH_OK:
+        raise VhError(st, "vh_ic_read_gubser")
+    return out
 
 
 def read_tmunu(path, nx, ny, nz):

@@ -26,11 +26,15 @@
 // then one RK2 step at t, then t = t0 + n dt.  Consecutive steps with no host
 // work between them are replayed from a HIP graph (vh_steps).
 //
-// Initial conditions: types 0 (constant energy density, :953-981), 2 (the
-// optical Glauber, :1069-1105), 4 (the reference's MC-Glauber, bit for bit) and 11 (input/Tmunu.dat under the
-// working directory, :41-105), with u = (1,0,0,0) and pi = 0 or its
-// Navier-Stokes value (:492-646) for 0, 2 and 4.  Other types exit as the
-// reference does for an undefined type.
+// Initial conditions (rhic/src/InitialConditions.cpp:1508-1581): types 0
+// (constant energy density, :953-981), 2 (the optical Glauber, :1069-1105),
+// 4 (the reference's MC-Glauber, bit for bit), with u = (1,0,0,0) and pi = 0
+// or its Navier-Stokes value (:492-646); the analytic test states 3 and 5-9
+// (vh_ic_test_problem); the files of types 1
+// (<root>/rhic/rhic-trunk/src/test/resources/gubser/viscous/gubserIC.dat),
+// 10 (input/e.dat, ux.dat, ...) and 11 (input/Tmunu.dat), under the working
+// directory.  Types 12-14 (dynamical sources, in-memory vectors of the
+// wrapper) exit as the reference does for an undefined type.
 //
 // Output: the snapshot files of rhic/src/HydroPlugin.cpp:40-72 in the format
 // of rhic/src/FileIO.cpp:14-59 (<name>_<t %.3f>.dat, "x y z value" lines, x
@@ -237,6 +241,55 @@ void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const Hydr
                                        L.dx, L.dy, L.dz, e.data(), &nw));
         std::printf("==> Found %d wounded nucleons.\n", nw);
         for (size_t s = 0; s < N; ++s) p[s] = vh_eos_pressure(h, e[s]);
+        break;
+    }
+    case 1: {
+        std::printf("Isreal-Stewart hydrodynamic Gubser flow test.\n");
+        const std::string path = root + "/rhic/rhic-trunk/src/test/resources/gubser/viscous/gubserIC.dat";
+        if (vh_ic_read_gubser(path.c_str(), L.nx, L.ny, L.nz, e.data(), p.data(), ut.data(), ux.data(), uy.data(),
+                              un.data(), pi.data()) != VH_OK) {
+            std::printf("Could not read %s\n", path.c_str());
+            std::exit(-1);
+        }
+        break;
+    }
+    case 3: case 5: case 6: case 7: case 8: case 9: {
+        static const char *name[10] = {"", "", "", "Ideal hydrodynamic Gubser flow test.",
+                                       "", "Relativistic Sod shock-tube test.", "Implosion in a box test.",
+                                       "Rayleigh-Taylor instability test.", "Implosion in a box test.",
+                                       "Relativistic 2d Sod shock-tube test."};
+        std::printf("%s\n", name[I.type]);
+        check("vh_ic_test_problem", vh_ic_test_problem(I.type, I.e0, L.nx, L.ny, L.nz, L.dx, L.dy, e.data(), p.data(),
+                                                        ut.data(), ux.data(), uy.data(), un.data()));
+        break;
+    }
+    case 10: {
+        std::printf("Reading initial T ^mu nu from input/e.dat , input/p.dat , etc... \n");
+        // The reference reports a missing file and then fails in fclose
+        // (InitialConditions.cpp:125-146); here the run stops at the report.
+        static const char *files[14] = {"e", "ux", "uy", "un", "pitt", "pitx", "pity", "pitn",
+                                        "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"};
+        for (const char *f : files) {
+            const bool eta = !std::strcmp(f, "un") || !std::strcmp(f, "pitn") || !std::strcmp(f, "pixn") ||
+                             !std::strcmp(f, "piyn");
+            if (eta && L.nz == 1) continue;
+            const std::string path = root + "/input/" + f + ".dat";
+            if (FILE *fp = std::fopen(path.c_str(), "r")) {
+                std::fclose(fp);
+            } else {
+                std::printf("Couldn't open %s.dat!\n", f);
+                std::exit(-1);
+            }
+        }
+        if (vh_ic_read_fields(root.c_str(), L.nx, L.ny, L.nz, H.t0, e.data(), ut.data(), ux.data(), uy.data(),
+                              un.data(), pi.data()) != VH_OK) {
+            std::printf("Could not read the input/*.dat files of %s\n", root.c_str());
+            std::exit(-1);
+        }
+        for (size_t s = 0; s < N; ++s) {
+            p[s] = vh_eos_pressure(h, e[s]);
+            if (bulk) Pi[s] = e[s] / 3. - p[s];  // InitialConditions.cpp:455-464
+        }
         break;
     }
     case 11: {

@@ -11,11 +11,16 @@
 //   parameter and a 64-bit seed so events are reproducible and independent.
 // * Type-11 file reader/writer: input/Tmunu.dat, 20 columns per line read
 //   through float (rhic/src/InitialConditions.cpp:41-105).
+// * The reference's own types 2 and 4 (optical and Monte-Carlo Glauber), its
+//   analytic test states (types 3, 5-9) and its other file formats (types 1
+//   and 10).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/cpuvh_amd.h"
@@ -348,6 +353,155 @@ extern "C" vh_status vh_ic_write_tmunu(const char *path, int nx, int ny, int nz,
                 for (int c = 0; c < 10; ++c) std::fprintf(f, " %.9g", (double)(float)pi10[c * N + s]);
                 std::fprintf(f, " %.9g\n", Pi ? (double)(float)Pi[s] : 0.0);
             }
+    std::fclose(f);
+    return VH_OK;
+}
+
+// The reference's analytic initial states (rhic/src/InitialConditions.cpp):
+// p = e/3 whatever the EoS, pi^{mu nu} and Pi zero, u^mu as written there.
+//   3: ideal Gubser flow at tau = 1, q = 1 (:1146-1190)
+//   5: Sod shock tube in x (:1259-1298)     9: Sod shock tube across y = x (:1300-1338)
+//   6: implosion in a box (:1340-1378)      7: Rayleigh-Taylor instability (:1380-1429)
+//   8: Gaussian pulse with a transverse shear flow (:1431-1478)
+// At r = 0 the Gubser flow is defined as u^mu = (1, 0, 0, 0); the reference
+// evaluates sinh(phi) x / r there and gets NaN on grids with a centre cell.
+extern "C" vh_status vh_ic_test_problem(int type, double e0, int nx, int ny, int nz, double dx, double dy,
+                                        double *e, double *p, double *ut, double *ux, double *uy, double *un) {
+    if (!e || !p || !ut || !ux || !uy || !un || nx < 1 || ny < 1 || nz < 1) return VH_ERR_ARG;
+    if (type != 3 && (type < 5 || type > 9)) return VH_ERR_ARG;
+    const double Lx = ((nx - 1) / 2.) * dx, Ly = ((ny - 1) / 2.) * dy;
+    for (int j = 0; j < ny; ++j) {
+        const double y = (j - (ny - 1) / 2.) * dy;
+        for (int i = 0; i < nx; ++i) {
+            const double x = (i - (nx - 1) / 2.) * dx;
+            double ed = 0, vx = 0, vy = 0;
+            switch (type) {
+            case 3: {
+                const double T = 1.9048812623618392 /
+                                 std::pow(1 + std::pow(1 - std::pow(x, 2) - std::pow(y, 2), 2) +
+                                              2 * (1 + std::pow(x, 2) + std::pow(y, 2)),
+                                          0.3333333333333333);
+                const double r = std::sqrt(x * x + y * y);
+                const double phi = std::atanh(2.0 * r / (1.0 + 1.0 + x * x + y * y));
+                ed = e0 * std::pow(T, 4);
+                if (r > 0) {
+                    vx = std::sinh(phi) * x / r;
+                    vy = std::sinh(phi) * y / r;
+                }
+                break;
+            }
+            case 5: ed = x > 0 ? 0.00778147 : 0.124503; break;
+            case 9: ed = y > x ? 0.00778147 : 0.124503; break;
+            case 6: ed = std::sqrt(x * x + y * y) <= 0.15 ? 0.124503 : 0.00778147; break;
+            case 7: {
+                const double gravity = 0.1, rhoTop = 2.0, rhoBot = 1.0, pert = 0.01;
+                const double yloc = 0.5 + pert * std::cos(M_PI * x);
+                ed = y > yloc ? rhoTop * gravity * (1 - y) : rhoTop * gravity * (1 - yloc) + rhoBot * gravity * (yloc - y);
+                break;
+            }
+            case 8: {
+                const double xc = 0.5, yc = 0.5, beta = 50.0;
+                const double pr = 1.0 + 1e-1 * std::exp(-beta * ((x - xc) * (x - xc) + (y - yc) * (y - yc)));
+                ed = pr / (1.4 - 1.);
+                vy = (1. + std::cos(2 * M_PI * x / Lx)) * (1 + std::cos(2 * M_PI * y / Ly));
+                break;
+            }
+            }
+            const double vt = std::sqrt(1 + vx * vx + vy * vy);
+            for (int k = 0; k < nz; ++k) {
+                const size_t s = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                e[s] = ed;
+                p[s] = ed / 3.0;
+                ux[s] = vx;
+                uy[s] = vy;
+                un[s] = 0.0;
+                ut[s] = vt;
+            }
+        }
+    }
+    return VH_OK;
+}
+
+namespace {
+
+// One input/<name>.dat of IC type 10: "x y z value" lines read through float,
+// i outer, k inner (rhic/src/InitialConditions.cpp:123-146).
+bool read_field_file(const char *dir, const char *name, int nx, int ny, int nz, double *out) {
+    std::string path = std::string(dir) + "/input/" + name + ".dat";
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    float x, y, z, v;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                if (std::fscanf(f, "%f %f %f %f\n", &x, &y, &z, &v) != 4) {
+                    std::fclose(f);
+                    return false;
+                }
+                out[(size_t)i + (size_t)nx * (j + (size_t)ny * k)] = v;
+            }
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace
+
+// IC type 10 (rhic/src/InitialConditions.cpp:108-466): input/e.dat, ux, uy,
+// un (nz > 1) and pitt..pinn (the eta components only for nz > 1; zero
+// otherwise), each "x y z value" per line.  u^tau = sqrt(1 + ux^2 + uy^2 +
+// t0^2 un^2): the reference stores the square (:226), which is not a unit
+// four-velocity.  The pressure and Pi depend on the EoS and are the
+// caller's.  VH_ERR_ARG names no file; the caller checks which exist.
+extern "C" vh_status vh_ic_read_fields(const char *dir, int nx, int ny, int nz, double t0, double *e, double *ut,
+                                       double *ux, double *uy, double *un, double *pi10) {
+    if (!dir || !e || !ut || !ux || !uy || !un || !pi10 || nx < 1 || ny < 1 || nz < 1) return VH_ERR_ARG;
+    const size_t N = (size_t)nx * ny * nz;
+    std::fill(un, un + N, 0.0);
+    std::fill(pi10, pi10 + 10 * N, 0.0);
+    if (!read_field_file(dir, "e", nx, ny, nz, e) || !read_field_file(dir, "ux", nx, ny, nz, ux) ||
+        !read_field_file(dir, "uy", nx, ny, nz, uy) || (nz > 1 && !read_field_file(dir, "un", nx, ny, nz, un)))
+        return VH_ERR_ARG;
+    for (size_t s = 0; s < N; ++s) ut[s] = std::sqrt(1.0 + ux[s] * ux[s] + uy[s] * uy[s] + t0 * t0 * un[s] * un[s]);
+    static const char *pin[10] = {"pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"};
+    for (int c = 0; c < 10; ++c) {
+        const bool eta = c == 3 || c == 6 || c == 8;
+        if (eta && nz == 1) continue;
+        if (!read_field_file(dir, pin[c], nx, ny, nz, pi10 + c * N)) return VH_ERR_ARG;
+    }
+    return VH_OK;
+}
+
+// IC type 1 (rhic/src/InitialConditions.cpp:1196-1257): the Israel-Stewart
+// Gubser profile from a file of one line per transverse cell, i outer, j
+// inner, columns x y e ux uy pixx piyy pixy pitt pitx pity pinn read as
+// double; the same values on every rapidity slice, p = e/3, un and the eta
+// components of pi zero.
+extern "C" vh_status vh_ic_read_gubser(const char *path, int nx, int ny, int nz, double *e, double *p, double *ut,
+                                       double *ux, double *uy, double *un, double *pi10) {
+    if (!path || !e || !p || !ut || !ux || !uy || !un || !pi10 || nx < 1 || ny < 1 || nz < 1) return VH_ERR_ARG;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return VH_ERR_ARG;
+    const size_t N = (size_t)nx * ny * nz;
+    double v[12];
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            if (std::fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", &v[0], &v[1], &v[2], &v[3], &v[4],
+                            &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11]) != 12) {
+                std::fclose(f);
+                return VH_ERR_ARG;
+            }
+            const double pv[10] = {v[8], v[9], v[10], 0.0, v[5], v[7], 0.0, v[6], 0.0, v[11]};
+            for (int k = 0; k < nz; ++k) {
+                const size_t s = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                e[s] = v[2];
+                p[s] = v[2] / 3.0;
+                ux[s] = v[3];
+                uy[s] = v[4];
+                un[s] = 0.0;
+                ut[s] = std::sqrt(1.0 + v[3] * v[3] + v[4] * v[4]);
+                for (int c = 0; c < 10; ++c) pi10[c * N + s] = pv[c];
+            }
+        }
     std::fclose(f);
     return VH_OK;
 }

@@ -209,6 +209,30 @@ vh_status vh_ic_optical_glauber(int A, double b, double sigma_nn_mb, double alph
                                 double eta_variance, int nx, int ny, int nz, double dx, double dy, double dz,
                                 double *e_out);
 
+/* The reference's analytic initial states, type = its initialConditionType
+ * (rhic/src/InitialConditions.cpp): 3 ideal Gubser flow (:1146-1190; u = 0
+ * at r = 0, where the reference divides 0 by 0), 5 Sod shock tube (:1259),
+ * 6 implosion in a box (:1340), 7 Rayleigh-Taylor (:1380), 8 Gaussian pulse
+ * (:1431), 9 diagonal Sod shock tube (:1300).  p = e/3 as there; pi and Pi
+ * are zero.  e0 is used by type 3 only. */
+vh_status vh_ic_test_problem(int type, double e0, int nx, int ny, int nz, double dx, double dy, double *e, double *p,
+                             double *ut, double *ux, double *uy, double *un);
+
+/* IC type 10 reader (rhic/src/InitialConditions.cpp:108-466): dir/input/
+ * e.dat, ux.dat, uy.dat, un.dat and pitt.dat..pinn.dat ("x y z value" lines
+ * through float, i outer, k inner; the eta files only when nz > 1).  u^tau is
+ * normalised (the reference stores 1 + u_perp^2 + t0^2 un^2 without the
+ * square root, :226).  The caller sets p = p(e) and, with bulk, Pi = e/3 - p,
+ * as :200 and :455-464 do. */
+vh_status vh_ic_read_fields(const char *dir, int nx, int ny, int nz, double t0, double *e, double *ut, double *ux,
+                            double *uy, double *un, double *pi10);
+
+/* IC type 1 reader (rhic/src/InitialConditions.cpp:1196-1257): one line per
+ * transverse cell, i outer, "x y e ux uy pixx piyy pixy pitt pitx pity pinn",
+ * repeated over rapidity; p = e/3. */
+vh_status vh_ic_read_gubser(const char *path, int nx, int ny, int nz, double *e, double *p, double *ut, double *ux,
+                            double *uy, double *un, double *pi10);
+
 /* input/Tmunu.dat (IC type 11) reader, values through float as the reference
  * reads them (rhic/src/InitialConditions.cpp:41-105); Pi may be NULL.  The
  * writer produces a file that reader (and the reference) accept. */

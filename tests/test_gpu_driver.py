@@ -187,21 +187,28 @@ def _oracle_loop(lat, hyd, bulk, table, ic):
     return files, nsteps
 
 
-def _run_case(tmp_path, lat, hyd, gpu, ic_fields, bulk, table):
+def _run_case(tmp_path, lat, hyd, gpu, ic_fields, bulk, table, ic_type=11):
+    """ic_type 11: ic_fields go through input/Tmunu.dat; otherwise the driver
+    builds the initial state itself and ic_fields is what it should be."""
     conf, out = str(tmp_path / "conf"), str(tmp_path / "out")
     os.makedirs(out)
-    _write_conf(conf, lat, {"type": 11}, hyd, gpu)
-    os.makedirs(tmp_path / "input")
+    _write_conf(conf, lat, {"type": ic_type}, hyd, gpu)
     nx, ny, nz, dx = lat["nx"], lat["ny"], lat["nz"], lat["dx"]
-    C.write_tmunu(str(tmp_path / "input" / "Tmunu.dat"), nx, ny, nz, dx, dx, dx, ic_fields)
-    print(f"case driver {nx}x{ny}x{nz} nt={lat['nt']} bulk={bulk} table={table is not None}", flush=True)
+    if ic_type == 11:
+        os.makedirs(tmp_path / "input")
+        C.write_tmunu(str(tmp_path / "input" / "Tmunu.dat"), nx, ny, nz, dx, dx, dx, ic_fields)
+    print(f"case driver type={ic_type} {nx}x{ny}x{nz} nt={lat['nt']} bulk={bulk} table={table is not None}",
+          flush=True)
     r = subprocess.run([DRIVER, "--hydro", "--config", conf, "--output", out], cwd=str(tmp_path),
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     steps = int(re.search(r"Steps taken: (\d+)", r.stdout).group(1))
-    ic = _read_tmunu_f32(str(tmp_path / "input" / "Tmunu.dat"), nx, ny, nz)
-    if not bulk:
-        ic.pop("Pi")
+    if ic_type == 11:
+        ic = _read_tmunu_f32(str(tmp_path / "input" / "Tmunu.dat"), nx, ny, nz)
+        if not bulk:
+            ic.pop("Pi")
+    else:
+        ic = dict(ic_fields)
     want, want_steps = _oracle_loop(lat, hyd, bulk, table, ic)
     assert steps == want_steps, (steps, want_steps)
     got = sorted(os.listdir(out))
@@ -229,6 +236,38 @@ def test_driver_3d_table_bulk_runs_nt_plus_one_steps(tmp_path):
     steps, _ = _run_case(tmp_path, lat, hyd, {"bulk": 1, "eos": "table", "eosFile": eos_file, "snapshotEvery": 5},
                          ic, True, table)
     assert steps == lat["nt"] + 1
+
+
+def test_driver_ideal_gubser_initial_state(tmp_path):
+    """IC type 3 built by the driver (initialEnergyDensity = 1, the Gubser
+    flow at tau = 1 with u = 0 at the centre cell): same files as the oracle
+    loop started from the same state."""
+    lat = dict(nx=21, ny=21, nz=1, nt=20, dx=0.2, dt=0.02, coarse=4, snap=10)
+    hyd = dict(t0=1.0, etas=0.08, TF=0.06, Treg=0.0)
+    ic = C.analytic_ic(3, 21, 21, 1, 0.2, 0.2, e0=1.0)
+    ic["pi"] = np.zeros(10 * 21 * 21)
+    steps, log = _run_case(tmp_path, lat, hyd, {"snapshotEvery": 10}, ic, False, None, ic_type=3)
+    assert steps == lat["nt"] + 1 and "Ideal hydrodynamic Gubser flow test." in log
+
+
+def test_driver_israel_stewart_gubser_file(tmp_path):
+    """IC type 1: the driver reads
+    rhic/rhic-trunk/src/test/resources/gubser/viscous/gubserIC.dat under the
+    working directory (the reference does not ship the file; it is made here
+    from the semi-analytic Israel-Stewart Gubser solution at tau = 1,
+    eta/s = 0.2), no regulation (the regulator's tanh may differ from the
+    C library's by an ulp, which a %.8f digit can show); same files as the
+    oracle loop."""
+    import gubser as Gb
+    from test_gubser import write_gubser_ic
+    d = tmp_path / "rhic" / "rhic-trunk" / "src" / "test" / "resources" / "gubser" / "viscous"
+    d.mkdir(parents=True)
+    write_gubser_ic(str(d / "gubserIC.dat"), Gb.solve(0.2), 21, 0.2)
+    ic = C.read_gubser(str(d / "gubserIC.dat"), 21, 21, 1)
+    lat = dict(nx=21, ny=21, nz=1, nt=20, dx=0.2, dt=0.02, coarse=4, snap=10)
+    hyd = dict(t0=1.0, etas=0.2, TF=0.15, Treg=0.0)
+    steps, log = _run_case(tmp_path, lat, hyd, {"snapshotEvery": 10}, ic, False, None, ic_type=1)
+    assert steps == lat["nt"] + 1 and "Gubser" in log
 
 
 def test_driver_runs_reference_default_configuration(tmp_path):
