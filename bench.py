@@ -258,16 +258,23 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
     fields = ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi")
     cap = sum(ev["N"] for ev in events)
     stops = [c for c in checkpoints if c < nsteps] + [nsteps]
-    variants = {"fp64": (C.PREC_FP64, False), "mixed": (C.PREC_MIXED, False), "fp64_f32_ic": (C.PREC_FP64, True)}
+    # FP64 from perturbed starts: the initial state rounded to FP32, and e
+    # scaled by 1 + 1e-7 u (u uniform in [-1, 1], two seeds)
+    variants = {"fp64": (C.PREC_FP64, None), "mixed": (C.PREC_MIXED, None), "fp64_f32_ic": (C.PREC_FP64, "f32"),
+                "fp64_pert1": (C.PREC_FP64, 1), "fp64_pert2": (C.PREC_FP64, 2)}
     snaps, cubes = {}, {}
-    for name, (prec, round_ic) in variants.items():
+    for name, (prec, pert) in variants.items():
         b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
         if prec == C.PREC_MIXED and FACE_MODE is not None:
             b.set_face_mode(FACE_MODE)
         ids = []
-        for ev in events:
-            if round_ic:
-                ev = dict(ev, ic={k: v.astype(np.float32).astype(np.float64) for k, v in ev["ic"].items()})
+        for k, ev in enumerate(events):
+            if pert == "f32":
+                ev = dict(ev, ic={f: v.astype(np.float32).astype(np.float64) for f, v in ev["ic"].items()})
+            elif pert is not None:
+                r = np.random.default_rng(1000 * pert + k)
+                e = ev["ic"]["e"] * (1 + 1e-7 * r.uniform(-1, 1, ev["ic"]["e"].size))
+                ev = dict(ev, ic=dict(ev["ic"], e=e, p=table_pressure(table, e)))
             ids.append(add_event(b, ev))
         done = 0
         for stop in stops:
@@ -301,7 +308,10 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
                        "fp64_sensitivity": errors(snaps[("fp64", stop)], snaps[("fp64_f32_ic", stop)]),
                        "fo_cubes_fp64": sum(len(a) for a in cubes[("fp64", stop)]),
                        "fo_cubes_differing_mixed": cube_diff("mixed"),
-                       "fo_cubes_differing_fp64_f32_ic": cube_diff("fp64_f32_ic")})
+                       "fo_cubes_differing_fp64_f32_ic": cube_diff("fp64_f32_ic"),
+                       "fp64_perturbed_e": [errors(snaps[("fp64", stop)], snaps[(v, stop)])["e"]
+                                            for v in ("fp64_pert1", "fp64_pert2")],
+                       "fo_cubes_differing_fp64_perturbed": [cube_diff(v) for v in ("fp64_pert1", "fp64_pert2")]})
     return {"steps": nsteps, "events": len(events), "checkpoints": report,
             "fields": report[-1]["mixed_vs_fp64"]}
 

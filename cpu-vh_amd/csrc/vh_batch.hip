@@ -47,6 +47,7 @@ struct Event {
     double e_reg = 0;
     double *fo_prev = nullptr, *fo_cur = nullptr;
     long long *fo_ids = nullptr;
+    bool bi = false;  // boost-invariant 2+1D: nz = 1, u^eta = pi^{tau eta} = pi^{x eta} = pi^{y eta} = 0
 };
 
 }  // namespace
@@ -71,7 +72,10 @@ struct vh_batch {
     std::vector<EvDesc> h_ev;
     StepTimes *d_times = nullptr;  // max_events x (max_steps + 1)
     int *d_blk_ev = nullptr, *d_blk_first = nullptr;
-    int nblk2 = 0, nblk3 = 0;      // block table: [2+1D blocks | 3+1D blocks]
+    // block table: [2+1D blocks | 3+1D blocks]; in a mixed batch the 2+1D
+    // part starts with the nblk2bi blocks of boost-invariant events, which
+    // have a launch of their own
+    int nblk2 = 0, nblk3 = 0, nblk2bi = 0;
     // 64-cell chunks of every event (counting kernels of the mixed form)
     int *d_cnt_ev = nullptr;
     long *d_cnt_start = nullptr;
@@ -137,30 +141,36 @@ vh_status rebuild_layout(vh_batch *b) {
     if (!b->layout_dirty) return VH_OK;
     std::vector<int> bev, bfirst, cev;
     std::vector<long> cstart;
-    int n2 = 0;
-    for (int pass = 0; pass < 2; ++pass)
+    int n2 = 0, n2bi = 0;
+    // pass 0: boost-invariant 2+1D events of a mixed batch; 1: other 2+1D; 2: 3+1D
+    for (int pass = 0; pass < 3; ++pass)
         for (int id = 0; id < (int)b->ev.size(); ++id) {
             const Event &E = b->ev[id];
-            if (!E.used || (E.prm.nz > 1) != (pass == 1)) continue;
+            if (!E.used) continue;
+            const int cls = E.prm.nz > 1 ? 2 : (b->mixed && E.bi ? 0 : 1);
+            if (cls != pass) continue;
             const long nb = event_blocks(b, E);
-            // 2+1D and 3+1D blocks go to separate launches, each indexing its
-            // part of the table from 0: first is relative to that part.
-            const int first = (int)bev.size() - (pass == 1 ? n2 : 0);
+            // each class goes to a launch of its own, indexing its part of
+            // the table from 0: first is relative to that part.
+            const int first = (int)bev.size() - (pass == 0 ? 0 : (pass == 1 ? n2bi : n2));
             for (long k = 0; k < nb; ++k) {
                 bev.push_back(id);
                 bfirst.push_back(first);
             }
-            if (pass == 0) n2 += (int)nb;
+            if (pass == 0) n2bi += (int)nb;
+            if (pass <= 1) n2 += (int)nb;
             for (long c = 0; c < E.N; c += 64) {
                 cev.push_back(id);
                 cstart.push_back(c);
             }
         }
     b->nblk2 = n2;
+    b->nblk2bi = n2bi;
     b->nblk3 = (int)bev.size() - n2;
     // every block must name a tile inside its event: 0 <= k - first < blocks
     for (size_t k = 0; k < bev.size(); ++k) {
-        const long local = (long)k - (k < (size_t)n2 ? 0 : n2) - bfirst[k];
+        const long seg = k < (size_t)n2bi ? 0 : (k < (size_t)n2 ? n2bi : n2);
+        const long local = (long)k - seg - bfirst[k];
         if (local < 0 || local >= event_blocks(b, b->ev[bev[k]])) {
             set_last_error("batch block table inconsistent");
             return VH_ERR_STATE;
@@ -214,21 +224,16 @@ StageParams batch_params(const vh_batch *b, int stage, int iq_in, int iu_in, int
 mx::MxFields mixed_fields(const vh_batch *b, int stage, int iq_in, int iu_in, int iu_prev_used, int ie_in,
                           int iq_out, int iu_out, int ie_out) {
     mx::MxFields F{};
-    for (int n = 0; n < 15; ++n) {
-        F.q[n] = b->qf[iq_in][n];
-        F.qn[n] = b->qf[b->iq_cur][n];
-        F.qo[n] = b->qf[iq_out][n];
-    }
-    for (int n = 0; n < 4; ++n) {
-        F.ql[n] = b->qfl[iq_in][n];
-        F.qnl[n] = b->qfl[b->iq_cur][n];
-        F.qol[n] = b->qfl[iq_out][n];
-    }
-    for (int c = 0; c < 4; ++c) {
-        F.u[c] = b->uf[iu_in][c];
-        F.up[c] = b->uf[iu_prev_used][c];
-        F.uo[c] = b->uf[iu_out][c];
-    }
+    F.q = b->qf[iq_in][0];
+    F.qn = b->qf[b->iq_cur][0];
+    F.qo = b->qf[iq_out][0];
+    F.ql = b->qfl[iq_in][0];
+    F.qnl = b->qfl[b->iq_cur][0];
+    F.qol = b->qfl[iq_out][0];
+    F.u = b->uf[iu_in][0];
+    F.up = b->uf[iu_prev_used][0];
+    F.uo = b->uf[iu_out][0];
+    F.cap = (unsigned long)b->cap;
     F.e = b->ef[ie_in];
     F.p = b->pf[ie_in];
     F.eo = b->ef[ie_out];
@@ -248,8 +253,11 @@ vh_status enqueue_stage(vh_batch *b, int stage, int off) {
         mx::MxFields F = stage == 0 ? mixed_fields(b, 0, b->iq_cur, b->iu_cur, b->iu_prev, 0, b->iq_stage, b->iu_stage, 1)
                                     : mixed_fields(b, 1, b->iq_stage, b->iu_stage, b->iu_cur, 1, b->iq_next, b->iu_prev, 0);
         F.off = off;
-        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, false, b->tile2, b->d_ev, be2, bf2, b->nblk2, b->stream));
-        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, true, b->tile3, b->d_ev, be3, bf3, b->nblk3, b->stream));
+        const int nbi = b->nblk2bi;
+        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, false, b->tile2, true, b->d_ev, be2, bf2, nbi, b->stream));
+        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, false, b->tile2, false, b->d_ev, be2 ? be2 + nbi : nullptr,
+                                bf2 ? bf2 + nbi : nullptr, b->nblk2 - nbi, b->stream));
+        BCHK(launch_stage_mixed(F, b->deos.E, b->nc, true, b->tile3, false, b->d_ev, be3, bf3, b->nblk3, b->stream));
         return VH_OK;
     }
     StageParams P = stage == 0 ? batch_params(b, 0, b->iq_cur, b->iu_cur, b->iu_prev, 0, b->iq_stage, b->iu_stage, 1)
@@ -423,6 +431,9 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
     E.N = N;
     E.steps_done = 0;
     E.e_reg = b->heos.energy_density(c.T_reg_GeV / 0.197326938);
+    E.bi = c.nz == 1;
+    for (long s = 0; s < N && E.bi; ++s)
+        E.bi = un[s] == 0.0 && pi10[3 * N + s] == 0.0 && pi10[6 * N + s] == 0.0 && pi10[8 * N + s] == 0.0;
     const size_t bytes = sizeof(double) * N;
     if (!b->mixed) {
         BCHK(hipMemcpy(b->e[0] + off, e, bytes, hipMemcpyHostToDevice));
@@ -458,9 +469,10 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
             at += len[k];
         }
         mx::MxFields F{};
-        for (int n = 0; n < 15; ++n) F.qo[n] = b->qf[b->iq_cur][n];
-        for (int n = 0; n < 4; ++n) F.qol[n] = b->qfl[b->iq_cur][n];
-        for (int k = 0; k < 4; ++k) F.uo[k] = b->uf[b->iu_cur][k];
+        F.qo = b->qf[b->iq_cur][0];
+        F.qol = b->qfl[b->iq_cur][0];
+        F.uo = b->uf[b->iu_cur][0];
+        F.cap = (unsigned long)b->cap;
         F.eo = b->ef[0];
         F.po = b->pf[0];
         if (r == hipSuccess) r = launch_init_mixed(F, st, st + N, st + 2 * N, st + 6 * N, st + 16 * N, N, off, b->nc,
@@ -716,10 +728,18 @@ vh_status vh_batch_autotune(vh_batch *b, int reps, int *tile2d, int *tile3d) {
             const int *be = b->d_blk_ev + (dim ? b->nblk2 : 0), *bf = b->d_blk_first + (dim ? b->nblk2 : 0);
             const int nb = dim ? b->nblk3 : b->nblk2;
             const int tile = dim ? b->tile3 : b->tile2;
-            BCHK(launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, b->d_ev, be, bf, nb, b->stream));
+            // the 2+1D part is two launches (boost-invariant events first)
+            const int nbi = dim ? 0 : b->nblk2bi;
+            auto launch = [&]() -> hipError_t {
+                hipError_t r = launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, true, b->d_ev, be, bf, nbi,
+                                                  b->stream);
+                if (r != hipSuccess) return r;
+                return launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, false, b->d_ev, be + nbi, bf + nbi,
+                                          nb - nbi, b->stream);
+            };
+            BCHK(launch());
             BCHK(hipEventRecord(b->ev0, b->stream));
-            for (int r = 0; r < reps; ++r)
-                BCHK(launch_stage_mixed(F, b->deos.E, b->nc, dim == 1, tile, b->d_ev, be, bf, nb, b->stream));
+            for (int r = 0; r < reps; ++r) BCHK(launch());
             BCHK(hipEventRecord(b->ev1, b->stream));
             BCHK(hipStreamSynchronize(b->stream));
             float ms = 0;

@@ -86,6 +86,11 @@ struct QuotExact {
     static __device__ __forceinline__ float sqrt_island(float x) { return (float)__builtin_sqrt((double)x); }
 };
 struct QuotFast {
+    // hardware reciprocal and one Newton step: relative error about 2^-46
+    static __device__ __forceinline__ double rcp1(double b) {
+        const double r = __builtin_amdgcn_rcp(b);
+        return __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    }
     static __device__ __forceinline__ double rcp(double b) {
         double r = __builtin_amdgcn_rcp(b);
         r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
@@ -164,6 +169,20 @@ __device__ __forceinline__ double eos_pressure(const EosDev &E, double e, DV &di
     }
 }
 
+// Mixed form: the same linear interpolation from one scaled coordinate
+// x = (e - e_min) / de (idx = trunc(x), frac = x - idx), within a few ulps of
+// the reference's two divisions.
+__device__ __forceinline__ double eos_pressure_fast(const EosDev &E, double e) {
+    const double x = (e - E.e_min) * E.r_spacing;
+    int idx = (int)x;
+    idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
+    idx = idx > 0 ? idx : 0;
+    const double frac = x - (double)idx;
+    const double t1 = E.p_tb[idx], t2 = E.p_tb[idx + 1];
+    const double f = __builtin_fma(frac, t2 - t1, t1);
+    return 1.e-15 < f ? f : 1.e-15;
+}
+
 template <bool TABLE, class DV>
 __device__ __forceinline__ double eos_cs2(const EosDev &E, double e, DV &div) {
     if constexpr (TABLE) {
@@ -219,20 +238,24 @@ template <class QD, class DV>
 __device__ __forceinline__ double nr_step(const EosDev &E, double e0, double M0, double M, double Pi, DV &dx,
                                           bool &bad) {
     if constexpr (std::is_same_v<QD, QuotFast>) {
-        // mixed form: the same iteration with four refined reciprocals in
-        // place of five quotients (1/e0, 1/p, 1/H, 1/fp; cst2 e0 H = p H)
-        const double p = eos_pressure<true>(E, e0, dx);
+        // mixed form: the same iteration with two reciprocals in place of
+        // five quotients.  R = 1/(e0 p) gives 1/e0 and 1/p; with
+        // X = c (B + D H - c cst2 D M0 / e0) and cst2 e0 H = p H the step
+        // f / fp = f (1 - X / (p H))^-1 is f p H / (p H - X).
+        const double p = eos_pressure_fast(E, e0);
         const double cs2 = eos_cs2<true>(E, e0, dx);
-        const double r0 = QuotFast::rcp(e0), rp = QuotFast::rcp(p);
-        const double cst2 = p * r0;
-        const double A = M0 * (1.0 - cst2) + Pi;
+        const double R = QuotFast::rcp1(e0 * p);
+        const double ie0 = p * R, ip = e0 * R;
+        const double cst2 = p * ie0;
+        const double A = __builtin_fma(-M0, cst2, M0) + Pi;
         const double B = M0 * (M0 + Pi) - M;
-        const double H = (double)QuotFast::sqrt_island(f32_fabs((float)(A * A + 4 * cst2 * B)));
-        const double D = (A - H) * (0.5 * e0 * rp);
+        const double H = (double)QuotFast::sqrt_island(f32_fabs((float)__builtin_fma(A, A, 4 * cst2 * B)));
+        const double D = (A - H) * (0.5 * e0 * ip);
         const double f = e0 + D;
         const double c = cs2 - cst2;
-        const double fp = 1.0 - c * (B + D * H - c * cst2 * D * M0 * r0) * (rp * QuotFast::rcp(H));
-        const double step = f * QuotFast::rcp(fp);
+        const double X = c * (B + D * H - c * cst2 * D * M0 * ie0);
+        const double pH = p * H;
+        const double step = f * pH * QuotFast::rcp1(pH - X);
         bad = __builtin_isnan(step);
         return e0 - step;
     }
@@ -250,8 +273,11 @@ __device__ __forceinline__ double nr_step(const EosDev &E, double e0, double M0,
     return e0 - step;
 }
 
-// the reference's stop test, evaluated in float (:44)
+// the reference's stop test, evaluated in float (:44); the mixed form
+// evaluates it in double
+template <class QD>
 __device__ __forceinline__ bool nr_converged(double e, double e0) {
+    if constexpr (std::is_same_v<QD, QuotFast>) return __builtin_fabs(e - e0) <= ACC * __builtin_fabs(e);
     return (double)f32_fabs((float)(e - e0)) <= ACC * (double)f32_fabs((float)e);
 }
 
@@ -302,7 +328,7 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
         bool bad;
         const double e = nr_step<QD, DV>(E, e0, M0, M, Pi, dx, bad);
         if (bad) return RootResult{ePrev, NAN_ROOT};
-        if (nr_converged(e, e0)) return RootResult{e, NAN_NONE};
+        if (nr_converged<QD>(e, e0)) return RootResult{e, NAN_NONE};
         e0 = e;
     }
     return RootResult{table_bisect<QD, DV>(E, ePrev, M0, M, Pi), NAN_NONE};
@@ -351,15 +377,21 @@ __device__ __forceinline__ Prim recover_finish(const EosDev &E, const Moments &m
     Prim r;
     r.e = e;
     if (__builtin_isnan(r.e)) nan = NAN_ENERGY;
-    r.p = eos_pressure<TABLE>(E, r.e, div);
+    constexpr bool FAST = std::is_same_v<QD, QuotFast>;
+    if constexpr (FAST && TABLE) r.p = eos_pressure_fast(E, r.e);
+    else r.p = eos_pressure<TABLE>(E, r.e, div);
     if (r.e < 1.e-7) {
         r.e = 1.e-7;
         r.p = 1.e-7;
     }
     double P = r.p + Pi;
-    double Ei = QD::q(1.0, r.e + P);
+    double Ei;
+    if constexpr (FAST) Ei = QuotFast::rcp1(r.e + P);
+    else Ei = QD::q(1.0, r.e + P);
     r.ut = (double)QD::sqrt_island(f32_fabs((float)((m.M0 + P) * Ei)));
-    double E2 = QD::q(Ei, r.ut);
+    double E2;
+    if constexpr (FAST) E2 = Ei * QuotFast::rcp1(r.ut);
+    else E2 = QD::q(Ei, r.ut);
     r.ux = m.M1 * E2;
     r.uy = m.M2 * E2;
     r.un = m.M3 * E2;
@@ -378,8 +410,10 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
                                         int &nan, DV &div) {
     const Moments m = moments(t, ttt, ttx, tty, ttn, pitt, pitx, pity, pitn);
     double e;
-    if (ePrev <= 0.1) e = m.M0 - QD::q(m.M, m.M0);
-    else  // the FP64 path keeps the division operator inside the root solve's table lookups
+    if (ePrev <= 0.1) {
+        if constexpr (std::is_same_v<QD, QuotFast>) e = m.M0 - m.M * QuotFast::rcp1(m.M0);
+        else e = m.M0 - QD::q(m.M, m.M0);
+    } else  // the FP64 path keeps the division operator inside the root solve's table lookups
         e = energy_from_conserved<TABLE, QD, std::conditional_t<std::is_same_v<QD, QuotFast>, DivRecip, DivExact>>(
             E, ePrev, m.M0, m.M, Pi, nan);
     return recover_finish<TABLE, DV, QD>(E, m, e, Pi, nan, div);

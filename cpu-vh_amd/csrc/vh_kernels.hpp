@@ -43,8 +43,10 @@ struct MxFields;
 constexpr int MIXED_NTILE = 3;
 extern const int MIXED_TILES_2D[MIXED_NTILE][3];  // (TX, TY, TZ) of each 2+1D tile variant
 extern const int MIXED_TILES_3D[MIXED_NTILE][3];
-hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, const EvDesc *ev,
-                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st);
+// bi: the launch holds only boost-invariant 2+1D events (nz = 1, u^eta and
+// the pi^{. eta} components zero), whose eta components stay zero.
+hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, bool bi,
+                              const EvDesc *ev, const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st);
 hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,
                              const double *pi10, const double *Pi, long N, long off, int nc, hipStream_t st);
 hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,

@@ -28,24 +28,24 @@ __global__ __launch_bounds__(256) void init_mixed_kernel(const MxFields F, const
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
         const float hi = (float)T4[n];
-        F.qo[n][s] = hi;
-        F.qol[n][s] = (float)(T4[n] - (double)hi);
+        F.at(F.qo, n)[s] = hi;
+        F.at(F.qol, n)[s] = (float)(T4[n] - (double)hi);
     }
-    for (int n = 0; n < 10; ++n) F.qo[4 + n][s] = (float)pi10[n * N + c];
-    if (nc == 15) F.qo[14][s] = (float)Pb;
+    for (int n = 0; n < 10; ++n) F.at(F.qo, 4 + n)[s] = (float)pi10[n * N + c];
+    if (nc == 15) F.at(F.qo, 14)[s] = (float)Pb;
     F.eo[s] = (float)e[c];
     F.po[s] = (float)p[c];
-    F.uo[0][s] = (float)ut;
-    F.uo[1][s] = (float)ux;
-    F.uo[2][s] = (float)uy;
-    F.uo[3][s] = (float)un;
+    F.at(F.uo, 0)[s] = (float)ut;
+    F.at(F.uo, 1)[s] = (float)ux;
+    F.at(F.uo, 2)[s] = (float)uy;
+    F.at(F.uo, 3)[s] = (float)un;
 }
 
 }  // namespace mx
 
 namespace {
 
-template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ>
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool BI = false>
 hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, const int *blk_ev,
                      const int *blk_first, int nblocks, hipStream_t st) {
     if constexpr (DIM3) {
@@ -57,10 +57,10 @@ hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, c
                                dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
     } else {
         if (F.face_mode == 0)
-            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, false>), dim3(nblocks),
+            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, false, BI>), dim3(nblocks),
                                dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
         else
-            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, true>), dim3(nblocks),
+            hipLaunchKernelGGL((mx::stage_mixed_kernel<NC, TABLE, DIM3, TX, TY, TZ, true, BI>), dim3(nblocks),
                                dim3(mx::MX_THREADS), 0, st, F, E, ev, blk_ev, blk_first);
     }
     return hipGetLastError();
@@ -68,10 +68,18 @@ hipError_t launch_mx(const mx::MxFields &F, const EosDev &E, const EvDesc *ev, c
 
 // Tile variants: the face list of a tile fills whole passes of 256 threads
 // closely (16x15: 511 faces, 240 cells; 6x6x6: 756 faces, 216 cells).
+// bi: every event of the launch is a boost-invariant 2+1D event (the
+// kernels that neither load nor evolve the eta components).
 template <int NC, bool TABLE>
-hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, int tile, const EvDesc *ev,
+hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, int tile, bool bi, const EvDesc *ev,
                           const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
-    if (!dim3d) {
+    if (!dim3d && bi) {
+        switch (tile) {
+        case 0: return launch_mx<NC, TABLE, false, 16, 15, 1, true>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 1: return launch_mx<NC, TABLE, false, 32, 7, 1, true>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        case 2: return launch_mx<NC, TABLE, false, 64, 3, 1, true>(F, E, ev, blk_ev, blk_first, nblocks, st);
+        }
+    } else if (!dim3d) {
         switch (tile) {
         case 0: return launch_mx<NC, TABLE, false, 16, 15, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
         case 1: return launch_mx<NC, TABLE, false, 32, 7, 1>(F, E, ev, blk_ev, blk_first, nblocks, st);
@@ -92,16 +100,16 @@ hipError_t launch_mx_tile(const mx::MxFields &F, const EosDev &E, bool dim3d, in
 const int MIXED_TILES_2D[MIXED_NTILE][3] = {{16, 15, 1}, {32, 7, 1}, {64, 3, 1}};
 const int MIXED_TILES_3D[MIXED_NTILE][3] = {{6, 6, 6}, {8, 8, 3}, {8, 4, 6}};
 
-hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, const EvDesc *ev,
-                              const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
+hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, bool bi,
+                              const EvDesc *ev, const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st) {
     if (nblocks <= 0) return hipSuccess;
-    if (tile < 0 || tile >= MIXED_NTILE) return hipErrorInvalidValue;
+    if (tile < 0 || tile >= MIXED_NTILE || (bi && dim3d)) return hipErrorInvalidValue;
     const bool tab = E.kind == 1;
     if (nc == 14)
-        return tab ? launch_mx_tile<14, true>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st)
-                   : launch_mx_tile<14, false>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st);
-    return tab ? launch_mx_tile<15, true>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st)
-               : launch_mx_tile<15, false>(F, E, dim3d, tile, ev, blk_ev, blk_first, nblocks, st);
+        return tab ? launch_mx_tile<14, true>(F, E, dim3d, tile, bi, ev, blk_ev, blk_first, nblocks, st)
+                   : launch_mx_tile<14, false>(F, E, dim3d, tile, bi, ev, blk_ev, blk_first, nblocks, st);
+    return tab ? launch_mx_tile<15, true>(F, E, dim3d, tile, bi, ev, blk_ev, blk_first, nblocks, st)
+               : launch_mx_tile<15, false>(F, E, dim3d, tile, bi, ev, blk_ev, blk_first, nblocks, st);
 }
 
 hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,

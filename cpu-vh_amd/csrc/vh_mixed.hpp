@@ -33,23 +33,29 @@ constexpr float delta_PiPi = 0.666667f;
 constexpr float lambda_piPi = 1.2f;
 constexpr float THETA_F = 1.1f;         // rhic/include/FluxLimiter.h:13
 
-// Device pointers of one stage (pooled state of a batch).
+// Device pointers of one stage (pooled state of a batch).  The fields of a
+// group lie cap cells apart in the pool, so a group is one base pointer and
+// field n of it is at(base, n): a few kernel arguments instead of one pointer
+// per field, which kept scalar registers from spilling.
 struct MxFields {
-    const float *q[15];   // stage input, high parts
-    const float *ql[4];   // stage input, low parts of T^{tau mu}
-    const float *qn[15];  // q^n (stage 2)
-    const float *qnl[4];
-    float *qo[15];
-    float *qol[4];
-    const float *u[4];    // stage input velocity
-    const float *up[4];   // velocity of the time derivative
-    const float *e, *p;   // stage input e, p
-    float *uo[4];
+    const float *q;    // stage input, high parts (15 fields)
+    const float *ql;   // stage input, low parts of T^{tau mu} (4)
+    const float *qn;   // q^n (stage 2)
+    const float *qnl;
+    float *qo;
+    float *qol;
+    const float *u;    // stage input velocity (4)
+    const float *up;   // velocity of the time derivative
+    const float *e, *p;  // stage input e, p
+    float *uo;
     float *eo, *po;
+    unsigned long cap;  // cells between consecutive fields of a group
     int stage;            // 0: first stage, 1: second stage
     int off;              // step offset inside a captured graph
     int face_mode;        // see faces_mx
     int *err;
+    template <class T>
+    __device__ __forceinline__ T *at(T *base, int n) const { return base + cap * (unsigned long)n; }
 };
 
 // rhic/src/FluxLimiter.cpp:14-32 in FP32: minmod(a, minmod(b, c)) of
@@ -77,6 +83,19 @@ __device__ __forceinline__ void stf(float *__restrict__ base, unsigned idx, floa
     *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (idx << 2)) = v;
 }
 
+// Boost-invariant 2+1D events (nz = 1 with u^eta, pi^{tau eta}, pi^{x eta}
+// and pi^{y eta} zero initially): T^{tau eta} (q[3]), pi^{tau eta} (q[7]),
+// pi^{x eta} (q[10]) and pi^{y eta} (q[12]) are zero and stay exactly zero
+// under the update (every term of their fluxes and sources carries one of
+// them, u^eta or an eta derivative, all zero), so the BI kernels neither load
+// nor evolve them and store zeros.
+constexpr bool dead_field(int n) { return n == 3 || n == 7 || n == 10 || n == 12; }
+template <bool BI>
+constexpr bool field_live(int n) { return !BI || !dead_field(n); }
+// LDS slot of a live field
+template <bool BI>
+constexpr int field_slot(int n) { return BI ? n - (n > 3) - (n > 7) - (n > 10) - (n > 12) : n; }
+
 template <bool TABLE>
 __device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
     if constexpr (TABLE) {
@@ -102,14 +121,14 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float pitt = q[4], pitx = q[5], pity = q[6], pitn = q[7], pixx = q[8], pixy = q[9], pixn = q[10],
                 piyy = q[11], piyn = q[12], pinn = q[13];
     const float Pi = NC == 15 ? q[NC - 1] : 0.0f;
-    const float *__restrict__ utv = F.u[0];
-    const float *__restrict__ uxv = F.u[1];
-    const float *__restrict__ uyv = F.u[2];
-    const float *__restrict__ unv = F.u[3];
+    const float *__restrict__ utv = F.at(F.u, 0);
+    const float *__restrict__ uxv = F.at(F.u, 1);
+    const float *__restrict__ uyv = F.at(F.u, 2);
+    const float *__restrict__ unv = F.at(F.u, 3);
     const float *__restrict__ pv = F.p;
     const float p = ldf(pv, s);
     const float ut = ldf(utv, s), ux = ldf(uxv, s), uy = ldf(uyv, s), un = ldf(unv, s);
-    const float utp = ldf(F.up[0], s), uxp = ldf(F.up[1], s), uyp = ldf(F.up[2], s), unp = ldf(F.up[3], s);
+    const float utp = ldf(F.at(F.up, 0), s), uxp = ldf(F.at(F.up, 1), s), uyp = ldf(F.at(F.up, 2), s), unp = ldf(F.at(F.up, 3), s);
     const unsigned strY = G.nx, strZ = (unsigned)G.nx * G.ny;
     const unsigned sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
     const unsigned sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
@@ -154,7 +173,7 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
         const float vd[3] = {vx, vy, vn};
 #pragma unroll
         for (int d = 0; d < (DIM3 ? 3 : 2); ++d) {
-            auto dq = [&](int n) { return (ldf(F.q[n], nbp[d]) - ldf(F.q[n], nbm[d])) * fac[d]; };
+            auto dq = [&](int n) { return (ldf(F.at(F.q, n), nbp[d]) - ldf(F.at(F.q, n), nbm[d])) * fac[d]; };
 #pragma unroll
             for (int mu = 0; mu < 4; ++mu) Sd[d][mu] = dq(4 + mu) * vd[d] - dq(pi_a_mu(d, mu));
             if constexpr (NC == 15) {
@@ -303,6 +322,163 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 
 }
 
+// source_mx for a boost-invariant 2+1D cell: the same expressions with
+// u^eta, pi^{tau eta}, pi^{x eta}, pi^{y eta} and every eta derivative zero
+// (S[3], S[7], S[10], S[12] and Sd[.][3] are zero and not formed).
+template <int NC, bool TABLE>
+__device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G, const EosDev &E, unsigned s, int i,
+                                             int j, float t, float ecell, const float (&q)[NC], float (&S)[NC],
+                                             float (&Sd)[3][4]) {
+#pragma clang fp contract(fast)
+    const float rdx = 1.0f / (float)G.dx, rdy = 1.0f / (float)G.dy;
+    const float rt = 1.0f / t, rdt = 1.0f / (float)G.dt;
+    const float r3 = 1.0f / 3.0f, r5 = 1.0f / 5.0f;
+    const float ttt = q[0], ttx = q[1], tty = q[2];
+    const float pitt = q[4], pitx = q[5], pity = q[6], pixx = q[8], pixy = q[9], piyy = q[11], pinn = q[13];
+    const float Pi = NC == 15 ? q[NC - 1] : 0.0f;
+    const float *__restrict__ utv = F.at(F.u, 0);
+    const float *__restrict__ uxv = F.at(F.u, 1);
+    const float *__restrict__ uyv = F.at(F.u, 2);
+    const float *__restrict__ pv = F.p;
+    const float p = ldf(pv, s);
+    const float ut = ldf(utv, s), ux = ldf(uxv, s), uy = ldf(uyv, s);
+    const float utp = ldf(F.at(F.up, 0), s), uxp = ldf(F.at(F.up, 1), s), uyp = ldf(F.at(F.up, 2), s);
+    const unsigned strY = G.nx;
+    const unsigned sl = i > 0 ? s - 1 : s, sr = i < G.nx - 1 ? s + 1 : s;
+    const unsigned sb = j > 0 ? s - strY : s, stp = j < G.ny - 1 ? s + strY : s;
+    const float dxut = deriv_f(ldf(utv, sl), ut, ldf(utv, sr)) * rdx;
+    const float dxux = deriv_f(ldf(uxv, sl), ux, ldf(uxv, sr)) * rdx;
+    const float dxuy = deriv_f(ldf(uyv, sl), uy, ldf(uyv, sr)) * rdx;
+    const float dyut = deriv_f(ldf(utv, sb), ut, ldf(utv, stp)) * rdy;
+    const float dyux = deriv_f(ldf(uxv, sb), ux, ldf(uxv, stp)) * rdy;
+    const float dyuy = deriv_f(ldf(uyv, sb), uy, ldf(uyv, stp)) * rdy;
+    const float dxp = deriv_f(ldf(pv, sl), p, ldf(pv, sr)) * rdx;
+    const float dyp = deriv_f(ldf(pv, sb), p, ldf(pv, stp)) * rdy;
+
+    const float pP = p + Pi;
+    const float tnn = pP * rt * rt + pinn;
+    const float rut = 1.0f / ut;
+    const float vx = ux * rut, vy = uy * rut;
+    const float dxvx = (dxux - vx * dxut) * rut;
+    const float dyvy = (dyuy - vy * dyut) * rut;
+    const float dkvk = dxvx + dyvy;
+    S[0] = -(ttt * rt + t * tnn) + dkvk * (pitt - p - Pi) - vx * dxp - vy * dyp;
+    S[1] = -ttx * rt - dxp + dkvk * pitx;
+    S[2] = -tty * rt - dyp + dkvk * pity;
+    S[3] = 0.0f;
+
+    {
+        const unsigned nbm[2] = {sl, sb}, nbp[2] = {sr, stp};
+        const float fac[2] = {0.5f * rdx, 0.5f * rdy};
+        const float vd[2] = {vx, vy};
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            auto dq = [&](int n) { return (ldf(F.at(F.q, n), nbp[d]) - ldf(F.at(F.q, n), nbm[d])) * fac[d]; };
+#pragma unroll
+            for (int mu = 0; mu < 3; ++mu) Sd[d][mu] = dq(4 + mu) * vd[d] - dq(pi_a_mu(d, mu));
+            Sd[d][3] = 0.0f;
+            if constexpr (NC == 15) {
+                const float dPi = dq(14);
+                Sd[d][0] -= vd[d] * dPi;
+                Sd[d][d + 1] -= dPi;
+            }
+        }
+        Sd[2][0] = Sd[2][1] = Sd[2][2] = Sd[2][3] = 0.f;
+    }
+
+    const float e = ecell;
+    const float T = temperature_f<TABLE>(E, e);
+    const float d_etabar = (float)etabar_of_T(G, (double)T);
+    const float taupiInv = T * r5 / d_etabar;
+    const float beta_pi = (e + p) * r5;
+    DivRecip dx64;
+    const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
+    const float a = 1.0f / 3.0f - cs2;
+    const float a2 = a * a;
+    const float beta_Pi = 15 * a2 * (e + p);
+    const float lambda_Pipi = 8 * a * r5;
+    float tauPiInv = 0.0f;
+    if constexpr (NC == 15) {
+        if (a2 != 0.0f) {
+            const float x = T / 1.01355f;
+            float zetabar;
+            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) / 0.025f) + 0.25f * __expf(-(x - 1) / 0.13f) + 0.001f;
+            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) / 0.0025f) + 0.22f * __expf((x - 1) / 0.022f) + 0.03f;
+            else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
+            tauPiInv = 15 * a2 * T / zetabar;
+        }
+    }
+    const float ut2 = ut * ut, t2 = t * t, t3 = t * t2;
+    const float rt2 = 1.0f / t2, rt3 = 1.0f / t3;
+    const float dtut = (ut - utp) * rdt;
+    const float dtux = (ux - uxp) * rdt;
+    const float dtuy = (uy - uyp) * rdt;
+    const float Dut = ut * dtut + ux * dxut + uy * dyut;
+    const float dux = ut * dtux + ux * dxux + uy * dyux;
+    const float duy = ut * dtuy + ux * dxuy + uy * dyuy;
+    const float Dux = -dux, Duy = -duy;
+    const float dut = Dut;
+    const float theta = (ut) * rt + dtut + dxux + dyuy;
+    const float stt = (dtut - ut * dut) + ((ut2 - 1) * theta) * r3;
+    const float stx = (dtux - dxut) / 2.0f - (ux * dut + ut * dux) / 2.0f + (ut * ux * theta) * r3;
+    const float sty = (dtuy - dyut) / 2.0f - (uy * dut + ut * duy) / 2.0f + (ut * uy * theta) * r3;
+    const float sxx = -(dxux + ux * dux) + ((1.0f + ux * ux) * theta) * r3;
+    const float sxy = -(dxuy + dyux) / 2.0f - (uy * dux + ux * duy) / 2.0f + (ux * uy * theta) * r3;
+    const float syy = -(dyuy + uy * duy) + ((1.0f + uy * uy) * theta) * r3;
+    const float snn = (-ut) * rt3 + (rt2 * theta) * r3;
+    const float wtx = (-dtux - dxut + ut * dux - ux * dut) / 2.0f;
+    const float wty = (-dtuy - dyut + ut * duy - uy * dut) / 2.0f;
+    const float wxy = (dxuy - dyux + ux * duy - uy * dux) / 2.0f;
+    const float wxt = wtx, wyt = wty, wyx = -wxy;
+    const float I1tt = 2.0f * ut * (pitt * Dut + pitx * Dux + pity * Duy);
+    const float I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy;
+    const float I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy;
+    const float I1xx = 2.0f * ux * (pitx * Dut + pixx * Dux + pixy * Duy);
+    const float I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy;
+    const float I1yy = 2.0f * uy * (pity * Dut + pixy * Dux + piyy * Duy);
+    const float I3tt = 2.0f * (pitx * wtx + pity * wty);
+    const float I3tx = pitt * wxt + pity * wxy + pixx * wtx + pixy * wty;
+    const float I3ty = pitt * wyt + pitx * wyx + pixy * wtx + piyy * wty;
+    const float I3xx = 2.0f * (pitx * wxt + pixy * wxy);
+    const float I3xy = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy;
+    const float I3yy = 2.0f * (pity * wyt + pixy * wyx);
+    const float ps = pitt * stt - 2.0f * pitx * stx - 2.0f * pity * sty + pixx * sxx + 2.0f * pixy * sxy + piyy * syy +
+                     pinn * snn * t2 * t2;
+    const float I4tt = (pitt * stt - pitx * stx - pity * sty) - ((1.0f - ut2) * ps) * r3;
+    const float I4tx = (pitt * stx + pitx * stt) / 2.0f - (pitx * sxx + pixx * stx) / 2.0f -
+                       (pity * sxy + pixy * sty) / 2.0f + ((ut * ux) * ps) * r3;
+    const float I4ty = (pitt * sty + pity * stt) / 2.0f - (pitx * sxy + pixy * stx) / 2.0f -
+                       (pity * syy + piyy * sty) / 2.0f + ((ut * uy) * ps) * r3;
+    const float I4xx = (pitx * stx - pixx * sxx - pixy * sxy) + ((1.0f + ux * ux) * ps) * r3;
+    const float I4xy = (pitx * sty + pity * stx) / 2.0f - (pixx * sxy + pixy * sxx) / 2.0f -
+                       (pixy * syy + piyy * sxy) / 2.0f + ((ux * uy) * ps) * r3;
+    const float I4yy = (pity * sty - pixy * sxy - piyy * syy) + ((1.0f + uy * uy) * ps) * r3;
+    const float I4nn = (-t2 * pinn * snn) + (rt2 * ps) * r3;
+    const float Itt = I1tt + delta_pipi * (theta * pitt) - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
+    const float Itx = I1tx + delta_pipi * (theta * pitx) - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
+    const float Ity = I1ty + delta_pipi * (theta * pity) - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
+    const float Ixx = I1xx + delta_pipi * (theta * pixx) - I3xx + tau_pipi * I4xx - lambda_piPi * Pi * sxx;
+    const float Ixy = I1xy + delta_pipi * (theta * pixy) - I3xy + tau_pipi * I4xy - lambda_piPi * Pi * sxy;
+    const float Iyy = I1yy + delta_pipi * (theta * piyy) - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
+    const float Inn = delta_pipi * (theta * pinn) + tau_pipi * I4nn - lambda_piPi * Pi * snn;
+    S[4] = (2.0f * beta_pi * stt - pitt * taupiInv - Itt) * rut + pitt * dkvk;
+    S[5] = (2.0f * beta_pi * stx - pitx * taupiInv - Itx) * rut + pitx * dkvk;
+    S[6] = (2.0f * beta_pi * sty - pity * taupiInv - Ity) * rut + pity * dkvk;
+    S[7] = 0.0f;
+    S[8] = (2.0f * beta_pi * sxx - pixx * taupiInv - Ixx) * rut + pixx * dkvk;
+    S[9] = (2.0f * beta_pi * sxy - pixy * taupiInv - Ixy) * rut + pixy * dkvk;
+    S[10] = 0.0f;
+    S[11] = (2.0f * beta_pi * syy - piyy * taupiInv - Iyy) * rut + piyy * dkvk;
+    S[12] = 0.0f;
+    S[13] = (2.0f * beta_pi * snn - pinn * taupiInv - Inn - (2 * (ut * pinn)) * rt) * rut + pinn * dkvk;
+    if constexpr (NC == 15) {
+        const float dPi = -beta_Pi * theta - Pi * tauPiInv - delta_PiPi * Pi * theta + lambda_Pipi * ps;
+        S[14] = (dPi) * rut + Pi * dkvk;
+    } else {
+        (void)beta_Pi; (void)lambda_Pipi; (void)tauPiInv;
+    }
+}
+
 // One face flux between cells a = b-1 and b along dir, b = (i, j, k).  The
 // limited slopes of the two cells come from the 4-point line (clamped at the
 // lattice edges, which equals the reference's ghost cells), the half-site
@@ -320,9 +496,9 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
 // reference's semantics (cell b's view is evaluated only where it can
 // differ); 2: cell a's view, plus cell b's view only where the two cells fall
 // on different sides of the e <= 0.1 branch.
-template <int NC, bool TABLE, bool TWO, int NF>
+template <int NC, bool TABLE, bool TWO, int NF, bool BI, int NL>
 __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, const EosDev &E, int dir, int i, int j,
-                                        int k, double t, float (&HA)[NC][NF], float (&HB)[TWO ? NC : 1][TWO ? NF : 1],
+                                        int k, double t, float (&HA)[NL][NF], float (&HB)[TWO ? NL : 1][TWO ? NF : 1],
                                         int f) {
 #pragma clang fp contract(fast)
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -336,7 +512,11 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
     float qL[NC], qR[NC];
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
-        const float *__restrict__ fld = F.q[n];
+        if (!field_live<BI>(n)) {
+            qL[n] = qR[n] = 0.0f;
+            continue;
+        }
+        const float *__restrict__ fld = F.at(F.q, n);
         const float x0 = ldf(fld, p0), x1 = ldf(fld, p1), x2 = ldf(fld, p2), x3 = ldf(fld, p3);
         qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
         qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
@@ -370,17 +550,23 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
             if (q == kk) v[q] = vs;
     }
     if (nan != NAN_NONE) report_nan(F.err, nan, p2);
+    // H = (v_R qR + v_L qL - a (qR - qL)) / 2, with the halves taken into
+    // the speeds (exact: a power of two)
     const float aA = __builtin_fmaxf(__builtin_fabsf(v[0]), __builtin_fabsf(v[1]));
+    const float hRA = 0.5f * v[1], hLA = 0.5f * v[0], haA = 0.5f * aA;
 #pragma unroll
-    for (int n = 0; n < NC; ++n) HA[n][f] = 0.5f * (v[1] * qR[n] + v[0] * qL[n] - aA * (qR[n] - qL[n]));
+    for (int n = 0; n < NC; ++n)
+        if (field_live<BI>(n)) HA[field_slot<BI>(n)][f] = hRA * qR[n] + hLA * qL[n] - haA * (qR[n] - qL[n]);
     if constexpr (TWO) {
         if (!viewB) {
             v[2] = v[0];
             v[3] = v[1];
         }
         const float aB = __builtin_fmaxf(__builtin_fabsf(v[2]), __builtin_fabsf(v[3]));
+        const float hRB = 0.5f * v[3], hLB = 0.5f * v[2], haB = 0.5f * aB;
 #pragma unroll
-        for (int n = 0; n < NC; ++n) HB[n][f] = 0.5f * (v[3] * qR[n] + v[2] * qL[n] - aB * (qR[n] - qL[n]));
+        for (int n = 0; n < NC; ++n)
+            if (field_live<BI>(n)) HB[field_slot<BI>(n)][f] = hRB * qR[n] + hLB * qL[n] - haB * (qR[n] - qL[n]);
     }
 }
 
@@ -443,13 +629,15 @@ struct Tile {
 // cell forms q + dt (S + divergence + S_x + S_y + S_eta) with the FP64
 // accumulation, recovers (e, p, u), regulates and stores.  TWO: both views
 // of a face are stored (face modes 1, 2).
-template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO, bool BI>
 __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev &E, const EvDesc *__restrict__ ev,
                                                  const int *__restrict__ blk_ev, const int *__restrict__ blk_first) {
 #pragma clang fp contract(fast)
+    static_assert(!(BI && DIM3), "boost invariance is a 2+1D property");
     using T = Tile<TX, TY, TZ, DIM3>;
-    __shared__ float HA[NC][T::FMAX];
-    __shared__ float HB[TWO ? NC : 1][TWO ? T::FMAX : 1];
+    constexpr int NL = BI ? NC - 4 : NC;  // fields evolved
+    __shared__ float HA[NL][T::FMAX];
+    __shared__ float HB[TWO ? NL : 1][TWO ? T::FMAX : 1];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blk]);
     const int tile = blk - __builtin_amdgcn_readfirstlane(blk_first[blk]);
@@ -481,7 +669,7 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
             const int fi = i0 + fx, fj = j0 + fy, fk = k0 + fz;
             const bool need = (d == 0 ? fi - 1 < G.nx : fi < G.nx) && (d == 1 ? fj - 1 < G.ny : fj < G.ny) &&
                               (d == 2 ? fk - 1 < G.nz : fk < G.nz);
-            if (need) face_mx<NC, TABLE, TWO, T::FMAX>(F, G, E, d, fi, fj, fk, t, HA, HB, f);
+            if (need) face_mx<NC, TABLE, TWO, T::FMAX, BI, NL>(F, G, E, d, fi, fj, fk, t, HA, HB, f);
         }
         __syncthreads();
         if (live) {
@@ -490,9 +678,11 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
             const int fstep = d == 0 ? 1 : (d == 1 ? ex : ex * ey);
 #pragma unroll
             for (int n = 0; n < NC; ++n) {
+                if (!field_live<BI>(n)) continue;
                 // the face before the cell is seen from its b side, the face after from its a side
-                const float hb = TWO ? HB[TWO ? n : 0][TWO ? fb : 0] : HA[n][fb];
-                fdiv[n] += (hb - HA[n][fb + fstep]) * rd;
+                const int sl = field_slot<BI>(n);
+                const float hb = TWO ? HB[TWO ? sl : 0][TWO ? fb : 0] : HA[sl][fb];
+                fdiv[n] += (hb - HA[sl][fb + fstep]) * rd;
             }
         }
         if (d + 1 < (DIM3 ? 3 : 2)) __syncthreads();  // the next direction reuses the face buffer
@@ -507,15 +697,19 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
     {
         float q[NC], S[NC], Sd[3][4];
 #pragma unroll
-        for (int n = 0; n < NC; ++n) q[n] = ldf(F.q[n], s);
-        source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, ldf(F.e, s), q, S, Sd);
+        for (int n = 0; n < NC; ++n) q[n] = field_live<BI>(n) ? ldf(F.at(F.q, n), s) : 0.0f;
+        if constexpr (BI) source_mx_bi<NC, TABLE>(F, G, E, s, i, j, (float)t, ldf(F.e, s), q, S, Sd);
+        else source_mx<NC, TABLE, DIM3>(F, G, E, s, i, j, k, (float)t, ldf(F.e, s), q, S, Sd);
         const float dt = (float)G.dt;
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
             float r = fdiv[n];
-            if (n < 4) {
+            if (!field_live<BI>(n)) {
+                if (n < 4) acc[n] = 0.0;
+                else pv[n - 4] = 0.0f;
+            } else if (n < 4) {
                 r += DIM3 ? (Sd[0][n] + Sd[1][n]) + Sd[2][n] : Sd[0][n] + Sd[1][n];
-                const double qn = (double)q[n] + (double)ldf(F.ql[n], s);
+                const double qn = (double)q[n] + (double)ldf(F.at(F.ql, n), s);
                 acc[n] = (qn + (double)(dt * S[n])) + (double)(dt * r);
             } else {
                 pv[n - 4] = (float)(((double)q[n] + (double)(dt * S[n])) + (double)(dt * r));
@@ -525,8 +719,9 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
     if (F.stage == 1) {
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
-            if (n < 4) acc[n] = (acc[n] + ((double)ldf(F.qn[n], s) + (double)ldf(F.qnl[n], s))) * 0.5;
-            else pv[n - 4] = (pv[n - 4] + ldf(F.qn[n], s)) * 0.5f;
+            if (!field_live<BI>(n)) continue;
+            if (n < 4) acc[n] = (acc[n] + ((double)ldf(F.at(F.qn, n), s) + (double)ldf(F.at(F.qnl, n), s))) * 0.5;
+            else pv[n - 4] = (pv[n - 4] + ldf(F.at(F.qn, n), s)) * 0.5f;
         }
     }
     int nan = NAN_NONE;
@@ -539,35 +734,35 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
         const float hi = (float)acc[n];
-        stf(F.qo[n], s, hi);
-        stf(F.qol[n], s, (float)(acc[n] - (double)hi));
+        stf(F.at(F.qo, n), s, hi);
+        stf(F.at(F.qol, n), s, (float)(acc[n] - (double)hi));
     }
 #pragma unroll
-    for (int n = 4; n < NC; ++n) stf(F.qo[n], s, pv[n - 4]);
+    for (int n = 4; n < NC; ++n) stf(F.at(F.qo, n), s, pv[n - 4]);
     stf(F.eo, s, (float)w.e);
     stf(F.po, s, (float)w.p);
-    stf(F.uo[0], s, (float)w.ut);
-    stf(F.uo[1], s, (float)w.ux);
-    stf(F.uo[2], s, (float)w.uy);
-    stf(F.uo[3], s, (float)w.un);
+    stf(F.at(F.uo, 0), s, (float)w.ut);
+    stf(F.at(F.uo, 1), s, (float)w.ux);
+    stf(F.at(F.uo, 2), s, (float)w.uy);
+    stf(F.at(F.uo, 3), s, BI ? 0.0f : (float)w.un);
     if (nan != NAN_NONE) report_nan(F.err, nan, s);
 }
 
 // 2+1D: 128 VGPRs (four waves per SIMD) with a 12-byte spill measured
 // faster than 3 waves without; 3+1D keeps the register budget the compiler
 // picks (it would spill ~110 bytes at four waves).
-template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
+template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO, bool BI>
 __global__ __launch_bounds__(MX_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void stage_mixed_kernel(
     const MxFields F, const EosDev E, const EvDesc *__restrict__ ev, const int *__restrict__ blk_ev,
     const int *__restrict__ blk_first) {
-    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO>(F, E, ev, blk_ev, blk_first);
+    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO, BI>(F, E, ev, blk_ev, blk_first);
 }
 template <int NC, bool TABLE, bool DIM3, int TX, int TY, int TZ, bool TWO>
 __global__ __launch_bounds__(MX_THREADS) void stage_mixed_kernel_3d(const MxFields F, const EosDev E,
                                                                     const EvDesc *__restrict__ ev,
                                                                     const int *__restrict__ blk_ev,
                                                                     const int *__restrict__ blk_first) {
-    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO>(F, E, ev, blk_ev, blk_first);
+    stage_mixed_body<NC, TABLE, DIM3, TX, TY, TZ, TWO, false>(F, E, ev, blk_ev, blk_first);
 }
 
 }  // namespace mx

@@ -129,3 +129,54 @@ def test_mixed_autotune_keeps_state():
     for i1, i2 in zip(ids1, ids2):
         for f in ("e", "ut", "pitt"):
             assert V.bits_equal(b1.get(i1, f), b2.get(i2, f)), f
+
+
+BI_EVENTS = [
+    (41, 37, 1, 0.10, 0.020, 0.60, 0.12, 0.10, (0.5, 1.0), 21),
+    (50, 9, 1, 0.08, 0.010, 1.00, 0.20, 0.00, (0.0, 0.0), 22),
+    (33, 45, 1, 0.10, 0.020, 0.40, 0.08, 0.15, (0.5, 1.0), 23),
+]
+ETA = ["ttn", "pitn", "pixn", "piyn", "un"]
+
+
+def _bi_ic(ev, lattice, bulk, tiny=0.0):
+    """A boost-invariant 2+1D state: u^eta and pi^{tau eta}, pi^{x eta},
+    pi^{y eta} zero (tiny > 0 puts that much pi^{tau eta} in one cell, which
+    makes the event general)."""
+    ic = _ic(ev, lattice, bulk)
+    N = ev[0] * ev[1] * ev[2]
+    for c in (3, 6, 8):
+        ic["pi"][c * N:(c + 1) * N] = 0.0
+    ic["pi"][3 * N + N // 2] = tiny
+    return ic
+
+
+@pytest.mark.parametrize("lattice,bulk", [(False, False), (True, True)])
+def test_boost_invariant_kernel(lattice, bulk):
+    """Boost-invariant 2+1D events run the kernel that neither loads nor
+    evolves the eta components: against FP64 to the mixed form's bar, their
+    eta components stay exactly zero, and they agree with the general kernel
+    (the same events made general by a 1e-30 pi^{tau eta} in one cell) to
+    that bar."""
+    print(f"case boost-invariant lattice={lattice} bulk={bulk}", flush=True)
+    nsteps = 30
+    ics = [_bi_ic(ev, lattice, bulk) for ev in BI_EVENTS]
+    b64, ids64 = _batch(BI_EVENTS, ics, lattice, bulk, 0, nsteps)
+    bmx, idsmx = _batch(BI_EVENTS, ics, lattice, bulk, PREC_MIXED, nsteps)
+    gics = [_bi_ic(ev, lattice, bulk, tiny=1e-30) for ev in BI_EVENTS]
+    bgen, idsgen = _batch(BI_EVENTS, gics, lattice, bulk, PREC_MIXED, nsteps)
+    for ev, i64, imx, igen in zip(BI_EVENTS, ids64, idsmx, idsgen):
+        ref = {f: b64.get(i64, f) for f in PRIM + PI}
+        got = {f: bmx.get(imx, f) for f in PRIM + PI}
+        gen = {f: bgen.get(igen, f) for f in PRIM + PI}
+        for f in ETA:
+            assert not np.any(bmx.get(imx, f)), f
+        for name, other in (("fp64", ref), ("general", gen)):
+            err = _rel_errors(other, got)
+            print(ev[:3], name, {k: f"{v:.2e}" for k, v in err.items() if k not in ("un", "pitn", "pixn", "piyn")},
+                  flush=True)
+            assert err["e_pointwise"] < 1e-3 and err["ut_pointwise"] < 1e-3, (name, err)
+            for f in ("ux", "uy"):
+                assert err[f] < 1e-3, (name, f, err[f])
+            for f in ("pitt", "pitx", "pity", "pixx", "pixy", "piyy", "pinn"):
+                assert err[f] < 1e-2, (name, f, err[f])
