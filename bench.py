@@ -522,9 +522,16 @@ def main():
         print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
         _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.steps, args.warmup)
 
-    # dominant kernel: the mixed stage kernel over the batch (two launches per step)
-    nc = 15
-    bytes_per_zone_mx = (5 * nc + 32 + 5 * 4) * 4  # FP32 fields + FP32 low parts of T^{tau mu}
+    # dominant kernel: the mixed stage kernel over the batch (two launches per
+    # step).  Algorithmic bytes of one zone-update (two stages): each stage
+    # reads q (nl FP32 fields + the low parts of T^{tau mu}), u, the previous
+    # u, e and p, and writes q, u, e and p; stage 2 also reads q^n.  The
+    # boost-invariant kernel moves the 11 fields without eta components, 3
+    # low parts and 3 velocity components; the general one 15, 4 and 4.
+    bi = all(not np.any(ev["ic"]["un"]) and not np.any(ev["ic"]["pi"].reshape(10, -1)[[3, 6, 8]])
+             for ev in events)
+    nl, nlo, nu = (11, 3, 3) if bi else (15, 4, 4)
+    bytes_per_zone_mx = (2 * (nl + nlo + 2 * nu + 2) + 2 * (nl + nlo + nu + 2) + (nl + nlo)) * 4
     achieved = mixed["zone_updates"] * bytes_per_zone_mx / (mixed["graph_ms"] * 1e-3) / 1e9
     launches = 2 * mixed["batched_steps"]
     per_cell = load_traffic("stage_mixed_kernel_batch_pmc.json", "hbm_bytes_per_cell_stage")
@@ -572,7 +579,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_mx,
                      "traffic_note": "HBM bytes per average launch: bytes per cell-stage measured with rocprofv3 "
                                      "(profiles/stage_mixed_kernel__batch256_2d.md) x the run's mean cells per launch",
-                     "kernel": "stage_mixed_kernel (one RK stage, all 2+1D events)",
+                     "kernel": "stage_mixed_kernel (one RK stage, all 2+1D events"
+                               + (", boost-invariant)" if bi else ")"),
                      "avg_launch_ms": mixed["graph_ms"] / launches,
                      "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},
         "cpu_baseline": cpu,

@@ -625,6 +625,11 @@ vh_status vh_batch_download(vh_batch *b, int id, int field, double *out) {
     }
     const float *hi = nullptr, *lo = nullptr;
     if (!field_of_mixed(b, field, &hi, &lo)) return VH_ERR_ARG;
+    // the eta components of a boost-invariant event are zero and not stored
+    if (E.bi && (field == 3 || field == 7 || field == 10 || field == 12 || field == 20 || field == 24)) {
+        std::fill(out, out + E.N, 0.0);
+        return vh_batch_sync(b);
+    }
     double *tmp = nullptr;
     BCHK(hipMalloc(&tmp, sizeof(double) * E.N));
     hipError_t r = launch_widen(hi + E.off, lo ? lo + E.off : nullptr, tmp, E.N, b->stream);

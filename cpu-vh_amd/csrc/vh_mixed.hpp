@@ -87,8 +87,8 @@ __device__ __forceinline__ void stf(float *__restrict__ base, unsigned idx, floa
 // and pi^{y eta} zero initially): T^{tau eta} (q[3]), pi^{tau eta} (q[7]),
 // pi^{x eta} (q[10]) and pi^{y eta} (q[12]) are zero and stay exactly zero
 // under the update (every term of their fluxes and sources carries one of
-// them, u^eta or an eta derivative, all zero), so the BI kernels neither load
-// nor evolve them and store zeros.
+// them, u^eta or an eta derivative, all zero), so the BI kernels neither
+// load, evolve nor store them.
 constexpr bool dead_field(int n) { return n == 3 || n == 7 || n == 10 || n == 12; }
 template <bool BI>
 constexpr bool field_live(int n) { return !BI || !dead_field(n); }
@@ -731,20 +731,24 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
                                                       NC == 15 ? (double)pv[NC - 5] : 0.0, (double)ldf(F.e, s), nan,
                                                       div);
     regulate_mx<NC>(G, t_new, pv, w, nan);
+    // a boost-invariant event's eta components are not stored: only BI
+    // kernels read its cells, and downloads give those fields as zero
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
+        if (!field_live<BI>(n)) continue;
         const float hi = (float)acc[n];
         stf(F.at(F.qo, n), s, hi);
         stf(F.at(F.qol, n), s, (float)(acc[n] - (double)hi));
     }
 #pragma unroll
-    for (int n = 4; n < NC; ++n) stf(F.at(F.qo, n), s, pv[n - 4]);
+    for (int n = 4; n < NC; ++n)
+        if (field_live<BI>(n)) stf(F.at(F.qo, n), s, pv[n - 4]);
     stf(F.eo, s, (float)w.e);
     stf(F.po, s, (float)w.p);
     stf(F.at(F.uo, 0), s, (float)w.ut);
     stf(F.at(F.uo, 1), s, (float)w.ux);
     stf(F.at(F.uo, 2), s, (float)w.uy);
-    stf(F.at(F.uo, 3), s, BI ? 0.0f : (float)w.un);
+    if constexpr (!BI) stf(F.at(F.uo, 3), s, (float)w.un);
     if (nan != NAN_NONE) report_nan(F.err, nan, s);
 }
 
