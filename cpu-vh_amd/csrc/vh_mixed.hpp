@@ -54,8 +54,15 @@ struct MxFields {
     int off;              // step offset inside a captured graph
     int face_mode;        // see faces_mx
     int *err;
+    // The empty asm makes the stride opaque at each use, so field addresses
+    // are formed (scalar instructions) where they are used; hoisted, all of
+    // them at once overflow the scalar registers into spills.
     template <class T>
-    __device__ __forceinline__ T *at(T *base, int n) const { return base + cap * (unsigned long)n; }
+    __device__ __forceinline__ T *at(T *base, int n) const {
+        unsigned long c = cap;
+        asm volatile("" : "+s"(c));
+        return base + c * (unsigned long)n;
+    }
 };
 
 // rhic/src/FluxLimiter.cpp:14-32 in FP32: minmod(a, minmod(b, c)) of
@@ -71,6 +78,10 @@ __device__ __forceinline__ float deriv_f(float x, float y, float z) {
 }
 
 __device__ __forceinline__ int clampi(int c, int n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+
+// FP32 reciprocal of the hardware (1 ulp), in place of a division sequence
+// that scales operands for subnormal results
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Field access by a 32-bit cell index: a scalar base plus a 32-bit byte
 // offset, which the hardware adds itself (global loads with a scalar base),
@@ -114,8 +125,8 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
                                           int k, float t, float ecell, const float (&q)[NC], float (&S)[NC],
                                           float (&Sd)[3][4]) {
 #pragma clang fp contract(fast)
-    const float rdx = 1.0f / (float)G.dx, rdy = 1.0f / (float)G.dy, rdz = 1.0f / (float)G.dz;
-    const float rt = 1.0f / t, rdt = 1.0f / (float)G.dt;
+    const float rdx = frcp((float)G.dx), rdy = frcp((float)G.dy), rdz = frcp((float)G.dz);
+    const float rt = frcp(t), rdt = frcp((float)G.dt);
     const float r3 = 1.0f / 3.0f, r5 = 1.0f / 5.0f;
     const float ttt = q[0], ttx = q[1], tty = q[2], ttn = q[3];
     const float pitt = q[4], pitx = q[5], pity = q[6], pitn = q[7], pixx = q[8], pixy = q[9], pixn = q[10],
@@ -152,10 +163,9 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float dxp = deriv_f(ldf(pv, sl), p, ldf(pv, sr)) * rdx;
     const float dyp = deriv_f(ldf(pv, sb), p, ldf(pv, stp)) * rdy;
 
-    const float t2pow = t * t;
     const float pP = p + Pi;
     const float tnn = (ecell + pP) * un * un + pP * rt * rt + pinn;
-    const float rut = 1.0f / ut;
+    const float rut = frcp(ut);
     const float vx = ux * rut, vy = uy * rut, vn = un * rut;
     const float dxvx = (dxux - vx * dxut) * rut;
     const float dyvy = (dyuy - vy * dyut) * rut;
@@ -164,7 +174,7 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     S[0] = -(ttt * rt + t * tnn) + dkvk * (pitt - p - Pi) - vx * dxp - vy * dyp - vn * dnp;
     S[1] = -ttx * rt - dxp + dkvk * pitx;
     S[2] = -tty * rt - dyp + dkvk * pity;
-    S[3] = -(3 * ttn) * rt - dnp / t2pow + dkvk * pitn;
+    S[3] = -(3 * ttn) * rt - dnp * (rt * rt) + dkvk * pitn;
 
     // pi-gradient terms of each direction: d_a(pi^{tau mu}) v_a - d_a(pi^{a mu})
     {
@@ -179,7 +189,7 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
             if constexpr (NC == 15) {
                 const float dPi = dq(14);
                 Sd[d][0] -= vd[d] * dPi;
-                Sd[d][d + 1] -= d == 2 ? dPi / t2pow : dPi;
+                Sd[d][d + 1] -= d == 2 ? dPi * (rt * rt) : dPi;
             }
         }
         if constexpr (!DIM3) Sd[2][0] = Sd[2][1] = Sd[2][2] = Sd[2][3] = 0.f;
@@ -188,7 +198,7 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float e = ecell;
     const float T = temperature_f<TABLE>(E, e);
     const float d_etabar = (float)etabar_of_T(G, (double)T);
-    const float taupiInv = T * r5 / d_etabar;
+    const float taupiInv = T * r5 * frcp(d_etabar);
     const float beta_pi = (e + p) * r5;
     DivRecip dx64;
     const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
@@ -200,16 +210,16 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     if constexpr (NC == 15) {
         // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48)
         if (a2 != 0.0f) {
-            const float x = T / 1.01355f;
+            const float x = T * (1.0f / 1.01355f);
             float zetabar;
-            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) / 0.025f) + 0.25f * __expf(-(x - 1) / 0.13f) + 0.001f;
-            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) / 0.0025f) + 0.22f * __expf((x - 1) / 0.022f) + 0.03f;
+            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) * 40.0f) + 0.25f * __expf(-(x - 1) * (1.0f / 0.13f)) + 0.001f;
+            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) * 400.0f) + 0.22f * __expf((x - 1) * (1.0f / 0.022f)) + 0.03f;
             else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
-            tauPiInv = 15 * a2 * T / zetabar;
+            tauPiInv = 15 * a2 * T * frcp(zetabar);
         }
     }
     const float ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
-    const float rt2 = 1.0f / t2, rt3 = 1.0f / t3, r2t = 1.0f / (2.0f * t);
+    const float rt2 = rt * rt, rt3 = rt2 * rt, r2t = 0.5f * rt;
     const float dtut = (ut - utp) * rdt;
     const float dtux = (ux - uxp) * rdt;
     const float dtuy = (uy - uyp) * rdt;
@@ -330,8 +340,8 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
                                              int j, float t, float ecell, const float (&q)[NC], float (&S)[NC],
                                              float (&Sd)[3][4]) {
 #pragma clang fp contract(fast)
-    const float rdx = 1.0f / (float)G.dx, rdy = 1.0f / (float)G.dy;
-    const float rt = 1.0f / t, rdt = 1.0f / (float)G.dt;
+    const float rdx = frcp((float)G.dx), rdy = frcp((float)G.dy);
+    const float rt = frcp(t), rdt = frcp((float)G.dt);
     const float r3 = 1.0f / 3.0f, r5 = 1.0f / 5.0f;
     const float ttt = q[0], ttx = q[1], tty = q[2];
     const float pitt = q[4], pitx = q[5], pity = q[6], pixx = q[8], pixy = q[9], piyy = q[11], pinn = q[13];
@@ -357,7 +367,7 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
 
     const float pP = p + Pi;
     const float tnn = pP * rt * rt + pinn;
-    const float rut = 1.0f / ut;
+    const float rut = frcp(ut);
     const float vx = ux * rut, vy = uy * rut;
     const float dxvx = (dxux - vx * dxut) * rut;
     const float dyvy = (dyuy - vy * dyut) * rut;
@@ -389,7 +399,7 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
     const float e = ecell;
     const float T = temperature_f<TABLE>(E, e);
     const float d_etabar = (float)etabar_of_T(G, (double)T);
-    const float taupiInv = T * r5 / d_etabar;
+    const float taupiInv = T * r5 * frcp(d_etabar);
     const float beta_pi = (e + p) * r5;
     DivRecip dx64;
     const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
@@ -400,16 +410,16 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
     float tauPiInv = 0.0f;
     if constexpr (NC == 15) {
         if (a2 != 0.0f) {
-            const float x = T / 1.01355f;
+            const float x = T * (1.0f / 1.01355f);
             float zetabar;
-            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) / 0.025f) + 0.25f * __expf(-(x - 1) / 0.13f) + 0.001f;
-            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) / 0.0025f) + 0.22f * __expf((x - 1) / 0.022f) + 0.03f;
+            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) * 40.0f) + 0.25f * __expf(-(x - 1) * (1.0f / 0.13f)) + 0.001f;
+            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) * 400.0f) + 0.22f * __expf((x - 1) * (1.0f / 0.022f)) + 0.03f;
             else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
-            tauPiInv = 15 * a2 * T / zetabar;
+            tauPiInv = 15 * a2 * T * frcp(zetabar);
         }
     }
     const float ut2 = ut * ut, t2 = t * t, t3 = t * t2;
-    const float rt2 = 1.0f / t2, rt3 = 1.0f / t3;
+    const float rt2 = rt * rt, rt3 = rt2 * rt;
     const float dtut = (ut - utp) * rdt;
     const float dtux = (ux - uxp) * rdt;
     const float dtuy = (uy - uyp) * rdt;
@@ -544,7 +554,7 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
         }
         const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
                                                           guess, nan, div);
-        const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) / (float)w.ut;
+        const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) * frcp((float)w.ut);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (q == kk) v[q] = vs;
@@ -592,19 +602,19 @@ __device__ __forceinline__ void regulate_mx(const EvDesc &G, double t64, float (
     const float piu1 = -(pixn * t2 * un) + pitx * ut - pixx * ux - pixy * uy;
     const float piu2 = -(piyn * t2 * un) + pity * ut - pixy * ux - piyy * uy;
     const float piu3 = -(pinn * t2 * un) + pitn * ut - pixn * ux - piyn * uy;
-    const float a1 = spipi / __builtin_sqrtf(e * e + 3 * p * p);
-    const float inv = 1.0f / (0.1f * spipi);  // 0 / 0 gives NaN, which fmax passes over, as in FP64
+    const float a1 = spipi * __builtin_amdgcn_rsqf(e * e + 3 * p * p);
+    const float inv = frcp(0.1f * spipi);  // pi = 0: 0 * inf gives NaN, which fmax passes over, as 0 / 0 in FP64
     const float a2 = pimumu * inv, a3 = piu0 * inv, a4 = piu1 * inv, a5 = piu2 * inv, a6 = piu3 * inv;
     const float rho = __builtin_fmaxf(__builtin_fmaxf(a1, a2), __builtin_fmaxf(__builtin_fmaxf(a3, a4),
                                                                                  __builtin_fmaxf(a5, a6)));
-    float facShear = __builtin_fabsf(rho) < 1.e-7f ? 1.0f : tanhf(rho) / rho;
+    float facShear = __builtin_fabsf(rho) < 1.e-7f ? 1.0f : tanhf(rho) * frcp(rho);
 #pragma unroll
     for (int n = 0; n < 10; ++n) pv[n] *= facShear;
     if (__builtin_isnan(facShear)) nan = NAN_REG_SHEAR;
     if constexpr (NC == 15) {
         const float Pi = pv[10];
-        const float rb = __builtin_fabsf(Pi) / p;
-        const float facBulk = __builtin_fabsf(rb) < 1.e-7f ? 1.0f : tanhf(rb) / rb;
+        const float rb = __builtin_fabsf(Pi) * frcp(p);
+        const float facBulk = __builtin_fabsf(rb) < 1.e-7f ? 1.0f : tanhf(rb) * frcp(rb);
         pv[10] *= facBulk;
         if (__builtin_isnan(facBulk)) nan = NAN_REG_BULK;
     }
