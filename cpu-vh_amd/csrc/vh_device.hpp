@@ -404,11 +404,10 @@ __device__ __forceinline__ Prim recover_finish(const EosDev &E, const Moments &m
     return r;
 }
 
+// getInferredVariables from the moments of the conserved variables.
 template <bool TABLE, class DV, class QD = QuotExact>
-__device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
-                                        double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
-                                        int &nan, DV &div) {
-    const Moments m = moments(t, ttt, ttx, tty, ttn, pitt, pitx, pity, pitn);
+__device__ __forceinline__ Prim recover_moments(const EosDev &E, const Moments &m, double Pi, double ePrev, int &nan,
+                                                DV &div) {
     double e;
     if (ePrev <= 0.1) {
         if constexpr (std::is_same_v<QD, QuotFast>) e = m.M0 - m.M * QuotFast::rcp1(m.M0);
@@ -417,6 +416,14 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
         e = energy_from_conserved<TABLE, QD, std::conditional_t<std::is_same_v<QD, QuotFast>, DivRecip, DivExact>>(
             E, ePrev, m.M0, m.M, Pi, nan);
     return recover_finish<TABLE, DV, QD>(E, m, e, Pi, nan, div);
+}
+
+template <bool TABLE, class DV, class QD = QuotExact>
+__device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, double ttx, double tty, double ttn,
+                                        double pitt, double pitx, double pity, double pitn, double Pi, double ePrev,
+                                        int &nan, DV &div) {
+    const Moments m = moments(t, ttt, ttx, tty, ttn, pitt, pitx, pity, pitn);
+    return recover_moments<TABLE, DV, QD>(E, m, Pi, ePrev, nan, div);
 }
 
 }  // namespace vh

@@ -538,26 +538,31 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
     // both states take the closed form whatever the start)
     const bool lowA = ea <= 0.1, lowB = eb <= 0.1;
     const bool viewB = TWO && (mode == 1 ? (ea != eb && !(lowA && lowB)) : (mode == 2 && lowA != lowB));
-    const int nsolve = viewB ? 4 : 2;
+    // v[0], v[1]: the left and right states' speeds in cell a's view (both
+    // solves started from ea); v[2], v[3] in cell b's view (from eb).  The
+    // moments of a state serve both of its views.
     float v[4];
     int nan = NAN_NONE;
     DivRecip div;
 #pragma unroll 1
-    for (int kk = 0; kk < nsolve; ++kk) {
-        const int side = kk & 1;  // 0: left state, 1: right state
-        const double guess = kk >= 2 ? eb : ((!TWO || mode == 0) && side == 1 ? eb : ea);
+    for (int side = 0; side < 2; ++side) {  // 0: left state, 1: right state
         double x[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) {
             const int n = m < 8 ? m : NC - 1;
             x[m] = (m == 8 && NC == 14) ? 0.0 : (double)(side == 0 ? qL[n] : qR[n]);
         }
-        const Prim w = recover<TABLE, DivRecip, QuotFast>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8],
-                                                          guess, nan, div);
-        const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) * frcp((float)w.ut);
+        const Moments mo = moments(t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
+#pragma unroll 1
+        for (int view = 0; view < (viewB ? 2 : 1); ++view) {
+            const double guess = view == 1 ? eb : ((!TWO || mode == 0) && side == 1 ? eb : ea);
+            const Prim w = recover_moments<TABLE, DivRecip, QuotFast>(E, mo, x[8], guess, nan, div);
+            const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) * frcp((float)w.ut);
+            const int kk = side + 2 * view;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q == kk) v[q] = vs;
+            for (int q = 0; q < 4; ++q)
+                if (q == kk) v[q] = vs;
+        }
     }
     if (nan != NAN_NONE) report_nan(F.err, nan, p2);
     // H = (v_R qR + v_L qL - a (qR - qL)) / 2, with the halves taken into
