@@ -76,6 +76,10 @@ struct vh_batch {
     // part starts with the nblk2bi blocks of boost-invariant events, which
     // have a launch of their own
     int nblk2 = 0, nblk3 = 0, nblk2bi = 0;
+    // blocks of removed events still in the table (their kernels return at
+    // once); the table is rebuilt when they exceed a tenth of it or an
+    // event is added, so removing an event keeps the captured graphs
+    long dead_blocks = 0;
     // 64-cell chunks of every event (counting kernels of the mixed form)
     int *d_cnt_ev = nullptr;
     long *d_cnt_start = nullptr;
@@ -194,6 +198,7 @@ vh_status rebuild_layout(vh_batch *b) {
         BCHK(hipMemcpy(b->d_cnt_start, cstart.data(), sizeof(long) * cstart.size(), hipMemcpyHostToDevice));
     }
     drop_graphs(b);
+    b->dead_blocks = 0;
     b->layout_dirty = false;
     ++b->layout_version;
     return VH_OK;
@@ -521,6 +526,7 @@ vh_status vh_batch_remove(vh_batch *b, int id) {
     BCHK(hipSetDevice(b->dev));
     BCHK(hipStreamSynchronize(b->stream));
     Event &E = b->ev[id];
+    if (!b->layout_dirty) b->dead_blocks += event_blocks(b, E);
     free_cells(b, E.off, E.N);
     (void)hipFree(E.fo_prev);
     (void)hipFree(E.fo_cur);
@@ -528,7 +534,7 @@ vh_status vh_batch_remove(vh_batch *b, int id) {
     E = Event();
     b->h_ev[id].active = 0;
     BCHK(hipMemcpy(b->d_ev + id, &b->h_ev[id], sizeof(EvDesc), hipMemcpyHostToDevice));
-    b->layout_dirty = true;
+    if (10 * b->dead_blocks > (long)b->nblk2 + b->nblk3) b->layout_dirty = true;
     return VH_OK;
 }
 

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(64) void count_above_batch_f_kernel(const float *__
                                                                  unsigned long long *__restrict__ counts) {
     const int id = ev_of[blockIdx.x];
     const EvDesc &D = ev[id];
+    if (!D.active) return;
     const long N = (long)D.nx * D.ny * D.nz;
     const long sl = start[blockIdx.x] + threadIdx.x;
     const bool above = sl < N && (double)e[D.off + sl] > eF;

@@ -657,6 +657,7 @@ __device__ __forceinline__ void stage_mixed_body(const MxFields &F, const EosDev
     const int eid = __builtin_amdgcn_readfirstlane(blk_ev[blk]);
     const int tile = blk - __builtin_amdgcn_readfirstlane(blk_first[blk]);
     const EvDesc &G = ev[eid];
+    if (!G.active) return;  // a removed event whose blocks are still in the table (block-uniform)
     const int ntx = (G.nx + TX - 1) / TX, nty = (G.ny + TY - 1) / TY, ntz = (G.nz + TZ - 1) / TZ;
     if (tile < 0 || tile >= ntx * nty * ntz) return;  // block-uniform: no barrier is skipped by part of a block
     const int i0 = (tile % ntx) * TX, j0 = ((tile / ntx) % nty) * TY, k0 = (tile / (ntx * nty)) * TZ;

@@ -600,6 +600,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     const int e = __builtin_amdgcn_readfirstlane(blk_ev[b]);
     const int b0 = __builtin_amdgcn_readfirstlane(blk_first[b]);
     const EvDesc &G = ev[e];
+    if (!G.active) return;  // a removed event whose blocks are still in the table (block-uniform)
     stage_body<NC, TABLE, DIM3>(P, E, G, (long)b - b0);
 }
 
