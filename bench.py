@@ -141,6 +141,9 @@ def add_event(b, ev):
                  etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
 
 
+CHECKS_PER_SYNC = 8
+
+
 def run_batched(events, table, precision, warmup, max_batched_steps=None):
     """All events resident in one batch, stepped in chunks of the coarse
     freezeout interval until each meets the stop rule; returns timings."""
@@ -169,38 +172,47 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
         if FACE_MODE is not None:
             b.set_face_mode(FACE_MODE)
     eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
-    strikes = {i: 0 for i in ids}
     cells = {i: ev["N"] for i, ev in zip(ids, events)}
+    running = set(ids)
+    done_steps = {i: 0 for i in ids}
     steps_done = {}
     zone_updates = 0
     graph_ms = 0.0
     batched_steps = 0
     t_start = time.perf_counter()
     failed = []
-    while strikes and (max_batched_steps is None or batched_steps < max_batched_steps):
-        b.steps(coarse)
-        while True:
-            try:
-                counts = b.counts_above(eF)  # synchronising
-                break
-            except C.VhError as ex:  # an event hit a NaN: it leaves the batch, the others go on
-                if ex.status != C.VH_ERR_NAN:
-                    raise
-                site, bad, cell = b.nan_info()
-                b.clear_error()
-                if bad in strikes:
-                    failed.append(bad)
-                    b.remove(bad)
-                    del strikes[bad]
-        graph_ms += b.last_ms()
-        zone_updates += coarse * sum(cells[i] for i in strikes)
-        batched_steps += coarse
-        for i in list(strikes):
-            strikes[i] += counts[i] == 0
-            if strikes[i] >= 3 or b.info(i)["steps_done"] + coarse > BATCH["max_steps"]:
-                steps_done[i] = b.info(i)["steps_done"]
+    # the stop rule runs on the device: one graph holds CHECKS_PER_SYNC rounds
+    # of (coarse steps, count, strike), and the host only removes the events
+    # that stopped
+    while running and (max_batched_steps is None or batched_steps < max_batched_steps):
+        room = (BATCH["max_steps"] - max(done_steps[i] for i in running)) // coarse
+        if room <= 0:  # step budget spent: the remaining events stop here
+            for i in list(running):
+                steps_done[i] = done_steps[i]
                 b.remove(i)
-                del strikes[i]
+            running.clear()
+            break
+        n = min(CHECKS_PER_SYNC, room)
+        if max_batched_steps is not None:
+            n = max(1, min(n, (max_batched_steps - batched_steps) // coarse))
+        st, stopped, steps = b.run(eF, coarse, n)
+        graph_ms += b.last_ms()
+        batched_steps += coarse * n
+        for i in running:
+            zone_updates += cells[i] * (int(steps[i]) - done_steps[i])
+            done_steps[i] = int(steps[i])
+        if st == C.VH_ERR_NAN:  # an event hit a NaN: it leaves the batch, the others go on
+            site, bad, cell = b.nan_info()
+            b.clear_error()
+            if bad in running:
+                failed.append(bad)
+                b.remove(bad)
+                running.discard(bad)
+        for i in list(running):
+            if stopped[i] == 1:
+                steps_done[i] = done_steps[i]
+                b.remove(i)
+                running.discard(i)
     wall = time.perf_counter() - t_start
     b.close()
     done = len(events) - len(failed)

@@ -33,7 +33,8 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
            "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
            "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube",
-           "vh_ic_optical_glauber", "vh_ic_test_problem", "vh_ic_read_fields", "vh_ic_read_gubser"]
+           "vh_ic_optical_glauber", "vh_ic_test_problem", "vh_ic_read_fields", "vh_ic_read_gubser",
+           "vh_batch_run"]
 
 
 class VhConfig(ctypes.Structure):
@@ -160,6 +161,7 @@ def _lib():
         L.vh_batch_nan_info.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
         L.vh_batch_clear_error.argtypes = [P]
         L.vh_batch_counts_above.argtypes = [P, D, P]
+        L.vh_batch_run.argtypes = [P, D, I, I, P, P]
         L.vh_batch_download.argtypes = [P, I, I, P]
         L.vh_batch_fo_flag.argtypes = [P, I, D, P, P, I64, ctypes.POINTER(I64)]
         L.vh_batch_event_info.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
@@ -369,6 +371,17 @@ class Batch:
         out = np.empty(self.max_events, dtype=np.int64)
         Hydro._check(_lib().vh_batch_counts_above(self.b, eF, _ptr(out)), "vh_batch_counts_above")
         return out
+
+    def run(self, eF, coarse, nchecks):
+        """vh_batch_run: nchecks x (coarse steps + the stop-rule check) on the
+        device.  Returns (status, stopped, steps_done) per id; status is
+        VH_OK or VH_ERR_NAN (other errors raise)."""
+        stopped = np.empty(self.max_events, dtype=np.int32)
+        steps = np.empty(self.max_events, dtype=np.int32)
+        st = _lib().vh_batch_run(self.b, eF, coarse, nchecks, _ptr(stopped), _ptr(steps))
+        if st not in (VH_OK, VH_ERR_NAN):
+            raise VhError(st, "vh_batch_run")
+        return st, stopped, steps
 
     def get(self, eid, field):
         f = FIELD[field] if isinstance(field, str) else field

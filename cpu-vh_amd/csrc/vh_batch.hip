@@ -48,6 +48,8 @@ struct Event {
     double *fo_prev = nullptr, *fo_cur = nullptr;
     long long *fo_ids = nullptr;
     bool bi = false;  // boost-invariant 2+1D: nz = 1, u^eta = pi^{tau eta} = pi^{x eta} = pi^{y eta} = 0
+    bool stopped = false;  // met the stop rule in vh_batch_run; no longer stepped
+    long stop_at = 0;      // the batch's step count when it stopped
 };
 
 }  // namespace
@@ -94,6 +96,14 @@ struct vh_batch {
     int layout_version = 0;
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     unsigned long long *d_counts = nullptr;
+    // device stop rule of vh_batch_run: strikes per event and the check at
+    // which an event stopped (-1: running)
+    int *d_strikes = nullptr, *d_stop = nullptr;
+    // steps taken by the batch: the state buffers swap roles every step, so
+    // a stopped event's latest state sits in the buffers that were current
+    // when it stopped, current again after an even number of steps
+    long step_count = 0;
+    std::map<std::tuple<int, int, int, uint64_t>, hipGraphExec_t> run_graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;  // ev0/ev1 bracket a graph launch not yet read
     double last_ms = 0;
@@ -104,6 +114,8 @@ namespace {
 void drop_graphs(vh_batch *b) {
     for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     b->graphs.clear();
+    for (auto &kv : b->run_graphs) (void)hipGraphExecDestroy(kv.second);
+    b->run_graphs.clear();
 }
 
 long alloc_cells(vh_batch *b, long n) {
@@ -283,26 +295,32 @@ vh_status enqueue_batch_step(vh_batch *b, int off) {
     return VH_OK;
 }
 
-double *field_of(vh_batch *b, int field) {
-    if (field >= 0 && field < b->nc) return b->q[b->iq_cur][field];
+// flip: an odd number of steps since the event stopped (its state is in the
+// buffers that are now the previous ones)
+double *field_of(vh_batch *b, int field, bool flip) {
+    const int iq = flip ? b->iq_next : b->iq_cur, iu = flip ? b->iu_prev : b->iu_cur;
+    const int iup = flip ? b->iu_cur : b->iu_prev;
+    if (field >= 0 && field < b->nc) return b->q[iq][field];
     if (field == 15) return b->e[0];
     if (field == 16) return b->p[0];
-    if (field >= 17 && field <= 20) return b->u[b->iu_cur][field - 17];
-    if (field >= 21 && field <= 24) return b->u[b->iu_prev][field - 21];
+    if (field >= 17 && field <= 20) return b->u[iu][field - 17];
+    if (field >= 21 && field <= 24) return b->u[iup][field - 21];
     return nullptr;
 }
 
-bool field_of_mixed(vh_batch *b, int field, const float **hi, const float **lo) {
+bool field_of_mixed(vh_batch *b, int field, bool flip, const float **hi, const float **lo) {
+    const int iq = flip ? b->iq_next : b->iq_cur, iu = flip ? b->iu_prev : b->iu_cur;
+    const int iup = flip ? b->iu_cur : b->iu_prev;
     *lo = nullptr;
     if (field >= 0 && field < b->nc) {
-        *hi = b->qf[b->iq_cur][field];
-        if (field < 4) *lo = b->qfl[b->iq_cur][field];
+        *hi = b->qf[iq][field];
+        if (field < 4) *lo = b->qfl[iq][field];
         return true;
     }
     if (field == 15) *hi = b->ef[0];
     else if (field == 16) *hi = b->pf[0];
-    else if (field >= 17 && field <= 20) *hi = b->uf[b->iu_cur][field - 17];
-    else if (field >= 21 && field <= 24) *hi = b->uf[b->iu_prev][field - 21];
+    else if (field >= 17 && field <= 20) *hi = b->uf[iu][field - 17];
+    else if (field >= 21 && field <= 24) *hi = b->uf[iup][field - 21];
     else return false;
     return true;
 }
@@ -366,6 +384,9 @@ vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out) {
         return fail(VH_ERR_NOMEM);
     if (hipMalloc(&b->d_ev, sizeof(EvDesc) * c.max_events) != hipSuccess) return fail(VH_ERR_NOMEM);
     if (hipMalloc(&b->d_counts, sizeof(unsigned long long) * c.max_events) != hipSuccess) return fail(VH_ERR_NOMEM);
+    if (hipMalloc(&b->d_strikes, sizeof(int) * c.max_events) != hipSuccess ||
+        hipMalloc(&b->d_stop, sizeof(int) * c.max_events) != hipSuccess)
+        return fail(VH_ERR_NOMEM);
     if (hipMalloc(&b->d_times, sizeof(StepTimes) * (size_t)c.max_events * (c.max_steps + 1)) != hipSuccess)
         return fail(VH_ERR_NOMEM);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(VH_ERR_HIP);
@@ -397,6 +418,8 @@ vh_status vh_batch_destroy(vh_batch *b) {
     (void)hipFree(b->d_err);
     (void)hipFree(b->d_ev);
     (void)hipFree(b->d_counts);
+    (void)hipFree(b->d_strikes);
+    (void)hipFree(b->d_stop);
     (void)hipFree(b->d_times);
     (void)hipFree(b->d_blk_ev);
     (void)hipFree(b->d_blk_first);
@@ -515,6 +538,9 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
     D.tab = b->d_times + (size_t)id * (b->cfg.max_steps + 1);
     D.base = 1;
     BCHK(hipMemcpyAsync(b->d_ev + id, &D, sizeof(EvDesc), hipMemcpyHostToDevice, b->stream));
+    const int zero = 0, running = -1;
+    BCHK(hipMemcpyAsync(b->d_strikes + id, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipMemcpyAsync(b->d_stop + id, &running, sizeof(int), hipMemcpyHostToDevice, b->stream));
     BCHK(hipStreamSynchronize(b->stream));
     b->layout_dirty = true;
     *id_out = id;
@@ -543,7 +569,7 @@ vh_status vh_batch_steps(vh_batch *b, int count) {
     if (count == 0) return VH_OK;
     BCHK(hipSetDevice(b->dev));
     for (const Event &E : b->ev)
-        if (E.used && E.steps_done + count > b->cfg.max_steps) return VH_ERR_CAPACITY;
+        if (E.used && !E.stopped && E.steps_done + count > b->cfg.max_steps) return VH_ERR_CAPACITY;
     vh_status s = rebuild_layout(b);
     if (s != VH_OK) return s;
     if (b->nblk2 + b->nblk3 == 0) return VH_OK;
@@ -578,8 +604,86 @@ vh_status vh_batch_steps(vh_batch *b, int count) {
         std::swap(b->iu_cur, b->iu_prev);
     }
     for (Event &E : b->ev)
-        if (E.used) E.steps_done += count;
+        if (E.used && !E.stopped) E.steps_done += count;
+    b->step_count += count;
     return VH_OK;
+}
+
+vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t *stopped, int32_t *steps_done) {
+    if (!b || coarse < 1 || nchecks < 1) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    const int total = coarse * nchecks;
+    for (const Event &E : b->ev)
+        if (E.used && !E.stopped && E.steps_done + total > b->cfg.max_steps) return VH_ERR_CAPACITY;
+    vh_status s = rebuild_layout(b);
+    if (s != VH_OK) return s;
+    const int nev = (int)b->ev.size();
+    if (b->nblk2 + b->nblk3 > 0) {
+        // the graph's count nodes carry eF: its bits are part of the key
+        uint64_t eF_bits = 0;
+        std::memcpy(&eF_bits, &eF, sizeof eF_bits);
+        const auto key = std::make_tuple(coarse * 65536 + nchecks, b->iq_cur * 3 + b->iu_cur, b->layout_version,
+                                         eF_bits);
+        auto it = b->run_graphs.find(key);
+        if (it == b->run_graphs.end()) {
+            const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
+            hipGraph_t g = nullptr;
+            BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            vh_status cs = VH_OK;
+            for (int c = 0; c < nchecks && cs == VH_OK; ++c) {
+                for (int k = 0; k < coarse && cs == VH_OK; ++k) cs = enqueue_batch_step(b, c * coarse + k);
+                hipError_t ke = launch_zero_counts(b->d_counts, nev, b->stream);
+                if (ke == hipSuccess)
+                    ke = b->mixed ? launch_count_above_batch_f(b->ef[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
+                                                               b->ncnt, b->d_counts, b->stream)
+                                  : launch_count_above_batch(b->e[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
+                                                             b->ncnt, b->d_counts, b->stream);
+                if (ke == hipSuccess) ke = launch_strikes(b->d_ev, b->d_counts, b->d_strikes, b->d_stop, nev, c, b->stream);
+                if (cs == VH_OK && ke != hipSuccess) cs = bfail(ke, "stop rule");
+            }
+            if (cs == VH_OK) {
+                hipError_t ae = launch_advance_clocks(b->d_ev, nev, total, b->stream);
+                if (ae != hipSuccess) cs = bfail(ae, "advance clocks");
+            }
+            hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
+            if (cs != VH_OK) return cs;
+            if (ce != hipSuccess) return bfail(ce, "hipStreamEndCapture");
+            hipGraphExec_t ge = nullptr;
+            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return bfail(ie, "hipGraphInstantiate");
+            it = b->run_graphs.emplace(key, ge).first;
+        }
+        BCHK(hipEventRecord(b->ev0, b->stream));
+        BCHK(hipGraphLaunch(it->second, b->stream));
+        BCHK(hipEventRecord(b->ev1, b->stream));
+        b->timed = true;
+        for (int k = 0; k < total; ++k) {
+            std::swap(b->iq_cur, b->iq_next);
+            std::swap(b->iu_cur, b->iu_prev);
+        }
+    }
+    const long step_count0 = b->step_count;
+    b->step_count += total;
+    std::vector<int> stop(nev, -1);
+    BCHK(hipMemcpyAsync(stop.data(), b->d_stop, sizeof(int) * nev, hipMemcpyDeviceToHost, b->stream));
+    const vh_status st = vh_batch_sync(b);
+    for (int id = 0; id < nev; ++id) {
+        Event &E = b->ev[id];
+        if (E.used && !E.stopped) {
+            if (stop[id] >= 0) {
+                E.stopped = true;
+                E.steps_done += coarse * (stop[id] + 1);
+                E.stop_at = step_count0 + coarse * (stop[id] + 1);
+            } else {
+                E.steps_done += total;
+            }
+        }
+        if (stopped) stopped[id] = E.used ? (E.stopped ? 1 : 0) : -1;
+        if (steps_done) steps_done[id] = E.used ? E.steps_done : -1;
+    }
+    return st;
 }
 
 vh_status vh_batch_sync(vh_batch *b) {
@@ -623,14 +727,15 @@ vh_status vh_batch_download(vh_batch *b, int id, int field, double *out) {
     if (!b || !out || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used) return VH_ERR_ARG;
     BCHK(hipSetDevice(b->dev));
     const Event &E = b->ev[id];
+    const bool flip = E.stopped && ((b->step_count - E.stop_at) & 1);
     if (!b->mixed) {
-        double *src = field_of(b, field);
+        double *src = field_of(b, field, flip);
         if (!src) return VH_ERR_ARG;
         BCHK(hipMemcpyAsync(out, src + E.off, sizeof(double) * E.N, hipMemcpyDeviceToHost, b->stream));
         return vh_batch_sync(b);
     }
     const float *hi = nullptr, *lo = nullptr;
-    if (!field_of_mixed(b, field, &hi, &lo)) return VH_ERR_ARG;
+    if (!field_of_mixed(b, field, flip, &hi, &lo)) return VH_ERR_ARG;
     // the eta components of a boost-invariant event are zero and not stored
     if (E.bi && (field == 3 || field == 7 || field == 10 || field == 12 || field == 20 || field == 24)) {
         std::fill(out, out + E.N, 0.0);

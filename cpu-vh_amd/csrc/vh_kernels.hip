@@ -167,6 +167,41 @@ hipError_t launch_count_above_batch(const double *e, double eF, const EvDesc *ev
     return hipGetLastError();
 }
 
+// Zero of the per-event counts inside a captured graph (a fill kernel: no
+// memset node in a graph).
+__global__ __launch_bounds__(64) void zero_counts_kernel(unsigned long long *__restrict__ counts, int n) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k < n) counts[k] = 0;
+}
+
+hipError_t launch_zero_counts(unsigned long long *counts, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_counts_kernel, dim3((n + 63) / 64), dim3(64), 0, st, counts, n);
+    return hipGetLastError();
+}
+
+// The reference's stop rule on the device (rhic/src/HydroPlugin.cpp:266-281):
+// an active event whose count of cells above e_F is zero gets a strike (the
+// strikes are never reset); at the third it stops (inactive: its blocks
+// return at once) and the check is recorded.
+__global__ __launch_bounds__(64) void strike_kernel(EvDesc *__restrict__ ev, const unsigned long long *__restrict__ counts,
+                                                    int *__restrict__ strikes, int *__restrict__ stop_check, int n,
+                                                    int check) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= n || !ev[k].active || counts[k] != 0) return;
+    if (++strikes[k] >= 3) {
+        ev[k].active = 0;
+        stop_check[k] = check;
+    }
+}
+
+hipError_t launch_strikes(EvDesc *ev, const unsigned long long *counts, int *strikes, int *stop_check, int n,
+                          int check, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(strike_kernel, dim3((n + 63) / 64), dim3(64), 0, st, ev, counts, strikes, stop_check, n, check);
+    return hipGetLastError();
+}
+
 // Every event's clock moves on by count steps (the last node of a batch graph).
 __global__ __launch_bounds__(64) void advance_clocks_kernel(EvDesc *__restrict__ ev, int n, int count) {
     const int k = blockIdx.x * 64 + threadIdx.x;

@@ -28,6 +28,9 @@ hipError_t launch_stage_batch(const StageParams &P, const EosDev &E, int nc, boo
 hipError_t launch_count_above_batch(const double *e, double eF, const EvDesc *ev, const int *ev_of,
                                     const long *start, int nchunks, unsigned long long *counts, hipStream_t st);
 hipError_t launch_advance_clocks(EvDesc *ev, int n, int count, hipStream_t st);
+hipError_t launch_zero_counts(unsigned long long *counts, int n, hipStream_t st);
+hipError_t launch_strikes(EvDesc *ev, const unsigned long long *counts, int *strikes, int *stop_check, int n,
+                          int check, hipStream_t st);
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,
                                  const double *const u[4], long N, hipStream_t st);
 hipError_t launch_count_above(const double *e, long N, double eF, unsigned long long *out, hipStream_t st);

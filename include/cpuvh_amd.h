@@ -303,6 +303,20 @@ vh_status vh_batch_nan_info(vh_batch *b, int *site, int *event, int64_t *cell);
 vh_status vh_batch_clear_error(vh_batch *b);
 /* counts[id] = cells of event id with e > eF (-1 for free slots); max_events entries. */
 vh_status vh_batch_counts_above(vh_batch *b, double eF, int64_t *counts);
+/* The reference's run loop for every resident event at once, stop rule on
+ * the device (rhic/src/HydroPlugin.cpp:226-308, the check and its 3-strike
+ * stop at :266-281): nchecks times, coarse steps then the count of cells
+ * above eF; an event whose count is zero gets a strike (strikes are never
+ * reset, counting from its vh_batch_add) and at the third it stops being
+ * stepped, at that check, without a host round trip.  One captured graph per
+ * (coarse, nchecks, eF, layout).  On return stopped[id] is 1 for an event
+ * that has stopped (now or in an earlier call), 0 for one still running, -1
+ * for a free id; steps_done[id] its steps (either array may be NULL).  A
+ * stopped event stays resident, and is no longer stepped, until
+ * vh_batch_remove.  VH_ERR_NAN as vh_batch_sync (the outputs are still
+ * filled); VH_ERR_CAPACITY if a running event would pass max_steps.  No
+ * counterpart in the reference, which checks one event on the host. */
+vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t *stopped, int32_t *steps_done);
 /* As vh_download / vh_fo_flag for one event. */
 vh_status vh_batch_download(vh_batch *b, int id, int field, double *out);
 vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,

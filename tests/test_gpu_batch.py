@@ -151,3 +151,56 @@ def test_step_budget_and_bad_arguments():
         b.remove(eid + 1)
     b.remove(eid)
     assert b.free_cells() == (1000, 1000)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_stop_rule_equals_host_loop(precision):
+    """vh_batch_run (the stop rule on the device, one graph for all checks)
+    against the host loop of steps, counts and strikes with removal at the
+    third zero count: the same events stop at the same checks, with the same
+    bits, and the others run on to the same bits."""
+    lattice, bulk, coarse, nchecks = True, True, 5, 12
+    print(f"case device stop rule precision={precision}", flush=True)
+    ics = [_ic(ev, lattice, bulk) for ev in EVENTS]
+    kw = dict(max_events=len(EVENTS), max_steps=200, bulk=bulk, eos_table=V.eos_table(), precision=precision)
+    cap = sum(ev[0] * ev[1] * ev[2] for ev in EVENTS)
+    eF = 9.0  # two of the five events fall below it within the 60 steps (stopping at 55 and 60)
+    fields = ["e", "ut", "ux", "uy", "pixx", "Pi", "ttt"]
+
+    hb = Batch(cap, **kw)
+    hid = [_add(hb, ev, ic) for ev, ic in zip(EVENTS, ics)]
+    strikes = {i: 0 for i in hid}
+    host_stop, host_state = {}, {}
+    for c in range(nchecks):
+        hb.steps(coarse)
+        counts = hb.counts_above(eF)
+        for i in list(strikes):
+            strikes[i] += counts[i] == 0
+            if strikes[i] >= 3:
+                host_stop[i] = hb.info(i)["steps_done"]
+                host_state[i] = {f: hb.get(i, f) for f in fields}
+                hb.remove(i)
+                del strikes[i]
+    for i in strikes:
+        host_state[i] = {f: hb.get(i, f) for f in fields}
+
+    db = Batch(cap, **kw)
+    did = [_add(db, ev, ic) for ev, ic in zip(EVENTS, ics)]
+    st, stopped, steps = db.run(eF, coarse, nchecks // 2)
+    assert st == 0
+    st, stopped, steps = db.run(eF, coarse, nchecks - nchecks // 2)  # a second call continues the rule
+    assert st == 0
+    assert 0 < len(host_stop) < len(EVENTS), host_stop  # the case exercises both outcomes
+    for h, d in zip(hid, did):
+        assert bool(stopped[d]) == (h in host_stop), (h, d)
+        assert steps[d] == host_stop.get(h, coarse * nchecks)
+        for f in fields:
+            assert V.bits_equal(host_state[h][f], db.get(d, f)), (h, f)
+    # a stopped event is no longer stepped by vh_batch_steps either, and its
+    # state reads the same after an odd number of further steps (the state
+    # buffers swap roles every step)
+    h_stopped, stopped_id = next((h, d) for h, d in zip(hid, did) if h in host_stop)
+    db.steps(coarse)
+    db.sync()
+    for f in fields:
+        assert V.bits_equal(host_state[h_stopped][f], db.get(stopped_id, f)), f
