@@ -103,7 +103,7 @@ struct vh_batch {
     // a stopped event's latest state sits in the buffers that were current
     // when it stopped, current again after an even number of steps
     long step_count = 0;
-    std::map<std::tuple<int, int, int, uint64_t>, hipGraphExec_t> run_graphs;
+    std::map<std::tuple<int, int, int, int, uint64_t>, hipGraphExec_t> run_graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;  // ev0/ev1 bracket a graph launch not yet read
     double last_ms = 0;
@@ -622,8 +622,7 @@ vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t 
         // the graph's count nodes carry eF: its bits are part of the key
         uint64_t eF_bits = 0;
         std::memcpy(&eF_bits, &eF, sizeof eF_bits);
-        const auto key = std::make_tuple(coarse * 65536 + nchecks, b->iq_cur * 3 + b->iu_cur, b->layout_version,
-                                         eF_bits);
+        const auto key = std::make_tuple(coarse, nchecks, b->iq_cur * 3 + b->iu_cur, b->layout_version, eF_bits);
         auto it = b->run_graphs.find(key);
         if (it == b->run_graphs.end()) {
             const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
