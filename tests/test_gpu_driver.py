@@ -270,6 +270,43 @@ def test_driver_israel_stewart_gubser_file(tmp_path):
     assert steps == lat["nt"] + 1 and "Gubser" in log
 
 
+def test_driver_type10_field_files_table_bulk(tmp_path):
+    """IC type 10: input/e.dat, ux.dat, uy.dat, un.dat and the ten pi files
+    (read through float, i outer), p from the table EoS, Pi = e/3 - p with
+    bulk (rhic/src/InitialConditions.cpp:108-466), u^tau normalised; 3+1D
+    with u^eta != 0.  Same files as the oracle loop from that state."""
+    nx, ny, nz, dx, t0 = 11, 9, 5, 0.2, 0.6
+    N = nx * ny * nz
+    ref = V.random_ic(nx, ny, nz, dx, t0, seed=33, lattice=True, bulk=True)
+    f32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    vals = {"e": f32(ref["e"]), "ux": f32(ref["ux"]), "uy": f32(ref["uy"]), "un": f32(ref["un"])}
+    names = ["pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"]
+    for c, n in enumerate(names):
+        vals[n] = f32(ref["pi"][c * N:(c + 1) * N])
+    os.makedirs(tmp_path / "input")
+    for n, v in vals.items():
+        with open(tmp_path / "input" / (n + ".dat"), "w") as f:
+            for i in range(nx):
+                for j in range(ny):
+                    for k in range(nz):
+                        f.write("%.3f %.3f %.3f %.9g\n" % (i, j, k, v[i + nx * (j + ny * k)]))
+    table = V.eos_table()
+    L = V.oracle_lib()
+    eos = L.orc_eos_create(table.ctypes.data)
+    p = np.array([L.orc_eos_pressure(eos, x) for x in vals["e"]])
+    L.orc_eos_destroy(eos)
+    ux, uy, un = vals["ux"], vals["uy"], vals["un"]
+    ic = {"e": vals["e"], "p": p, "ux": ux, "uy": uy, "un": un,
+          "ut": np.sqrt(((1.0 + ux * ux) + uy * uy) + ((t0 * t0) * un) * un),
+          "pi": np.concatenate([vals[n] for n in names]), "Pi": vals["e"] / 3. - p}
+    lat = dict(nx=nx, ny=ny, nz=nz, nt=10, dx=dx, dt=0.02, coarse=3, snap=5)
+    hyd = dict(t0=t0, etas=0.12, TF=0.150, Treg=0.0)
+    eos_file = os.path.join(os.path.dirname(os.path.dirname(C.LIB_PATH)), "EoS", "hrg_hotqcd_eos_binary.dat")
+    steps, log = _run_case(tmp_path, lat, hyd, {"bulk": 1, "eos": "table", "eosFile": eos_file, "snapshotEvery": 5},
+                           ic, True, table, ic_type=10)
+    assert steps == lat["nt"] + 1 and "input/e.dat" in log
+
+
 def test_driver_runs_reference_default_configuration(tmp_path):
     """The reference's own rhic-conf directory (copied as data): IC type 2
     (optical Glauber Pb+Pb, e0 = 70), 101 x 101, conformal EoS, regulation
