@@ -204,3 +204,39 @@ def test_device_stop_rule_equals_host_loop(precision):
     db.sync()
     for f in fields:
         assert V.bits_equal(host_state[h_stopped][f], db.get(stopped_id, f)), f
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nan_event_is_isolated(precision):
+    """An event whose initial state holds a NaN (set from the host) reports
+    VH_ERR_NAN through vh_batch_run with its id; removed, the others run on
+    and end bit-identical to a batch that never held it."""
+    lattice, bulk, coarse = True, True, 5
+    print(f"case nan isolation precision={precision}", flush=True)
+    evs = [EVENTS[0], EVENTS[1], EVENTS[2]]
+    ics = [_ic(ev, lattice, bulk) for ev in evs]
+    bad_ic = dict(ics[1], e=ics[1]["e"].copy())
+    bad_ic["e"][len(bad_ic["e"]) // 2] = np.nan
+    kw = dict(max_events=3, max_steps=100, bulk=bulk, eos_table=V.eos_table(), precision=precision)
+    cap = sum(ev[0] * ev[1] * ev[2] for ev in evs)
+    fields = ["e", "ut", "ux", "pixx", "Pi", "ttt"]
+
+    b = Batch(cap, **kw)
+    ids = [_add(b, ev, ic) for ev, ic in zip(evs, [ics[0], bad_ic, ics[2]])]
+    st, stopped, steps = b.run(1e-9, coarse, 2)
+    assert st == -4  # VH_ERR_NAN
+    site, bad, cell = b.nan_info()
+    assert bad == ids[1] and site != 0, (site, bad, cell)
+    b.remove(bad)
+    b.clear_error()
+    st, stopped, steps = b.run(1e-9, coarse, 2)
+    assert st == 0
+
+    ref = Batch(cap, **kw)
+    rid = [_add(ref, ev, ic) for ev, ic in zip([evs[0], evs[2]], [ics[0], ics[2]])]
+    st, _, rsteps = ref.run(1e-9, coarse, 4)
+    assert st == 0
+    for i, r in zip([ids[0], ids[2]], rid):
+        assert steps[i] == rsteps[r] == 4 * coarse
+        for f in fields:
+            assert V.bits_equal(b.get(i, f), ref.get(r, f)), (i, f)
