@@ -489,6 +489,34 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
     }
 }
 
+// The face speed u^dir / u^tau of one half-site state (localPropagationSpeed
+// and the flux functions' u^i/u^tau, LocalPropagationSpeed.cpp:14-22,
+// FluxFunctions.cpp:15-25) from its recovered e, without forming u: with
+// u^tau^2 = (M0 + P)/(e + P) and u^i = M_i/((e + P) u^tau) the speed is
+// M_i/(M0 + P).  The recovery's rest-frame cap (u^tau > 1e3 gives u = (1,0,0,0),
+// EnergyMomentumTensor.cpp:223-232) and its floors are kept.
+template <bool TABLE>
+__device__ __forceinline__ float face_speed_mx(const EosDev &E, const Moments &m, double Pi, double ePrev, int dir,
+                                               int &nan) {
+    double e;
+    if (ePrev <= 0.1) e = m.M0 - m.M * QuotFast::rcp1(m.M0);
+    else e = energy_from_conserved<TABLE, QuotFast, DivRecip>(E, ePrev, m.M0, m.M, Pi, nan);
+    if (__builtin_isnan(e)) nan = NAN_ENERGY;
+    DivRecip div;
+    double p;
+    if constexpr (TABLE) p = eos_pressure_fast(E, e);
+    else p = eos_pressure<false>(E, e, div);
+    if (e < 1.e-7) {
+        e = 1.e-7;
+        p = 1.e-7;
+    }
+    const double P = p + Pi;
+    const double A = m.M0 + P;
+    if (__builtin_fabs(A) > 1.0e6 * __builtin_fabs(e + P)) return 0.0f;
+    const double Mi = dir == 0 ? m.M1 : (dir == 1 ? m.M2 : m.M3);
+    return (float)Mi * frcp((float)A);
+}
+
 // One face flux between cells a = b-1 and b along dir, b = (i, j, k).  The
 // limited slopes of the two cells come from the 4-point line (clamped at the
 // lattice edges, which equals the reference's ghost cells), the half-site
@@ -543,7 +571,6 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
     // moments of a state serve both of its views.
     float v[4];
     int nan = NAN_NONE;
-    DivRecip div;
 #pragma unroll 1
     for (int side = 0; side < 2; ++side) {  // 0: left state, 1: right state
         double x[9];
@@ -556,8 +583,7 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
 #pragma unroll 1
         for (int view = 0; view < (viewB ? 2 : 1); ++view) {
             const double guess = view == 1 ? eb : ((!TWO || mode == 0) && side == 1 ? eb : ea);
-            const Prim w = recover_moments<TABLE, DivRecip, QuotFast>(E, mo, x[8], guess, nan, div);
-            const float vs = (float)(dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un)) * frcp((float)w.ut);
+            const float vs = face_speed_mx<TABLE>(E, mo, x[8], guess, dir, nan);
             const int kk = side + 2 * view;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
