@@ -329,17 +329,29 @@ void write_snapshot(const std::vector<double> &v, double t, const std::string &d
         std::printf("cannot write %s\n", fname);
         std::exit(-1);
     }
-    size_t s = 0;
-    for (int k = 0; k < L.nz; ++k) {
-        const double z = (k - (L.nz - 1) / 2.) * L.dz;
-        for (int j = 0; j < L.ny; ++j) {
+    // the lines (k, then j, then i, x fastest) are formatted in parallel in
+    // blocks of rows and written in order
+    const long rows = (long)L.ny * L.nz;
+    constexpr long ROWS_PER_BLOCK = 64;
+    const long nblocks = (rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+    std::vector<std::string> text(nblocks);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long b = 0; b < nblocks; ++b) {
+        std::string &out = text[b];
+        char line[160];
+        for (long r = b * ROWS_PER_BLOCK; r < std::min(rows, (b + 1) * ROWS_PER_BLOCK); ++r) {
+            const int j = (int)(r % L.ny), k = (int)(r / L.ny);
+            const double z = (k - (L.nz - 1) / 2.) * L.dz;
             const double y = (j - (L.ny - 1) / 2.) * L.dy;
-            for (int i = 0; i < L.nx; ++i, ++s) {
+            const size_t s0 = (size_t)r * L.nx;
+            for (int i = 0; i < L.nx; ++i) {
                 const double x = (i - (L.nx - 1) / 2.) * L.dx;
-                std::fprintf(fp, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z, v[s]);
+                const int len = std::snprintf(line, sizeof line, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z, v[s0 + i]);
+                out.append(line, (size_t)len);
             }
         }
     }
+    for (const std::string &t : text) std::fwrite(t.data(), 1, t.size(), fp);
     std::fclose(fp);
 }
 
@@ -412,7 +424,7 @@ void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, co
 #pragma omp parallel for schedule(dynamic, 64) reduction(max : failed)
     for (int64_t m = 0; m < nf; ++m) {
         double el[16 * 8];
-        std::ostringstream os;
+        std::string &os = text[m];
         const int64_t c = ids[m];
         const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
         auto at = [&](int x, int y, int z) { return (size_t)(ix + x) + (size_t)nx * (iy + y) + sxy * (iz + z); };
@@ -459,12 +471,15 @@ void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, co
             const double ds0 = t * nor[0], ds1 = t * nor[1], ds2 = t * nor[2], ds3 = dim == 4 ? t * nor[3] : 0.0;
             const double eps = var[3];
             const double T = vh_eos_temperature(h, eps), P = vh_eos_pressure(h, eps);
-            os << tau << " " << x << " " << y << " " << eta << " " << ds0 << " " << ds1 << " " << ds2 << " " << ds3
-               << " " << var[0] << " " << var[1] << " " << var[2] << " " << eps << " " << T << " " << P << " "
-               << var[4] << " " << var[5] << " " << var[6] << " " << var[7] << " " << var[8] << " " << var[9]
-               << '\n';
+            // an ostream's default double output is printf's %g (6 significant digits)
+            const double col[20] = {tau,    x,      y,      eta,    ds0,    ds1, ds2, ds3,    var[0], var[1],
+                                    var[2], eps,    T,      P,      var[4], var[5], var[6], var[7], var[8], var[9]};
+            char buf[20 * 16 + 2];
+            int len = 0;
+            for (int q = 0; q < 20; ++q)
+                len += std::snprintf(buf + len, sizeof buf - (size_t)len, q < 19 ? "%g " : "%g\n", col[q]);
+            os.append(buf, (size_t)len);
         }
-        text[m] = os.str();
     }
     if (failed) std::exit(-1);
     for (const std::string &line : text) out << line;
@@ -540,6 +555,8 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
 
     double t = t0, total_ms = 0;
+    double fo_fields_ms = 0, fo_surface_ms = 0;  // freezeout: field downloads; flags, elements and output
+    double snap_ms = 0;
     int nsteps = 0, zero_counts = 0, pend_first = 0, pend_count = 0;
     using clk = std::chrono::steady_clock;
     // queued steps run as one graph replay; their wall time counts as step time
@@ -563,19 +580,25 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
             const double pc = buf[sctr];
             std::printf("n = %d:%d (t = %.3f),\t (e, p) = (%.3f, %.3f) [fm^-4],\t (T = %.3f [GeV]),\t", n - 1, nt, t,
                         ec, pc, vh_eos_temperature(h, ec) * HBARC);
+            const auto a = clk::now();
             snapshot(h, t, out, L, X.bulk, n == 1, buf);
+            snap_ms += std::chrono::duration<double, std::milli>(clk::now() - a).count();
         }
         if (fo) {
             if (H.do_freezeout && dim) {
                 // swapAndSetHydroVariables (FreezeOut.cpp:57-97), then the cut cubes
+                const auto a = clk::now();
                 for (int v = 0; v < FO_VARS; ++v) {
                     std::swap(slice[0][v], slice[1][v]);
                     if (FO_FIELD[v] == 14 && !X.bulk) std::fill(slice[1][v].begin(), slice[1][v].end(), 0.0);
                     else check_sync(h, vh_download(h, FO_FIELD[v], slice[1][v].data()));
                 }
+                const auto b = clk::now();
                 int64_t nf = 0;
                 check_sync(h, vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf));
                 write_surface(surface, ids, nf, slice, dim, L, spacing, t0 + (double)n * dt, t, eF, h);
+                fo_fields_ms += std::chrono::duration<double, std::milli>(b - a).count();
+                fo_surface_ms += std::chrono::duration<double, std::milli>(clk::now() - b).count();
             }
             int64_t above = 0;
             check_sync(h, vh_count_above(h, eF, &above));
@@ -597,6 +620,10 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     }
     flush();
     std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
+    std::printf("Snapshots: %.3f s\n", snap_ms * 1e-3);
+    if (H.do_freezeout && dim)
+        std::printf("Freezeout: %.3f s downloading fields, %.3f s flagging, building and writing the surface\n",
+                    fo_fields_ms * 1e-3, fo_surface_ms * 1e-3);
     std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
     surface.close();
     check("vh_destroy", vh_destroy(h));
