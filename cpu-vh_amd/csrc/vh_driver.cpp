@@ -48,6 +48,7 @@
 // vh_surface_cube / vh_surface_hypercube.
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -66,6 +67,25 @@
 namespace {
 
 constexpr double HBARC = 0.197326938;  // rhic/src/HydroPlugin.cpp:112
+
+// printf's "%.Nf" and "%g" (6 significant digits, an ostream's default output
+// of a double) appended to out through std::to_chars, which writes the same
+// characters several times faster.
+void put_fixed(std::string &out, double v, int prec) {
+    char tmp[400];
+    const auto r = std::to_chars(tmp, tmp + sizeof tmp, v, std::chars_format::fixed, prec);
+    if (r.ec == std::errc()) {
+        out.append(tmp, r.ptr);
+    } else {  // beyond 1e390: cannot happen for a finite double at these precisions
+        const int len = std::snprintf(tmp, sizeof tmp, "%.*f", prec, v);
+        out.append(tmp, (size_t)std::min(len, (int)sizeof tmp - 1));
+    }
+}
+void put_g(std::string &out, double v) {
+    char tmp[32];
+    const auto r = std::to_chars(tmp, tmp + sizeof tmp, v, std::chars_format::general, 6);
+    out.append(tmp, r.ptr);
+}
 
 // One positional value per line: whitespace removed, the text after the
 // first '=' (the whole line when there is none).  When the file has run out,
@@ -338,16 +358,21 @@ void write_snapshot(const std::vector<double> &v, double t, const std::string &d
 #pragma omp parallel for schedule(dynamic, 1)
     for (long b = 0; b < nblocks; ++b) {
         std::string &out = text[b];
-        char line[160];
         for (long r = b * ROWS_PER_BLOCK; r < std::min(rows, (b + 1) * ROWS_PER_BLOCK); ++r) {
             const int j = (int)(r % L.ny), k = (int)(r / L.ny);
             const double z = (k - (L.nz - 1) / 2.) * L.dz;
             const double y = (j - (L.ny - 1) / 2.) * L.dy;
             const size_t s0 = (size_t)r * L.nx;
-            for (int i = 0; i < L.nx; ++i) {
+            for (int i = 0; i < L.nx; ++i) {  // "%.3f\t%.3f\t%.3f\t%.8f\n"
                 const double x = (i - (L.nx - 1) / 2.) * L.dx;
-                const int len = std::snprintf(line, sizeof line, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z, v[s0 + i]);
-                out.append(line, (size_t)len);
+                put_fixed(out, x, 3);
+                out += '\t';
+                put_fixed(out, y, 3);
+                out += '\t';
+                put_fixed(out, z, 3);
+                out += '\t';
+                put_fixed(out, v[s0 + i], 8);
+                out += '\n';
             }
         }
     }
@@ -474,11 +499,10 @@ void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, co
             // an ostream's default double output is printf's %g (6 significant digits)
             const double col[20] = {tau,    x,      y,      eta,    ds0,    ds1, ds2, ds3,    var[0], var[1],
                                     var[2], eps,    T,      P,      var[4], var[5], var[6], var[7], var[8], var[9]};
-            char buf[20 * 16 + 2];
-            int len = 0;
-            for (int q = 0; q < 20; ++q)
-                len += std::snprintf(buf + len, sizeof buf - (size_t)len, q < 19 ? "%g " : "%g\n", col[q]);
-            os.append(buf, (size_t)len);
+            for (int q = 0; q < 20; ++q) {
+                put_g(os, col[q]);
+                os += q < 19 ? ' ' : '\n';
+            }
         }
     }
     if (failed) std::exit(-1);
