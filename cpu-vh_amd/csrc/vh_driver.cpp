@@ -63,6 +63,7 @@
 #include <vector>
 
 #include "../../include/cpuvh_amd.h"
+#include "../../include/cpuvh_hydro.hpp"
 
 namespace {
 
@@ -234,7 +235,7 @@ void navier_stokes_pi(vh_handle *h, const HydroPar &H, int bulk, size_t N, const
 
 // Initial state on the host, interior-only, x fastest.
 void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const HydroPar &H, int bulk,
-                   const std::string &root) {
+                   const std::string &root, const cpuvh::InitialTmunu *vec) {
     const size_t N = (size_t)L.nx * L.ny * L.nz;
     std::vector<double> e(N), p(N), ut(N, 1.0), ux(N, 0.0), uy(N, 0.0), un(N, 0.0), pi(10 * N, 0.0), Pi(N, 0.0);
     std::printf("Setting initial conditions: ");
@@ -321,6 +322,48 @@ void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const Hydr
             std::exit(-1);
         }
         if (!bulk) std::fill(Pi.begin(), Pi.end(), 0.0);
+        break;
+    }
+    case 13: case 14: {
+        // HYDRO's vectors (rhic/src/InitialConditions.cpp:1593-1711): e in
+        // GeV/fm^3 over hbar c plus the 1e-3 floor, p(e); type 13 at rest with
+        // pi = Pi = 0, type 14 with u (components below 1e-7 taken as 0) and
+        // pi, Pi over hbar c
+        std::printf(I.type == 13 ? "Initialize from initial energy density vector ...\n"
+                                 : "Initialize from preequilibrium vectors ...\n");
+        auto need = [&](const std::vector<double> &v, const char *name) {
+            if (v.size() != N) {
+                std::printf("HYDRO vector %s has %zu values, the lattice %zu\n", name, v.size(), N);
+                std::exit(-1);
+            }
+        };
+        if (!vec) {
+            std::printf("Initial condition type %d takes its state from the HYDRO wrapper's vectors\n", I.type);
+            std::exit(-1);
+        }
+        need(vec->e, "initial_energy_density");
+        for (size_t s = 0; s < N; ++s) {
+            e[s] = (vec->e[s] / HBARC) + 1.0e-3;
+            p[s] = vh_eos_pressure(h, e[s]);
+        }
+        if (I.type == 14) {
+            const std::vector<double> *comp[4] = {&vec->ux, &vec->uy, &vec->un, nullptr};
+            const char *cname[3] = {"initial_ux", "initial_uy", "initial_un"};
+            for (int c = 0; c < 3; ++c) need(*comp[c], cname[c]);
+            const std::vector<double> *piv[10] = {&vec->pitt, &vec->pitx, &vec->pity, &vec->pitn, &vec->pixx,
+                                                  &vec->pixy, &vec->pixn, &vec->piyy, &vec->piyn, &vec->pinn};
+            for (int c = 0; c < 10; ++c) need(*piv[c], "initial_pi");
+            if (bulk) need(vec->Pi, "initial_Pi");
+            for (size_t s = 0; s < N; ++s) {
+                auto cut = [](double x) { return std::fabs(x) < 1.0e-7 ? 0.0 : x; };
+                ux[s] = cut(vec->ux[s]);
+                uy[s] = cut(vec->uy[s]);
+                un[s] = cut(vec->un[s]);
+                ut[s] = std::sqrt(1.0 + ux[s] * ux[s] + uy[s] * uy[s] + H.t0 * H.t0 * un[s] * un[s]);
+                for (int c = 0; c < 10; ++c) pi[c * N + s] = (*piv[c])[s] / HBARC;
+                if (bulk) Pi[s] = vec->Pi[s] / HBARC;
+            }
+        }
         break;
     }
     default:
@@ -511,7 +554,7 @@ void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, co
 }
 
 void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X, const std::string &root,
-         const std::string &out) {
+         const std::string &out, const cpuvh::InitialTmunu *vec) {
     std::vector<double> table;
     if (X.eos_table) {
         table.resize(4 * 100000);
@@ -559,7 +602,7 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
         std::exit(-1);
     }
 
-    initial_state(h, L, I, H, X.bulk, root);
+    initial_state(h, L, I, H, X.bulk, root, vec);
     check("vh_init_conserved", vh_init_conserved(h, t0));
 
     const size_t N = (size_t)nx * ny * nz;
@@ -655,7 +698,11 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
 
 }  // namespace
 
-int main(int argc, char **argv) {
+namespace cpuvh {
+
+// HydroWrapper.cpp:106-136 (and the reference's main): the options, the
+// rhic-conf readers and the run.
+int run_main(int argc, char **argv, const InitialTmunu *vec) {
     bool run_hydro = false, run_test = false, print_config = false;
     const char *config = nullptr, *output = nullptr;
     static const option opts[] = {{"test", optional_argument, nullptr, 't'},
@@ -696,10 +743,33 @@ int main(int argc, char **argv) {
     }
     const auto a = std::chrono::steady_clock::now();
     if (run_hydro) {
-        run(L, I, H, X, root, output ? output : ".");
+        run(L, I, H, X, root, output ? output : ".", vec);
         std::printf("Done hydro.\n");
     }
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     std::printf("Hydro took %f seconds\n", sec);
     return 0;
 }
+
+// HydroWrapper.cpp:64-104: config "rhic-conf", output "cpu-vh_output" under
+// the working directory.
+int run_no_cli(const InitialTmunu *vec) {
+    const char *config = "rhic-conf", *output = "cpu-vh_output";
+    char cwd[4096];
+    const std::string root = getcwd(cwd, sizeof cwd) ? cwd : ".";
+    std::printf("configDirectory = %s\n", config);
+    std::printf("outputDirectory = %s\n", output);
+    Lattice L;
+    InitCond I;
+    HydroPar H;
+    Extra X;
+    read_config(config, L, I, H, X);
+    const auto a = std::chrono::steady_clock::now();
+    run(L, I, H, X, root, output, vec);
+    std::printf("Done hydro.\n");
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    std::printf("Hydro took %f seconds\n", sec);
+    return 0;
+}
+
+}  // namespace cpuvh

@@ -307,6 +307,55 @@ def test_driver_type10_field_files_table_bulk(tmp_path):
     assert steps == lat["nt"] + 1 and "input/e.dat" in log
 
 
+HYDRO_DEMO = os.path.join(os.path.dirname(C.LIB_PATH), "hydro_vectors")
+
+
+@pytest.mark.parametrize("ic_type", [13, 14])
+def test_hydro_wrapper_vectors(tmp_path, ic_type):
+    """The reference's HYDRO class (include/cpuvh_hydro.hpp) with its vectors
+    handed over in memory (cpu-vh_amd/examples/hydro_vectors): type 13, an
+    energy density in GeV/fm^3 at rest; type 14, all sixteen vectors; run by
+    run_hydro_no_cli() from rhic-conf/ into cpu-vh_output/.  Same files as the
+    oracle loop from the state the reference builds from those vectors
+    (rhic/src/InitialConditions.cpp:1593-1711; u^tau normalised)."""
+    nx, ny, nz, dx, t0 = 17, 15, 1, 0.2, 0.6
+    N = nx * ny * nz
+    ref = V.random_ic(nx, ny, nz, dx, t0, seed=40 + ic_type, bulk=True)
+    hbarc = 0.197326938
+    e_gev = ref["e"] * hbarc
+    lat = dict(nx=nx, ny=ny, nz=nz, nt=12, dx=dx, dt=0.02, coarse=3, snap=5)
+    hyd = dict(t0=t0, etas=0.2, TF=0.150, Treg=0.0)
+    _write_conf(str(tmp_path / "rhic-conf"), lat, {"type": ic_type}, hyd, {"bulk": 1, "snapshotEvery": 5})
+    (tmp_path / "cpu-vh_output").mkdir()
+    e = e_gev / hbarc + 1.0e-3
+    ic = {"e": e, "p": e / 3.0, "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N), "un": np.zeros(N),
+          "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
+    if ic_type == 13:
+        rows = e_gev[:, None]
+    else:
+        ux = np.where(np.abs(ref["ux"]) < 1e-7, 0.0, ref["ux"])
+        uy = np.where(np.abs(ref["uy"]) < 1e-7, 0.0, ref["uy"])
+        un = np.zeros(N)
+        pi_gev = ref["pi"].reshape(10, N) * hbarc
+        Pi_gev = ref["Pi"] * hbarc
+        rows = np.column_stack([e_gev, np.ones(N), ux, uy, un, pi_gev.T, Pi_gev])
+        ic.update(ux=ux, uy=uy, un=un, ut=np.sqrt(1.0 + ux * ux + uy * uy + t0 * t0 * un * un),
+                  pi=(pi_gev / hbarc).ravel(), Pi=Pi_gev / hbarc)
+    np.savetxt(tmp_path / "vectors.txt", rows, fmt="%.17g")
+    print(f"case hydro wrapper type={ic_type}", flush=True)
+    r = subprocess.run([HYDRO_DEMO, str(tmp_path / "vectors.txt")], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    steps = int(re.search(r"Steps taken: (\d+)", r.stdout).group(1))
+    want, want_steps = _oracle_loop(lat, hyd, True, None, ic)
+    assert steps == want_steps == lat["nt"] + 1
+    out = tmp_path / "cpu-vh_output"
+    assert sorted(os.listdir(out)) == sorted(want)
+    for name, text in want.items():
+        with open(out / name) as f:
+            assert f.read() == text, name
+
+
 def test_driver_runs_reference_default_configuration(tmp_path):
     """The reference's own rhic-conf directory (copied as data): IC type 2
     (optical Glauber Pb+Pb, e0 = 70), 101 x 101, conformal EoS, regulation
