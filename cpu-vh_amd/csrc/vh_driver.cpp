@@ -33,8 +33,9 @@
 // (vh_ic_test_problem); the files of types 1
 // (<root>/rhic/rhic-trunk/src/test/resources/gubser/viscous/gubserIC.dat),
 // 10 (input/e.dat, ux.dat, ...) and 11 (input/Tmunu.dat), under the working
-// directory.  Types 12-14 (dynamical sources, in-memory vectors of the
-// wrapper) exit as the reference does for an undefined type.
+// directory; 13 and 14 from the vectors of the HYDRO class
+// (include/cpuvh_hydro.hpp).  Type 12 (dynamical sources) exits as the
+// reference does for an undefined type.
 //
 // Output: the snapshot files of rhic/src/HydroPlugin.cpp:40-72 in the format
 // of rhic/src/FileIO.cpp:14-59 (<name>_<t %.3f>.dat, "x y z value" lines, x
