@@ -24,7 +24,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 # The exported entry points, as declared in include/cpuvh_amd.h.
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
-           "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag",
+           "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag", "vh_fo_interp",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
@@ -128,6 +128,7 @@ def _lib():
         L.vh_upload_field.argtypes = [P, ctypes.c_int, P]
         L.vh_count_above.argtypes = [P, D, ctypes.POINTER(ctypes.c_int64)]
         L.vh_fo_flag.argtypes = [P, D, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.vh_fo_interp.argtypes = [P, P, P, ctypes.c_int64, P]
         for f in ("vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature"):
             getattr(L, f).argtypes = [P, D]
             getattr(L, f).restype = D
@@ -283,6 +284,18 @@ class Hydro:
         if with_corners:
             return ids, corners[:n.value * per].reshape(n.value, per).copy()
         return ids
+
+    def fo_interp(self, cubes, frac):
+        """The ten freezeout variables at points inside cubes (vh_fo_interp):
+        cubes (n,) int64 cube ids, frac (n, 4) offsets in units of the
+        coarse time step and the cell sizes; returns (n, 10)."""
+        cubes = np.ascontiguousarray(cubes, dtype=np.int64)
+        frac = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 4)
+        if frac.shape[0] != cubes.shape[0]:
+            raise ValueError("fo_interp: one row of frac per cube id")
+        out = np.empty((max(cubes.shape[0], 1), 10), dtype=np.float64)
+        self._check(_lib().vh_fo_interp(self.h, _ptr(cubes), _ptr(frac), cubes.shape[0], _ptr(out)), "vh_fo_interp")
+        return out[:cubes.shape[0]].copy()
 
     def eos_energy_density(self, T):
         return _lib().vh_eos_energy_density(self.h, T)

@@ -44,6 +44,12 @@ struct vh_handle {
     // freezeout slices
     double *fo_prev = nullptr, *fo_cur = nullptr;
     long long *fo_ids = nullptr;
+    // the ten interpolated variables' slices, [2][10][N]; slot fo_vslot is current
+    double *fo_vars = nullptr;
+    int fo_vslot = 0;
+    // staging for corners and interpolation points, grown on demand
+    double *fo_buf = nullptr;
+    size_t fo_buf_bytes = 0;
     // equation of state
     HostEos heos;
     EosDev eos{};
@@ -269,6 +275,8 @@ vh_status vh_destroy(vh_handle *h) {
     hipFree(h->fo_prev);
     hipFree(h->fo_cur);
     hipFree(h->fo_ids);
+    hipFree(h->fo_vars);
+    hipFree(h->fo_buf);
     h->deos.release();
     hipFree(h->d_times);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -457,6 +465,24 @@ vh_status vh_count_above(vh_handle *h, double eF, int64_t *count) {
     return s;
 }
 
+// surface.dat's interpolated variables as field ids, in the reference's order
+// (FreezeOut.cpp:57-97): u^x u^y u^eta e pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi
+static constexpr int FO_FIELD[FO_NVARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
+
+// device staging of at least bytes (the stream is idle when it grows)
+static hipError_t fo_staging(vh_handle *h, size_t bytes) {
+    if (bytes <= h->fo_buf_bytes) return hipSuccess;
+    hipError_t r = hipStreamSynchronize(h->stream);
+    if (r != hipSuccess) return r;
+    const size_t want = std::max(bytes, h->fo_buf_bytes * 2);
+    hipFree(h->fo_buf);
+    h->fo_buf = nullptr;
+    h->fo_buf_bytes = 0;
+    if ((r = hipMalloc(&h->fo_buf, want)) != hipSuccess) return r;
+    h->fo_buf_bytes = want;
+    return hipSuccess;
+}
+
 vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int64_t cap, int64_t *n_flagged) {
     if (!h || !n_flagged || (cap > 0 && !ids)) return VH_ERR_ARG;
     const vh_config &c = h->cfg;
@@ -473,6 +499,18 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     }
     std::swap(h->fo_prev, h->fo_cur);
     HIPCHK(hipMemcpyAsync(h->fo_cur, h->e[0], b, hipMemcpyDeviceToDevice, h->stream));
+    // the interpolated variables move with e (Pi stays zero without bulk)
+    if (!h->fo_vars) {
+        HIPCHK(hipMalloc(&h->fo_vars, 2 * FO_NVARS * b));
+        HIPCHK(hipMemsetAsync(h->fo_vars, 0, 2 * FO_NVARS * b, h->stream));
+    }
+    h->fo_vslot ^= 1;
+    for (int v = 0; v < FO_NVARS; ++v) {
+        const double *src = field_ptr(h, FO_FIELD[v]);
+        if (src)
+            HIPCHK(hipMemcpyAsync(h->fo_vars + (h->fo_vslot * FO_NVARS + v) * h->N, src, b, hipMemcpyDeviceToDevice,
+                                  h->stream));
+    }
     HIPCHK(hipMemsetAsync(h->d_count, 0, sizeof(unsigned long long), h->stream));
     HIPCHK(launch_fo_flag(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, eF, h->fo_ids, M, h->d_count, h->stream));
     unsigned long long n = 0;
@@ -489,17 +527,49 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     if (corners) {
         HIPCHK(hipMemcpy(h->fo_ids, tmp.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
         const int per = c.nz > 1 ? 16 : 8;
-        double *d_corners = nullptr;
-        HIPCHK(hipMalloc(&d_corners, sizeof(double) * per * n));
-        hipError_t le = launch_fo_corners(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, h->fo_ids, (long long)n, d_corners,
-                                          h->stream);
-        if (le == hipSuccess)
-            le = hipMemcpyAsync(corners, d_corners, sizeof(double) * per * n, hipMemcpyDeviceToHost, h->stream);
-        if (le == hipSuccess) le = hipStreamSynchronize(h->stream);
-        hipFree(d_corners);
-        if (le != hipSuccess) return hip_fail(le, "fo corners");
+        HIPCHK(fo_staging(h, sizeof(double) * per * n));
+        HIPCHK(launch_fo_corners(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, h->fo_ids, (long long)n, h->fo_buf,
+                                 h->stream));
+        HIPCHK(hipMemcpyAsync(corners, h->fo_buf, sizeof(double) * per * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     return VH_OK;
+}
+
+vh_status vh_fo_interp(vh_handle *h, const int64_t *cubes, const double *frac, int64_t n, double *vars) {
+    if (!h || n < 0 || (n > 0 && (!cubes || !frac || !vars))) return VH_ERR_ARG;
+    const vh_config &c = h->cfg;
+    if (c.nx < 2 || c.ny < 2) return VH_ERR_ARG;
+    if (!h->fo_vars) {
+        vh::set_last_error("vh_fo_interp: no vh_fo_flag call yet");
+        return VH_ERR_ARG;
+    }
+    const int64_t M = (int64_t)(c.nx - 1) * (c.ny - 1) * (c.nz > 1 ? c.nz - 1 : 1);
+    for (int64_t m = 0; m < n; ++m)
+        if (cubes[m] < 0 || cubes[m] >= M) {
+            vh::set_last_error("vh_fo_interp: cube id out of range");
+            return VH_ERR_ARG;
+        }
+    if (n == 0) return VH_OK;
+    HIPCHK(hipSetDevice(h->dev));
+    // staging: ids, then 4 fractions and 10 variables per point
+    const size_t ib = sizeof(long long) * n, fb = sizeof(double) * 4 * n, vb = sizeof(double) * FO_NVARS * n;
+    HIPCHK(fo_staging(h, ib + fb + vb));
+    char *base = (char *)h->fo_buf;
+    long long *d_cubes = (long long *)base;
+    double *d_frac = (double *)(base + ib), *d_vars = (double *)(base + ib + fb);
+    static_assert(sizeof(long long) == sizeof(int64_t), "cube ids are 64-bit");
+    HIPCHK(hipMemcpyAsync(d_cubes, cubes, ib, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(d_frac, frac, fb, hipMemcpyHostToDevice, h->stream));
+    FoSlices S;
+    for (int v = 0; v < FO_NVARS; ++v) {
+        S.v[1][v] = h->fo_vars + (h->fo_vslot * FO_NVARS + v) * h->N;
+        S.v[0][v] = h->fo_vars + ((h->fo_vslot ^ 1) * FO_NVARS + v) * h->N;
+    }
+    HIPCHK(launch_fo_interp(S, c.nx, c.ny, c.nz, d_cubes, d_frac, (long long)n, d_vars, h->stream));
+    HIPCHK(hipMemcpyAsync(vars, d_vars, vb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_err(h);
 }
 
 double vh_eos_energy_density(const vh_handle *h, double T) { return h ? h->heos.energy_density(T) : NAN; }

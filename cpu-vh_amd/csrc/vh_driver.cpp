@@ -442,99 +442,94 @@ void snapshot(vh_handle *h, double t, const std::string &dir, const Lattice &L, 
         for (const F &f : bulkf) one(f);
 }
 
-// FreezeOut.cpp:22-56: multilinear interpolation inside the cell, the
-// terms summed in the reference's order.  v[t][x][y][z] corner values.
-double interp3(double x0, double x1, double x2, const double (&a)[2][2][2]) {
-    double r = 0;
-    r = ((1 - x0) * (1 - x1) * (1 - x2) * a[0][0][0]) + ((x0) * (1 - x1) * (1 - x2) * a[1][0][0]) +
-        ((1 - x0) * (x1) * (1 - x2) * a[0][1][0]) + ((1 - x0) * (1 - x1) * (x2) * a[0][0][1]) +
-        ((x0) * (x1) * (1 - x2) * a[1][1][0]) + ((x0) * (1 - x1) * (x2) * a[1][0][1]) +
-        ((1 - x0) * (x1) * (x2) * a[0][1][1]) + ((x0) * (x1) * (x2) * a[1][1][1]);
-    return r;
-}
-double interp4(double x0, double x1, double x2, double x3, const double (&a)[2][2][2][2]) {
-    double r = 0;
-    r = ((1 - x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[0][0][0][0]) +
-        ((x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[1][0][0][0]) +
-        ((1 - x0) * (x1) * (1 - x2) * (1 - x3) * a[0][1][0][0]) +
-        ((1 - x0) * (1 - x1) * (x2) * (1 - x3) * a[0][0][1][0]) +
-        ((1 - x0) * (1 - x1) * (1 - x2) * (x3) * a[0][0][0][1]) + ((x0) * (x1) * (1 - x2) * (1 - x3) * a[1][1][0][0]) +
-        ((x0) * (1 - x1) * (x2) * (1 - x3) * a[1][0][1][0]) + ((x0) * (1 - x1) * (1 - x2) * (x3) * a[1][0][0][1]) +
-        ((1 - x0) * (x1) * (x2) * (1 - x3) * a[0][1][1][0]) + ((1 - x0) * (x1) * (1 - x2) * (x3) * a[0][1][0][1]) +
-        ((1 - x0) * (1 - x1) * (x2) * (x3) * a[0][0][1][1]) + ((x0) * (x1) * (x2) * (1 - x3) * a[1][1][1][0]) +
-        ((x0) * (x1) * (1 - x2) * (x3) * a[1][1][0][1]) + ((x0) * (1 - x1) * (x2) * (x3) * a[1][0][1][1]) +
-        ((1 - x0) * (x1) * (x2) * (x3) * a[0][1][1][1]) + ((x0) * (x1) * (x2) * (x3) * a[1][1][1][1]);
-    return r;
-}
+// Per-coarse-step work space of the surface writer, kept across calls.
+struct SurfaceWork {
+    std::vector<int64_t> order, cubes;
+    std::vector<double> el, frac, vars;
+    std::vector<int> ne;
+    std::vector<int64_t> off;
+    std::vector<std::string> text;
+};
 
-// The hydro variables surface.dat interpolates, in the reference's order
-// (FreezeOut.cpp:57-97): u^x u^y u^eta e pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi.
-constexpr int FO_VARS = 10;
-constexpr int FO_FIELD[FO_VARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
-
-// callFOFinder2p1D / callFOFinder3p1D (FreezeOut.cpp:200-361) for the cut
-// cubes ids (x-fastest cube index, increasing): visited in the reference's
-// order (x outer, eta inner), each surface element written as one line.
-void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, const std::vector<double> (&sl)[2][FO_VARS],
+// callFOFinder2p1D / callFOFinder3p1D (FreezeOut.cpp:200-361) for the nf cut
+// cubes ids (x-fastest cube index, increasing) with their corner energies as
+// vh_fo_flag returns them (previous slice first, x fastest within a slice):
+// visited in the reference's order (x outer, eta inner); the elements come
+// from vh_surface_*, the ten variables at their centroids from vh_fo_interp
+// (interpolateVariable3D/4D on the device's slices), one line per element.
+// Cubes are independent: their elements and text are built on all host
+// threads and written afterwards in the reference's order.  Returns the
+// seconds spent building (elements, interpolation, text) in *build_s.
+void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const std::vector<double> &ecorn, int64_t nf,
                    int dim, const Lattice &L, const double *spacing, double cell_tau, double t, double eF,
-                   const vh_handle *h) {
+                   vh_handle *h, SurfaceWork &W, double *build_s) {
+    const auto t_start = std::chrono::steady_clock::now();
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int64_t cx = nx - 1, cy = ny - 1;
     auto key = [&](int64_t c) {  // (ix, iy, iz) lexicographic
         const int64_t ix = c % cx, iy = (c / cx) % cy, iz = c / (cx * cy);
         return (ix * cy + iy) * (int64_t)std::max(nz - 1, 1) + iz;
     };
-    std::sort(ids.begin(), ids.begin() + nf, [&](int64_t a, int64_t b) { return key(a) < key(b); });
-    const size_t sxy = (size_t)nx * ny;
-    // cubes are independent: each thread formats its cubes' lines, written
-    // afterwards in the reference's order
-    std::vector<std::string> text(nf);
+    W.order.resize(nf);
+    for (int64_t m = 0; m < nf; ++m) W.order[m] = m;
+    std::sort(W.order.begin(), W.order.end(), [&](int64_t a, int64_t b) { return key(ids[a]) < key(ids[b]); });
+    constexpr int MAXEL = 16;
+    const int per = dim == 4 ? 16 : 8, half = per / 2, nzc = dim == 4 ? 2 : 1;
+    W.el.resize((size_t)nf * MAXEL * 8);
+    W.ne.assign(nf, 0);
     int failed = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(max : failed)
-    for (int64_t m = 0; m < nf; ++m) {
-        double el[16 * 8];
-        std::string &os = text[m];
-        const int64_t c = ids[m];
-        const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
-        auto at = [&](int x, int y, int z) { return (size_t)(ix + x) + (size_t)nx * (iy + y) + sxy * (iz + z); };
-        double corners[16];
-        const int nzc = dim == 4 ? 2 : 1;
+    for (int64_t r = 0; r < nf; ++r) {
+        const int64_t m = W.order[r], c = ids[m];
+        const double *ec = ecorn.data() + (size_t)m * per;
+        double corners[16];  // [t][x][y][z]
         for (int it = 0; it < 2; ++it)
             for (int x = 0; x < 2; ++x)
                 for (int y = 0; y < 2; ++y)
                     for (int z = 0; z < nzc; ++z)
-                        corners[((it * 2 + x) * 2 + y) * nzc + z] = sl[it][3][at(x, y, z)];
+                        corners[((it * 2 + x) * 2 + y) * nzc + z] = ec[it * half + (z * 2 + y) * 2 + x];
         int ne = 0;
-        const vh_status st = dim == 4 ? vh_surface_hypercube(corners, eF, spacing, el, 16, &ne)
-                                      : vh_surface_cube(corners, eF, spacing, el, 16, &ne);
+        double *el = W.el.data() + (size_t)r * MAXEL * 8;
+        const vh_status st = dim == 4 ? vh_surface_hypercube(corners, eF, spacing, el, MAXEL, &ne)
+                                      : vh_surface_cube(corners, eF, spacing, el, MAXEL, &ne);
         if (st != VH_OK) {
-            std::printf("freezeout surface: %s at cube (%d, %d, %d)\n", vh_status_string(st), ix, iy, iz);
+            std::printf("freezeout surface: %s at cube (%d, %d, %d)\n", vh_status_string(st), (int)(c % cx),
+                        (int)((c / cx) % cy), dim == 4 ? (int)(c / (cx * cy)) : 0);
             failed = 1;
             continue;
         }
+        W.ne[r] = ne;
+    }
+    if (failed) std::exit(-1);
+    W.off.resize(nf + 1);
+    W.off[0] = 0;
+    for (int64_t r = 0; r < nf; ++r) W.off[r + 1] = W.off[r] + W.ne[r];
+    const int64_t total = W.off[nf];
+    W.cubes.resize(total);
+    W.frac.assign((size_t)total * 4, 0.0);
+    W.vars.resize((size_t)total * 10);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nf; ++r)
+        for (int e = 0; e < W.ne[r]; ++e) {
+            const int64_t k = W.off[r] + e;
+            const double *cen = W.el.data() + (size_t)r * MAXEL * 8 + 2 * dim * e;
+            W.cubes[k] = ids[W.order[r]];
+            for (int d = 0; d < dim; ++d) W.frac[4 * k + d] = cen[d] / spacing[d];
+        }
+    if (total) check_sync(h, vh_fo_interp(h, W.cubes.data(), W.frac.data(), total, W.vars.data()));
+    W.text.resize(nf);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t r = 0; r < nf; ++r) {
+        std::string &os = W.text[r];
+        os.clear();
+        const int64_t c = ids[W.order[r]];
+        const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
         const double cell_x = (double)ix * L.dx - (((double)(nx - 1)) / 2.0 * L.dx);
         const double cell_y = (double)iy * L.dy - (((double)(ny - 1)) / 2.0 * L.dy);
         const double cell_z = dim == 4 ? (double)iz * L.dz - (((double)(nz - 1)) / 2.0 * L.dz) : 0.0;
-        for (int e = 0; e < ne; ++e) {
-            const double *cen = el + 2 * dim * e, *nor = cen + dim;
-            double frac[4] = {0, 0, 0, 0}, var[FO_VARS];
-            for (int k = 0; k < dim; ++k) frac[k] = cen[k] / spacing[k];
-            for (int v = 0; v < FO_VARS; ++v) {
-                if (dim == 4) {
-                    double a[2][2][2][2];
-                    for (int it = 0; it < 2; ++it)
-                        for (int x = 0; x < 2; ++x)
-                            for (int y = 0; y < 2; ++y)
-                                for (int z = 0; z < 2; ++z) a[it][x][y][z] = sl[it][v][at(x, y, z)];
-                    var[v] = interp4(frac[0], frac[1], frac[2], frac[3], a);
-                } else {
-                    double a[2][2][2];
-                    for (int it = 0; it < 2; ++it)
-                        for (int x = 0; x < 2; ++x)
-                            for (int y = 0; y < 2; ++y) a[it][x][y] = sl[it][v][at(x, y, 0)];
-                    var[v] = interp3(frac[0], frac[1], frac[2], a);
-                }
-            }
+        for (int e = 0; e < W.ne[r]; ++e) {
+            const double *cen = W.el.data() + (size_t)r * MAXEL * 8 + 2 * dim * e, *nor = cen + dim;
+            const double *var = W.vars.data() + (size_t)(W.off[r] + e) * 10;
             const double tau = cen[0] + cell_tau, x = cen[1] + cell_x, y = cen[2] + cell_y;
             const double eta = dim == 4 ? cen[3] + cell_z : 0.0;
             const double ds0 = t * nor[0], ds1 = t * nor[1], ds2 = t * nor[2], ds3 = dim == 4 ? t * nor[3] : 0.0;
@@ -549,8 +544,8 @@ void write_surface(std::ofstream &out, std::vector<int64_t> &ids, int64_t nf, co
             }
         }
     }
-    if (failed) std::exit(-1);
-    for (const std::string &line : text) out << line;
+    *build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    for (int64_t r = 0; r < nf; ++r) out << W.text[r];
     out.flush();
 }
 
@@ -611,19 +606,22 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     check("vh_download", vh_download(h, 25, buf.data()));  // record regFacShear from the first step on
     if (X.bulk) check("vh_download", vh_download(h, 26, buf.data()));
     const size_t ncubes = (size_t)std::max(nx - 1, 1) * std::max(ny - 1, 1) * std::max(nz - 1, 1);
-    std::vector<int64_t> ids(ncubes);
-    // the previous and current coarse slices of the interpolated variables
-    // (the previous one starts at zero, HydroPlugin.cpp:159-170)
-    std::vector<double> slice[2][FO_VARS];
-    if (H.do_freezeout && dim)
-        for (int s2 = 0; s2 < 2; ++s2)
-            for (int v = 0; v < FO_VARS; ++v) slice[s2][v].assign(N, 0.0);
+    // the flagged cubes and their corner energies; the device keeps the
+    // previous and current coarse slices of e and the interpolated variables
+    // (the previous ones start at zero, HydroPlugin.cpp:159-170)
+    std::vector<int64_t> ids;
+    std::vector<double> ecorn;
+    SurfaceWork work;
+    if (H.do_freezeout && dim) {
+        ids.resize(ncubes);
+        ecorn.resize(ncubes * (dim == 4 ? 16 : 8));
+    }
     const double spacing[4] = {dt * coarse, L.dx, L.dy, L.dz};  // HydroPlugin.cpp:130-145
     const int sc = nx / 2, sj = ny / 2, sk = nz / 2;  // the reference's centre cell (HydroPlugin.cpp:211-214)
     const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
 
     double t = t0, total_ms = 0;
-    double fo_fields_ms = 0, fo_surface_ms = 0;  // freezeout: field downloads; flags, elements and output
+    double fo_flag_s = 0, fo_build_s = 0, fo_write_s = 0;  // freezeout: flags; elements, variables, text; output
     double snap_ms = 0;
     int nsteps = 0, zero_counts = 0, pend_first = 0, pend_count = 0;
     using clk = std::chrono::steady_clock;
@@ -654,19 +652,16 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
         }
         if (fo) {
             if (H.do_freezeout && dim) {
-                // swapAndSetHydroVariables (FreezeOut.cpp:57-97), then the cut cubes
+                // swapAndSetHydroVariables (FreezeOut.cpp:57-97) on the device, then the cut cubes
                 const auto a = clk::now();
-                for (int v = 0; v < FO_VARS; ++v) {
-                    std::swap(slice[0][v], slice[1][v]);
-                    if (FO_FIELD[v] == 14 && !X.bulk) std::fill(slice[1][v].begin(), slice[1][v].end(), 0.0);
-                    else check_sync(h, vh_download(h, FO_FIELD[v], slice[1][v].data()));
-                }
-                const auto b = clk::now();
                 int64_t nf = 0;
-                check_sync(h, vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf));
-                write_surface(surface, ids, nf, slice, dim, L, spacing, t0 + (double)n * dt, t, eF, h);
-                fo_fields_ms += std::chrono::duration<double, std::milli>(b - a).count();
-                fo_surface_ms += std::chrono::duration<double, std::milli>(clk::now() - b).count();
+                check_sync(h, vh_fo_flag(h, eF, ids.data(), ecorn.data(), (int64_t)ids.size(), &nf));
+                const auto b = clk::now();
+                double build = 0;
+                write_surface(surface, ids, ecorn, nf, dim, L, spacing, t0 + (double)n * dt, t, eF, h, work, &build);
+                fo_flag_s += std::chrono::duration<double>(b - a).count();
+                fo_build_s += build;
+                fo_write_s += std::chrono::duration<double>(clk::now() - b).count() - build;
             }
             int64_t above = 0;
             check_sync(h, vh_count_above(h, eF, &above));
@@ -690,8 +685,8 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
     std::printf("Snapshots: %.3f s\n", snap_ms * 1e-3);
     if (H.do_freezeout && dim)
-        std::printf("Freezeout: %.3f s downloading fields, %.3f s flagging, building and writing the surface\n",
-                    fo_fields_ms * 1e-3, fo_surface_ms * 1e-3);
+        std::printf("Freezeout: %.3f s flagging, %.3f s building elements and text, %.3f s writing\n", fo_flag_s,
+                    fo_build_s, fo_write_s);
     std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
     surface.close();
     check("vh_destroy", vh_destroy(h));

@@ -101,6 +101,68 @@ __global__ __launch_bounds__(256) void fo_corners_kernel(const double *__restric
             }
 }
 
+// Multilinear interpolation inside a cell (FreezeOut.cpp:13-56), the terms
+// summed in the reference's order; a[t][x][y][z] corner values.
+__device__ double fo_lin3(double x0, double x1, double x2, const double (&a)[2][2][2]) {
+    return ((1 - x0) * (1 - x1) * (1 - x2) * a[0][0][0]) + ((x0) * (1 - x1) * (1 - x2) * a[1][0][0]) +
+           ((1 - x0) * (x1) * (1 - x2) * a[0][1][0]) + ((1 - x0) * (1 - x1) * (x2) * a[0][0][1]) +
+           ((x0) * (x1) * (1 - x2) * a[1][1][0]) + ((x0) * (1 - x1) * (x2) * a[1][0][1]) +
+           ((1 - x0) * (x1) * (x2) * a[0][1][1]) + ((x0) * (x1) * (x2) * a[1][1][1]);
+}
+__device__ double fo_lin4(double x0, double x1, double x2, double x3, const double (&a)[2][2][2][2]) {
+    return ((1 - x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[0][0][0][0]) +
+           ((x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[1][0][0][0]) +
+           ((1 - x0) * (x1) * (1 - x2) * (1 - x3) * a[0][1][0][0]) +
+           ((1 - x0) * (1 - x1) * (x2) * (1 - x3) * a[0][0][1][0]) +
+           ((1 - x0) * (1 - x1) * (1 - x2) * (x3) * a[0][0][0][1]) + ((x0) * (x1) * (1 - x2) * (1 - x3) * a[1][1][0][0]) +
+           ((x0) * (1 - x1) * (x2) * (1 - x3) * a[1][0][1][0]) + ((x0) * (1 - x1) * (1 - x2) * (x3) * a[1][0][0][1]) +
+           ((1 - x0) * (x1) * (x2) * (1 - x3) * a[0][1][1][0]) + ((1 - x0) * (x1) * (1 - x2) * (x3) * a[0][1][0][1]) +
+           ((1 - x0) * (1 - x1) * (x2) * (x3) * a[0][0][1][1]) + ((x0) * (x1) * (x2) * (1 - x3) * a[1][1][1][0]) +
+           ((x0) * (x1) * (1 - x2) * (x3) * a[1][1][0][1]) + ((x0) * (1 - x1) * (x2) * (x3) * a[1][0][1][1]) +
+           ((1 - x0) * (x1) * (x2) * (x3) * a[0][1][1][1]) + ((x0) * (x1) * (x2) * (x3) * a[1][1][1][1]);
+}
+
+// interpolateVariable3D/4D (FreezeOut.cpp:182-198) of the ten freezeout
+// variables at points inside cubes: one thread per (point, variable), so the
+// output vars[10 m + v] is written contiguously.  cubes[] were checked to lie
+// in [0, (nx-1)(ny-1)(nz-1 or 1)) by the caller.
+__global__ __launch_bounds__(256) void fo_interp_kernel(FoSlices S, int nx, int ny, int nz,
+                                                        const long long *__restrict__ cubes,
+                                                        const double *__restrict__ frac, long long n,
+                                                        double *__restrict__ vars) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * FO_NVARS) return;
+    const long long m = g / FO_NVARS;
+    const int v = (int)(g - m * FO_NVARS);
+    const long mx = nx - 1, my = ny - 1;
+    const long c = cubes[m];
+    const long ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
+    const long sy = nx, sz = (long)nx * ny;
+    const long base = ix + sy * iy + sz * iz;
+    const double *f = frac + 4 * m;
+    const double *p0 = S.v[0][v], *p1 = S.v[1][v];
+    if (nz > 1) {
+        double a[2][2][2][2];
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y)
+                for (int z = 0; z < 2; ++z) {
+                    const long s = base + x + sy * y + sz * z;
+                    a[0][x][y][z] = p0[s];
+                    a[1][x][y][z] = p1[s];
+                }
+        vars[g] = fo_lin4(f[0], f[1], f[2], f[3], a);
+    } else {
+        double a[2][2][2];
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y) {
+                const long s = base + x + sy * y;
+                a[0][x][y] = p0[s];
+                a[1][x][y] = p1[s];
+            }
+        vars[g] = fo_lin3(f[0], f[1], f[2], a);
+    }
+}
+
 template <int NC, bool TABLE, bool DIM3>
 static hipError_t launch_stage_t(const StageParams &P, const EosDev &E, hipStream_t st) {
     const long N = (long)P.nx * P.ny * P.nz;
@@ -242,6 +304,14 @@ hipError_t launch_fo_corners(const double *eprev, const double *ecur, int nx, in
     if (n <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(fo_corners_kernel, dim3(grid), dim3(256), 0, st, eprev, ecur, nx, ny, nz, ids, n, corners);
+    return hipGetLastError();
+}
+
+hipError_t launch_fo_interp(const FoSlices &S, int nx, int ny, int nz, const long long *cubes, const double *frac,
+                            long long n, double *vars, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n * FO_NVARS + 255) / 256);
+    hipLaunchKernelGGL(fo_interp_kernel, dim3(grid), dim3(256), 0, st, S, nx, ny, nz, cubes, frac, n, vars);
     return hipGetLastError();
 }
 

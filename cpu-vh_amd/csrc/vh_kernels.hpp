@@ -38,6 +38,15 @@ hipError_t launch_fo_flag(const double *eprev, const double *ecur, int nx, int n
                           long long *ids, long long cap, unsigned long long *count, hipStream_t st);
 hipError_t launch_fo_corners(const double *eprev, const double *ecur, int nx, int ny, int nz, const long long *ids,
                              long long n, double *corners, hipStream_t st);
+// The ten variables the freezeout surface interpolates (u^x u^y u^eta e pi^xx
+// pi^xy pi^xeta pi^yy pi^yeta Pi, FreezeOut.cpp:57-97): previous (v[0]) and
+// current (v[1]) coarse slices, nx*ny*nz doubles each.
+constexpr int FO_NVARS = 10;
+struct FoSlices {
+    const double *v[2][FO_NVARS];
+};
+hipError_t launch_fo_interp(const FoSlices &S, int nx, int ny, int nz, const long long *cubes, const double *frac,
+                            long long n, double *vars, hipStream_t st);
 
 // Mixed-precision form (FP32 storage; FP64 root solve, EoS and accumulation).
 namespace mx {

@@ -127,6 +127,19 @@ vh_status vh_count_above(vh_handle *h, double eF, int64_t *count);
  * too small (*n_flagged still set). */
 vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int64_t cap, int64_t *n_flagged);
 
+/* The ten variables surface.dat interpolates (u^x u^y u^eta e pi^xx pi^xy
+ * pi^xeta pi^yy pi^yeta Pi): the reference keeps their previous and current
+ * coarse slices beside e's (swapAndSetHydroVariables, FreezeOut.cpp:57-97),
+ * and vh_fo_flag shifts them with e's (Pi stays zero without bulk).  At n
+ * points inside cubes: cubes[m] a cube id numbered as vh_fo_flag numbers
+ * them, frac[4m .. 4m+3] the point's offset from the cube's first corner in
+ * units of (coarse time step, dx, dy, deta), the fourth unused in 2+1D;
+ * vars[10m + v] = interpolateVariable3D/4D (FreezeOut.cpp:182-198, called at
+ * :248-262 and :327-343), the terms of the multilinear interpolation summed
+ * in the reference's order.  VH_ERR_ARG before the first vh_fo_flag call or
+ * for a cube id out of range.  Synchronising. */
+vh_status vh_fo_interp(vh_handle *h, const int64_t *cubes, const double *frac, int64_t n, double *vars);
+
 /* Host-side EoS of the handle (EquationOfState.cpp:44-290): used by drivers for
  * e_F = e(T_F) and by tests. */
 double vh_eos_energy_density(const vh_handle *h, double T);

@@ -107,6 +107,73 @@ def test_count_and_freezeout_flags_match_oracle():
         o.step(t)
 
 
+FO_VARS = ["ux", "uy", "un", "e", "pixx", "pixy", "pixn", "piyy", "piyn", "Pi"]
+
+
+def _lin(f, a, dim):
+    """interpolateVariable3D/4D (freezeout/FreezeOut.cpp:13-56) with numpy
+    float64 operations in the reference's term order; a[t][x][y][z] (n, ...)."""
+    x = [f[:, k] for k in range(dim)]
+    w = [(1 - xk, xk) for xk in x]
+    if dim == 3:
+        order = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+    else:
+        order = [(0, 0, 0, 0), (1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0), (0, 0, 0, 1), (1, 1, 0, 0), (1, 0, 1, 0),
+                 (1, 0, 0, 1), (0, 1, 1, 0), (0, 1, 0, 1), (0, 0, 1, 1), (1, 1, 1, 0), (1, 1, 0, 1), (1, 0, 1, 1),
+                 (0, 1, 1, 1), (1, 1, 1, 1)]
+    r = None
+    for idx in order:
+        term = w[0][idx[0]]
+        for k in range(1, dim):
+            term = term * w[k][idx[k]]
+        term = term * a[(slice(None),) + idx]
+        r = term if r is None else r + term
+    return r
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(17, 13, 1), (13, 11, 7)])
+def test_freezeout_interpolation(nx, ny, nz):
+    """vh_fo_interp on the device's previous/current slices: bit-identical to
+    the reference's interpolation of the oracle's fields at random points, and
+    exactly the corner value at every corner of the cube (which pins the
+    corner indexing and the slice order)."""
+    dim = 4 if nz > 1 else 3
+    g, o = _pair(nx, ny, nz, True, True, T_reg=0.1)
+    eF = float(np.median(o.get("e")))  # a level that cuts cubes at every step
+    rng = np.random.default_rng(11)
+    prev = {v: np.zeros(g.N) for v in FO_VARS}
+    t0, dt = 0.6, 0.02
+    for n in range(1, 4):
+        ids = g.fo_flag_ids(eF)
+        cur = {v: o.get(v) for v in FO_VARS}
+        assert len(ids) > 0
+        cubes = np.concatenate([ids, ids[:1].repeat(2 ** dim)])
+        frac = np.zeros((len(cubes), 4))
+        frac[:len(ids), :dim] = rng.random((len(ids), dim))
+        corners = np.array(list(np.ndindex(*([2] * dim))), dtype=np.float64)
+        frac[len(ids):, :dim] = corners
+        got = g.fo_interp(cubes, frac)
+        mx, my = nx - 1, ny - 1
+        ix, iy, iz = cubes % mx, (cubes // mx) % my, cubes // (mx * my)
+        for v, name in enumerate(FO_VARS):
+            sl = (prev[name], cur[name])
+            shape = (len(cubes),) + (2,) * dim
+            a = np.empty(shape)
+            for c in np.ndindex(*([2] * dim)):
+                s = (ix + c[1]) + nx * ((iy + c[2]) + ny * ((iz + c[3]) if dim == 4 else iz))
+                a[(slice(None),) + c] = sl[c[0]][s]
+            want = _lin(frac, a, dim)
+            assert V.bits_equal(got[:, v], want), (n, name)
+            for q, c in enumerate(np.ndindex(*([2] * dim))):
+                assert got[len(ids) + q, v] == a[len(ids) + q][c], (n, name, c)
+        prev = cur
+        t = t0 + (n - 1) * dt
+        g.step(t)
+        o.step(t)
+    with pytest.raises(RuntimeError):
+        g.fo_interp(np.array([(nx - 1) * (ny - 1) * max(nz - 1, 1)]), np.zeros((1, 4)))
+
+
 def test_gubser_configuration():
     """The Gubser configuration (201 x 201, dx = 0.05 fm, dtau = 0.0125 fm/c,
     conformal EoS, eta/s = 0.2, regulation below 100 MeV) on the GPU for 40
