@@ -25,11 +25,41 @@
 //     normal is the sum of the triangles' (segment, centroid) vector areas,
 //     each turned towards its segment's low-side point (:478-624).
 #include <cmath>
-#include <vector>
+#include <cstddef>
+#include <utility>
 
 #include "../../include/cpuvh_amd.h"
 
 namespace {
+
+// A list of at most N elements held in place: no allocation per cell.  The
+// capacities used below are the most one cell can produce (two segments per
+// square, six faces per cube, polygons of at least three segments, eight
+// cubes per hypercube), so push_back never meets a full list.
+template <class T, int N>
+struct Small {
+    T v[N];
+    int n = 0;
+    size_t size() const { return (size_t)n; }
+    bool empty() const { return n == 0; }
+    void clear() { n = 0; }
+    void push_back(const T &x) {
+        if (n < N) v[n++] = x;
+    }
+    T &emplace_back() {
+        v[n < N ? n : N - 1] = T();
+        if (n < N) ++n;
+        return v[n - 1];
+    }
+    T &back() { return v[n - 1]; }
+    const T &back() const { return v[n - 1]; }
+    T &operator[](size_t i) { return v[i]; }
+    const T &operator[](size_t i) const { return v[i]; }
+    T *begin() { return v; }
+    T *end() { return v + n; }
+    const T *begin() const { return v; }
+    const T *end() const { return v + n; }
+};
 
 // Points live in the four coordinates (tau, x, y, eta) of the largest cell;
 // a square or cube of a lower-dimensional cell is embedded with its unused
@@ -38,12 +68,16 @@ struct Seg {
     double a[4], b[4];  // end points ("start", "end")
     double out[4];      // a point on the low side
 };
+constexpr int MAX_SEGS = 12;  // per cube: six faces, two segments each
+using SegList = Small<Seg, MAX_SEGS>;
+using Poly = Small<int, MAX_SEGS>;            // segment indices, in chain order
+using PolyList = Small<Poly, MAX_SEGS / 3>;  // per cube
 
 // Contour segments of one square.  p[iu][iv] are the corner values at
 // (iu du, iv dv); put(u, v, dst) places a square-local point.  Returns
 // whether the square is ambiguous (four crossings).
 template <class PUT>
-bool square_segments(const double p[2][2], double E0, double du, double dv, std::vector<Seg> &segs, PUT put) {
+bool square_segments(const double p[2][2], double E0, double du, double dv, SegList &segs, PUT put) {
     int above = 0;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
@@ -137,8 +171,8 @@ bool same_point(const double *p, const double *q) {
 // f[0] < f[1] < f[2], c[i][j][k] the value at (i, j, k) along them.
 struct CubeSurface {
     int f[3];
-    std::vector<Seg> segs;
-    std::vector<std::vector<int>> polys;  // segment indices, in chain order
+    SegList segs;
+    PolyList polys;
     bool ambiguous = false;
 };
 
@@ -172,7 +206,7 @@ void cube_surface(const double c[2][2][2], double E0, const double *d, int cf, d
     }
     // chain segments end to start; a segment joined by its end is flipped
     cs.ambiguous = true;
-    std::vector<char> used(n, 0);
+    char used[MAX_SEGS] = {};
     size_t nused = 0;
     while (nused < n) {
         if (n - nused < 3) {
@@ -206,7 +240,7 @@ void cube_surface(const double c[2][2][2], double E0, const double *d, int cf, d
 
 // Centroid of one polygon: the mean of its corners for a triangle, else the
 // area-weighted mean of the triangles (segment, mean).
-void polygon_centroid(const CubeSurface &cs, const std::vector<int> &poly, double *centroid) {
+void polygon_centroid(const CubeSurface &cs, const Poly &poly, double *centroid) {
     double mean[4] = {0, 0, 0, 0};
     for (int si : poly)
         for (int k = 0; k < 4; ++k) mean[k] += cs.segs[si].a[k] + cs.segs[si].b[k];
@@ -236,7 +270,7 @@ void polygon_centroid(const CubeSurface &cs, const std::vector<int> &poly, doubl
 
 // Normal of one polygon: the sum of the (segment, centroid) triangles' vector
 // areas, each turned towards its segment's low-side point.
-void polygon_normal(const CubeSurface &cs, const std::vector<int> &poly, const double *centroid, double *normal) {
+void polygon_normal(const CubeSurface &cs, const Poly &poly, const double *centroid, double *normal) {
     const int x1 = cs.f[0], x2 = cs.f[1], x3 = cs.f[2];
     for (int k = 0; k < 4; ++k) normal[k] = 0;
     for (int si : poly) {
@@ -274,7 +308,7 @@ extern "C" vh_status vh_surface_square(const double *corners, double E0, const d
                                        int cap, int *n_elements) {
     if (!corners || !d || !n_elements) return VH_ERR_ARG;
     const double p[2][2] = {{corners[0], corners[1]}, {corners[2], corners[3]}};
-    std::vector<Seg> segs;
+    SegList segs;
     square_segments(p, E0, d[0], d[1], segs, [](double u, double v, double *dst) {
         dst[0] = 0;
         dst[1] = 0;
@@ -338,7 +372,7 @@ extern "C" vh_status vh_surface_hypercube(const double *corners, double E0, cons
     *n_elements = 0;
     if (above == 0 || above == 16) return VH_OK;
     // the eight cubes: coordinate i fixed at 0 then at its width
-    std::vector<CubeSurface> cubes(8);
+    CubeSurface cubes[8];
     bool ambiguous = false;
     int nlines = 0;
     for (int i = 0; i < 4; ++i)
@@ -363,17 +397,18 @@ extern "C" vh_status vh_surface_hypercube(const double *corners, double E0, cons
     }
     struct PolyRef {
         const CubeSurface *cs;
-        const std::vector<int> *segs;
+        const Poly *segs;
         double centroid[4];
     };
-    std::vector<PolyRef> polys;
+    constexpr int MAX_POLYS = 8 * (MAX_SEGS / 3);
+    Small<PolyRef, MAX_POLYS> polys;
     for (const CubeSurface &cs : cubes)
         for (const auto &p : cs.polys) {
             PolyRef r{&cs, &p, {0, 0, 0, 0}};
             polygon_centroid(cs, p, r.centroid);
             polys.push_back(r);
         }
-    std::vector<std::vector<int>> hedra;
+    Small<Small<int, MAX_POLYS>, MAX_POLYS> hedra;
     if (!ambiguous) {
         hedra.emplace_back();
         for (size_t i = 0; i < polys.size(); ++i) hedra.back().push_back((int)i);
@@ -388,7 +423,7 @@ extern "C" vh_status vh_surface_hypercube(const double *corners, double E0, cons
                 }
             return false;
         };
-        std::vector<char> used(polys.size(), 0);
+        char used[MAX_POLYS] = {};
         size_t nused = 0;
         while (nused < polys.size()) {
             hedra.emplace_back();
