@@ -469,6 +469,28 @@ vh_status vh_count_above(vh_handle *h, double eF, int64_t *count) {
 // (FreezeOut.cpp:57-97): u^x u^y u^eta e pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi
 static constexpr int FO_FIELD[FO_NVARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
 
+// Sorts cube ids (all in [0, limit)) increasingly: least-significant digit
+// first, 11 bits per stable counting pass (the flag kernel's atomics leave
+// them in any order; tens of thousands per coarse step).
+static void radix_sort_ids(std::vector<long long> &a, long long limit) {
+    const size_t n = a.size();
+    std::vector<long long> tmp(n);
+    constexpr int BITS = 11, R = 1 << BITS;
+    std::vector<size_t> count(R);
+    for (int shift = 0; shift < 63 && (limit - 1) >> shift; shift += BITS) {
+        std::fill(count.begin(), count.end(), 0);
+        for (size_t i = 0; i < n; ++i) ++count[(a[i] >> shift) & (R - 1)];
+        size_t sum = 0;
+        for (int d = 0; d < R; ++d) {
+            const size_t c = count[d];
+            count[d] = sum;
+            sum += c;
+        }
+        for (size_t i = 0; i < n; ++i) tmp[count[(a[i] >> shift) & (R - 1)]++] = a[i];
+        a.swap(tmp);
+    }
+}
+
 // device staging of at least bytes (the stream is idle when it grows)
 static hipError_t fo_staging(vh_handle *h, size_t bytes) {
     if (bytes <= h->fo_buf_bytes) return hipSuccess;
@@ -522,7 +544,7 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     if (n == 0) return VH_OK;
     std::vector<long long> tmp(n);
     HIPCHK(hipMemcpy(tmp.data(), h->fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
-    std::sort(tmp.begin(), tmp.end());
+    radix_sort_ids(tmp, M);
     for (size_t m = 0; m < n; ++m) ids[m] = tmp[m];
     if (corners) {
         HIPCHK(hipMemcpy(h->fo_ids, tmp.data(), sizeof(long long) * n, hipMemcpyHostToDevice));

@@ -65,29 +65,16 @@
 
 #include "../../include/cpuvh_amd.h"
 #include "../../include/cpuvh_hydro.hpp"
+#include "vh_format.hpp"
 
 namespace {
 
 constexpr double HBARC = 0.197326938;  // rhic/src/HydroPlugin.cpp:112
 
 // printf's "%.Nf" and "%g" (6 significant digits, an ostream's default output
-// of a double) appended to out through std::to_chars, which writes the same
-// characters several times faster.
-void put_fixed(std::string &out, double v, int prec) {
-    char tmp[400];
-    const auto r = std::to_chars(tmp, tmp + sizeof tmp, v, std::chars_format::fixed, prec);
-    if (r.ec == std::errc()) {
-        out.append(tmp, r.ptr);
-    } else {  // beyond 1e390: cannot happen for a finite double at these precisions
-        const int len = std::snprintf(tmp, sizeof tmp, "%.*f", prec, v);
-        out.append(tmp, (size_t)std::min(len, (int)sizeof tmp - 1));
-    }
-}
-void put_g(std::string &out, double v) {
-    char tmp[32];
-    const auto r = std::to_chars(tmp, tmp + sizeof tmp, v, std::chars_format::general, 6);
-    out.append(tmp, r.ptr);
-}
+// of a double) appended to a string, exactly and without printf's cost
+using cpuvh::put_fixed;
+using cpuvh::put_g;
 
 // One positional value per line: whitespace removed, the text after the
 // first '=' (the whole line when there is none).  When the file has run out,
@@ -399,22 +386,26 @@ void write_snapshot(const std::vector<double> &v, double t, const std::string &d
     constexpr long ROWS_PER_BLOCK = 64;
     const long nblocks = (rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
     std::vector<std::string> text(nblocks);
+    // each coordinate's "%.3f\t" text, formatted once
+    auto coords = [](int n, double d) {
+        std::vector<std::string> c(n);
+        for (int i = 0; i < n; ++i) {
+            put_fixed(c[i], (i - (n - 1) / 2.) * d, 3);
+            c[i] += '\t';
+        }
+        return c;
+    };
+    const std::vector<std::string> xs = coords(L.nx, L.dx), ys = coords(L.ny, L.dy), zs = coords(L.nz, L.dz);
 #pragma omp parallel for schedule(dynamic, 1)
     for (long b = 0; b < nblocks; ++b) {
         std::string &out = text[b];
         for (long r = b * ROWS_PER_BLOCK; r < std::min(rows, (b + 1) * ROWS_PER_BLOCK); ++r) {
             const int j = (int)(r % L.ny), k = (int)(r / L.ny);
-            const double z = (k - (L.nz - 1) / 2.) * L.dz;
-            const double y = (j - (L.ny - 1) / 2.) * L.dy;
+            const std::string yz = ys[j] + zs[k];
             const size_t s0 = (size_t)r * L.nx;
             for (int i = 0; i < L.nx; ++i) {  // "%.3f\t%.3f\t%.3f\t%.8f\n"
-                const double x = (i - (L.nx - 1) / 2.) * L.dx;
-                put_fixed(out, x, 3);
-                out += '\t';
-                put_fixed(out, y, 3);
-                out += '\t';
-                put_fixed(out, z, 3);
-                out += '\t';
+                out += xs[i];
+                out += yz;
                 put_fixed(out, v[s0 + i], 8);
                 out += '\n';
             }
@@ -442,9 +433,32 @@ void snapshot(vh_handle *h, double t, const std::string &dir, const Lattice &L, 
         for (const F &f : bulkf) one(f);
 }
 
+// Sorts a[0..n) by .first (all keys in [0, limit)): least-significant digit
+// first, 11 bits per stable counting pass; tmp is scratch of the same size.
+template <class T>
+void radix_sort_by_key(std::vector<T> &a, std::vector<T> &tmp, int64_t limit) {
+    const size_t n = a.size();
+    tmp.resize(n);
+    constexpr int BITS = 11, R = 1 << BITS;
+    std::vector<size_t> count(R);
+    for (int shift = 0; shift < 63 && (limit - 1) >> shift; shift += BITS) {
+        std::fill(count.begin(), count.end(), 0);
+        for (size_t i = 0; i < n; ++i) ++count[(a[i].first >> shift) & (R - 1)];
+        size_t sum = 0;
+        for (int d = 0; d < R; ++d) {
+            const size_t c = count[d];
+            count[d] = sum;
+            sum += c;
+        }
+        for (size_t i = 0; i < n; ++i) tmp[count[(a[i].first >> shift) & (R - 1)]++] = a[i];
+        a.swap(tmp);
+    }
+}
+
 // Per-coarse-step work space of the surface writer, kept across calls.
 struct SurfaceWork {
-    std::vector<int64_t> order, cubes;
+    std::vector<std::pair<int64_t, int64_t>> order, scratch;  // (reference-order key, index into ids)
+    std::vector<int64_t> cubes;
     std::vector<double> el, frac, vars;
     std::vector<int> ne;
     std::vector<int64_t> off;
@@ -458,21 +472,30 @@ struct SurfaceWork {
 // from vh_surface_*, the ten variables at their centroids from vh_fo_interp
 // (interpolateVariable3D/4D on the device's slices), one line per element.
 // Cubes are independent: their elements and text are built on all host
-// threads and written afterwards in the reference's order.  Returns the
-// seconds spent building (elements, interpolation, text) in *build_s.
+// threads and written afterwards in the reference's order.  Adds the
+// seconds spent sorting, building elements, interpolating and formatting to
+// phase[0..3].
 void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const std::vector<double> &ecorn, int64_t nf,
                    int dim, const Lattice &L, const double *spacing, double cell_tau, double t, double eF,
-                   vh_handle *h, SurfaceWork &W, double *build_s) {
-    const auto t_start = std::chrono::steady_clock::now();
+                   vh_handle *h, SurfaceWork &W, double *phase) {
+    using clk = std::chrono::steady_clock;
+    auto lap = [](clk::time_point &a) {
+        const clk::time_point b = clk::now();
+        const double s = std::chrono::duration<double>(b - a).count();
+        a = b;
+        return s;
+    };
+    clk::time_point t_lap = clk::now();
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int64_t cx = nx - 1, cy = ny - 1;
-    auto key = [&](int64_t c) {  // (ix, iy, iz) lexicographic
-        const int64_t ix = c % cx, iy = (c / cx) % cy, iz = c / (cx * cy);
-        return (ix * cy + iy) * (int64_t)std::max(nz - 1, 1) + iz;
-    };
     W.order.resize(nf);
-    for (int64_t m = 0; m < nf; ++m) W.order[m] = m;
-    std::sort(W.order.begin(), W.order.end(), [&](int64_t a, int64_t b) { return key(ids[a]) < key(ids[b]); });
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < nf; ++m) {  // key: (ix, iy, iz) lexicographic
+        const int64_t c = ids[m], ix = c % cx, iy = (c / cx) % cy, iz = c / (cx * cy);
+        W.order[m] = {(ix * cy + iy) * (int64_t)std::max(nz - 1, 1) + iz, m};
+    }
+    radix_sort_by_key(W.order, W.scratch, cx * cy * std::max(nz - 1, 1));
+    phase[0] += lap(t_lap);
     constexpr int MAXEL = 16;
     const int per = dim == 4 ? 16 : 8, half = per / 2, nzc = dim == 4 ? 2 : 1;
     W.el.resize((size_t)nf * MAXEL * 8);
@@ -480,7 +503,7 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
     int failed = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(max : failed)
     for (int64_t r = 0; r < nf; ++r) {
-        const int64_t m = W.order[r], c = ids[m];
+        const int64_t m = W.order[r].second, c = ids[m];
         const double *ec = ecorn.data() + (size_t)m * per;
         double corners[16];  // [t][x][y][z]
         for (int it = 0; it < 2; ++it)
@@ -501,6 +524,7 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
         W.ne[r] = ne;
     }
     if (failed) std::exit(-1);
+    phase[1] += lap(t_lap);
     W.off.resize(nf + 1);
     W.off[0] = 0;
     for (int64_t r = 0; r < nf; ++r) W.off[r + 1] = W.off[r] + W.ne[r];
@@ -513,16 +537,17 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
         for (int e = 0; e < W.ne[r]; ++e) {
             const int64_t k = W.off[r] + e;
             const double *cen = W.el.data() + (size_t)r * MAXEL * 8 + 2 * dim * e;
-            W.cubes[k] = ids[W.order[r]];
+            W.cubes[k] = ids[W.order[r].second];
             for (int d = 0; d < dim; ++d) W.frac[4 * k + d] = cen[d] / spacing[d];
         }
     if (total) check_sync(h, vh_fo_interp(h, W.cubes.data(), W.frac.data(), total, W.vars.data()));
+    phase[2] += lap(t_lap);
     W.text.resize(nf);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t r = 0; r < nf; ++r) {
         std::string &os = W.text[r];
         os.clear();
-        const int64_t c = ids[W.order[r]];
+        const int64_t c = ids[W.order[r].second];
         const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
         const double cell_x = (double)ix * L.dx - (((double)(nx - 1)) / 2.0 * L.dx);
         const double cell_y = (double)iy * L.dy - (((double)(ny - 1)) / 2.0 * L.dy);
@@ -544,7 +569,7 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
             }
         }
     }
-    *build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    phase[3] += lap(t_lap);
     for (int64_t r = 0; r < nf; ++r) out << W.text[r];
     out.flush();
 }
@@ -621,7 +646,8 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
 
     double t = t0, total_ms = 0;
-    double fo_flag_s = 0, fo_build_s = 0, fo_write_s = 0;  // freezeout: flags; elements, variables, text; output
+    // freezeout: flags; sort, elements, variables, text; output
+    double fo_flag_s = 0, fo_phase[4] = {0, 0, 0, 0}, fo_write_s = 0;
     double snap_ms = 0;
     int nsteps = 0, zero_counts = 0, pend_first = 0, pend_count = 0;
     using clk = std::chrono::steady_clock;
@@ -657,10 +683,11 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
                 int64_t nf = 0;
                 check_sync(h, vh_fo_flag(h, eF, ids.data(), ecorn.data(), (int64_t)ids.size(), &nf));
                 const auto b = clk::now();
-                double build = 0;
-                write_surface(surface, ids, ecorn, nf, dim, L, spacing, t0 + (double)n * dt, t, eF, h, work, &build);
+                const double before = fo_phase[0] + fo_phase[1] + fo_phase[2] + fo_phase[3];
+                write_surface(surface, ids, ecorn, nf, dim, L, spacing, t0 + (double)n * dt, t, eF, h, work,
+                              fo_phase);
+                const double build = fo_phase[0] + fo_phase[1] + fo_phase[2] + fo_phase[3] - before;
                 fo_flag_s += std::chrono::duration<double>(b - a).count();
-                fo_build_s += build;
                 fo_write_s += std::chrono::duration<double>(clk::now() - b).count() - build;
             }
             int64_t above = 0;
@@ -685,8 +712,9 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
     std::printf("Snapshots: %.3f s\n", snap_ms * 1e-3);
     if (H.do_freezeout && dim)
-        std::printf("Freezeout: %.3f s flagging, %.3f s building elements and text, %.3f s writing\n", fo_flag_s,
-                    fo_build_s, fo_write_s);
+        std::printf("Freezeout: %.3f s flagging, %.3f s ordering, %.3f s elements, %.3f s interpolating, "
+                    "%.3f s text, %.3f s writing\n",
+                    fo_flag_s, fo_phase[0], fo_phase[1], fo_phase[2], fo_phase[3], fo_write_s);
     std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
     surface.close();
     check("vh_destroy", vh_destroy(h));

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cpuvh_amd.h"
@@ -284,17 +285,25 @@ extern "C" vh_status vh_ic_optical_glauber(int A, double b, double sigma_nn_mb, 
     const double nbc_norm = 1. / (sigma_nn_mb * T0 * T0);
     const double wn_norm = 1. / wounded(T0, T0, T0, T0, Ad, Ad, sigma_nn_mb);
     std::vector<double> eT((size_t)nx * ny);
-    for (int i = 0; i < nx; ++i) {
-        const double x = (i - (nx - 1) / 2.) * dx;
-        for (int j = 0; j < ny; ++j) {
-            const double y = (j - (ny - 1) / 2.) * dy;
-            const double TAminus = thickness(x - b / 2, y, Ad), TAplus = thickness(x + b / 2, y, Ad);
-            double ed = nbc_norm * (sigma_nn_mb * TAplus * TAminus);
-            ed *= alpha;
-            ed += (1 - alpha) * wn_norm * wounded(TAminus, TAplus, TAminus, TAplus, Ad, Ad, sigma_nn_mb);
-            eT[(size_t)i + (size_t)j * nx] = ed;
+    // transverse points are independent: columns i spread over host threads
+    auto columns = [&](int first, int stride) {
+        for (int i = first; i < nx; i += stride) {
+            const double x = (i - (nx - 1) / 2.) * dx;
+            for (int j = 0; j < ny; ++j) {
+                const double y = (j - (ny - 1) / 2.) * dy;
+                const double TAminus = thickness(x - b / 2, y, Ad), TAplus = thickness(x + b / 2, y, Ad);
+                double ed = nbc_norm * (sigma_nn_mb * TAplus * TAminus);
+                ed *= alpha;
+                ed += (1 - alpha) * wn_norm * wounded(TAminus, TAplus, TAminus, TAplus, Ad, Ad, sigma_nn_mb);
+                eT[(size_t)i + (size_t)j * nx] = ed;
+            }
         }
-    }
+    };
+    const int nthreads = std::max(1, std::min({16, nx, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(columns, t, nthreads);
+    columns(0, nthreads);
+    for (std::thread &th : pool) th.join();
     for (int k = 0; k < nz; ++k) {
         const double eta = (k - (nz - 1) / 2) * dz;
         const double es = std::fabs(eta) - eta_flat / 2;
