@@ -576,6 +576,7 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
 
 void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X, const std::string &root,
          const std::string &out, const cpuvh::InitialTmunu *vec) {
+    const auto run_start = std::chrono::steady_clock::now();
     std::vector<double> table;
     if (X.eos_table) {
         table.resize(4 * 100000);
@@ -625,6 +626,7 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
 
     initial_state(h, L, I, H, X.bulk, root, vec);
     check("vh_init_conserved", vh_init_conserved(h, t0));
+    const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - run_start).count();
 
     const size_t N = (size_t)nx * ny * nz;
     std::vector<double> buf(N);
@@ -710,6 +712,7 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     }
     flush();
     std::printf("Average time/step: %.3f ms\n", nsteps ? total_ms / nsteps : 0.0);
+    std::printf("Setup: %.3f s (EoS table, device, initial state)\n", setup_s);
     std::printf("Snapshots: %.3f s\n", snap_ms * 1e-3);
     if (H.do_freezeout && dim)
         std::printf("Freezeout: %.3f s flagging, %.3f s ordering, %.3f s elements, %.3f s interpolating, "
