@@ -33,12 +33,26 @@ __global__ __launch_bounds__(256) void init_conserved_kernel(double *__restrict_
 
 // checkForCellsAboveTc (freezeout/FreezeOut.cpp:364-381) as a wavefront
 // reduction: one 64-bit atomic per wave.
+// A bounded grid strides over the cells; each wave sums its ballots, the
+// block its four waves in LDS, and one atomic per block adds the total (an
+// atomic per wave on the one counter serialises and sets the time).
+constexpr int COUNT_BLOCKS = 1024;
 __global__ __launch_bounds__(256) void count_above_kernel(const double *__restrict__ e, long N, double eF,
                                                           unsigned long long *__restrict__ out) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool above = s < N && e[s] > eF;
-    const unsigned long long mask = __ballot(above);
-    if ((threadIdx.x & 63) == 0 && mask) atomicAdd(out, (unsigned long long)__popcll(mask));
+    __shared__ unsigned long long part[4];
+    unsigned long long n = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long base = (long)blockIdx.x * blockDim.x; base < N; base += stride) {  // the same trip count block-wide
+        const long s = base + threadIdx.x;
+        const bool above = s < N && e[s] > eF;
+        n += (unsigned long long)__popcll(__ballot(above));
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(out, t);
+    }
 }
 
 // Freezeout cube flags (the crossing test of cornelius.cpp:2201-2212 /
@@ -285,7 +299,8 @@ hipError_t launch_init_conserved(double *const q[15], const double *Pi, const do
 }
 
 hipError_t launch_count_above(const double *e, long N, double eF, unsigned long long *out, hipStream_t st) {
-    const unsigned grid = (unsigned)((N + 255) / 256);
+    if (N <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<long>((N + 255) / 256, COUNT_BLOCKS);
     hipLaunchKernelGGL(count_above_kernel, dim3(grid), dim3(256), 0, st, e, N, eF, out);
     return hipGetLastError();
 }
