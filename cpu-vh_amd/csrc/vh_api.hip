@@ -469,28 +469,6 @@ vh_status vh_count_above(vh_handle *h, double eF, int64_t *count) {
 // (FreezeOut.cpp:57-97): u^x u^y u^eta e pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi
 static constexpr int FO_FIELD[FO_NVARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
 
-// Sorts cube ids (all in [0, limit)) increasingly: least-significant digit
-// first, 11 bits per stable counting pass (the flag kernel's atomics leave
-// them in any order; tens of thousands per coarse step).
-static void radix_sort_ids(std::vector<long long> &a, long long limit) {
-    const size_t n = a.size();
-    std::vector<long long> tmp(n);
-    constexpr int BITS = 11, R = 1 << BITS;
-    std::vector<size_t> count(R);
-    for (int shift = 0; shift < 63 && (limit - 1) >> shift; shift += BITS) {
-        std::fill(count.begin(), count.end(), 0);
-        for (size_t i = 0; i < n; ++i) ++count[(a[i] >> shift) & (R - 1)];
-        size_t sum = 0;
-        for (int d = 0; d < R; ++d) {
-            const size_t c = count[d];
-            count[d] = sum;
-            sum += c;
-        }
-        for (size_t i = 0; i < n; ++i) tmp[count[(a[i] >> shift) & (R - 1)]++] = a[i];
-        a.swap(tmp);
-    }
-}
-
 // device staging of at least bytes (the stream is idle when it grows)
 static hipError_t fo_staging(vh_handle *h, size_t bytes) {
     if (bytes <= h->fo_buf_bytes) return hipSuccess;
@@ -515,7 +493,8 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     if (!h->fo_prev) {
         HIPCHK(hipMalloc(&h->fo_prev, b));
         HIPCHK(hipMalloc(&h->fo_cur, b));
-        HIPCHK(hipMalloc(&h->fo_ids, sizeof(long long) * std::max(1L, M)));
+        // the ids, then the flag kernels' per-block scratch
+        HIPCHK(hipMalloc(&h->fo_ids, sizeof(long long) * (std::max(1L, M) + fo_flag_blocks(M))));
         HIPCHK(hipMemsetAsync(h->fo_prev, 0, b, h->stream));  // calloc'd slices, HydroPlugin.cpp:159
         HIPCHK(hipMemsetAsync(h->fo_cur, 0, b, h->stream));
     }
@@ -534,7 +513,8 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
                                   h->stream));
     }
     HIPCHK(hipMemsetAsync(h->d_count, 0, sizeof(unsigned long long), h->stream));
-    HIPCHK(launch_fo_flag(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, eF, h->fo_ids, M, h->d_count, h->stream));
+    HIPCHK(launch_fo_flag(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, eF, h->fo_ids, M, h->d_count,
+                          (unsigned long long *)(h->fo_ids + std::max(1L, M)), h->stream));
     unsigned long long n = 0;
     HIPCHK(hipMemcpyAsync(&n, h->d_count, sizeof(n), hipMemcpyDeviceToHost, h->stream));
     vh_status s = check_err(h);
@@ -542,12 +522,10 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     *n_flagged = (int64_t)n;
     if ((int64_t)n > cap) return VH_ERR_CAPACITY;
     if (n == 0) return VH_OK;
-    std::vector<long long> tmp(n);
-    HIPCHK(hipMemcpy(tmp.data(), h->fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
-    radix_sort_ids(tmp, M);
-    for (size_t m = 0; m < n; ++m) ids[m] = tmp[m];
+    // the ids come out in increasing order
+    static_assert(sizeof(long long) == sizeof(int64_t), "cube ids are 64-bit");
+    HIPCHK(hipMemcpy(ids, h->fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
     if (corners) {
-        HIPCHK(hipMemcpy(h->fo_ids, tmp.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
         const int per = c.nz > 1 ? 16 : 8;
         HIPCHK(fo_staging(h, sizeof(double) * per * n));
         HIPCHK(launch_fo_corners(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, h->fo_ids, (long long)n, h->fo_buf,

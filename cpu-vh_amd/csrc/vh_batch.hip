@@ -784,7 +784,8 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     if (!E.fo_prev) {
         BCHK(hipMalloc(&E.fo_prev, bytes));
         BCHK(hipMalloc(&E.fo_cur, bytes));
-        BCHK(hipMalloc(&E.fo_ids, sizeof(long long) * std::max(1L, M)));
+        // the ids, then the flag kernels' per-block scratch
+        BCHK(hipMalloc(&E.fo_ids, sizeof(long long) * (std::max(1L, M) + fo_flag_blocks(M))));
         BCHK(hipMemsetAsync(E.fo_prev, 0, bytes, b->stream));
         BCHK(hipMemsetAsync(E.fo_cur, 0, bytes, b->stream));
     }
@@ -792,7 +793,8 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     if (b->mixed) BCHK(launch_widen(b->ef[0] + E.off, nullptr, E.fo_cur, E.N, b->stream));
     else BCHK(hipMemcpyAsync(E.fo_cur, b->e[0] + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
     BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
-    BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts, b->stream));
+    BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts,
+                        (unsigned long long *)(E.fo_ids + std::max(1L, M)), b->stream));
     unsigned long long n = 0;
     BCHK(hipMemcpyAsync(&n, b->d_counts, sizeof(n), hipMemcpyDeviceToHost, b->stream));
     vh_status s = vh_batch_sync(b);
@@ -800,12 +802,9 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     *n_flagged = (int64_t)n;
     if ((int64_t)n > cap) return VH_ERR_CAPACITY;
     if (n == 0) return VH_OK;
-    std::vector<long long> tmp(n);
-    BCHK(hipMemcpy(tmp.data(), E.fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
-    std::sort(tmp.begin(), tmp.end());
-    for (size_t m = 0; m < n; ++m) ids[m] = tmp[m];
+    // the ids come out in increasing order
+    BCHK(hipMemcpy(ids, E.fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
     if (corners) {
-        BCHK(hipMemcpy(E.fo_ids, tmp.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
         const int per = c.nz > 1 ? 16 : 8;
         double *dc = nullptr;
         BCHK(hipMalloc(&dc, sizeof(double) * per * n));

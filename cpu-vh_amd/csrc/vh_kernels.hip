@@ -55,39 +55,90 @@ __global__ __launch_bounds__(256) void count_above_kernel(const double *__restri
     }
 }
 
-// Freezeout cube flags (the crossing test of cornelius.cpp:2201-2212 /
-// :2257-2272 on the cubes FreezeOut.cpp:207-212 / :288-291 visits), with
-// wave-aggregated append of the flagged cube ids.
-__global__ __launch_bounds__(256) void fo_flag_kernel(const double *__restrict__ eprev,
-                                                      const double *__restrict__ ecur, int nx, int ny, int nz,
-                                                      double eF, long long *__restrict__ ids, long long cap,
-                                                      unsigned long long *__restrict__ count) {
+// Freezeout cube flags: the crossing test of cornelius.cpp:2201-2212 /
+// :2257-2272 on the cubes FreezeOut.cpp:207-212 / :288-291 visits (cube c
+// below (nx-1)(ny-1)(nz-1 or 1), x fastest).
+__device__ __forceinline__ bool fo_cut(const double *__restrict__ eprev, const double *__restrict__ ecur, int nx,
+                                       int ny, int nz, double eF, long c) {
     const bool d4 = nz > 1;
-    const long mx = nx - 1, my = ny - 1, mz = d4 ? nz - 1 : 1;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool flag = false;
-    if (c < mx * my * mz) {
-        const long ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
-        const long base = ix + (long)nx * (iy + (long)ny * iz);
-        const long sx = 1, sy = nx, sz = (long)nx * ny;
-        int above = 0, tot = 0;
-        for (int dz = 0; dz <= (d4 ? 1 : 0); ++dz)
-            for (int dy = 0; dy <= 1; ++dy)
-                for (int dx = 0; dx <= 1; ++dx) {
-                    const long s = base + dx * sx + dy * sy + dz * sz;
-                    above += (eprev[s] >= eF) + (ecur[s] >= eF);
-                    tot += 2;
-                }
-        flag = above != 0 && above != tot;
-    }
+    const long mx = nx - 1, my = ny - 1;
+    const long ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
+    const long base = ix + (long)nx * (iy + (long)ny * iz);
+    const long sx = 1, sy = nx, sz = (long)nx * ny;
+    int above = 0, tot = 0;
+    for (int dz = 0; dz <= (d4 ? 1 : 0); ++dz)
+        for (int dy = 0; dy <= 1; ++dy)
+            for (int dx = 0; dx <= 1; ++dx) {
+                const long s = base + dx * sx + dy * sy + dz * sz;
+                above += (eprev[s] >= eF) + (ecur[s] >= eF);
+                tot += 2;
+            }
+    return above != 0 && above != tot;
+}
+
+// The flagged ids in increasing order without atomics (an append with a
+// returning atomic per wave serialises on the one counter): pass 1 counts
+// the cut cubes of each block of 256, pass 2 turns the counts into offsets,
+// pass 3 recomputes the flags and writes each id at its block's offset plus
+// its rank in the block.
+__global__ __launch_bounds__(256) void fo_flag_count_kernel(const double *__restrict__ eprev,
+                                                            const double *__restrict__ ecur, int nx, int ny, int nz,
+                                                            double eF, long M, unsigned long long *__restrict__ blk) {
+    __shared__ unsigned int part[4];
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    const bool flag = c < M && fo_cut(eprev, ecur, nx, ny, nz, eF, c);
     const unsigned long long mask = __ballot(flag);
-    const int lane = threadIdx.x & 63;
-    unsigned long long basepos = 0;
-    if (lane == 0 && mask) basepos = atomicAdd(count, (unsigned long long)__popcll(mask));
-    basepos = __shfl(basepos, 0);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = (unsigned long long)part[0] + part[1] + part[2] + part[3];
+}
+
+// exclusive prefix sum of blk[0 .. nb) in place and the total into *count;
+// one block of 1024 threads walks the counts in chunks of 1024
+__global__ __launch_bounds__(1024) void fo_flag_scan_kernel(unsigned long long *__restrict__ blk, long nb,
+                                                           unsigned long long *__restrict__ count) {
+    __shared__ unsigned long long wsum[16];
+    __shared__ unsigned long long carry_s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (long base = 0; base < nb; base += 1024) {
+        const long i = base + threadIdx.x;
+        const unsigned long long v = i < nb ? blk[i] : 0ull;
+        unsigned long long x = v;  // inclusive scan within the wave
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned long long y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        unsigned long long woff = 0;
+        for (int k = 0; k < w; ++k) woff += wsum[k];
+        const unsigned long long carry = carry_s;
+        if (i < nb) blk[i] = carry + woff + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s = carry + woff + x;  // the chunk's total added
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = carry_s;
+}
+
+__global__ __launch_bounds__(256) void fo_flag_write_kernel(const double *__restrict__ eprev,
+                                                            const double *__restrict__ ecur, int nx, int ny, int nz,
+                                                            double eF, long M,
+                                                            const unsigned long long *__restrict__ blk,
+                                                            long long *__restrict__ ids, long long cap) {
+    __shared__ unsigned int part[4];
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    const bool flag = c < M && fo_cut(eprev, ecur, nx, ny, nz, eF, c);
+    const unsigned long long mask = __ballot(flag);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    unsigned long long off = blk[blockIdx.x];
+    for (int k = 0; k < w; ++k) off += part[k];
     if (flag) {
-        const unsigned long long below = mask & ((1ull << lane) - 1ull);
-        const unsigned long long pos = basepos + __popcll(below);
+        const unsigned long long pos = off + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
         if ((long long)pos < cap) ids[pos] = c;
     }
 }
@@ -306,11 +357,16 @@ hipError_t launch_count_above(const double *e, long N, double eF, unsigned long 
 }
 
 hipError_t launch_fo_flag(const double *eprev, const double *ecur, int nx, int ny, int nz, double eF,
-                          long long *ids, long long cap, unsigned long long *count, hipStream_t st) {
+                          long long *ids, long long cap, unsigned long long *count, unsigned long long *blk,
+                          hipStream_t st) {
     const long M = (long)(nx - 1) * (ny - 1) * (nz > 1 ? nz - 1 : 1);
     if (M <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((M + 255) / 256);
-    hipLaunchKernelGGL(fo_flag_kernel, dim3(grid), dim3(256), 0, st, eprev, ecur, nx, ny, nz, eF, ids, cap, count);
+    const long nb = fo_flag_blocks(M);
+    hipLaunchKernelGGL(fo_flag_count_kernel, dim3((unsigned)nb), dim3(256), 0, st, eprev, ecur, nx, ny, nz, eF, M,
+                       blk);
+    hipLaunchKernelGGL(fo_flag_scan_kernel, dim3(1), dim3(1024), 0, st, blk, nb, count);
+    hipLaunchKernelGGL(fo_flag_write_kernel, dim3((unsigned)nb), dim3(256), 0, st, eprev, ecur, nx, ny, nz, eF, M,
+                       (const unsigned long long *)blk, ids, cap);
     return hipGetLastError();
 }
 

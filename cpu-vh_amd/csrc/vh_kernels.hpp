@@ -34,8 +34,12 @@ hipError_t launch_strikes(EvDesc *ev, const unsigned long long *counts, int *str
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,
                                  const double *const u[4], long N, hipStream_t st);
 hipError_t launch_count_above(const double *e, long N, double eF, unsigned long long *out, hipStream_t st);
+// The cut cubes' ids in increasing order into ids (capacity cap), their
+// number into *count; blk is scratch of fo_flag_blocks(cubes) entries.
+inline long fo_flag_blocks(long cubes) { return (cubes + 255) / 256; }
 hipError_t launch_fo_flag(const double *eprev, const double *ecur, int nx, int ny, int nz, double eF,
-                          long long *ids, long long cap, unsigned long long *count, hipStream_t st);
+                          long long *ids, long long cap, unsigned long long *count, unsigned long long *blk,
+                          hipStream_t st);
 hipError_t launch_fo_corners(const double *eprev, const double *ecur, int nx, int ny, int nz, const long long *ids,
                              long long n, double *corners, hipStream_t st);
 // The ten variables the freezeout surface interpolates (u^x u^y u^eta e pi^xx

@@ -107,6 +107,24 @@ def test_count_and_freezeout_flags_match_oracle():
         o.step(t)
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(71, 71, 61), (601, 601, 1)])
+def test_freezeout_flags_many_blocks(nx, ny, nz):
+    """Over 1024 blocks of 256 cubes (294,000 and 360,000 cubes), so the
+    offsets scan of the ordered flag compaction walks several chunks: the ids
+    equal the oracle's crossing test on the same two slices."""
+    g, _ = _pair(nx, ny, nz, False, False, T_reg=0.1)
+    eF = float(np.median(g.get("e")))
+    prev = np.zeros(g.N)
+    for n in range(3):
+        ids = g.fo_flag_ids(eF)
+        cur = g.get("e")
+        want = V.oracle_fo_flag(prev, cur, nx, ny, nz, eF)
+        assert len(want) > 0
+        assert np.array_equal(ids, want), n
+        prev = cur
+        g.step(0.6 + n * 0.02)
+
+
 FO_VARS = ["ux", "uy", "un", "e", "pixx", "pixy", "pixn", "piyy", "piyn", "Pi"]
 
 
