@@ -120,26 +120,57 @@ hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const doubl
     return hipGetLastError();
 }
 
-// Per-event count of cells with e > eF (FP32 state, compared in FP64).
-__global__ __launch_bounds__(64) void count_above_batch_f_kernel(const float *__restrict__ e, double eF,
-                                                                 const EvDesc *__restrict__ ev,
-                                                                 const int *__restrict__ ev_of,
-                                                                 const long *__restrict__ start,
-                                                                 unsigned long long *__restrict__ counts) {
-    const int id = ev_of[blockIdx.x];
-    const EvDesc &D = ev[id];
-    if (!D.active) return;
-    const long N = (long)D.nx * D.ny * D.nz;
-    const long sl = start[blockIdx.x] + threadIdx.x;
-    const bool above = sl < N && (double)e[D.off + sl] > eF;
+// Per-event count of cells with e > eF (FP32 state, compared in FP64).  Each
+// wave takes one 64-cell entry of the block table, the block four; one
+// thread adds the waves' counts per run of equal events, so one atomic per
+// event run replaces one per 64 cells.
+__global__ __launch_bounds__(256) void count_above_batch_f_kernel(const float *__restrict__ e, double eF,
+                                                                  const EvDesc *__restrict__ ev,
+                                                                  const int *__restrict__ ev_of,
+                                                                  const long *__restrict__ start, int nblocks,
+                                                                  unsigned long long *__restrict__ counts) {
+    __shared__ int wid[4];
+    __shared__ unsigned long long wcnt[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long k = (long)blockIdx.x * 4 + w;  // this wave's table entry
+    int id = -1;
+    bool above = false;
+    if (k < nblocks) {
+        const int eid = ev_of[k];
+        const EvDesc &D = ev[eid];
+        if (D.active) {
+            id = eid;
+            const long N = (long)D.nx * D.ny * D.nz;
+            const long sl = start[k] + lane;
+            above = sl < N && (double)e[D.off + sl] > eF;
+        }
+    }
     const unsigned long long mask = __ballot(above);
-    if (threadIdx.x == 0 && mask) atomicAdd(&counts[id], (unsigned long long)__popcll(mask));
+    if (lane == 0) {
+        wid[w] = id;
+        wcnt[w] = (unsigned long long)__popcll(mask);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cur = -1;
+        unsigned long long sum = 0;
+        for (int q = 0; q < 4; ++q) {
+            if (wid[q] != cur) {
+                if (cur >= 0 && sum) atomicAdd(&counts[cur], sum);
+                cur = wid[q];
+                sum = 0;
+            }
+            sum += wcnt[q];
+        }
+        if (cur >= 0 && sum) atomicAdd(&counts[cur], sum);
+    }
 }
 
 hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
                                       const long *start, int nblocks, unsigned long long *counts, hipStream_t st) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(count_above_batch_f_kernel, dim3(nblocks), dim3(64), 0, st, e, eF, ev, ev_of, start, counts);
+    hipLaunchKernelGGL(count_above_batch_f_kernel, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, st, e, eF, ev,
+                       ev_of, start, nblocks, counts);
     return hipGetLastError();
 }
 
