@@ -159,7 +159,7 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
             add_event(wb, ev)
         if precision == C.PREC_MIXED and FACE_MODE is not None:
             wb.set_face_mode(FACE_MODE)
-        tiles = wb.autotune(reps=2)
+        tiles = wb.autotune(reps=5)
         wb.steps(warmup)
         wb.sync()
         wb.close()
