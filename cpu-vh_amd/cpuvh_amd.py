@@ -26,6 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag", "vh_fo_interp",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
+           "vh_inferred_variables",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
@@ -134,6 +135,7 @@ def _lib():
             getattr(L, f).restype = D
         L.vh_last_kernel_ms.argtypes = [P]
         L.vh_last_kernel_ms.restype = D
+        L.vh_inferred_variables.argtypes = [P, D, P, P, ctypes.c_int64, P, P]
         L.vh_status_string.argtypes = [ctypes.c_int]
         L.vh_status_string.restype = ctypes.c_char_p
         L.vh_build_info.restype = ctypes.c_char_p
@@ -308,6 +310,22 @@ class Hydro:
 
     def last_kernel_ms(self):
         return _lib().vh_last_kernel_ms(self.h)
+
+    def inferred(self, t, q, ePrev):
+        """getInferredVariables on the device for n states: q is (nc, n),
+        ePrev (n,).  Returns (out (6, n): e, p, u^tau, u^x, u^y, u^eta;
+        NaN site per state, 0 if none)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        ePrev = np.ascontiguousarray(ePrev, dtype=np.float64).ravel()
+        n = ePrev.size
+        if q.shape != (self.nc, n):
+            raise ValueError(f"q must be ({self.nc}, {n})")
+        out = np.empty((6, n), dtype=np.float64)
+        nan = np.zeros(n, dtype=np.int32)
+        st = _lib().vh_inferred_variables(self.h, t, _ptr(q), _ptr(ePrev), n, _ptr(out), _ptr(nan))
+        if st not in (VH_OK, VH_ERR_NAN):
+            self._check(st, "vh_inferred_variables")
+        return out, nan
 
 
 class Batch:

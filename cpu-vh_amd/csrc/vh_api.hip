@@ -330,7 +330,8 @@ vh_status vh_step(vh_handle *h, double t) {
     return enqueue_step(h, st, nullptr, 0);
 }
 
-// Graph of [stage 1, stage 2] x count for the current buffer-role state.
+// Graph of [stage 1, stage 2] x count for the current buffer-role state,
+// count <= MAX_GRAPH_STEPS, ending with a node that moves the step clock on.
 static vh_status get_graph(vh_handle *h, int count, hipGraphExec_t *out) {
     const int parity = h->iq_cur * 3 + h->iu_cur;  // buffer-role state the graph was captured in
     const auto key = std::make_pair(count, parity);
@@ -341,6 +342,10 @@ static vh_status get_graph(vh_handle *h, int count, hipGraphExec_t *out) {
         HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         vh_status cs = VH_OK;
         for (int k = 0; k < count && cs == VH_OK; ++k) cs = enqueue_step(h, StepTimes{}, h->d_times, k);
+        if (cs == VH_OK) {  // the next graph replayed continues at step *d_base + count
+            const hipError_t ae = launch_advance_base(h->d_base, count, h->stream);
+            if (ae != hipSuccess) cs = hip_fail(ae, "advance step clock");
+        }
         hipError_t ce = hipStreamEndCapture(h->stream, &g);
         // capture only records the launches: restore the roles, replay applies them
         h->iq_cur = iq0; h->iq_next = iqn0; h->iu_cur = iu0; h->iu_prev = iup0;
@@ -356,13 +361,32 @@ static vh_status get_graph(vh_handle *h, int count, hipGraphExec_t *out) {
     return VH_OK;
 }
 
+// The graphs that replay `count` steps from the current buffer roles, in
+// order (captured where missing; the stream must be idle).
+static vh_status graphs_for(vh_handle *h, int count, std::vector<hipGraphExec_t> *out) {
+    const int iq0 = h->iq_cur, iqn0 = h->iq_next, iu0 = h->iu_cur, iup0 = h->iu_prev;
+    vh_status s = VH_OK;
+    for (int done = 0; done < count && s == VH_OK;) {
+        const int c = std::min(count - done, MAX_GRAPH_STEPS);
+        hipGraphExec_t ge = nullptr;
+        s = get_graph(h, c, &ge);
+        if (out) out->push_back(ge);
+        for (int k = 0; k < c; ++k) {
+            std::swap(h->iq_cur, h->iq_next);
+            std::swap(h->iu_cur, h->iu_prev);
+        }
+        done += c;
+    }
+    h->iq_cur = iq0; h->iq_next = iqn0; h->iu_cur = iu0; h->iu_prev = iup0;
+    return s;
+}
+
 vh_status vh_prepare_steps(vh_handle *h, int count) {
     if (!h || count < 1) return VH_ERR_ARG;
     if (!h->conserved_ready) return VH_ERR_STATE;
     HIPCHK(hipSetDevice(h->dev));
     HIPCHK(hipStreamSynchronize(h->stream));
-    hipGraphExec_t ge;
-    return get_graph(h, count, &ge);
+    return graphs_for(h, count, nullptr);
 }
 
 vh_status vh_steps(vh_handle *h, int n_first, int count) {
@@ -373,13 +397,15 @@ vh_status vh_steps(vh_handle *h, int n_first, int count) {
     vh_status s = ensure_times(h, n_first + count);
     if (s != VH_OK) return s;
     HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<hipGraphExec_t> ges;
+    s = graphs_for(h, count, &ges);
+    if (s != VH_OK) return s;
     *h->h_base = n_first;
     HIPCHK(hipMemcpyAsync(h->d_base, h->h_base, sizeof(int), hipMemcpyHostToDevice, h->stream));
-    hipGraphExec_t ge = nullptr;
-    s = get_graph(h, count, &ge);
-    if (s != VH_OK) return s;
+    // graphs of at most MAX_GRAPH_STEPS steps, back to back: each one's last
+    // node moves *d_base on, so the next continues the clock
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    HIPCHK(hipGraphLaunch(ge, h->stream));
+    for (hipGraphExec_t ge : ges) HIPCHK(hipGraphLaunch(ge, h->stream));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     for (int k = 0; k < count; ++k) {
         std::swap(h->iq_cur, h->iq_next);
@@ -393,6 +419,34 @@ vh_status vh_steps(vh_handle *h, int n_first, int count) {
 }
 
 double vh_last_kernel_ms(const vh_handle *h) { return h ? h->last_kernel_ms : 0.0; }
+
+vh_status vh_inferred_variables(vh_handle *h, double t, const double *q, const double *ePrev, int64_t n, double *out,
+                                int32_t *nan_site) {
+    if (!h || !q || !ePrev || !out || n < 0) return VH_ERR_ARG;
+    if (n == 0) return VH_OK;
+    HIPCHK(hipSetDevice(h->dev));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t nq = (size_t)h->nc * n;
+    double *dq = nullptr;
+    HIPCHK(hipMalloc(&dq, sizeof(double) * (nq + 7 * (size_t)n) + sizeof(int) * n));
+    double *de = dq + nq, *dout = de + n;
+    int *dnan = reinterpret_cast<int *>(dout + 6 * (size_t)n);
+    hipError_t r = hipMemcpy(dq, q, sizeof(double) * nq, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemcpy(de, ePrev, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = launch_inferred(h->eos, h->nc, t, dq, de, n, dout, dnan, h->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(h->stream);
+    if (r == hipSuccess) r = hipMemcpy(out, dout, sizeof(double) * 6 * n, hipMemcpyDeviceToHost);
+    std::vector<int> nan(n);
+    if (r == hipSuccess) r = hipMemcpy(nan.data(), dnan, sizeof(int) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dq);
+    if (r != hipSuccess) return hip_fail(r, "vh_inferred_variables");
+    bool any = false;
+    for (int64_t i = 0; i < n; ++i) {
+        if (nan_site) nan_site[i] = nan[i];
+        any = any || nan[i] != 0;
+    }
+    return any ? VH_ERR_NAN : VH_OK;
+}
 
 vh_status vh_sync(vh_handle *h) {
     if (!h) return VH_ERR_ARG;

@@ -103,7 +103,7 @@ struct vh_batch {
     // a stopped event's latest state sits in the buffers that were current
     // when it stopped, current again after an even number of steps
     long step_count = 0;
-    std::map<std::tuple<int, int, int, int, uint64_t>, hipGraphExec_t> run_graphs;
+    std::map<std::tuple<int, int, int, int, int, uint64_t>, hipGraphExec_t> run_graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;  // ev0/ev1 bracket a graph launch not yet read
     double last_ms = 0;
@@ -193,6 +193,9 @@ vh_status rebuild_layout(vh_batch *b) {
         }
     }
     b->ncnt = (int)cev.size();
+    // captured graphs and queued launches read the old tables: let them
+    // finish before the tables are freed and the graphs destroyed
+    BCHK(hipStreamSynchronize(b->stream));
     (void)hipFree(b->d_blk_ev);
     (void)hipFree(b->d_blk_first);
     (void)hipFree(b->d_cnt_ev);
@@ -564,15 +567,11 @@ vh_status vh_batch_remove(vh_batch *b, int id) {
     return VH_OK;
 }
 
-vh_status vh_batch_steps(vh_batch *b, int count) {
-    if (!b || count < 0) return VH_ERR_ARG;
-    if (count == 0) return VH_OK;
-    BCHK(hipSetDevice(b->dev));
-    for (const Event &E : b->ev)
-        if (E.used && !E.stopped && E.steps_done + count > b->cfg.max_steps) return VH_ERR_CAPACITY;
-    vh_status s = rebuild_layout(b);
-    if (s != VH_OK) return s;
-    if (b->nblk2 + b->nblk3 == 0) return VH_OK;
+namespace {
+
+// Graph of `count` batched steps (count <= MAX_GRAPH_STEPS) from the current
+// buffer roles, ending with the node that moves every event's clock on.
+vh_status steps_graph(vh_batch *b, int count, hipGraphExec_t *out) {
     const auto key = std::make_tuple(count, b->iq_cur * 3 + b->iu_cur, b->layout_version);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
@@ -595,14 +594,92 @@ vh_status vh_batch_steps(vh_batch *b, int count) {
         if (ie != hipSuccess) return bfail(ie, "hipGraphInstantiate");
         it = b->graphs.emplace(key, ge).first;
     }
-    BCHK(hipEventRecord(b->ev0, b->stream));
-    BCHK(hipGraphLaunch(it->second, b->stream));
-    BCHK(hipEventRecord(b->ev1, b->stream));
-    b->timed = true;
-    for (int k = 0; k < count; ++k) {
+    *out = it->second;
+    return VH_OK;
+}
+
+// Graph of `nchecks` rounds of (coarse steps, per-event count above eF,
+// strike) numbered from check_off, ending with the clock node
+// (coarse * nchecks <= MAX_GRAPH_STEPS unless one round is longer).
+vh_status run_graph(vh_batch *b, double eF, int coarse, int nchecks, int check_off, hipGraphExec_t *out) {
+    const int nev = (int)b->ev.size();
+    // the graph's count nodes carry eF: its bits are part of the key
+    uint64_t eF_bits = 0;
+    std::memcpy(&eF_bits, &eF, sizeof eF_bits);
+    const auto key = std::make_tuple(coarse, nchecks, check_off, b->iq_cur * 3 + b->iu_cur, b->layout_version, eF_bits);
+    auto it = b->run_graphs.find(key);
+    if (it == b->run_graphs.end()) {
+        const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
+        hipGraph_t g = nullptr;
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        vh_status cs = VH_OK;
+        for (int c = 0; c < nchecks && cs == VH_OK; ++c) {
+            for (int k = 0; k < coarse && cs == VH_OK; ++k) cs = enqueue_batch_step(b, c * coarse + k);
+            hipError_t ke = launch_zero_counts(b->d_counts, nev, b->stream);
+            if (ke == hipSuccess)
+                ke = b->mixed ? launch_count_above_batch_f(b->ef[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
+                                                           b->ncnt, b->d_counts, b->stream)
+                              : launch_count_above_batch(b->e[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
+                                                         b->ncnt, b->d_counts, b->stream);
+            if (ke == hipSuccess)
+                ke = launch_strikes(b->d_ev, b->d_counts, b->d_strikes, b->d_stop, nev, check_off + c, b->stream);
+            if (cs == VH_OK && ke != hipSuccess) cs = bfail(ke, "stop rule");
+        }
+        if (cs == VH_OK) {
+            hipError_t ae = launch_advance_clocks(b->d_ev, nev, coarse * nchecks, b->stream);
+            if (ae != hipSuccess) cs = bfail(ae, "advance clocks");
+        }
+        hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
+        if (cs != VH_OK) return cs;
+        if (ce != hipSuccess) return bfail(ce, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return bfail(ie, "hipGraphInstantiate");
+        it = b->run_graphs.emplace(key, ge).first;
+    }
+    *out = it->second;
+    return VH_OK;
+}
+
+void rotate_roles(vh_batch *b, int steps) {
+    for (int k = 0; k < steps; ++k) {
         std::swap(b->iq_cur, b->iq_next);
         std::swap(b->iu_cur, b->iu_prev);
     }
+}
+
+}  // namespace
+
+vh_status vh_batch_steps(vh_batch *b, int count) {
+    if (!b || count < 0) return VH_ERR_ARG;
+    if (count == 0) return VH_OK;
+    BCHK(hipSetDevice(b->dev));
+    for (const Event &E : b->ev)
+        if (E.used && !E.stopped && E.steps_done + count > b->cfg.max_steps) return VH_ERR_CAPACITY;
+    vh_status s = rebuild_layout(b);
+    if (s != VH_OK) return s;
+    if (b->nblk2 + b->nblk3 == 0) return VH_OK;
+    // capture first (on an idle stream), then replay the chunks back to back
+    BCHK(hipStreamSynchronize(b->stream));
+    std::vector<hipGraphExec_t> ges;
+    const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
+    for (int done = 0; done < count && s == VH_OK;) {
+        const int c = std::min(count - done, MAX_GRAPH_STEPS);
+        hipGraphExec_t ge = nullptr;
+        s = steps_graph(b, c, &ge);
+        ges.push_back(ge);
+        rotate_roles(b, c);
+        done += c;
+    }
+    b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
+    if (s != VH_OK) return s;
+    BCHK(hipEventRecord(b->ev0, b->stream));
+    for (hipGraphExec_t ge : ges) BCHK(hipGraphLaunch(ge, b->stream));
+    BCHK(hipEventRecord(b->ev1, b->stream));
+    b->timed = true;
+    rotate_roles(b, count);
     for (Event &E : b->ev)
         if (E.used && !E.stopped) E.steps_done += count;
     b->step_count += count;
@@ -619,49 +696,27 @@ vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t 
     if (s != VH_OK) return s;
     const int nev = (int)b->ev.size();
     if (b->nblk2 + b->nblk3 > 0) {
-        // the graph's count nodes carry eF: its bits are part of the key
-        uint64_t eF_bits = 0;
-        std::memcpy(&eF_bits, &eF, sizeof eF_bits);
-        const auto key = std::make_tuple(coarse, nchecks, b->iq_cur * 3 + b->iu_cur, b->layout_version, eF_bits);
-        auto it = b->run_graphs.find(key);
-        if (it == b->run_graphs.end()) {
-            const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
-            hipGraph_t g = nullptr;
-            BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            vh_status cs = VH_OK;
-            for (int c = 0; c < nchecks && cs == VH_OK; ++c) {
-                for (int k = 0; k < coarse && cs == VH_OK; ++k) cs = enqueue_batch_step(b, c * coarse + k);
-                hipError_t ke = launch_zero_counts(b->d_counts, nev, b->stream);
-                if (ke == hipSuccess)
-                    ke = b->mixed ? launch_count_above_batch_f(b->ef[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
-                                                               b->ncnt, b->d_counts, b->stream)
-                                  : launch_count_above_batch(b->e[0], eF, b->d_ev, b->d_cnt_ev, b->d_cnt_start,
-                                                             b->ncnt, b->d_counts, b->stream);
-                if (ke == hipSuccess) ke = launch_strikes(b->d_ev, b->d_counts, b->d_strikes, b->d_stop, nev, c, b->stream);
-                if (cs == VH_OK && ke != hipSuccess) cs = bfail(ke, "stop rule");
-            }
-            if (cs == VH_OK) {
-                hipError_t ae = launch_advance_clocks(b->d_ev, nev, total, b->stream);
-                if (ae != hipSuccess) cs = bfail(ae, "advance clocks");
-            }
-            hipError_t ce = hipStreamEndCapture(b->stream, &g);
-            b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
-            if (cs != VH_OK) return cs;
-            if (ce != hipSuccess) return bfail(ce, "hipStreamEndCapture");
+        // whole checks per graph, at most MAX_GRAPH_STEPS steps (one check if
+        // a check is longer); captured first on an idle stream
+        const int per = std::max(1, MAX_GRAPH_STEPS / coarse);
+        BCHK(hipStreamSynchronize(b->stream));
+        std::vector<hipGraphExec_t> ges;
+        const int iq0 = b->iq_cur, iqn0 = b->iq_next, iu0 = b->iu_cur, iup0 = b->iu_prev;
+        for (int done = 0; done < nchecks && s == VH_OK;) {
+            const int c = std::min(nchecks - done, per);
             hipGraphExec_t ge = nullptr;
-            hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) return bfail(ie, "hipGraphInstantiate");
-            it = b->run_graphs.emplace(key, ge).first;
+            s = run_graph(b, eF, coarse, c, done, &ge);
+            ges.push_back(ge);
+            rotate_roles(b, c * coarse);
+            done += c;
         }
+        b->iq_cur = iq0; b->iq_next = iqn0; b->iu_cur = iu0; b->iu_prev = iup0;
+        if (s != VH_OK) return s;
         BCHK(hipEventRecord(b->ev0, b->stream));
-        BCHK(hipGraphLaunch(it->second, b->stream));
+        for (hipGraphExec_t ge : ges) BCHK(hipGraphLaunch(ge, b->stream));
         BCHK(hipEventRecord(b->ev1, b->stream));
         b->timed = true;
-        for (int k = 0; k < total; ++k) {
-            std::swap(b->iq_cur, b->iq_next);
-            std::swap(b->iu_cur, b->iu_prev);
-        }
+        rotate_roles(b, total);
     }
     const long step_count0 = b->step_count;
     b->step_count += total;
@@ -886,6 +941,8 @@ vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d) {
 vh_status vh_batch_set_face_mode(vh_batch *b, int mode) {
     if (!b || mode < 0 || mode > 2) return VH_ERR_ARG;
     if (b->face_mode != mode) {
+        BCHK(hipSetDevice(b->dev));
+        BCHK(hipStreamSynchronize(b->stream));  // no graph is destroyed while it runs
         b->face_mode = mode;
         drop_graphs(b);  // the mode is baked into captured launches
     }

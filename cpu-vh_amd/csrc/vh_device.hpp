@@ -51,7 +51,7 @@ struct Recip {
     double b, r;
     bool ok;
 };
-__device__ __forceinline__ Recip make_recip(double b) {
+__host__ __device__ __forceinline__ Recip make_recip(double b) {
     Recip R;
     R.b = b;
     R.r = 1.0 / b;
@@ -60,7 +60,7 @@ __device__ __forceinline__ Recip make_recip(double b) {
     return R;
 }
 struct DivFast {
-    __device__ __forceinline__ double operator()(double a, const Recip &R) {
+    __host__ __device__ __forceinline__ double operator()(double a, const Recip &R) {
         const double q0 = a * R.r;
         const double rem = __builtin_fma(-R.b, q0, a);
         double q = __builtin_fma(rem, R.r, q0);

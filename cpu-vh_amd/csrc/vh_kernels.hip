@@ -335,6 +335,52 @@ __global__ __launch_bounds__(64) void advance_clocks_kernel(EvDesc *__restrict__
     if (k < n) ev[k].base += count;
 }
 
+// getInferredVariables (rhic/src/EnergyMomentumTensor.cpp:117-233) on
+// independent states, one thread each, through the same recover() the stage
+// kernel applies to its cell.
+template <bool TABLE>
+__global__ __launch_bounds__(256) void inferred_kernel(const EosDev E, int nc, double t, const double *__restrict__ q,
+                                                       const double *__restrict__ ePrev, long n,
+                                                       double *__restrict__ out, int *__restrict__ nan_out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = q[k * n + i];
+    const double Pi = nc == 15 ? q[14 * n + i] : 0.0;
+    DivFast div;
+    int nan = NAN_NONE;
+    const Prim w = recover<TABLE>(E, t, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], Pi, ePrev[i], nan, div);
+    out[i] = w.e;
+    out[n + i] = w.p;
+    out[2 * n + i] = w.ut;
+    out[3 * n + i] = w.ux;
+    out[4 * n + i] = w.uy;
+    out[5 * n + i] = w.un;
+    nan_out[i] = nan;
+}
+
+hipError_t launch_inferred(const EosDev &E, int nc, double t, const double *q, const double *ePrev, long n,
+                           double *out, int *nan, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (E.kind == 1)
+        hipLaunchKernelGGL(inferred_kernel<true>, dim3(grid), dim3(256), 0, st, E, nc, t, q, ePrev, n, out, nan);
+    else
+        hipLaunchKernelGGL(inferred_kernel<false>, dim3(grid), dim3(256), 0, st, E, nc, t, q, ePrev, n, out, nan);
+    return hipGetLastError();
+}
+
+// The single-event step clock moves on by count steps (last node of a graph).
+__global__ void advance_base_kernel(int *__restrict__ base, int count) {
+    if (threadIdx.x == 0) *base += count;
+}
+
+hipError_t launch_advance_base(int *base, int count, hipStream_t st) {
+    hipLaunchKernelGGL(advance_base_kernel, dim3(1), dim3(64), 0, st, base, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_advance_clocks(EvDesc *ev, int n, int count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(advance_clocks_kernel, dim3((n + 63) / 64), dim3(64), 0, st, ev, n, count);

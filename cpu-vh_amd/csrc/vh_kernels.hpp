@@ -28,7 +28,17 @@ hipError_t launch_stage_batch(const StageParams &P, const EosDev &E, int nc, boo
 hipError_t launch_count_above_batch(const double *e, double eF, const EvDesc *ev, const int *ev_of,
                                     const long *start, int nchunks, unsigned long long *counts, hipStream_t st);
 hipError_t launch_advance_clocks(EvDesc *ev, int n, int count, hipStream_t st);
+hipError_t launch_advance_base(int *base, int count, hipStream_t st);
+
+// Steps per captured HIP graph.  Longer runs replay several graphs back to
+// back (each one's last node moves the step clock on), so no single launch
+// puts more than about a hundred kernel dispatches in flight.
+constexpr int MAX_GRAPH_STEPS = 40;
 hipError_t launch_zero_counts(unsigned long long *counts, int n, hipStream_t st);
+// getInferredVariables on n states (SoA: q[k*n + i], k < nc; out[j*n + i],
+// j = e, p, u^tau, u^x, u^y, u^eta; nan[i] the NaN site, 0 if none)
+hipError_t launch_inferred(const EosDev &E, int nc, double t, const double *q, const double *ePrev, long n,
+                           double *out, int *nan, hipStream_t st);
 hipError_t launch_strikes(EvDesc *ev, const unsigned long long *counts, int *strikes, int *stop_check, int n,
                           int check, hipStream_t st);
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,

@@ -15,6 +15,7 @@
 // stage's input state while the new state is written.
 #pragma once
 #include "vh_device.hpp"
+#include "vh_libm.hpp"
 
 namespace vh {
 
@@ -448,11 +449,11 @@ __device__ __forceinline__ double regulate(const GEO &G, double t, double (&acc)
         const double a6 = piu3 / xi0 / rhomax / spipi;
         const double a12 = __builtin_fmax(a1, a2), a34 = __builtin_fmax(a3, a4), a56 = __builtin_fmax(a5, a6);
         const double rho = __builtin_fmax(a12, __builtin_fmax(a34, a56));
-        facShear = tanh(rho) / rho;
+        facShear = glibc_tanh(rho) / rho;
         if (__builtin_fabs(rho) < 1.e-7) facShear = 1.0;
         if constexpr (NC == 15) {
             const double rhoBulk = __builtin_fabs(Pi) / w.p;
-            facBulk = tanh(rhoBulk) / rhoBulk;
+            facBulk = glibc_tanh(rhoBulk) / rhoBulk;
             if (__builtin_fabs(rhoBulk) < 1.e-7) facBulk = 1.0;
         }
     }

@@ -150,6 +150,19 @@ double vh_eos_temperature(const vh_handle *h, double e);
  * from HIP events on the handle's stream; 0 if none. */
 double vh_last_kernel_ms(const vh_handle *h);
 
+/* getInferredVariables (rhic/include/EnergyMomentumTensor.h:13-16,
+ * rhic/src/EnergyMomentumTensor.cpp:117-233) for n states at time t, on the
+ * device, with the handle's EoS and bulk setting and the arithmetic the stage
+ * kernels apply to every cell: q[k*n + i] is conserved variable k of state i
+ * (k < 14, or 15 with bulk: ttt ttx tty ttn pitt ... pinn [Pi]), ePrev[i] its
+ * previous energy density; out[j*n + i] = e, p, u^tau, u^x, u^y, u^eta
+ * (j = 0..5).  Where the reference prints and exits on a NaN
+ * (EnergyMomentumTensor.cpp:42,197-203) nan_site[i] (optional) records the
+ * site -- 1 e, 2 the root solve's f/fp -- and the call returns VH_ERR_NAN
+ * after writing every state.  Synchronising. */
+vh_status vh_inferred_variables(vh_handle *h, double t, const double *q, const double *ePrev, int64_t n, double *out,
+                                int32_t *nan_site);
+
 /* ---- initial conditions (host side; no device needed) ------------------ */
 
 /* Seeded Monte-Carlo Glauber energy density (the reference's IC type 4,

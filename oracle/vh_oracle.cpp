@@ -196,10 +196,17 @@ struct Orc {
     }
 
     // rhic/src/EnergyMomentumTensor.cpp:22-115
+    // iterations the last table root solve of orc_inferred took (cell -1;
+    // tests classify states by them)
+    int last_nr = -1, last_bi = -1;
+
     double energy_from_conserved(double ePrev, double M0, double M, double Pi, long cell) {
         if (eos.kind == 0) return fabsf(sqrtf(fabsf(4 * M0 * M0 - 3 * M)) - M0);
         double e0 = ePrev;
+        const bool stats = cell < 0;
+        if (stats) last_nr = last_bi = 0;
         for (int j = 0; j < MAX_ITERS_NR; ++j) {
+            if (stats) last_nr = j + 1;
             double pp = eos.pressure(e0);
             double cs2 = eos.cs2(e0);
             double cst2 = pp / e0;
@@ -216,6 +223,7 @@ struct Orc {
         }
         double e_lo = 0.0, e_mid = ePrev, e_hi = 2.0 * (ePrev + 1.0);
         for (int j = 0; j < MAX_ITERS_BI; ++j) {
+            if (stats) last_bi = j + 1;
             double p_lo = eos.pressure(e_lo), p_mid = eos.pressure(e_mid), p_hi = eos.pressure(e_hi);
             double cst2_lo = p_lo / e_lo, cst2_mid = p_mid / e_mid, cst2_hi = p_hi / e_hi;
             double A_lo = M0 * (1.0 - cst2_lo) + Pi;
@@ -752,6 +760,9 @@ ORC_API int orc_set(void *h, int field, const double *in) {
 
 ORC_API double orc_e_reg(void *h) { return static_cast<Orc *>(h)->e_reg; }
 
+// eta/s at temperature T (fm^-1), as the shear relaxation time uses it
+ORC_API double orc_etas(void *h, double T) { return static_cast<Orc *>(h)->etabar_of_T(T); }
+
 // checkForCellsAboveTc: freezeout/FreezeOut.cpp:364-381
 ORC_API long orc_count_above(const double *e, long n, double eF) {
     long c = 0;
@@ -807,8 +818,23 @@ ORC_API double orc_eos_energy_density(void *h, double T) { return static_cast<Eo
 // One-cell primitive recovery through a throwaway 1-cell lattice.
 ORC_API int orc_inferred(void *h, double t, const double *qc, double ePrev, double *out6) {
     Orc *o = static_cast<Orc *>(h);
-    o->inferred(t, qc, ePrev, &out6[0], &out6[1], &out6[2], &out6[3], &out6[4], &out6[5], 0);
+    o->last_nr = o->last_bi = -1;
+    o->inferred(t, qc, ePrev, &out6[0], &out6[1], &out6[2], &out6[3], &out6[4], &out6[5], -1);
     return o->err;
+}
+
+// Newton and bisection iterations of the last orc_inferred call's root solve
+// (-1: no table root solve ran)
+ORC_API void orc_last_root(void *h, int *nr, int *bi) {
+    const Orc *o = static_cast<Orc *>(h);
+    *nr = o->last_nr;
+    *bi = o->last_bi;
+}
+
+ORC_API void orc_clear_error(void *h) {
+    Orc *o = static_cast<Orc *>(h);
+    o->err = 0;
+    o->err_cell = -1;
 }
 
 // Fourth root as the conformal temperature computes it: glibc powf(x, 0.25f).

@@ -221,7 +221,7 @@ def _run_case(tmp_path, lat, hyd, gpu, ic_fields, bulk, table, ic_type=11):
 
 def test_driver_2d_stops_on_third_cold_check(tmp_path):
     lat = dict(nx=21, ny=17, nz=1, nt=60, dx=0.2, dt=0.05, coarse=2, snap=10)
-    hyd = dict(t0=0.6, etas=0.2, TF=0.172, Treg=0.0)
+    hyd = dict(t0=0.6, etas=0.2, TF=0.172, Treg=0.1)
     ic = V.random_ic(21, 17, 1, 0.2, 0.6, seed=31)
     steps, log = _run_case(tmp_path, lat, hyd, {"snapshotEvery": 10}, ic, False, None)
     assert steps < lat["nt"] + 1 and "All cells have dropped below freezeout energy density" in log
@@ -229,7 +229,7 @@ def test_driver_2d_stops_on_third_cold_check(tmp_path):
 
 def test_driver_3d_table_bulk_runs_nt_plus_one_steps(tmp_path):
     lat = dict(nx=11, ny=9, nz=7, nt=12, dx=0.2, dt=0.02, coarse=3, snap=5)
-    hyd = dict(t0=0.6, etas=0.12, TF=0.150, Treg=0.0)
+    hyd = dict(t0=0.6, etas=0.12, TF=0.150, Treg=0.15)
     table = V.eos_table()
     ic = V.random_ic(11, 9, 7, 0.2, 0.6, seed=32, lattice=True, bulk=True)
     eos_file = os.path.join(os.path.dirname(os.path.dirname(C.LIB_PATH)), "EoS", "hrg_hotqcd_eos_binary.dat")
@@ -255,9 +255,8 @@ def test_driver_israel_stewart_gubser_file(tmp_path):
     rhic/rhic-trunk/src/test/resources/gubser/viscous/gubserIC.dat under the
     working directory (the reference does not ship the file; it is made here
     from the semi-analytic Israel-Stewart Gubser solution at tau = 1,
-    eta/s = 0.2), no regulation (the regulator's tanh may differ from the
-    C library's by an ulp, which a %.8f digit can show); same files as the
-    oracle loop."""
+    eta/s = 0.2), with regulation below 100 MeV; same files as the oracle
+    loop."""
     import gubser as Gb
     from test_gubser import write_gubser_ic
     d = tmp_path / "rhic" / "rhic-trunk" / "src" / "test" / "resources" / "gubser" / "viscous"
@@ -265,7 +264,7 @@ def test_driver_israel_stewart_gubser_file(tmp_path):
     write_gubser_ic(str(d / "gubserIC.dat"), Gb.solve(0.2), 21, 0.2)
     ic = C.read_gubser(str(d / "gubserIC.dat"), 21, 21, 1)
     lat = dict(nx=21, ny=21, nz=1, nt=20, dx=0.2, dt=0.02, coarse=4, snap=10)
-    hyd = dict(t0=1.0, etas=0.2, TF=0.15, Treg=0.0)
+    hyd = dict(t0=1.0, etas=0.2, TF=0.15, Treg=0.1)
     steps, log = _run_case(tmp_path, lat, hyd, {"snapshotEvery": 10}, ic, False, None, ic_type=1)
     assert steps == lat["nt"] + 1 and "Gubser" in log
 

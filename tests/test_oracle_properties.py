@@ -189,3 +189,87 @@ def test_bjorken_flow_against_ode():
     sol2 = solve_ivp(lambda tau, y: [rhs(tau, y)[0], rhs(tau, y)[1] + t_pp / 3 * y[1] / tau], (t0, tau),
                      [e0, phi0], rtol=1e-11, atol=1e-12)
     assert abs(sol2.y[1, -1] - phi_got) > 50 * abs(phi_want - phi_got)
+
+
+ETAS_MIN, ETAS_TK, SLOPE_LO, SLOPE_HI = 0.08, 0.154, 0.5, 1.0
+
+
+def _etas_oracle():
+    return V.Oracle(3, 3, 3, 0.1, 0.1, 0.1, 0.005, 0.6, ETAS_MIN, etas_Tk_GeV=ETAS_TK, etas_slope_lo=SLOPE_LO,
+                    etas_slope_hi=SLOPE_HI)
+
+
+def test_eta_over_s_of_T_minimum_in_GeV():
+    """eta/s(T) (the extension the drop-in configuration names: minimum 0.08
+    at T = 154 MeV) scanned over T = 50...400 MeV with T handed over in
+    fm^-1, as the source terms do: the minimum sits at 154 MeV to the scan's
+    resolution and equals 0.08, and the two sides rise with the configured
+    slopes per GeV.  A fm^-1/GeV slip would put the minimum at 30 MeV or
+    30 GeV."""
+    o = _etas_oracle()
+    TGeV = np.linspace(0.05, 0.4, 3501)  # 0.1 MeV steps
+    vals = np.array([o.etas(t / V.HBARC) for t in TGeV])
+    k = int(np.argmin(vals))
+    assert abs(TGeV[k] - ETAS_TK) <= 1e-4
+    assert vals[k] == pytest.approx(ETAS_MIN, abs=1e-5)
+    assert o.etas(ETAS_TK / V.HBARC) == pytest.approx(ETAS_MIN, abs=1e-12)
+    lo, hi = TGeV < ETAS_TK - 1e-3, TGeV > ETAS_TK + 1e-3
+    assert np.polyfit(TGeV[lo], vals[lo], 1)[0] == pytest.approx(-SLOPE_LO, rel=1e-9)
+    assert np.polyfit(TGeV[hi], vals[hi], 1)[0] == pytest.approx(SLOPE_HI, rel=1e-9)
+    # zero slopes: the reference's constant, bit for bit
+    c = V.Oracle(3, 3, 3, 0.1, 0.1, 0.1, 0.005, 0.6, 0.2)
+    assert all(c.etas(t / V.HBARC) == 0.2 for t in (0.05, 0.154, 0.3))
+
+
+def _bjorken_ode(etas_of_TGeV, e0, phi0, t0, t1):
+    from scipy.integrate import solve_ivp
+    d_pp, t_pp = 1.33333, 1.42857
+
+    def rhs(tau, y):
+        e, phi = y
+        p = e / 3
+        T = (e / 13.8997) ** 0.25  # fm^-1
+        return [-(e + p - phi) / tau,
+                4 * (e + p) / 5 / (3 * tau) - phi * T / (5 * etas_of_TGeV(T * 0.197326938))
+                - (d_pp + t_pp / 3) * phi / tau]
+
+    return solve_ivp(rhs, (t0, t1), [e0, phi0], rtol=1e-11, atol=1e-12, dense_output=True)
+
+
+def test_bjorken_flow_with_eta_over_s_of_T():
+    """0+1D Bjorken flow (as test_bjorken_flow_against_ode) with eta/s(T):
+    the fluid cools from 240 to 135 MeV, through the minimum at 154 MeV, and
+    the lattice must follow an ODE solved independently with
+    eta/s(T) = 0.08 + 0.5 (0.154 - T) below and 0.08 + 1.0 (T - 0.154) above
+    154 MeV (T in GeV).  The same ODE with the minimum placed at 0.154 fm^-1
+    (a unit slip) or with constant eta/s = 0.08 misses the lattice by far
+    more than the scheme's error."""
+    t0, dt, nsteps = 0.6, 0.005, 600
+    e0, phi0 = 30.0, 1.0
+
+    def etas_T(TGeV):
+        return ETAS_MIN + (SLOPE_HI * (TGeV - ETAS_TK) if TGeV > ETAS_TK else SLOPE_LO * (ETAS_TK - TGeV))
+
+    tau = t0 + nsteps * dt
+    sol = _bjorken_ode(etas_T, e0, phi0, t0, tau)
+    T_end = (sol.sol(tau)[0] / 13.8997) ** 0.25 * V.HBARC
+    assert (e0 / 13.8997) ** 0.25 * V.HBARC > 0.2 and T_end < 0.14  # the run crosses the minimum
+    N = 27
+    o = _etas_oracle()
+    pi = np.zeros((10, N))
+    pi[4] = pi[7] = phi0 / 2
+    pi[9] = -phi0 / t0 ** 2
+    o.set_ic({"e": np.full(N, e0), "p": np.full(N, e0 / 3), "ut": np.ones(N), "ux": np.zeros(N),
+              "uy": np.zeros(N), "un": np.zeros(N), "pi": pi.ravel()})
+    o.init_conserved(t0)
+    for n in range(1, nsteps + 1):
+        assert o.step(t0 + (n - 1) * dt) == 0
+    e_want, phi_want = sol.sol(tau)
+    e_got, phi_got = o.get("e")[0], -tau ** 2 * o.get("pinn")[0]
+    assert e_got == pytest.approx(e_want, rel=2e-5)
+    assert phi_got == pytest.approx(phi_want, rel=5e-4)
+    err = abs(phi_got - phi_want)
+    slip = _bjorken_ode(lambda TGeV: etas_T(TGeV / 0.197326938), e0, phi0, t0, tau).sol(tau)[1]
+    const = _bjorken_ode(lambda TGeV: ETAS_MIN, e0, phi0, t0, tau).sol(tau)[1]
+    assert abs(slip - phi_got) > 20 * err
+    assert abs(const - phi_got) > 20 * err

@@ -53,6 +53,8 @@ def oracle_lib():
         L.orc_set.argtypes = [VP, I, VP]
         L.orc_e_reg.argtypes = [VP]
         L.orc_e_reg.restype = D
+        L.orc_etas.argtypes = [VP, D]
+        L.orc_etas.restype = D
         L.orc_count_above.argtypes = [VP, ctypes.c_long, D]
         L.orc_count_above.restype = ctypes.c_long
         L.orc_fo_flag.argtypes = [VP, VP, I, I, I, D, VP, ctypes.c_long]
@@ -116,6 +118,10 @@ class Oracle:
 
     def e_reg(self):
         return oracle_lib().orc_e_reg(self.h)
+
+    def etas(self, T):
+        """eta/s at T in fm^-1."""
+        return oracle_lib().orc_etas(self.h, T)
 
 
 def oracle_fo_flag(eprev, ecur, nx, ny, nz, eF):
@@ -222,3 +228,86 @@ def max_rel(a, b, floor=1e-300):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(a), floor))) if a.size else 0.0
+
+
+# ------------------------------------------------ recovery edge states --
+# Conserved states that drive each branch of getInferredVariables
+# (rhic/src/EnergyMomentumTensor.cpp:22-233).  Labels:
+#   closed    ePrev <= 0.1: e = M0 - M/M0
+#   newton    ePrev > 0.1: the Newton iteration converges (table EoS)
+#   stuck     Newton fails after 10^4 steps and the bisection bracket has no
+#             sign change: the solver returns ePrev (table EoS)
+#   floor     e < 1e-7: e = p = 1e-7
+#   ut_reset  u^tau > 1e3: u := (1, 0, 0, 0)
+#   nan_e     e is NaN (closed form of a NaN state)
+#   nan_root  the Newton step f/fp is NaN
+# The stuck parameters (M0, M/M0^2, Pi, ePrev) come from a seeded search of
+# random states through the oracle's solver.
+STUCK_STATES = [(0.01546516889287413, 1.5556463828627634, -0.0046590459518938985, 5.308885459042157),
+                (0.05382590063955815, 0.6764068221343584, 0.0, 0.1853999997838796),
+                (0.07133211806725949, 0.6990414320888148, 0.0, 9.449399126342959)]
+
+
+def recovery_states(nc, t=1.0, n_random=64, seed=5, table=True):
+    rng = np.random.default_rng(seed)
+    cols, eprev, labels = [], [], []
+    L = oracle_lib()
+    tab = eos_table() if table else None
+    eos = L.orc_eos_create(None if tab is None else tab.ctypes.data)
+
+    def add(q, ep, label):
+        v = np.zeros(15)
+        v[:len(q)] = q
+        cols.append(v[:nc] if nc == 15 else v[:14])
+        eprev.append(ep)
+        labels.append(label)
+
+    for k in range(n_random):  # physical states, velocity up to 0.9 c, shear and bulk
+        e = 10 ** rng.uniform(-1.5, 1.5)
+        pp = L.orc_eos_pressure(eos, e)
+        ux, uy, un = rng.uniform(-2, 2, 3) * [1, 1, 1 / t]
+        ut = np.sqrt(1 + ux * ux + uy * uy + t * t * un * un)
+        pi = rng.uniform(-0.05, 0.05, 10) * e
+        Pi = rng.uniform(-0.1, 0.1) * e if nc == 15 else 0.0
+        w = e + pp + Pi
+        q = np.zeros(15)
+        q[0] = w * ut * ut - (pp + Pi) + pi[0]
+        q[1] = w * ut * ux + pi[1]
+        q[2] = w * ut * uy + pi[2]
+        q[3] = w * ut * un + pi[3]
+        q[4:14] = pi
+        q[14] = Pi
+        if k % 4 == 3:
+            add(q, rng.uniform(0.01, 0.1), "closed")
+        else:
+            add(q, e * rng.uniform(0.8, 1.25) + 0.11, "newton" if table else "closed_form")
+    if table:
+        for M0, ratio, Pi, ep in STUCK_STATES:
+            if Pi != 0.0 and nc != 15:
+                continue  # the state is stuck only with its bulk pressure
+            q = np.zeros(15)
+            q[0], q[1], q[14] = M0, np.sqrt(ratio) * M0, Pi
+            add(q, ep, "stuck")
+    q = np.zeros(15)
+    q[0] = 1e-9
+    add(q, 0.05, "floor")
+    q = np.zeros(15)
+    q[0], q[1] = 1.0, np.sqrt(1.0 - 1e-7)
+    add(q, 0.05, "ut_reset")
+    q = np.zeros(15)
+    q[0], q[2] = 2.0, 2.0 * np.sqrt(1.0 - 4e-8)
+    add(q, 0.08, "ut_reset")
+    q = np.zeros(15)
+    q[0], q[1] = np.nan, 0.5
+    add(q, 0.05, "nan_e")
+    if table:
+        q = np.zeros(15)
+        q[0], q[1] = np.nan, 0.5
+        add(q, 5.0, "nan_root")
+    L.orc_eos_destroy(eos)
+    return np.array(cols).T.copy(), np.array(eprev), labels
+
+
+def same_or_both_nan(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
