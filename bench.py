@@ -8,12 +8,16 @@ Latin-hypercube per-event parameters ((eta/s)_min in [0.04, 0.2], zeta/s peak
 in [0, 0.1], tau0 in [0.3, 1.0] fm/c), all resident in one batch and stepped
 together until each is below T_F = 150 MeV (the third coarse check, every 5
 steps, with no cell above e_F -- the reference never resets that count;
-finished events leave the batch).
+finished events leave the batch).  A step of --steps/--warmup is one pass of
+that workload: admission of every event (upload and conserved variables),
+then the run to the last event's freezeout.
 Beside it: the plain FP64 drop-in looping over events one at a time (a
-sample of the same events), the same batch in FP64, the 3+1D single event at
-real size (131x131x61) through the plain FP64 path, the mixed form's error
-against FP64 on sample events, the dominant kernel's roofline and the
-reference CPU build timed on this host.
+sample of the same events, creation to release inside the timed region), the
+same batch in FP64, the mixed form's error against FP64 on every event beside
+the FP64 path's own sensitivity, the serving sweep, the 201x201x101
+precision form, the 3+1D single event at real size (131x131x61, 600 steps)
+through the plain FP64 path, the dominant kernel's roofline and the reference
+CPU build timed on this host's cores.
 """
 import argparse
 import json
@@ -63,27 +67,45 @@ def make_ic(w, table):
     return ic, info
 
 
-def cpu_baseline(w, ic, threads, min_seconds=12.0, max_steps=40):
-    variant = ("lat" if w["lattice"] else "conf") + ("_bulk" if w["bulk"] else "")
-    lib = os.path.join(REPO, "oracle", "_ref", f"libcpuvh_ref_{variant}.so")
-    if not os.path.exists(lib):
-        return None
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cases, seconds=10.0, max_steps=400):
+    """The reference build (oracle/_ref, -O3 -ffp-contract=off) on this
+    host: one single-threaded process per case, all running at once, each
+    timing up to max_steps RK2 steps of its event for about `seconds` after
+    one untimed step.  cases: [(workload dict, initial state)].  Returns the
+    summed zone-updates/s and the per-process results."""
+    procs = []
     with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "ic.npz")
-        np.savez(path, **ic)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), path, variant, str(w["nx"]),
-               str(w["ny"]), str(w["nz"]), str(w["dx"]), str(w["dt"]), str(w["t0"]), str(w["etabar"]),
-               str(w["T_reg_GeV"]), str(min_seconds), str(max_steps)]
-        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
-        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
-        if r.returncode != 0 or not lines:
-            return {"error": (r.stderr or r.stdout)[-300:]}
-        res = json.loads(lines[-1])
-    return {"value": res["zone_upd_per_s"], "unit": "zone-upd/s", "cores": threads, "kind": "reference",
-            "sample": f"{res['steps']} RK2 steps of the same event after one untimed step, "
-                      f"{res['seconds']:.1f} s wall, OpenMP reference build (-O3 -ffp-contract=off), "
-                      f"constant eta/s={w['etabar']} (the reference has no eta/s(T))"}
+        for k, (w, ic) in enumerate(cases):
+            variant = ("lat" if w["lattice"] else "conf") + ("_bulk" if w["bulk"] else "")
+            lib = os.path.join(REPO, "oracle", "_ref", f"libcpuvh_ref_{variant}.so")
+            if not os.path.exists(lib):
+                return None
+            path = os.path.join(td, f"ic{k}.npz")
+            np.savez(path, **ic)
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), path, variant, str(w["nx"]),
+                   str(w["ny"]), str(w["nz"]), str(w["dx"]), str(w["dt"]), str(w["t0"]), str(w["etabar"]),
+                   str(w["T_reg_GeV"]), str(seconds), str(max_steps)]
+            procs.append(subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+        results = []
+        for pr in procs:
+            out, err = pr.communicate(timeout=600)
+            lines = [l for l in out.splitlines() if l.startswith("{")]
+            if pr.returncode != 0 or not lines:
+                return {"error": (err or out)[-300:]}
+            results.append(json.loads(lines[-1]))
+    return {"zone_upd_per_s": sum(r["zone_upd_per_s"] for r in results), "processes": results}
 
 
 def load_traffic(name, key="hbm_bytes_per_launch"):
@@ -136,41 +158,25 @@ def make_batch_events(table, n_events):
 
 def add_event(b, ev):
     d = BATCH["dx"]
+    const = ev.get("const_etas", False)
     return b.add(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], ev["ic"],
                  T_reg_GeV=BATCH["T_reg_GeV"], etas_Tk_GeV=BATCH["etas_Tk_GeV"],
-                 etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
+                 etas_slope_lo=0.0 if const else BATCH["etas_slope_lo"],
+                 etas_slope_hi=0.0 if const else BATCH["etas_slope_hi"])
 
 
 CHECKS_PER_SYNC = 8
 
 
-def run_batched(events, table, precision, warmup, max_batched_steps=None):
-    """All events resident in one batch, stepped in chunks of the coarse
-    freezeout interval until each meets the stop rule; returns timings."""
+def one_pass(b, events, coarse, max_batched_steps=None):
+    """One pass of the batching workload through batch b: every event
+    admitted (upload of its initial state, conserved variables formed on the
+    device), all stepped together until each meets the stop rule, finished
+    events removed.  Returns the pass's timings."""
     import cpuvh_amd as C
-    cap = sum(ev["N"] for ev in events)
-    kw = dict(max_events=len(events), max_steps=BATCH["max_steps"], bulk=True, eos_table=table,
-              precision=precision)
-    coarse = BATCH["coarse"]
-    tiles = (-1, -1)
-    if warmup > 0:  # throwaway batch: kernels, graphs and caches warm
-        wb = C.Batch(cap, **kw)
-        for ev in events:
-            add_event(wb, ev)
-        if precision == C.PREC_MIXED and FACE_MODE is not None:
-            wb.set_face_mode(FACE_MODE)
-        tiles = wb.autotune(reps=5)
-        wb.steps(warmup)
-        wb.sync()
-        wb.close()
-    b = C.Batch(cap, **kw)
+    t_start = time.perf_counter()
     ids = [add_event(b, ev) for ev in events]
-    if precision == C.PREC_MIXED:
-        if TILES_OVERRIDE is not None:
-            tiles = TILES_OVERRIDE
-        b.set_tiles(max(tiles[0], 0), max(tiles[1], 0))
-        if FACE_MODE is not None:
-            b.set_face_mode(FACE_MODE)
+    t_admitted = time.perf_counter()
     eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
     cells = {i: ev["N"] for i, ev in zip(ids, events)}
     running = set(ids)
@@ -179,9 +185,8 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
     zone_updates = 0
     graph_ms = 0.0
     batched_steps = 0
-    t_start = time.perf_counter()
     failed = []
-    # the stop rule runs on the device: one graph holds CHECKS_PER_SYNC rounds
+    # the stop rule runs on the device: one call holds CHECKS_PER_SYNC rounds
     # of (coarse steps, count, strike), and the host only removes the events
     # that stopped
     while running and (max_batched_steps is None or batched_steps < max_batched_steps):
@@ -205,7 +210,7 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
             site, bad, cell = b.nan_info()
             b.clear_error()
             if bad in running:
-                failed.append(bad)
+                failed.append(ids.index(bad))
                 b.remove(bad)
                 running.discard(bad)
         for i in list(running):
@@ -213,25 +218,74 @@ def run_batched(events, table, precision, warmup, max_batched_steps=None):
                 steps_done[i] = done_steps[i]
                 b.remove(i)
                 running.discard(i)
+    for i in list(running):  # max_batched_steps reached
+        steps_done[i] = done_steps[i]
+        b.remove(i)
     wall = time.perf_counter() - t_start
+    return dict(wall_s=wall, admission_s=t_admitted - t_start, events=len(events) - len(failed),
+                events_failed=len(failed), failed=sorted(failed), batched_steps=batched_steps,
+                zone_updates=zone_updates, graph_ms=graph_ms,
+                steps_to_freezeout=[None if k in failed else steps_done.get(i) for k, i in enumerate(ids)])
+
+
+def run_batched(events, table, precision, passes, warmup, max_batched_steps=None, const_etas=False):
+    """The batching workload: `warmup` untimed passes, then `passes` timed
+    passes, each from admission of all events to the last one's freezeout,
+    through one batch object.  const_etas: eta/s(T) slopes zero (each event's
+    (eta/s)_min as a constant, the reference's transport)."""
+    import cpuvh_amd as C
+    if const_etas:
+        events = [dict(ev, const_etas=True) for ev in events]
+    cap = sum(ev["N"] for ev in events)
+    kw = dict(max_events=len(events), max_steps=BATCH["max_steps"], bulk=True, eos_table=table,
+              precision=precision)
+    tiles = (-1, -1)
+    if precision == C.PREC_MIXED:
+        if TILES_OVERRIDE is not None:
+            tiles = TILES_OVERRIDE
+        else:  # the tile autotuner on a throwaway batch holding the workload
+            wb = C.Batch(cap, **kw)
+            for ev in events:
+                add_event(wb, ev)
+            tiles = wb.autotune(reps=5)
+            wb.close()
+    b = C.Batch(cap, **kw)
+    if precision == C.PREC_MIXED:
+        b.set_tiles(max(tiles[0], 0), max(tiles[1], 0))
+        if FACE_MODE is not None:
+            b.set_face_mode(FACE_MODE)
+    for _ in range(warmup):
+        one_pass(b, events, BATCH["coarse"], max_batched_steps)
+    runs = [one_pass(b, events, BATCH["coarse"], max_batched_steps) for _ in range(passes)]
     b.close()
-    done = len(events) - len(failed)
-    return dict(wall_s=wall, events=done, events_failed=len(failed), events_per_s=done / wall,
-                batched_steps=batched_steps, zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall,
-                graph_ms=graph_ms, steps_to_freezeout=[steps_done[i] for i in ids if i in steps_done],
-                failed_ids=sorted(failed), tiles=tiles)
+    wall = sum(r["wall_s"] for r in runs)
+    adm = sum(r["admission_s"] for r in runs)
+    done = sum(r["events"] for r in runs)
+    steps = sum(r["batched_steps"] for r in runs)
+    zu = sum(r["zone_updates"] for r in runs)
+    return dict(passes=passes, warmup_passes=warmup, wall_s=wall, admission_s=adm, events=done,
+                events_failed=sum(r["events_failed"] for r in runs), failed_ids=runs[-1]["failed"],
+                events_per_s=done / wall, events_per_s_excl_admission=done / (wall - adm),
+                batched_steps=steps, batched_steps_per_pass=runs[-1]["batched_steps"],
+                ms_per_batched_step=(wall - adm) / steps * 1e3, zone_updates=zu, zone_upd_per_s=zu / wall,
+                graph_ms=sum(r["graph_ms"] for r in runs), steps_to_freezeout=runs[-1]["steps_to_freezeout"],
+                pass_wall_s=[r["wall_s"] for r in runs], tiles=tiles)
 
 
 def run_plain_loop(events, table):
-    """The plain FP64 drop-in, one event at a time, same stop rule."""
+    """The plain FP64 drop-in, one event at a time, same stop rule; the
+    timed region holds each event's handle creation, upload, conserved
+    variables, steps and release (setup_s: all but the steps and checks)."""
     import cpuvh_amd as C
     coarse = BATCH["coarse"]
     d = BATCH["dx"]
     zone_updates = 0
     steps = []
     failed = 0
+    setup = 0.0
     t_start = time.perf_counter()
     for ev in events:
+        t0 = time.perf_counter()
         h = C.Hydro(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"],
                     bulk=True, eos_table=table, etas_Tk_GeV=BATCH["etas_Tk_GeV"],
                     etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
@@ -239,6 +293,7 @@ def run_plain_loop(events, table):
         h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic["Pi"])
         h.init_conserved(ev["t0"])
         eF = h.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+        setup += time.perf_counter() - t0
         n, strikes = 1, 0
         try:
             while strikes < 3 and n + coarse - 1 <= BATCH["max_steps"]:
@@ -251,30 +306,38 @@ def run_plain_loop(events, table):
                 raise
             failed += 1
         zone_updates += ev["N"] * (n - 1)
+        t1 = time.perf_counter()
         h.close()
+        setup += time.perf_counter() - t1
     wall = time.perf_counter() - t_start
     done = len(events) - failed
-    return dict(wall_s=wall, events=done, events_failed=failed, events_per_s=done / wall,
-                zone_updates=zone_updates, zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
+    return dict(wall_s=wall, setup_s=setup, events=done, events_failed=failed, events_per_s=done / wall,
+                events_per_s_excl_setup=done / (wall - setup), zone_updates=zone_updates,
+                zone_upd_per_s=zone_updates / wall, steps_to_freezeout=steps)
 
 
-def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
-    """Mixed form against FP64 on the same events, at checkpoints and after
-    nsteps steps; beside it the FP64 path's own sensitivity: FP64 started from
-    the initial state rounded to FP32 (the mixed form's stored state at t0)
-    against FP64 from the exact one.  The reference's root solver stops at a
-    relative change of 1e-3 and switches formula at e = 0.1, so differences
-    at the rounding level grow by orders of magnitude over hundreds of steps
-    in any arithmetic that is not bit-identical."""
+ACC_FIELDS = ("e", "ut", "ux", "uy")
+ENVELOPE_K = 2.0  # the attainable accuracy bar: mixed error <= K x the FP64 perturbed-start envelope
+
+
+def accuracy_all(events, table, checkpoints=(50, 100, 200, 300)):
+    """The mixed form against FP64 on every event of the workload, at fixed
+    steps, beside the FP64 path's own sensitivity: FP64 from the initial state
+    rounded to FP32 (the mixed form's stored state at t0) and from two states
+    with e scaled by 1 + 1e-7 u (u uniform in [-1, 1]).  The envelope of an
+    event's field is the largest error of those three runs against FP64 from
+    the exact state.  The reference's root solver stops at a relative change
+    of 1e-3 and switches formula at e = 0.1, so differences at the rounding
+    level grow by orders of magnitude over hundreds of steps in any arithmetic
+    that is not bit-identical to it.  All five batches stay resident and step
+    together; the freezeout cubes are flagged at each checkpoint (previous
+    slice: the last checkpoint's) and compared per event."""
     import cpuvh_amd as C
-    fields = ("e", "ut", "ux", "uy", "pixx", "pixy", "Pi")
     cap = sum(ev["N"] for ev in events)
-    stops = [c for c in checkpoints if c < nsteps] + [nsteps]
-    # FP64 from perturbed starts: the initial state rounded to FP32, and e
-    # scaled by 1 + 1e-7 u (u uniform in [-1, 1], two seeds)
-    variants = {"fp64": (C.PREC_FP64, None), "mixed": (C.PREC_MIXED, None), "fp64_f32_ic": (C.PREC_FP64, "f32"),
-                "fp64_pert1": (C.PREC_FP64, 1), "fp64_pert2": (C.PREC_FP64, 2)}
-    snaps, cubes = {}, {}
+    nsteps = max(checkpoints)
+    variants = {"fp64": (C.PREC_FP64, None), "mixed": (C.PREC_MIXED, None), "f32_ic": (C.PREC_FP64, "f32"),
+                "pert1": (C.PREC_FP64, 1), "pert2": (C.PREC_FP64, 2)}
+    batches = {}
     for name, (prec, pert) in variants.items():
         b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
         if prec == C.PREC_MIXED and FACE_MODE is not None:
@@ -288,44 +351,78 @@ def precision_check(events, table, nsteps, checkpoints=(50, 100, 200)):
                 e = ev["ic"]["e"] * (1 + 1e-7 * r.uniform(-1, 1, ev["ic"]["e"].size))
                 ev = dict(ev, ic=dict(ev["ic"], e=e, p=table_pressure(table, e)))
             ids.append(add_event(b, ev))
-        done = 0
-        for stop in stops:
+        batches[name] = (b, ids)
+    eF = batches["fp64"][0].eos_energy_density(BATCH["T_F_GeV"] / HBARC)
+    envs = ("f32_ic", "pert1", "pert2")
+    report, done = [], 0
+    dropped = {}  # event index -> (variant, step): a NaN in one run drops the event from the comparison
+    for stop in checkpoints:
+        for name, (b, ids) in batches.items():
             b.steps(stop - done)
-            b.sync()
-            done = stop
-            snaps[(name, stop)] = [{f: b.get(i, f) for f in fields} for i in ids]
-            eF = b.eos_energy_density(BATCH["T_F_GeV"] / HBARC)
-            cubes[(name, stop)] = [set(b.fo_flag_ids(i, eF).tolist()) for i in ids]
+            try:
+                b.sync()
+            except C.VhError as ex:
+                if ex.status != C.VH_ERR_NAN:
+                    raise
+                _, bad, _ = b.nan_info()
+                b.clear_error()
+                b.remove(bad)
+                dropped.setdefault(ids.index(bad), (name, stop))
+        done = stop
+        per_event = []
+        for k in range(len(events)):
+            if k in dropped:
+                continue
+            got = {name: {f: b.get(ids[k], f) for f in ACC_FIELDS} for name, (b, ids) in batches.items()}
+            cubes = {name: set(b.fo_flag_ids(ids[k], eF).tolist()) for name, (b, ids) in batches.items()}
+            ref = got["fp64"]
+            row = {}
+            for name in ("mixed",) + envs:
+                err = {}
+                for f in ACC_FIELDS:
+                    a, c = ref[f], got[name][f]
+                    err[f] = float(np.max(np.abs(a - c))) / (float(np.max(np.abs(a))) or 1.0)
+                    if f in ("e", "ut"):
+                        err[f + "_pt"] = float(np.max(np.abs(a - c) / np.abs(a)))
+                err["cubes"] = len(cubes["fp64"] ^ cubes[name])
+                row[name] = err
+            row["cubes_fp64"] = len(cubes["fp64"])
+            per_event.append(row)
+        keys = [f for f in ACC_FIELDS] + ["e_pt", "ut_pt"]
+        mixed_max = {f: max(r["mixed"][f] for r in per_event) for f in keys}
+        env = [{f: max(r[v][f] for v in envs) for f in keys + ["cubes"]} for r in per_event]
+        env_max = {f: max(x[f] for x in env) for f in keys}
+        within = {f: sum(r["mixed"][f] <= ENVELOPE_K * x[f] for r, x in zip(per_event, env)) for f in keys}
+        cubes_mixed = [r["mixed"]["cubes"] for r in per_event]
+        cubes_env = [x["cubes"] for x in env]
+        report.append({
+            "step": stop,
+            "mixed_vs_fp64_max_over_events": mixed_max,
+            "fp64_envelope_max_over_events": env_max,
+            "events_within_k_envelope": within,
+            "fo_cubes_fp64": sum(r["cubes_fp64"] for r in per_event),
+            "fo_cubes_differing_mixed": sum(cubes_mixed),
+            "fo_cubes_differing_fp64_envelope": sum(cubes_env),
+            "events_with_differing_cubes_mixed": sum(c > 0 for c in cubes_mixed),
+            "events_with_differing_cubes_fp64_envelope": sum(c > 0 for c in cubes_env),
+            "events_mixed_cubes_beyond_envelope": sum(m > x for m, x in zip(cubes_mixed, cubes_env)),
+            "fo_cubes_differing_mixed_per_event": cubes_mixed,
+        })
+    for b, _ in batches.values():
         b.close()
-
-    def errors(ref, got):
-        res = {}
-        for f in fields:
-            worst_pt, worst_norm = 0.0, 0.0
-            for r, g in zip(ref, got):
-                a, c = r[f], g[f]
-                scale = float(np.max(np.abs(a))) or 1.0
-                worst_norm = max(worst_norm, float(np.max(np.abs(a - c))) / scale)
-                if f in ("e", "ut"):
-                    worst_pt = max(worst_pt, float(np.max(np.abs(a - c) / np.abs(a))))
-            res[f] = {"max_abs_over_max": worst_norm}
-            if f in ("e", "ut"):
-                res[f]["max_pointwise_rel"] = worst_pt
-        return res
-    report = []
-    for stop in stops:
-        def cube_diff(other):
-            return sum(len(a ^ c) for a, c in zip(cubes[("fp64", stop)], cubes[(other, stop)]))
-        report.append({"step": stop, "mixed_vs_fp64": errors(snaps[("fp64", stop)], snaps[("mixed", stop)]),
-                       "fp64_sensitivity": errors(snaps[("fp64", stop)], snaps[("fp64_f32_ic", stop)]),
-                       "fo_cubes_fp64": sum(len(a) for a in cubes[("fp64", stop)]),
-                       "fo_cubes_differing_mixed": cube_diff("mixed"),
-                       "fo_cubes_differing_fp64_f32_ic": cube_diff("fp64_f32_ic"),
-                       "fp64_perturbed_e": [errors(snaps[("fp64", stop)], snaps[(v, stop)])["e"]
-                                            for v in ("fp64_pert1", "fp64_pert2")],
-                       "fo_cubes_differing_fp64_perturbed": [cube_diff(v) for v in ("fp64_pert1", "fp64_pert2")]})
-    return {"steps": nsteps, "events": len(events), "checkpoints": report,
-            "fields": report[-1]["mixed_vs_fp64"]}
+    n = len(events) - len(dropped)
+    pass_metric = all(r["mixed_vs_fp64_max_over_events"][f] <= 1e-3 for r in report for f in ("e_pt", "ut_pt", "ux", "uy"))
+    pass_metric = pass_metric and all(r["fo_cubes_differing_mixed"] == 0 for r in report)
+    pass_env = all(r["events_within_k_envelope"][f] == n for r in report for f in ("e", "e_pt", "ut_pt", "ux", "uy"))
+    pass_env = pass_env and all(r["events_mixed_cubes_beyond_envelope"] == 0 for r in report)
+    return {"events": n, "events_dropped_nan": {str(k): v for k, v in dropped.items()}, "checkpoints": list(checkpoints),
+            "bar_metric": "max relative error <= 1e-3 in e and u^mu (e, u^tau pointwise; u^x, u^y against the "
+                          "field's largest magnitude) and identical freezeout cube sets, every event, every checkpoint",
+            "pass_metric_bar": bool(pass_metric),
+            "bar_attainable": f"per event and field: mixed error <= {ENVELOPE_K:g} x the largest error of FP64 "
+                              "from three perturbed starts (FP32-rounded, two 1e-7 e perturbations), and no more "
+                              "differing freezeout cubes than the largest of those runs",
+            "pass_attainable_bar": bool(pass_env), "report": report}
 
 
 PRECISION_3D = dict(nx=201, ny=201, nz=101, dx=0.08, dt=0.02, t0=0.6, steps=600, etabar=0.08, etas_Tk_GeV=0.154,
@@ -394,22 +491,24 @@ SERVE = dict(frac_3d=0.2, side2_min=101, side2_max=201, side3_min=101, side3_max
              bulk=1, max_steps=3000, max_resident=256, collect_surface=1)
 
 
-def run_serving(table, rates, n_requests, share_3d):
+def run_serving(table, rates, n_requests, share_3d, budget):
     """Open-loop serving at each arrival rate (vh_serve_run)."""
     import cpuvh_amd as C
     out = []
-    for k, rate in enumerate(rates):
-        stats, reqs = C.serve(table, rate_per_s=float(rate), n_requests=n_requests, seed=1000 + k,
-                              precision=C.PREC_MIXED, **dict(SERVE, frac_3d=share_3d))
+    for k, (rate, nreq) in enumerate(zip(rates, n_requests)):
+        stats, reqs = C.serve(table, rate_per_s=float(rate), n_requests=nreq, seed=1000 + k,
+                              precision=C.PREC_MIXED, **dict(SERVE, frac_3d=share_3d, hbm_budget_bytes=budget))
         done = [r for r in reqs if r["status"] == C.VH_OK]
-        out.append({"rate_per_s": rate, "requests": n_requests, "completed": stats["completed"],
+        out.append({"rate_per_s": rate, "requests": nreq, "completed": stats["completed"],
                     "failed": stats["failed"], "events_per_s": stats["events_per_s"],
                     "zone_upd_per_s": stats["zone_upd_per_s"], "p50_s": stats["p50_s"], "p99_s": stats["p99_s"],
                     "mean_s": stats["mean_s"], "hbm_occupancy_mean": stats["occupancy_mean"],
                     "hbm_occupancy_peak": stats["occupancy_peak"],
                     "mean_wait_for_admission_s": float(np.mean([r["admit_s"] - r["arrival_s"] for r in done]))
                     if done else None,
-                    "share_3d": float(np.mean([r["nz"] > 1 for r in reqs]))})
+                    "share_3d": float(np.mean([r["nz"] > 1 for r in reqs])),
+                    "failed_requests": [{"nx": r["nx"], "nz": r["nz"], "glauber_seed": r["glauber_seed"],
+                                         "status": r["status"]} for r in reqs if r["status"] != C.VH_OK]})
     return out
 
 
@@ -453,28 +552,29 @@ def run_dropin_3d(w, table, steps, warmup):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600, help="timed steps of the 3+1D FP64 drop-in line")
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed passes of the batching workload (each: admission of every event, all stepped "
+                         "to freezeout)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before them")
     ap.add_argument("--events", type=int, default=BATCH["n_events"])
     ap.add_argument("--plain-sample", type=int, default=32, help="events of the plain FP64 loop")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="concurrent single-thread reference processes")
     ap.add_argument("--no-fp64-batch", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--skip-precision-3d", action="store_true")
     ap.add_argument("--max-batch-steps", type=int, default=None,
                     help="stop the batched runs after this many steps (profiling; events/s then not to freezeout)")
-    ap.add_argument("--serve-rates", default="4,16,64", help="arrival rates of the serving sweep (events/s)")
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rates", default="1,10,50,200", help="arrival rates of the serving sweep (events/s)")
+    ap.add_argument("--serve-requests", default="6,16,32,48", help="requests at each rate")
     ap.add_argument("--skip-serve", action="store_true")
-    ap.add_argument("--serve-3d-share", type=float, default=0.0,
-                    help="share of 3+1D requests in the serving sweep (default 0: 2+1D requests only)")
+    ap.add_argument("--serve-3d-share", type=float, default=0.2, help="share of 3+1D requests in the serving sweep")
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0, help="HBM budget of the serving loop")
     ap.add_argument("--tiles", default=None,
                     help="mixed-kernel tile variants 'T2,T3' for the timed batch (counter runs without the autotuner)")
     ap.add_argument("--face-mode", type=int, default=None, help="face semantics of the mixed kernel (0, 1, 2)")
-    ap.add_argument("--precision-3d", action="store_true",
-                    help="also run the 201x201x101 precision-form comparison (off by default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -494,6 +594,9 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: one GPU only", file=sys.stderr)
         return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be at least 1 and --warmup at least 0", file=sys.stderr)
+        return 2
 
     import cpuvh_amd as C
     global TILES_OVERRIDE, FACE_MODE
@@ -502,37 +605,40 @@ def main():
     if args.tiles:
         TILES_OVERRIDE = tuple(int(x) for x in args.tiles.split(","))
     table = C.load_eos_table(os.path.join(REPO, "EoS", "hrg_hotqcd_eos_binary.dat"))
-    print("bench: building initial states", file=sys.stderr, flush=True)
+    log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+    log("building initial states")
     events = make_batch_events(table, args.events)
-    print("bench: batched mixed", file=sys.stderr, flush=True)
-    mixed = run_batched(events, table, C.PREC_MIXED, args.warmup, args.max_batch_steps)
+    log(f"batched mixed: {args.warmup} untimed + {args.steps} timed passes")
+    mixed = run_batched(events, table, C.PREC_MIXED, args.steps, args.warmup, args.max_batch_steps)
     fp64b = None
     if not args.no_fp64_batch:
-        print("bench: batched fp64", file=sys.stderr, flush=True)
-        fp64b = run_batched(events, table, C.PREC_FP64, args.warmup, args.max_batch_steps)
+        log("batched fp64")
+        fp64b = run_batched(events, table, C.PREC_FP64, 1, 0, args.max_batch_steps)
     stride = max(1, len(events) // max(1, args.plain_sample))
     sample = events[::stride][:args.plain_sample]
     plain = None
     if not args.skip_plain:
-        print("bench: plain fp64 loop", file=sys.stderr, flush=True)
+        log("plain fp64 loop")
         plain = run_plain_loop(sample, table)
-    prec = None
-    if not args.skip_precision:
-        print("bench: precision check", file=sys.stderr, flush=True)
-        prec = precision_check(events[:: max(1, len(events) // 4)][:4], table, 300)
+    acc = None
+    if not args.skip_accuracy:
+        log("accuracy over every event")
+        acc = accuracy_all(events, table)
     serving = None
     if not args.skip_serve:
-        print("bench: serving sweep", file=sys.stderr, flush=True)
-        serving = run_serving(table, [float(r) for r in args.serve_rates.split(",") if r],
-                              args.serve_requests, args.serve_3d_share)
+        log("serving sweep")
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        nreq = [int(x) for x in args.serve_requests.split(",") if x]
+        nreq = (nreq + nreq[-1:] * len(rates))[:len(rates)]
+        serving = run_serving(table, rates, nreq, args.serve_3d_share, int(args.serve_budget_gib * (1 << 30)))
     precision3d = None
-    if args.precision_3d:
-        print("bench: precision form 3+1D", file=sys.stderr, flush=True)
+    if not args.skip_precision_3d:
+        log("precision form 3+1D 201x201x101")
         precision3d = run_precision_3d(table, PRECISION_3D["steps"])
     dropin = None
     if not args.skip_dropin:
-        print("bench: 3+1D drop-in", file=sys.stderr, flush=True)
-        _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.steps, args.warmup)
+        log("3+1D drop-in 131x131x61, 600 steps")
+        _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, WORKLOADS["glauber3d"]["steps_per_event"], 20)
 
     # dominant kernel: the mixed stage kernel over the batch (two launches per
     # step).  Algorithmic bytes of one zone-update (two stages): each stage
@@ -548,44 +654,64 @@ def main():
     launches = 2 * mixed["batched_steps"]
     per_cell = load_traffic("stage_mixed_kernel_batch_pmc.json", "hbm_bytes_per_cell_stage")
     traffic_mx = per_cell * mixed["zone_updates"] / mixed["batched_steps"] if per_cell else None
-    cpu = None
+    cpu, gpu_const = None, None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print("bench: cpu reference", file=sys.stderr, flush=True)
-        med = sorted(events, key=lambda ev: ev["N"])[len(events) // 2]
-        wcpu = dict(nx=med["n"], ny=med["n"], nz=1, dx=BATCH["dx"], dt=BATCH["dt"], t0=med["t0"], lattice=True,
-                    bulk=True, etabar=med["etabar"], T_reg_GeV=BATCH["T_reg_GeV"])
-        cpu = cpu_baseline(wcpu, med["ic"], threads, min_seconds=12.0, max_steps=400)
-        if cpu and "value" in cpu:
-            mean_steps = float(np.mean(mixed["steps_to_freezeout"] or [BATCH["max_steps"]]))
-            mean_cells = float(np.mean([ev["N"] for ev in events]))
-            cpu["events_per_s"] = cpu["value"] / (mean_steps * mean_cells)
-            cpu["sample"] += (f"; median-size event of the batch ({med['n']}^2 cells); events/s = zone-upd/s / "
-                              f"(mean cells {mean_cells:.0f} x mean steps to freezeout {mean_steps:.0f})")
+        nproc = args.cpu_procs or min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu reference: {nproc} single-thread processes")
+        by_size = sorted(events, key=lambda ev: ev["N"])
+        picks = [by_size[int((k + 0.5) * len(by_size) / nproc)] for k in range(nproc)]
+        cases = [(dict(nx=ev["n"], ny=ev["n"], nz=1, dx=BATCH["dx"], dt=BATCH["dt"], t0=ev["t0"], lattice=True,
+                       bulk=True, etabar=ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"]), ev["ic"]) for ev in picks]
+        res = cpu_baseline(cases, seconds=10.0, max_steps=400)
+        log("batched mixed with constant eta/s (the CPU comparison's transport)")
+        gpu_const = run_batched(events, table, C.PREC_MIXED, 1, 0, args.max_batch_steps, const_etas=True)
+        if res and "zone_upd_per_s" in res:
+            work = float(np.mean([ev["N"] * s for ev, s in zip(events, gpu_const["steps_to_freezeout"])
+                                  if s is not None]))
+            cpu = {"value": res["zone_upd_per_s"], "unit": "zone-upd/s", "cores": nproc, "kind": "reference",
+                   "cpu_model": cpu_model(), "logical_cpus_available": len(os.sched_getaffinity(0)),
+                   "events_per_s": res["zone_upd_per_s"] / work,
+                   "sample": f"{nproc} single-thread processes of the reference build (oracle/_ref, -O3 "
+                             "-ffp-contract=off) at once, one event of the workload each (spread over its sizes), "
+                             "constant eta/s = the event's (eta/s)_min (the reference has no eta/s(T)), ~10 s of "
+                             "steps each after one untimed step; events/s = summed zone-upd/s / the workload's mean "
+                             f"zone-updates per event to freezeout ({work:.3g}, from the GPU run with the same "
+                             "transport)",
+                   "per_process": [{"cells": c[0]["nx"] ** 2, "steps": r["steps"], "zone_upd_per_s": r["zone_upd_per_s"]}
+                                   for c, r in zip(cases, res["processes"])]}
+        elif res:
+            cpu = res
+    ratio = None if plain is None else mixed["events_per_s"] / plain["events_per_s"]
     out = {
         "metric": METRIC, "value": mixed["events_per_s"], "unit": "events/s", "n_gpus": 1,
-        "steps": mixed["batched_steps"], "warmup": args.warmup,
-        "ms_per_step": mixed["wall_s"] / mixed["batched_steps"] * 1e3, "higher_is_better": True,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": mixed["wall_s"] / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 storage/flux/source + f64 root solve/EoS/accumulation",
         "data": "synthetic (seeded MC-Glauber initial states)",
         "config": {"workload": f"{len(events)} 2+1D MC-Glauber Au+Au events, grids {BATCH['n_min']}^2..{BATCH['n_max']}^2, "
                                "dx=0.1, dtau=0.02, LHS (eta/s)_min [0.04,0.2], zeta/s peak [0,0.1], tau0 [0.3,1.0], "
-                               "HotQCD+HRG table EoS, bulk on, to freezeout at T=150 MeV (coarse check every 5 steps, "
-                               "3 strikes), one batch, mixed precision",
+                               "HotQCD+HRG table EoS, bulk on, eta/s(T) min at 154 MeV, to freezeout at T=150 MeV "
+                               "(coarse check every 5 steps, 3 strikes), one batch, mixed precision",
+                   "step": "one pass of the workload: every event admitted (initial state uploaded, conserved "
+                           "variables formed), all advanced together to freezeout, finished events removed",
                    "cells_total": int(sum(ev["N"] for ev in events)), "form": "batched mixed precision",
                    "tiles": mixed["tiles"]},
+        "accuracy": None if acc is None else {k: acc[k] for k in ("bar_metric", "pass_metric_bar", "bar_attainable",
+                                                                   "pass_attainable_bar")},
         "batched_mixed": {k: v for k, v in mixed.items() if k != "steps_to_freezeout"},
         "plain_fp64_loop": None if plain is None else {k: v for k, v in plain.items() if k != "steps_to_freezeout"},
-        "speedup_vs_plain_fp64": None if plain is None else mixed["events_per_s"] / plain["events_per_s"],
+        "speedup_vs_plain_fp64": ratio,
+        "speedup_vs_plain_fp64_excl_setup": None if plain is None else
+        mixed["events_per_s_excl_admission"] / plain["events_per_s_excl_setup"],
         "batched_fp64": None if fp64b is None else {k: v for k, v in fp64b.items() if k != "steps_to_freezeout"},
-        "mixed_vs_fp64_error": prec,
+        "mixed_vs_fp64_error": acc,
         "precision_form_3d": precision3d,
         "serving": None if serving is None else {
             "workload": f"Poisson arrivals of seeded MC-Glauber Au+Au events, {100 * (1 - args.serve_3d_share):.0f}% "
                         f"2+1D (101^2-201^2), {100 * args.serve_3d_share:.0f}% 3+1D (101x101x41 - 161x161x81), "
-                        "dx=0.1, table EoS, bulk, mixed precision, admitted into one batch by an 8 GiB HBM budget, "
-                        "resident until no cell is above T=150 MeV, freezeout cubes flagged per request at every "
-                        "check", "sweep": serving},
+                        f"dx=0.1, table EoS, bulk, mixed precision, admitted into one batch by a "
+                        f"{args.serve_budget_gib:g} GiB HBM budget, resident until no cell is above T=150 MeV, "
+                        "freezeout cubes flagged per request at every check", "sweep": serving},
         "fp64_dropin_3d": dropin,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_mx,
@@ -596,9 +722,11 @@ def main():
                      "avg_launch_ms": mixed["graph_ms"] / launches,
                      "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},
         "cpu_baseline": cpu,
+        "batched_mixed_const_etas": None if gpu_const is None else
+        {k: v for k, v in gpu_const.items() if k != "steps_to_freezeout"},
     }
-    if cpu and "events_per_s" in cpu:
-        out["speedup_vs_cpu_reference"] = mixed["events_per_s"] / cpu["events_per_s"]
+    if cpu and "events_per_s" in cpu and gpu_const:
+        out["speedup_vs_cpu_reference"] = gpu_const["events_per_s"] / cpu["events_per_s"]
     print(json.dumps(out))
     return 0
 

@@ -72,14 +72,14 @@ class ServeConfig(ctypes.Structure):
                 ("coarse", ctypes.c_int), ("e0", ctypes.c_double), ("e_floor", ctypes.c_double),
                 ("hbm_budget_bytes", ctypes.c_int64), ("precision", ctypes.c_int), ("bulk", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("max_resident", ctypes.c_int), ("collect_surface", ctypes.c_int),
-                ("device", ctypes.c_int)]
+                ("device", ctypes.c_int), ("digest", ctypes.c_int)]
 
 
 class ServeRequest(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
                 ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("steps", ctypes.c_int),
                 ("status", ctypes.c_int), ("glauber_seed", ctypes.c_uint64), ("fo_cubes", ctypes.c_int64),
-                ("e_sum", ctypes.c_double)]
+                ("e_sum", ctypes.c_double), ("digest", ctypes.c_uint64)]
 
 
 class ServeStats(ctypes.Structure):
@@ -463,7 +463,7 @@ SERVE_DEFAULTS = dict(rate_per_s=10.0, n_requests=50, seed=1, frac_3d=0.2, side2
                       side3_min=101, side3_max=161, nz3_min=41, nz3_max=81, dx=0.1, dt=0.02, t0=0.6, etabar=0.08,
                       etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0, T_reg_GeV=0.1, T_F_GeV=0.150,
                       coarse=5, e0=10.0, e_floor=0.01, hbm_budget_bytes=64 << 30, precision=PREC_MIXED, bulk=1,
-                      max_steps=3000, max_resident=256, collect_surface=1, device=0)
+                      max_steps=3000, max_resident=256, collect_surface=1, device=0, digest=0)
 
 
 def serve(eos_table=None, **kw):
@@ -487,6 +487,19 @@ def serve(eos_table=None, **kw):
 
 def build_info():
     return _lib().vh_build_info().decode()
+
+
+def state_digest(fields):
+    """The serving loop's digest (vh_serve_request.digest) of a state given as
+    the arrays of fields 0..nc-1, 15..20 in that order."""
+    M = (1 << 64) - 1
+    h = 14695981039346656037
+    for a in fields:
+        bits = np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+        w = 2 * np.arange(bits.size, dtype=np.uint64) + np.uint64(1)
+        d = int(np.sum(bits * w, dtype=np.uint64))
+        h = (h * 1099511628211 + d) & M
+    return h
 
 
 def mc_glauber(nx, ny, nz, dx, dy, dz, seed, A=197, R=6.38, a=0.535, sigma_nn_fm2=4.2, b=-1.0, b_max=10.0,

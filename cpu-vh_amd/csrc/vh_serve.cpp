@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <deque>
 #include <thread>
 #include <vector>
@@ -31,6 +32,26 @@ struct Rng {  // SplitMix64
 struct Payload {
     std::vector<double> e, p;
 };
+
+// Hash of an event's state bits (the vh_serve_request digest): fields in
+// vh_download's numbering, each weighted by position so that order counts.
+uint64_t state_digest(vh_batch *b, int id, int nc, int64_t cells, std::vector<double> &buf) {
+    uint64_t h = 14695981039346656037ull;
+    std::vector<int> fields;
+    for (int f = 0; f < nc; ++f) fields.push_back(f);
+    for (int f = 15; f <= 20; ++f) fields.push_back(f);
+    for (int f : fields) {
+        if (vh_batch_download(b, id, f, buf.data()) != VH_OK) return 0;
+        uint64_t d = 0;
+        for (int64_t i = 0; i < cells; ++i) {
+            uint64_t bits;
+            std::memcpy(&bits, &buf[i], sizeof bits);
+            d += bits * (2 * (uint64_t)i + 1);
+        }
+        h = h * 1099511628211ull + d;
+    }
+    return h;
+}
 
 double quantile(std::vector<double> v, double q) {  // nearest rank
     if (v.empty()) return 0.0;
@@ -123,6 +144,7 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             double sum = 0;
             for (int64_t m = 0; m < cells; ++m) sum += ebuf[m];
             r.e_sum = sum;
+            if (c.digest) r.digest = state_digest(b, id, c.bulk ? 15 : 14, cells, ebuf);
         }
         vh_batch_remove(b, id);
         r.done_s = now_s();

@@ -140,6 +140,7 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
  * for a cube id out of range.  Synchronising. */
 vh_status vh_fo_interp(vh_handle *h, const int64_t *cubes, const double *frac, int64_t n, double *vars);
 
+
 /* Host-side EoS of the handle (EquationOfState.cpp:44-290): used by drivers for
  * e_F = e(T_F) and by tests. */
 double vh_eos_energy_density(const vh_handle *h, double T);
@@ -397,6 +398,7 @@ typedef struct {
     int max_resident;          /* event slots */
     int collect_surface;       /* 1: flag freezeout cubes of every resident request at every check */
     int device;
+    int digest;                /* 1: a completed request's final state is downloaded and hashed (digest) */
 } vh_serve_config;
 
 typedef struct {
@@ -407,6 +409,10 @@ typedef struct {
     uint64_t glauber_seed;
     int64_t fo_cubes;                   /* freezeout cubes flagged, summed over checks */
     double e_sum;                       /* sum of e over the final state (identity checks) */
+    uint64_t digest;                    /* with cfg.digest: hash of the final state's bits, 0 otherwise:
+                                         * h = 14695981039346656037; for each field f of vh_download's
+                                         * numbering 0..nc-1, 15, 16, 17..20: h = h * 1099511628211 +
+                                         * sum_i bits(x_i) * (2i + 1), all mod 2^64 */
 } vh_serve_request;
 
 typedef struct {

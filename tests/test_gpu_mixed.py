@@ -180,3 +180,43 @@ def test_boost_invariant_kernel(lattice, bulk):
                 assert err[f] < 1e-3, (name, f, err[f])
             for f in ("pitt", "pitx", "pity", "pixx", "pixy", "piyy", "pinn"):
                 assert err[f] < 1e-2, (name, f, err[f])
+
+
+def test_mixed_3d_at_precision_form_size():
+    """The 3+1D mixed kernel on the precision form's lattice (201 x 201 x 101
+    = 4,080,501 cells, dx = deta = 0.08, seeded MC-Glauber, table EoS, bulk,
+    eta/s(T)) for 10 steps against the FP64 path on the same event: e and u^tau
+    within 1e-3 pointwise, u^x, u^y, u^eta within 1e-3 of their largest
+    magnitude, and the same freezeout cube set."""
+    import cpuvh_amd as C
+    nx = ny = 201
+    nz, dx, dt, t0, nsteps = 101, 0.08, 0.02, 0.6, 10
+    print(f"case mixed 3d {nx}x{ny}x{nz} steps={nsteps}", flush=True)
+    table = V.eos_table()
+    e, _ = C.mc_glauber(nx, ny, nz, dx, dx, dx, seed=4242, e0=10.0)
+    N = e.size
+    h = C.Hydro(2, 2, 1, 0.1, 0.1, 0.1, 0.02, 0.6, 0.2, eos_table=table)
+    eF = h.eos_energy_density(0.15 / V.HBARC)
+    h.close()
+    import sys
+    sys.path.insert(0, V.REPO)
+    import bench as B
+    ic = {"e": e, "p": B.table_pressure(table, e), "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N),
+          "un": np.zeros(N), "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
+    out = {}
+    for prec in (0, PREC_MIXED):
+        b = Batch(N, max_events=1, max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
+        eid = b.add(nx, ny, nz, dx, dx, dx, dt, t0, 0.08, ic, T_reg_GeV=0.1, etas_slope_lo=0.5, etas_slope_hi=1.0)
+        b.steps(nsteps)
+        b.sync()
+        out[prec] = ({f: b.get(eid, f) for f in PRIM}, b.fo_flag_ids(eid, eF))
+        b.close()
+    ref, got = out[0][0], out[PREC_MIXED][0]
+    err = {f: float(np.max(np.abs(ref[f] - got[f]))) / (float(np.max(np.abs(ref[f]))) or 1.0) for f in PRIM}
+    for f in ("e", "ut"):
+        err[f + "_pointwise"] = float(np.max(np.abs(ref[f] - got[f]) / np.abs(ref[f])))
+    print({k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    assert err["e_pointwise"] < 1e-3 and err["ut_pointwise"] < 1e-3, err
+    for f in ("ux", "uy", "un"):
+        assert err[f] < 1e-3, (f, err[f])
+    assert np.array_equal(out[0][1], out[PREC_MIXED][1])

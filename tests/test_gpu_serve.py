@@ -65,3 +65,64 @@ def test_request_larger_than_budget_is_refused():
     cfg = dict(CFG, n_requests=4, frac_3d=1.0)
     stats, reqs = C.serve(None, hbm_budget_bytes=10 * 10 * 10 * BYTES_PER_CELL_MIXED, **cfg)
     assert stats["failed"] == 4 and all(r["status"] == C.VH_ERR_CAPACITY for r in reqs)
+
+
+C5 = dict(rate_per_s=1000.0, n_requests=6, seed=3, frac_3d=0.3, side2_min=101, side2_max=141, side3_min=101,
+          side3_max=105, nz3_min=41, nz3_max=43, dx=0.1, dt=0.02, t0=0.6, etabar=0.08, T_reg_GeV=0.1,
+          T_F_GeV=0.150, coarse=5, e0=10.0, e_floor=0.01, precision=C.PREC_MIXED, bulk=1, max_steps=3000,
+          max_resident=8, collect_surface=1, digest=1)
+
+
+def _alone_state(r, cfg, table):
+    """One request alone in its own batch with the loop's stop rule; the
+    digest of its final state (fields 0..14, 15..20), or the failure."""
+    import vhtest as V
+    nx, ny, nz, dx = r["nx"], r["ny"], r["nz"], cfg["dx"]
+    e, _ = C.mc_glauber(nx, ny, nz, dx, dx, dx, seed=r["glauber_seed"], e0=cfg["e0"], e_floor=cfg["e_floor"])
+    N = e.size
+    h = C.Hydro(2, 2, 1, 0.1, 0.1, 0.1, 0.02, 0.6, 0.2, eos_table=table)  # the host EoS the loop uses
+    p = np.array([h.eos_pressure(x) for x in e])
+    h.close()
+    ic = {"e": e, "p": p, "ut": np.ones(N), "ux": np.zeros(N), "uy": np.zeros(N), "un": np.zeros(N),
+          "pi": np.zeros(10 * N), "Pi": np.zeros(N)}
+    b = C.Batch(N, 1, cfg["max_steps"], bulk=True, eos_table=table, precision=cfg["precision"])
+    eid = b.add(nx, ny, nz, dx, dx, dx, cfg["dt"], cfg["t0"], cfg["etabar"], ic, T_reg_GeV=cfg["T_reg_GeV"],
+                etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0)
+    eF = b.eos_energy_density(cfg["T_F_GeV"] / 0.197326938)
+    strikes, steps = 0, 0
+    while True:
+        b.steps(cfg["coarse"])
+        steps += cfg["coarse"]
+        try:
+            counts = b.counts_above(eF)
+        except C.VhError as ex:
+            assert ex.status == C.VH_ERR_NAN
+            b.close()
+            return C.VH_ERR_NAN, steps, 0
+        strikes += counts[eid] == 0
+        if strikes >= 3 or steps + cfg["coarse"] > cfg["max_steps"]:
+            break
+    fields = [b.get(eid, f) for f in list(range(15)) + list(range(15, 21))]
+    b.close()
+    return C.VH_OK, steps, C.state_digest(fields)
+
+
+def test_served_at_workload_size_equal_alone():
+    """The serving workload's sizes (2+1D 101^2-141^2 and two 3+1D requests of
+    104x104x41 and 101x101x42, table EoS, bulk, mixed form, all arriving
+    within a few ms into a budget of 1.3 of the largest request): every
+    request ends with the same status, step count and bits of its whole final
+    state (e, p, u^mu and the 15 conserved variables) as when it runs alone;
+    no request may fail."""
+    import vhtest as V
+    table = V.eos_table()
+    big = 104 * 104 * 41
+    stats, reqs = C.serve(table, hbm_budget_bytes=int(1.3 * big) * BYTES_PER_CELL_MIXED, **C5)
+    print(stats, flush=True)
+    assert sum(r["nz"] > 1 for r in reqs) == 2 and max(r["nx"] * r["ny"] * r["nz"] for r in reqs) == big
+    assert stats["completed"] == C5["n_requests"] and stats["failed"] == 0
+    assert any(r["admit_s"] > r["arrival_s"] + 1e-3 for r in reqs)  # some waited for memory
+    for r in reqs:
+        print(f"request {r['nx']}x{r['ny']}x{r['nz']}: {r['steps']} steps", flush=True)
+        status, steps, digest = _alone_state(r, C5, table)
+        assert (r["status"], r["steps"], r["digest"]) == (status, steps, digest), (r, status, steps, digest)
