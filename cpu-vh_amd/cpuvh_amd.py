@@ -26,7 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag", "vh_fo_interp",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
-           "vh_inferred_variables",
+           "vh_inferred_variables", "vh_fo_flag_noalloc",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
@@ -136,6 +136,7 @@ def _lib():
         L.vh_last_kernel_ms.argtypes = [P]
         L.vh_last_kernel_ms.restype = D
         L.vh_inferred_variables.argtypes = [P, D, P, P, ctypes.c_int64, P, P]
+        L.vh_fo_flag_noalloc.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, P, ctypes.c_int64, P, P]
         L.vh_status_string.argtypes = [ctypes.c_int]
         L.vh_status_string.restype = ctypes.c_char_p
         L.vh_build_info.restype = ctypes.c_char_p

@@ -140,6 +140,23 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
  * for a cube id out of range.  Synchronising. */
 vh_status vh_fo_interp(vh_handle *h, const int64_t *cubes, const double *frac, int64_t n, double *vars);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): vh_fo_flag's
+ * crossing test and ordered compaction on caller-owned DEVICE memory,
+ * allocating nothing.  eprev, ecur: the previous and current coarse slices,
+ * nx*ny*nz doubles each (x fastest); ids: at most cap cube ids, numbered and
+ * ordered as vh_fo_flag's (ids beyond cap are not written); *n_flagged (a
+ * device int64) receives the full count.  Enqueued on `stream` (a
+ * hipStream_t, NULL for the null stream) of the current device, with no
+ * synchronisation, so the call can be captured into a graph; the results are
+ * there once the stream reaches them.  The kernels' per-block counts live in
+ * one fixed device array of the library per GPU (512 KiB, kept until the
+ * process ends): at most VH_FO_NOALLOC_MAX_CUBES cubes ((nx-1)(ny-1)(nz-1),
+ * nz-1 read as 1 in 2+1D), else VH_ERR_ARG; calls that overlap on one GPU race
+ * for that array. */
+#define VH_FO_NOALLOC_MAX_CUBES 16777216L
+vh_status vh_fo_flag_noalloc(const double *eprev, const double *ecur, int nx, int ny, int nz, double eF,
+                             int64_t *ids, int64_t cap, int64_t *n_flagged, void *stream);
 
 /* Host-side EoS of the handle (EquationOfState.cpp:44-290): used by drivers for
  * e_F = e(T_F) and by tests. */
