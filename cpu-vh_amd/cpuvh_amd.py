@@ -190,6 +190,23 @@ def load_eos_table(path):
     return np.ascontiguousarray(raw)
 
 
+def _ic_arrays(arrs, N):
+    """e, p, ut, ux, uy, un, pi (10 N), Pi as contiguous float64 arrays of the
+    lattice's size; None stays None (e and p are required)."""
+    out = []
+    for k, a in enumerate(arrs):
+        if a is None:
+            if k < 2:
+                raise ValueError("e and p are required")
+            out.append(None)
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != (10 * N if k == 6 else N):
+            raise ValueError("initial-condition arrays do not match the lattice")
+        out.append(a)
+    return out
+
+
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
@@ -230,12 +247,9 @@ class Hydro:
             pass
 
     def upload_ic(self, e, p, ut, ux, uy, un, pi10, Pi=None):
-        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (e, p, ut, ux, uy, un)]
-        pi = np.ascontiguousarray(pi10, dtype=np.float64).ravel()
-        if any(a.size != self.N for a in arrs) or pi.size != 10 * self.N:
-            raise ValueError("initial-condition arrays do not match the lattice")
-        Pa = None if Pi is None else np.ascontiguousarray(Pi, dtype=np.float64).ravel()
-        self._check(_lib().vh_upload_ic(self.h, *[_ptr(a) for a in arrs], _ptr(pi), _ptr(Pa)), "vh_upload_ic")
+        """x-fastest arrays; any of ut .. Pi may be None (u^tau 1, the others 0)."""
+        arrs = _ic_arrays([e, p, ut, ux, uy, un, pi10, Pi], self.N)
+        self._check(_lib().vh_upload_ic(self.h, *[_ptr(a) for a in arrs]), "vh_upload_ic")
 
     def init_conserved(self, t):
         self._check(_lib().vh_init_conserved(self.h, t), "vh_init_conserved")
@@ -363,17 +377,11 @@ class Batch:
         """Admits one event; returns its id, or None when the batch is full
         (VH_ERR_CAPACITY).  ic: dict of x-fastest arrays e, p, ut, ux, uy, un, pi (10*N), Pi (optional)."""
         N = nx * ny * nz
-        arrs = [np.ascontiguousarray(ic[k], dtype=np.float64).ravel() for k in ("e", "p", "ut", "ux", "uy", "un")]
-        pi = np.ascontiguousarray(ic["pi"], dtype=np.float64).ravel()
-        if any(a.size != N for a in arrs) or pi.size != 10 * N:
-            raise ValueError("initial-condition arrays do not match the lattice")
-        Pi = ic.get("Pi")
-        Pa = None if Pi is None else np.ascontiguousarray(Pi, dtype=np.float64).ravel()
+        arrs = _ic_arrays([ic.get(k) for k in ("e", "p", "ut", "ux", "uy", "un", "pi", "Pi")], N)
         prm = EventParams(nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV, etas_Tk_GeV, etas_slope_lo,
                           etas_slope_hi)
         eid = ctypes.c_int(-1)
-        st = _lib().vh_batch_add(self.b, ctypes.byref(prm), *[_ptr(a) for a in arrs], _ptr(pi), _ptr(Pa),
-                                 ctypes.byref(eid))
+        st = _lib().vh_batch_add(self.b, ctypes.byref(prm), *[_ptr(a) for a in arrs], ctypes.byref(eid))
         if st == VH_ERR_CAPACITY:
             return None
         Hydro._check(st, "vh_batch_add")

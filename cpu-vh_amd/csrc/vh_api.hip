@@ -286,22 +286,32 @@ vh_status vh_destroy(vh_handle *h) {
 
 vh_status vh_upload_ic(vh_handle *h, const double *e, const double *p, const double *ut, const double *ux,
                        const double *uy, const double *un, const double *pi10, const double *Pi) {
-    if (!h || !e || !p || !ut || !ux || !uy || !un || !pi10) return VH_ERR_ARG;
+    if (!h || !e || !p) return VH_ERR_ARG;
     HIPCHK(hipSetDevice(h->dev));
     const size_t b = sizeof(double) * h->N;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(h->e[0], e, b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->p[0], p, b, hipMemcpyHostToDevice));
+    // a NULL array is zero (u^tau: one)
     const double *uu[4] = {ut, ux, uy, un};
     for (int c = 0; c < 4; ++c) {
-        HIPCHK(hipMemcpy(h->u[h->iu_cur][c], uu[c], b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->u[h->iu_prev][c], uu[c], b, hipMemcpyHostToDevice));  // up = u
+        if (uu[c]) {
+            HIPCHK(hipMemcpy(h->u[h->iu_cur][c], uu[c], b, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->u[h->iu_prev][c], uu[c], b, hipMemcpyHostToDevice));  // up = u
+        } else {
+            HIPCHK(launch_fill(h->u[h->iu_cur][c], h->N, c == 0 ? 1.0 : 0.0, h->stream));
+            HIPCHK(launch_fill(h->u[h->iu_prev][c], h->N, c == 0 ? 1.0 : 0.0, h->stream));
+        }
     }
-    for (int n = 0; n < 10; ++n) HIPCHK(hipMemcpy(h->q[h->iq_cur][4 + n], pi10 + (size_t)n * h->N, b, hipMemcpyHostToDevice));
+    for (int n = 0; n < 10; ++n) {
+        if (pi10) HIPCHK(hipMemcpy(h->q[h->iq_cur][4 + n], pi10 + (size_t)n * h->N, b, hipMemcpyHostToDevice));
+        else HIPCHK(launch_fill(h->q[h->iq_cur][4 + n], h->N, 0.0, h->stream));
+    }
     if (h->nc == 15) {
         if (Pi) HIPCHK(hipMemcpy(h->q[h->iq_cur][14], Pi, b, hipMemcpyHostToDevice));
-        else HIPCHK(hipMemset(h->q[h->iq_cur][14], 0, b));
+        else HIPCHK(launch_fill(h->q[h->iq_cur][14], h->N, 0.0, h->stream));
     }
+    HIPCHK(hipStreamSynchronize(h->stream));
     h->conserved_ready = false;
     return VH_OK;
 }

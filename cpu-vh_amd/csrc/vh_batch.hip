@@ -89,6 +89,9 @@ struct vh_batch {
     // mixed form: FP32 fields, tile variants of the 2+1D and 3+1D launches
     bool mixed = false;
     float *fpool = nullptr;
+    // device staging of a mixed batch's FP64 initial states (grown on demand)
+    double *stage_buf = nullptr;
+    size_t stage_bytes = 0;
     float *qf[3][15] = {}, *qfl[3][4] = {}, *uf[3][4] = {}, *ef[2] = {}, *pf[2] = {};
     int tile2 = 0, tile3 = 0;
     int face_mode = 1;  // the reference's face semantics (vh_batch_set_face_mode)
@@ -416,6 +419,7 @@ vh_status vh_batch_destroy(vh_batch *b) {
     }
     (void)hipFree(b->pool);
     (void)hipFree(b->fpool);
+    (void)hipFree(b->stage_buf);
     (void)hipFree(b->d_cnt_ev);
     (void)hipFree(b->d_cnt_start);
     (void)hipFree(b->d_err);
@@ -437,7 +441,7 @@ vh_status vh_batch_destroy(vh_batch *b) {
 vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e, const double *p, const double *ut,
                        const double *ux, const double *uy, const double *un, const double *pi10, const double *Pi,
                        int *id_out) {
-    if (!b || !prm || !e || !p || !ut || !ux || !uy || !un || !pi10 || !id_out) return VH_ERR_ARG;
+    if (!b || !prm || !e || !p || !id_out) return VH_ERR_ARG;
     const vh_event_params &c = *prm;
     if (c.nx < 1 || c.ny < 1 || c.nz < 1 || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0) || !(c.dt > 0) ||
         !(c.t0 > 0) || !(c.etabar > 0))
@@ -462,23 +466,32 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
     E.N = N;
     E.steps_done = 0;
     E.e_reg = b->heos.energy_density(c.T_reg_GeV / 0.197326938);
+    // a NULL array is zero (u^tau: one)
     E.bi = c.nz == 1;
-    for (long s = 0; s < N && E.bi; ++s)
-        E.bi = un[s] == 0.0 && pi10[3 * N + s] == 0.0 && pi10[6 * N + s] == 0.0 && pi10[8 * N + s] == 0.0;
+    for (long s = 0; s < N && E.bi && un; ++s) E.bi = un[s] == 0.0;
+    for (long s = 0; s < N && E.bi && pi10; ++s)
+        E.bi = pi10[3 * N + s] == 0.0 && pi10[6 * N + s] == 0.0 && pi10[8 * N + s] == 0.0;
     const size_t bytes = sizeof(double) * N;
     if (!b->mixed) {
         BCHK(hipMemcpy(b->e[0] + off, e, bytes, hipMemcpyHostToDevice));
         BCHK(hipMemcpy(b->p[0] + off, p, bytes, hipMemcpyHostToDevice));
         const double *uu[4] = {ut, ux, uy, un};
         for (int k = 0; k < 4; ++k) {
-            BCHK(hipMemcpy(b->u[b->iu_cur][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
-            BCHK(hipMemcpy(b->u[b->iu_prev][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
+            if (uu[k]) {
+                BCHK(hipMemcpy(b->u[b->iu_cur][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
+                BCHK(hipMemcpy(b->u[b->iu_prev][k] + off, uu[k], bytes, hipMemcpyHostToDevice));
+            } else {
+                BCHK(launch_fill(b->u[b->iu_cur][k] + off, N, k == 0 ? 1.0 : 0.0, b->stream));
+                BCHK(launch_fill(b->u[b->iu_prev][k] + off, N, k == 0 ? 1.0 : 0.0, b->stream));
+            }
         }
-        for (int n = 0; n < 10; ++n)
-            BCHK(hipMemcpy(b->q[b->iq_cur][4 + n] + off, pi10 + (size_t)n * N, bytes, hipMemcpyHostToDevice));
+        for (int n = 0; n < 10; ++n) {
+            if (pi10) BCHK(hipMemcpy(b->q[b->iq_cur][4 + n] + off, pi10 + (size_t)n * N, bytes, hipMemcpyHostToDevice));
+            else BCHK(launch_fill(b->q[b->iq_cur][4 + n] + off, N, 0.0, b->stream));
+        }
         if (b->nc == 15) {
             if (Pi) BCHK(hipMemcpy(b->q[b->iq_cur][14] + off, Pi, bytes, hipMemcpyHostToDevice));
-            else BCHK(hipMemset(b->q[b->iq_cur][14] + off, 0, bytes));
+            else BCHK(launch_fill(b->q[b->iq_cur][14] + off, N, 0.0, b->stream));
         }
         double *qq[15];
         for (int n = 0; n < 15; ++n) qq[n] = b->q[b->iq_cur][n] + off;
@@ -487,16 +500,23 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
         BCHK(launch_init_conserved(qq, b->nc == 15 ? qq[14] : nullptr, b->e[0] + off, b->p[0] + off, uc, N,
                                    b->stream));
     } else {
-        // FP64 primitives staged on the device; conserved variables formed in FP64
-        double *st = nullptr;
-        BCHK(hipMalloc(&st, bytes * 17));
+        // FP64 primitives staged on the device (a staging buffer the batch
+        // keeps); conserved variables formed in FP64
+        if (b->stage_bytes < bytes * 17) {
+            (void)hipFree(b->stage_buf);
+            b->stage_buf = nullptr;
+            b->stage_bytes = 0;
+            BCHK(hipMalloc(&b->stage_buf, bytes * 17));
+            b->stage_bytes = bytes * 17;
+        }
+        double *st = b->stage_buf;
         const double *src[8] = {e, p, ut, ux, uy, un, pi10, Pi};
         const size_t len[8] = {1, 1, 1, 1, 1, 1, 10, 1};
         size_t at = 0;
         hipError_t r = hipSuccess;
         for (int k = 0; k < 8 && r == hipSuccess; ++k) {
-            if (src[k]) r = hipMemcpy(st + at * N, src[k], bytes * len[k], hipMemcpyHostToDevice);
-            else r = hipMemset(st + at * N, 0, bytes * len[k]);
+            if (src[k]) r = hipMemcpyAsync(st + at * N, src[k], bytes * len[k], hipMemcpyHostToDevice, b->stream);
+            else r = launch_fill(st + at * N, (long)(N * len[k]), k == 2 ? 1.0 : 0.0, b->stream);
             at += len[k];
         }
         mx::MxFields F{};
@@ -512,7 +532,6 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
             r = hipMemcpyAsync(b->uf[b->iu_prev][k] + off, b->uf[b->iu_cur][k] + off, sizeof(float) * N,
                                hipMemcpyDeviceToDevice, b->stream);
         if (r == hipSuccess) r = hipStreamSynchronize(b->stream);
-        (void)hipFree(st);
         if (r != hipSuccess) {
             free_cells(b, off, N);
             E = Event();

@@ -334,6 +334,47 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
     return RootResult{table_bisect<QD, DV>(E, ePrev, M0, M, Pi), NAN_NONE};
 }
 
+// The mixed form's Newton iteration in FP32: the reference's iteration
+// (EnergyMomentumTensor.cpp:26-47, the formulas of nr_step's QuotFast form)
+// in single precision from the FP64 moments, with B = M0 (M0 + Pi) - M
+// formed in FP64 (it cancels for fast flow), hardware reciprocals, and the
+// reference's stop test |e - e0| <= 1e-3 |e| evaluated in FP32, as the
+// reference evaluates it (fabsf).  Table EoS with a constant c_s^2 only; the
+// bisection fallback is the FP64 one.
+__device__ __forceinline__ RootResult table_root_f32(const EosDev &E, double ePrev, double M0d, double Md, double Pid) {
+    const float M0 = (float)M0d, Pi = (float)Pid;
+    const float B = (float)(M0d * (M0d + Pid) - Md);
+    const float emin = (float)E.e_min, rsp = (float)E.r_spacing;
+    const float cs2 = 1.0f / 3.0f;
+    float e0 = (float)ePrev;
+    for (int j = 0; j < MAX_ITERS_NR; ++j) {
+        const float x = (e0 - emin) * rsp;
+        int idx = (int)x;
+        idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
+        idx = idx > 0 ? idx : 0;
+        const float frac = x - (float)idx;
+        const float t1 = (float)E.p_tb[idx], t2 = (float)E.p_tb[idx + 1];
+        float p = __builtin_fmaf(frac, t2 - t1, t1);
+        p = 1.e-15f < p ? p : 1.e-15f;
+        const float R = __builtin_amdgcn_rcpf(e0 * p);
+        const float ie0 = p * R, ip = e0 * R;
+        const float cst2 = p * ie0;
+        const float A = __builtin_fmaf(-M0, cst2, M0) + Pi;
+        const float H = __builtin_sqrtf(__builtin_fabsf(__builtin_fmaf(A, A, 4 * cst2 * B)));
+        const float D = (A - H) * (0.5f * e0 * ip);
+        const float f = e0 + D;
+        const float c = cs2 - cst2;
+        const float X = c * (B + D * H - c * cst2 * D * M0 * ie0);
+        const float pH = p * H;
+        const float step = f * pH * __builtin_amdgcn_rcpf(pH - X);
+        if (__builtin_isnan(step)) return RootResult{ePrev, NAN_ROOT};
+        const float e = e0 - step;
+        if (__builtin_fabsf(e - e0) <= 1.0e-3f * __builtin_fabsf(e)) return RootResult{(double)e, NAN_NONE};
+        e0 = e;
+    }
+    return RootResult{table_bisect<QuotFast, DivRecip>(E, ePrev, M0d, Md, Pid), NAN_NONE};
+}
+
 // energyDensityFromConservedVariables, rhic/src/EnergyMomentumTensor.cpp:22-115
 template <bool TABLE, class QD = QuotExact, class DV = DivExact>
 __device__ __forceinline__ double energy_from_conserved(const EosDev &E, double ePrev, double M0, double M, double Pi,
@@ -424,6 +465,86 @@ __device__ __forceinline__ Prim recover(const EosDev &E, double t, double ttt, d
                                         int &nan, DV &div) {
     const Moments m = moments(t, ttt, ttx, tty, ttn, pitt, pitx, pity, pitn);
     return recover_moments<TABLE, DV, QD>(E, m, Pi, ePrev, nan, div);
+}
+
+// Four root solves from one start value, stepped together: the table-EoS
+// Newton iteration and bisection of table_root for each state, unchanged,
+// with the four iterations interleaved so that their table lookups and
+// divisions overlap (independent chains).  Each state takes exactly the
+// steps table_root takes for it; rn[k] is its NaN site.
+template <class DV>
+__device__ __forceinline__ void table_root4(const EosDev &E, double ePrev, const Moments (&m)[4], const double (&Pi)[4],
+                                            double (&e)[4], int (&rn)[4]) {
+    DV dx;
+    double e0[4];
+    bool done[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        e0[k] = ePrev;
+        done[k] = false;
+        rn[k] = NAN_NONE;
+        e[k] = ePrev;
+    }
+    for (int j = 0; j < MAX_ITERS_NR; ++j) {
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!done[k]) {
+                bool bad;
+                const double en = nr_step<QuotExact, DV>(E, e0[k], m[k].M0, m[k].M, Pi[k], dx, bad);
+                if (bad) {
+                    e[k] = ePrev;
+                    rn[k] = NAN_ROOT;
+                    done[k] = true;
+                } else if (nr_converged<QuotExact>(en, e0[k])) {
+                    e[k] = en;
+                    done[k] = true;
+                } else {
+                    e0[k] = en;
+                }
+            }
+            any = any || !done[k];
+        }
+        if (!any) break;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k)
+        if (!done[k]) e[k] = table_bisect<QuotExact, DV>(E, ePrev, m[k].M0, m[k].M, Pi[k]);
+}
+
+// getInferredVariables for four states sharing ePrev (the four half-site
+// states of one cell's two faces along one direction): recover() of each,
+// with the four root solves interleaved.  NaN sites are recorded in the
+// order the four recover() calls would record them.
+template <bool TABLE, class DV>
+__device__ __forceinline__ void recover4(const EosDev &E, double t, const double (&x)[4][9], double ePrev, int &nan,
+                                         DV &div, Prim (&out)[4]) {
+    Moments m[4];
+    double Pi[4], e[4];
+    int rn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        m[k] = moments(t, x[k][0], x[k][1], x[k][2], x[k][3], x[k][4], x[k][5], x[k][6], x[k][7]);
+        Pi[k] = x[k][8];
+        rn[k] = NAN_NONE;
+    }
+    if (ePrev <= 0.1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = m[k].M0 - QuotExact::q(m[k].M, m[k].M0);
+    } else if constexpr (TABLE) {
+        table_root4<DivExact>(E, ePrev, m, Pi, e, rn);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int dummy = NAN_NONE;
+            e[k] = energy_from_conserved<false>(E, ePrev, m[k].M0, m[k].M, Pi[k], dummy);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (rn[k]) nan = rn[k];
+        out[k] = recover_finish<TABLE, DV, QuotExact>(E, m[k], e[k], Pi[k], nan, div);
+    }
 }
 
 }  // namespace vh

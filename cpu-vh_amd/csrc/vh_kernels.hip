@@ -371,6 +371,17 @@ hipError_t launch_inferred(const EosDev &E, int nc, double t, const double *q, c
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void fill_kernel(double *__restrict__ out, long n, double v) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
+hipError_t launch_fill(double *out, long n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, v);
+    return hipGetLastError();
+}
+
 // The single-event step clock moves on by count steps (last node of a graph).
 __global__ void advance_base_kernel(int *__restrict__ base, int count) {
     if (threadIdx.x == 0) *base += count;

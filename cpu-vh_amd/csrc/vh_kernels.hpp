@@ -78,5 +78,7 @@ hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const doubl
 hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
                                       const long *start, int nblocks, unsigned long long *counts, hipStream_t st);
 hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, hipStream_t st);
+// out[0 .. n) = v (a fill kernel: usable where a memset node is not)
+hipError_t launch_fill(double *out, long n, double v, hipStream_t st);
 
 }  // namespace vh

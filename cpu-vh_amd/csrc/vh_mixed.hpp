@@ -7,9 +7,10 @@
 //     is the FP64 accumulation to ~2^-48 (compensated storage);
 //   * limited slopes, half-site values, face fluxes and the source terms run
 //     in FP32 with fused multiply-adds and reciprocal multiplies;
-//   * the conserved-to-primitive root solve, the EoS table lookups and the
-//     accumulation acc = q + dt*S + r_x + r_y + r_eta (and the stage-2 convex
-//     combination) run in FP64; the regulator's factors are FP32;
+//   * the cell's conserved-to-primitive root solve, the EoS table lookups and
+//     the accumulation acc = q + dt*S + r_x + r_y + r_eta (and the stage-2
+//     convex combination) run in FP64, the face states' Newton iterations in
+//     FP32 (from FP64 moments); the regulator's factors are FP32;
 //   * each face flux is evaluated once and shared by the two cells it
 //     separates (the reference evaluates it from both sides, with each
 //     cell's own e as the root solve's starting value; here each side of a
@@ -500,7 +501,11 @@ __device__ __forceinline__ float face_speed_mx(const EosDev &E, const Moments &m
                                                int &nan) {
     double e;
     if (ePrev <= 0.1) e = m.M0 - m.M * QuotFast::rcp1(m.M0);
-    else e = energy_from_conserved<TABLE, QuotFast, DivRecip>(E, ePrev, m.M0, m.M, Pi, nan);
+    else if (TABLE && E.cs2_const) {  // the Newton iteration in FP32
+        const RootResult r = table_root_f32(E, ePrev, m.M0, m.M, Pi);
+        if (r.nan) nan = r.nan;
+        e = r.e;
+    } else e = energy_from_conserved<TABLE, QuotFast, DivRecip>(E, ePrev, m.M0, m.M, Pi, nan);
     if (__builtin_isnan(e)) nan = NAN_ENERGY;
     DivRecip div;
     double p;

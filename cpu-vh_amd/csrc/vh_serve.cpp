@@ -106,7 +106,6 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
         pay[k].p.resize(N);
         for (size_t m = 0; m < N; ++m) pay[k].p[m] = heos.pressure(pay[k].e[m]);
     }
-    std::vector<double> ones(nmax, 1.0), zeros(10 * nmax, 0.0);
 
     vh_batch_config bc{};
     bc.device = c.device;
@@ -172,8 +171,9 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             vh_event_params ep{r.nx, r.ny, r.nz, c.dx, c.dx, c.dx, c.dt, c.t0, c.etabar, c.T_reg_GeV,
                                c.etas_Tk_GeV, c.etas_slope_lo, c.etas_slope_hi};
             int id = -1;
-            vh_status s = vh_batch_add(b, &ep, pay[k].e.data(), pay[k].p.data(), ones.data(), zeros.data(),
-                                       zeros.data(), zeros.data(), zeros.data(), c.bulk ? zeros.data() : nullptr, &id);
+            // at rest without dissipative currents: only e and p are uploaded
+            vh_status s = vh_batch_add(b, &ep, pay[k].e.data(), pay[k].p.data(), nullptr, nullptr, nullptr, nullptr,
+                                       nullptr, nullptr, &id);
             if (s == VH_ERR_CAPACITY) {
                 ++it;
                 continue;

@@ -178,18 +178,22 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const GEO &G, 
         qRf[m] = right_fwd(l); qLf[m] = left_fwd(l);
         qRb[m] = right_bwd(l); qLb[m] = left_bwd(l);
     }
-    // Four root solves: forward-right, forward-left, backward-right, backward-left.
+    // Four root solves: forward-right, forward-left, backward-right,
+    // backward-left, all from the cell's e, stepped together.
     Side sd[4];
-#pragma unroll 1
-    for (int kk = 0; kk < 4; ++kk) {
-        double x[9];
+    {
+        double x[4][9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) x[m] = kk == 0 ? qRf[m] : (kk == 1 ? qLf[m] : (kk == 2 ? qRb[m] : qLb[m]));
-        const Prim w = recover<TABLE>(E, t, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], ePrev, nan, div);
-        const Side side{dir == 0 ? w.ux : (dir == 1 ? w.uy : w.un), make_recip(w.ut)};
+        for (int m = 0; m < 9; ++m) {
+            x[0][m] = qRf[m];
+            x[1][m] = qLf[m];
+            x[2][m] = qRb[m];
+            x[3][m] = qLb[m];
+        }
+        Prim w[4];
+        recover4<TABLE>(E, t, x, ePrev, nan, div, w);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q == kk) sd[q] = side;
+        for (int q = 0; q < 4; ++q) sd[q] = Side{dir == 0 ? w[q].ux : (dir == 1 ? w[q].uy : w[q].un), make_recip(w[q].ut)};
     }
     const Side Rf = sd[0], Lf = sd[1], Rb = sd[2], Lb = sd[3];
     const double af = face_speed(Rf, Lf, div), ab = face_speed(Rb, Lb, div);

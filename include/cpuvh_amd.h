@@ -83,7 +83,10 @@ vh_status vh_destroy(vh_handle *h);
 
 /* Initial state, the type-11 reader's assignments (rhic/src/InitialConditions.cpp:41-105):
  * e, p, u^mu, previous u^mu := u^mu, pi^{mu nu} (10 arrays back to back, pitt..pinn),
- * Pi (may be NULL: zero).  Host arrays of nx*ny*nz doubles. */
+ * Pi.  Host arrays of nx*ny*nz doubles.  Every array but e and p may be NULL:
+ * u^tau is then 1 everywhere, the others 0 (a fluid at rest without
+ * dissipative currents, as the Glauber and constant initial states start),
+ * and nothing is copied for it. */
 vh_status vh_upload_ic(vh_handle *h, const double *e, const double *p, const double *ut, const double *ux,
                        const double *uy, const double *un, const double *pi10, const double *Pi);
 
@@ -330,9 +333,10 @@ typedef struct vh_batch vh_batch;
 
 vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out);
 vh_status vh_batch_destroy(vh_batch *b);
-/* Admits one event (vh_upload_ic + vh_init_conserved at t0 for it): first-fit
- * placement of its nx*ny*nz cells; VH_ERR_CAPACITY when no free range or
- * event slot is large enough.  Synchronising. */
+/* Admits one event (vh_upload_ic + vh_init_conserved at t0 for it, with
+ * vh_upload_ic's NULL arrays): first-fit placement of its nx*ny*nz cells;
+ * VH_ERR_CAPACITY when no free range or event slot is large enough.
+ * Synchronising. */
 vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e, const double *p, const double *ut,
                        const double *ux, const double *uy, const double *un, const double *pi10, const double *Pi,
                        int *id_out);
