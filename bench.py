@@ -571,6 +571,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0, help="concurrent single-thread reference processes")
     ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--skip-dropin", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=WORKLOADS["glauber3d"]["steps_per_event"],
+                    help="timed steps of the 131x131x61 FP64 line (600: the whole run; fewer for profiling)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-accuracy", action="store_true")
     ap.add_argument("--skip-precision-3d", action="store_true")
@@ -646,8 +648,8 @@ def main():
         precision3d = run_precision_3d(table, PRECISION_3D["steps"])
     dropin = None
     if not args.skip_dropin:
-        log("3+1D drop-in 131x131x61, 600 steps")
-        _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, WORKLOADS["glauber3d"]["steps_per_event"], 20)
+        log(f"3+1D drop-in 131x131x61, {args.dropin_steps} steps")
+        _, dropin = run_dropin_3d(WORKLOADS["glauber3d"], table, args.dropin_steps, 20 if args.dropin_steps >= 100 else 0)
 
     # dominant kernel: the mixed stage kernel over the batch (two launches per
     # step).  Algorithmic bytes of one zone-update (two stages): each stage

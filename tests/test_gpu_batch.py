@@ -240,3 +240,45 @@ def test_nan_event_is_isolated(precision):
         assert steps[i] == rsteps[r] == 4 * coarse
         for f in fields:
             assert V.bits_equal(b.get(i, f), ref.get(r, f)), (i, f)
+
+
+def test_workload_events_fp64_bitwise_and_mixed_within_envelope():
+    """Eight events of the batching workload's own generator (bench.py: MC-Glauber
+    Au+Au, grids spread over 81^2..201^2, Latin-hypercube eta/s, tau0; table
+    EoS, bulk, eta/s(T), regulation below 100 MeV): the FP64 batch equals the
+    oracle bit for bit after 20 steps, every field of every event; and the
+    mixed batch stays within twice the FP64 path's own sensitivity (FP64
+    from the FP32-rounded and two 1e-7-perturbed starts, largest error over
+    the events) in e and u^mu, with no more differing freezeout cubes."""
+    import sys
+    sys.path.insert(0, V.REPO)
+    import bench as B
+    table = V.eos_table()
+    allev = B.make_batch_events(table, 256)
+    events = sorted(allev, key=lambda ev: ev["N"])[::32][:8]
+    assert events[0]["n"] < 100 and events[-1]["n"] > 180
+    nsteps = 20
+    print("case workload events", [ev["n"] for ev in events], flush=True)
+    b = Batch(sum(ev["N"] for ev in events), max_events=8, max_steps=nsteps, bulk=True, eos_table=table)
+    ids = [B.add_event(b, ev) for ev in events]
+    b.steps(nsteps)
+    b.sync()
+    for ev, eid in zip(events, ids):
+        n, N = ev["n"], ev["N"]
+        o = V.Oracle(n, n, 1, B.BATCH["dx"], B.BATCH["dx"], B.BATCH["dx"], B.BATCH["dt"], ev["t0"], ev["etabar"],
+                     T_reg_GeV=B.BATCH["T_reg_GeV"], bulk=True, table=table, etas_Tk_GeV=B.BATCH["etas_Tk_GeV"],
+                     etas_slope_lo=B.BATCH["etas_slope_lo"], etas_slope_hi=B.BATCH["etas_slope_hi"])
+        o.set_ic(B.full_ic(ev["ic"], N))
+        o.init_conserved(ev["t0"])
+        for s in range(1, nsteps + 1):
+            assert o.step(ev["t0"] + (s - 1) * B.BATCH["dt"]) == 0
+        for f in _fields(True):
+            a, c = o.get(f), b.get(eid, f)
+            assert V.bits_equal(a, c), f"event {n}^2 {f}: max rel {V.max_rel(a, c):.3e}"
+    b.close()
+    acc = B.accuracy_all(events, table, checkpoints=(nsteps,))
+    r = acc["report"][0]
+    print(r["mixed_vs_fp64_max_over_events"], r["fp64_envelope_max_over_events"], flush=True)
+    for f in ("e", "e_pt", "ut_pt", "ux", "uy"):
+        assert r["mixed_vs_fp64_max_over_events"][f] <= 2 * r["fp64_envelope_max_over_events"][f], f
+    assert r["fo_cubes_differing_mixed"] <= r["fo_cubes_differing_fp64_envelope"]
