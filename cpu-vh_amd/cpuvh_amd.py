@@ -45,13 +45,14 @@ class VhConfig(ctypes.Structure):
                 ("T_reg_GeV", ctypes.c_double), ("bulk", ctypes.c_int), ("eos_kind", ctypes.c_int),
                 ("eos_table", ctypes.c_void_p),
                 ("etas_Tk_GeV", ctypes.c_double), ("etas_slope_lo", ctypes.c_double),
-                ("etas_slope_hi", ctypes.c_double), ("device", ctypes.c_int)]
+                ("etas_slope_hi", ctypes.c_double), ("device", ctypes.c_int), ("cs2_mode", ctypes.c_int),
+                ("zeta_Tpeak_GeV", ctypes.c_double), ("zeta_peak", ctypes.c_double)]
 
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("capacity_cells", ctypes.c_int64), ("max_events", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("bulk", ctypes.c_int), ("eos_kind", ctypes.c_int),
-                ("eos_table", ctypes.c_void_p), ("precision", ctypes.c_int)]
+                ("eos_table", ctypes.c_void_p), ("precision", ctypes.c_int), ("cs2_mode", ctypes.c_int)]
 
 
 class EventParams(ctypes.Structure):
@@ -59,7 +60,8 @@ class EventParams(ctypes.Structure):
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
                 ("dt", ctypes.c_double), ("t0", ctypes.c_double), ("etabar", ctypes.c_double),
                 ("T_reg_GeV", ctypes.c_double), ("etas_Tk_GeV", ctypes.c_double),
-                ("etas_slope_lo", ctypes.c_double), ("etas_slope_hi", ctypes.c_double)]
+                ("etas_slope_lo", ctypes.c_double), ("etas_slope_hi", ctypes.c_double),
+                ("zeta_Tpeak_GeV", ctypes.c_double), ("zeta_peak", ctypes.c_double)]
 
 
 class ServeConfig(ctypes.Structure):
@@ -215,7 +217,8 @@ class Hydro:
     """One lattice on one GPU (a vh_handle)."""
 
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV=0.0, bulk=False,
-                 eos_table=None, etas_Tk_GeV=0.154, etas_slope_lo=0.0, etas_slope_hi=0.0, device=0):
+                 eos_table=None, etas_Tk_GeV=0.154, etas_slope_lo=0.0, etas_slope_hi=0.0, device=0,
+                 cs2_mode=0, zeta_Tpeak_GeV=0.0, zeta_peak=0.0):
         L = _lib()
         self.shape = (nz, ny, nx)
         self.N = nx * ny * nz
@@ -224,7 +227,7 @@ class Hydro:
         cfg = VhConfig(nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV, int(bool(bulk)),
                        EOS_CONFORMAL if self._table is None else EOS_TABLE,
                        None if self._table is None else self._table.ctypes.data,
-                       etas_Tk_GeV, etas_slope_lo, etas_slope_hi, device)
+                       etas_Tk_GeV, etas_slope_lo, etas_slope_hi, device, cs2_mode, zeta_Tpeak_GeV, zeta_peak)
         self.cfg = cfg
         h = ctypes.c_void_p()
         self._check(L.vh_create(ctypes.byref(cfg), ctypes.byref(h)), "vh_create")
@@ -347,7 +350,7 @@ class Batch:
     """Many events resident on one GPU (a vh_batch), stepped together."""
 
     def __init__(self, capacity_cells, max_events, max_steps, bulk=False, eos_table=None, device=0,
-                 precision=PREC_FP64):
+                 precision=PREC_FP64, cs2_mode=0):
         L = _lib()
         self.nc = 15 if bulk else 14
         self.precision = precision
@@ -355,7 +358,7 @@ class Batch:
         self._table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)
         cfg = BatchConfig(device, capacity_cells, max_events, max_steps, int(bool(bulk)),
                           EOS_CONFORMAL if self._table is None else EOS_TABLE,
-                          None if self._table is None else self._table.ctypes.data, precision)
+                          None if self._table is None else self._table.ctypes.data, precision, cs2_mode)
         b = ctypes.c_void_p()
         Hydro._check(L.vh_batch_create(ctypes.byref(cfg), ctypes.byref(b)), "vh_batch_create")
         self.b = b
@@ -373,13 +376,13 @@ class Batch:
             pass
 
     def add(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, ic, T_reg_GeV=0.0, etas_Tk_GeV=0.154,
-            etas_slope_lo=0.0, etas_slope_hi=0.0):
+            etas_slope_lo=0.0, etas_slope_hi=0.0, zeta_Tpeak_GeV=0.0, zeta_peak=0.0):
         """Admits one event; returns its id, or None when the batch is full
         (VH_ERR_CAPACITY).  ic: dict of x-fastest arrays e, p, ut, ux, uy, un, pi (10*N), Pi (optional)."""
         N = nx * ny * nz
         arrs = _ic_arrays([ic.get(k) for k in ("e", "p", "ut", "ux", "uy", "un", "pi", "Pi")], N)
         prm = EventParams(nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV, etas_Tk_GeV, etas_slope_lo,
-                          etas_slope_hi)
+                          etas_slope_hi, zeta_Tpeak_GeV, zeta_peak)
         eid = ctypes.c_int(-1)
         st = _lib().vh_batch_add(self.b, ctypes.byref(prm), *[_ptr(a) for a in arrs], ctypes.byref(eid))
         if st == VH_ERR_CAPACITY:

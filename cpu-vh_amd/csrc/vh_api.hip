@@ -146,6 +146,8 @@ StageParams stage_params(const vh_handle *h, int stage, int iq_in, int iu_in, in
     P.etas_Tk_GeV = h->cfg.etas_Tk_GeV;
     P.etas_slope_lo = h->cfg.etas_slope_lo;
     P.etas_slope_hi = h->cfg.etas_slope_hi;
+    P.zeta_Tpeak_GeV = h->cfg.zeta_Tpeak_GeV;
+    P.zeta_peak = h->cfg.zeta_peak;
     P.nx = h->cfg.nx;
     P.ny = h->cfg.ny;
     P.nz = h->cfg.nz;
@@ -206,7 +208,8 @@ vh_status vh_create(const vh_config *cfg, vh_handle **out) {
     const vh_config &c = *cfg;
     if (c.nx < 1 || c.ny < 1 || c.nz < 1 || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0) || !(c.dt > 0) ||
         !(c.t0 > 0) || !(c.etabar > 0) || (c.eos_kind != VH_EOS_CONFORMAL && c.eos_kind != VH_EOS_TABLE) ||
-        (c.eos_kind == VH_EOS_TABLE && !c.eos_table) || (long)c.nx * c.ny * c.nz >= (1L << 31))
+        (c.eos_kind == VH_EOS_TABLE && !c.eos_table) || (long)c.nx * c.ny * c.nz >= (1L << 31) ||
+        (c.cs2_mode != 0 && c.cs2_mode != 1) || !(c.zeta_Tpeak_GeV >= 0.0) || !(c.zeta_peak >= 0.0))
         return VH_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -249,7 +252,7 @@ vh_status vh_create(const vh_config *cfg, vh_handle **out) {
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(VH_ERR_HIP);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(VH_ERR_HIP);
 
-    if (hipError_t r = h->deos.init(h->heos, c.eos_kind); r != hipSuccess)
+    if (hipError_t r = h->deos.init(h->heos, c.eos_kind, c.cs2_mode); r != hipSuccess)
         return fail(r == hipErrorOutOfMemory ? VH_ERR_NOMEM : hip_fail(r, "equation of state upload"));
     h->eos = h->deos.E;
     vh_status s = ensure_times(h, 1);

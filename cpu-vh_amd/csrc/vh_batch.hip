@@ -360,7 +360,8 @@ vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out) {
     b->cap = (long)c.capacity_cells;
     if (hipSetDevice(b->dev) != hipSuccess) return fail(VH_ERR_HIP);
     if (c.eos_kind == VH_EOS_TABLE) b->heos.load(c.eos_table);
-    if (b->deos.init(b->heos, c.eos_kind) != 0) return fail(VH_ERR_NOMEM);
+    if (c.cs2_mode != 0 && c.cs2_mode != 1) return fail(VH_ERR_ARG);
+    if (b->deos.init(b->heos, c.eos_kind, c.cs2_mode) != 0) return fail(VH_ERR_NOMEM);
     if (c.precision != VH_PREC_FP64 && c.precision != VH_PREC_MIXED) return fail(VH_ERR_ARG);
     b->mixed = c.precision == VH_PREC_MIXED;
     if (b->mixed && b->cap >= (1L << 30)) return fail(VH_ERR_ARG);  // 32-bit byte offsets per field
@@ -444,7 +445,7 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
     if (!b || !prm || !e || !p || !id_out) return VH_ERR_ARG;
     const vh_event_params &c = *prm;
     if (c.nx < 1 || c.ny < 1 || c.nz < 1 || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0) || !(c.dt > 0) ||
-        !(c.t0 > 0) || !(c.etabar > 0))
+        !(c.t0 > 0) || !(c.etabar > 0) || !(c.zeta_Tpeak_GeV >= 0.0) || !(c.zeta_peak >= 0.0))
         return VH_ERR_ARG;
     int id = -1;
     for (int k = 0; k < (int)b->ev.size(); ++k)
@@ -557,6 +558,8 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
     D.etas_Tk_GeV = c.etas_Tk_GeV;
     D.etas_slope_lo = c.etas_slope_lo;
     D.etas_slope_hi = c.etas_slope_hi;
+    D.zeta_Tpeak_GeV = c.zeta_Tpeak_GeV;
+    D.zeta_peak = c.zeta_peak;
     D.tab = b->d_times + (size_t)id * (b->cfg.max_steps + 1);
     D.base = 1;
     BCHK(hipMemcpyAsync(b->d_ev + id, &D, sizeof(EvDesc), hipMemcpyHostToDevice, b->stream));

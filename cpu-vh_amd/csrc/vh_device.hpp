@@ -113,6 +113,7 @@ struct DivRecip {
 struct EosDev {
     int kind;        // 0 conformal, 1 table
     int cs2_const;   // table: host verified that the reference's c_s^2 is 1/3 for every e
+    int cs2_mode;    // table: 0 the reference's delta_e/delta_p, 1 the sound speed delta_p/delta_e
     double e_min, e_spacing, e_max;
     double r_spacing;  // RN(1/e_spacing)
     const double *p_tb;
@@ -192,7 +193,7 @@ __device__ __forceinline__ double eos_cs2(const EosDev &E, double e, DV &div) {
         double p = eos_pressure<true>(E, e, div);
         double p_r = eos_pressure<true>(E, e + delta_e, div);
         double delta_p = p_r - p;
-        double v = delta_e / delta_p;
+        double v = E.cs2_mode ? delta_p / delta_e : delta_e / delta_p;
         v = v_max < v ? v_max : v;  // std::min(v_max, v)
         v = v_min < v ? v : v_min;  // std::max(v_min, .)
         return v;

@@ -443,12 +443,13 @@ hipError_t launch_fo_interp(const FoSlices &S, int nx, int ny, int nz, const lon
     return hipGetLastError();
 }
 
-hipError_t DeviceEos::init(const HostEos &h, int kind) {
+hipError_t DeviceEos::init(const HostEos &h, int kind, int cs2_mode) {
     E = EosDev{};
     E.kind = kind;
     hipError_t r;
     if (kind == 1) {
-        E.cs2_const = h.cs2_is_one_third() ? 1 : 0;
+        E.cs2_mode = cs2_mode;
+        E.cs2_const = cs2_mode == 0 && h.cs2_is_one_third() ? 1 : 0;
         E.e_min = h.e_min;
         E.e_spacing = h.e_spacing;
         E.r_spacing = 1.0 / h.e_spacing;

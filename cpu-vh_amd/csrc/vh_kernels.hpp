@@ -18,7 +18,7 @@ struct DeviceEos {
     double *ptb = nullptr, *Ttb = nullptr;
     uint2 *exc = nullptr;
     uint32_t *bucket = nullptr;
-    hipError_t init(const HostEos &h, int kind);  // on the current device
+    hipError_t init(const HostEos &h, int kind, int cs2_mode = 0);  // on the current device
     void release();
 };
 

@@ -1,4 +1,5 @@
-// Hyperbolic tangent that rounds exactly like the host C library's tanh.
+// Hyperbolic tangent that rounds exactly like the host C library's tanh, and
+// an exponential whose every bit is fixed by its source (fdlibm_exp, below).
 //
 // The regulator of the viscous step (REG_SCHEME_1,
 // rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:775,780) calls the C
@@ -146,6 +147,57 @@ __host__ __device__ inline double glibc_tanh(double x) {
         z = one - tiny;
     }
     return pos ? z : -z;
+}
+
+// exp(x) by the fdlibm method (Sun's e_exp.c): x = k ln2 + r with ln2
+// split into a 32-bit-exact high part and a low part, |r| <= ln2/2, then
+// exp(r) = 1 + r + r c/(2 - c) with c = r - r^2 (P1 + r^2 (P2 + ... P5 r^2))
+// from the published minimax coefficients, scaled by 2^k.  Within an ulp
+// of exp.  It serves the bulk-viscosity extension (zeta/s, reached only with
+// the table's own sound speed, which the reference never uses), where the
+// device and the tests must agree bit for bit; the host C library's exp is
+// a different algorithm.
+__host__ __device__ inline double fdlibm_exp(double x) {
+    const double one = 1.0, huge = 1.0e+300, twom1000 = 9.33263618503218878990e-302;
+    const double o_threshold = 7.09782712893383973096e+02, u_threshold = -7.45133219101941108420e+02;
+    const double ln2HI = 6.93147180369123816490e-01, ln2LO = 1.90821492927058770002e-10;
+    const double invln2 = 1.44269504088896338700e+00;
+    const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03, P3 = 6.61375632143793436117e-05,
+                 P4 = -1.65339022054652515390e-06, P5 = 4.13813679705723846039e-08;
+    uint32_t hx = hi_word(x);
+    const int xsb = (int)((hx >> 31) & 1);
+    hx &= 0x7fffffffu;
+    if (hx >= 0x40862E42u) {  // |x| >= 709.78...
+        if (hx >= 0x7ff00000u) {
+            if (((hx & 0xfffffu) | lo_word(x)) != 0) return x + x;  // NaN
+            return xsb == 0 ? x : 0.0;                             // exp(+-inf) = {inf, 0}
+        }
+        if (x > o_threshold) return huge * huge;
+        if (x < u_threshold) return twom1000 * twom1000;
+    }
+    double hi = 0.0, lo = 0.0;
+    int k = 0;
+    if (hx > 0x3fd62e42u) {      // |x| > ln2 / 2
+        if (hx < 0x3FF0A2B2u) {  // and |x| < 3 ln2 / 2
+            hi = xsb == 0 ? x - ln2HI : x + ln2HI;
+            lo = xsb == 0 ? ln2LO : -ln2LO;
+            k = 1 - xsb - xsb;
+        } else {
+            k = (int)(invln2 * x + (xsb == 0 ? 0.5 : -0.5));
+            const double t = (double)k;
+            hi = x - t * ln2HI;  // exact
+            lo = t * ln2LO;
+        }
+        x = hi - lo;
+    } else if (hx < 0x3e300000u) {  // |x| < 2^-28
+        return one + x;
+    }
+    const double t = x * x;
+    const double c = x - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+    if (k == 0) return one - ((x * c) / (c - 2.0) - x);
+    const double y = one - ((lo - (x * c) / (2.0 - c)) - hi);
+    if (k >= -1021) return with_hi_word(y, hi_word(y) + ((uint32_t)k << 20));
+    return with_hi_word(y, hi_word(y) + ((uint32_t)(k + 1000) << 20)) * twom1000;
 }
 
 }  // namespace vh

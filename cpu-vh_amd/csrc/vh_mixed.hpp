@@ -118,6 +118,29 @@ __device__ __forceinline__ float temperature_f(const EosDev &E, float e) {
     }
 }
 
+// zeta_over_s in FP32 (the hardware's exponential)
+__device__ __forceinline__ float zeta_over_s_f(const EvDesc &G, float T) {
+    const bool ext = G.zeta_Tpeak_GeV > 0.0;
+    const float x = ext ? T * (float)(HBARC / G.zeta_Tpeak_GeV) : T * (1.0f / 1.01355f);
+    float z;
+    if (x > 1.05f) z = 0.9f * __expf(-(x - 1) * 40.0f) + 0.25f * __expf(-(x - 1) * (1.0f / 0.13f)) + 0.001f;
+    else if (x < 0.995f) z = 0.9f * __expf((x - 1) * 400.0f) + 0.22f * __expf((x - 1) * (1.0f / 0.022f)) + 0.03f;
+    else z = -13.77f * x * x + 27.55f * x - 13.45f;
+    return ext ? (float)(G.zeta_peak / 0.33) * z : z;
+}
+
+// tau_Pi^-1 = 15 a^2 T / (zeta/s), held at 1/dt at most with beta_Pi scaled
+// alike (the Navier-Stokes value -zeta theta is kept; explicit RK2 is unstable
+// for dt/tau_Pi > 2)
+__device__ __forceinline__ void bulk_relaxation(const EvDesc &G, float a2, float T, float rdt, float &tauPiInv,
+                                                float &beta_Pi) {
+    tauPiInv = 15 * a2 * T * frcp(zeta_over_s_f(G, T));
+    if (tauPiInv > rdt) {
+        beta_Pi *= rdt * frcp(tauPiInv);
+        tauPiInv = rdt;
+    }
+}
+
 // Source terms of one cell (loadSourceTerms2 + setPimunuSourceTerms,
 // rhic/src/SourceTerms.cpp:55-313,421-612) in FP32, and the pi-gradient
 // terms of loadSourceTermsX/Y/Z (:316-419) for each direction: Sd[d][mu].
@@ -205,19 +228,14 @@ __device__ __forceinline__ void source_mx(const MxFields &F, const EvDesc &G, co
     const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
     const float a = 1.0f / 3.0f - cs2;
     const float a2 = a * a;
-    const float beta_Pi = 15 * a2 * (e + p);
+    float beta_Pi = 15 * a2 * (e + p);
     const float lambda_Pipi = 8 * a * r5;
     float tauPiInv = 0.0f;
     if constexpr (NC == 15) {
-        // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48)
-        if (a2 != 0.0f) {
-            const float x = T * (1.0f / 1.01355f);
-            float zetabar;
-            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) * 40.0f) + 0.25f * __expf(-(x - 1) * (1.0f / 0.13f)) + 0.001f;
-            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) * 400.0f) + 0.22f * __expf((x - 1) * (1.0f / 0.022f)) + 0.03f;
-            else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
-            tauPiInv = 15 * a2 * T * frcp(zetabar);
-        }
+        // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48), or its
+        // rescaled form with the event's peak (zeta_over_s); relaxation no
+        // faster than 1/dt, as in the FP64 step
+        if (a2 != 0.0f) bulk_relaxation(G, a2, T, rdt, tauPiInv, beta_Pi);
     }
     const float ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
     const float rt2 = rt * rt, rt3 = rt2 * rt, r2t = 0.5f * rt;
@@ -406,18 +424,11 @@ __device__ __forceinline__ void source_mx_bi(const MxFields &F, const EvDesc &G,
     const float cs2 = (float)eos_cs2<TABLE>(E, (double)e, dx64);
     const float a = 1.0f / 3.0f - cs2;
     const float a2 = a * a;
-    const float beta_Pi = 15 * a2 * (e + p);
+    float beta_Pi = 15 * a2 * (e + p);
     const float lambda_Pipi = 8 * a * r5;
     float tauPiInv = 0.0f;
     if constexpr (NC == 15) {
-        if (a2 != 0.0f) {
-            const float x = T * (1.0f / 1.01355f);
-            float zetabar;
-            if (x > 1.05f) zetabar = 0.9f * __expf(-(x - 1) * 40.0f) + 0.25f * __expf(-(x - 1) * (1.0f / 0.13f)) + 0.001f;
-            else if (x < 0.995f) zetabar = 0.9f * __expf((x - 1) * 400.0f) + 0.22f * __expf((x - 1) * (1.0f / 0.022f)) + 0.03f;
-            else zetabar = -13.77f * x * x + 27.55f * x - 13.45f;
-            tauPiInv = 15 * a2 * T * frcp(zetabar);
-        }
+        if (a2 != 0.0f) bulk_relaxation(G, a2, T, rdt, tauPiInv, beta_Pi);
     }
     const float ut2 = ut * ut, t2 = t * t, t3 = t * t2;
     const float rt2 = rt * rt, rt3 = rt2 * rt;

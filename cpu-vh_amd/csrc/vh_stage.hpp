@@ -45,6 +45,7 @@ struct StageParams {
     StepTimes st;
     double dt, dx, dy, dz, e_reg;
     double etas_min, etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    double zeta_Tpeak_GeV, zeta_peak;
     int nx, ny, nz;
     int *err;              // err[0] site, err[1..2] cell (64-bit)
 };
@@ -58,6 +59,7 @@ struct EvDesc {
     long off;
     double dx, dy, dz, dt, e_reg;
     double etas_min, etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    double zeta_Tpeak_GeV, zeta_peak;
     const StepTimes *tab;
     int base;  // step index of graph node offset 0
     int pad;
@@ -88,6 +90,22 @@ __device__ __forceinline__ double etabar_of_T(const GEO &G, double T) {
     double TGeV = T * HBARC;
     if (TGeV > G.etas_Tk_GeV) return G.etas_min + G.etas_slope_hi * (TGeV - G.etas_Tk_GeV);
     return G.etas_min + G.etas_slope_lo * (G.etas_Tk_GeV - TGeV);
+}
+
+// zeta/s at T (fm^-1): the reference's bulkViscosityToEntropyDensity
+// (rhic/src/SourceTerms.cpp:40-48) of x = T/1.01355, or, with
+// zeta_Tpeak_GeV > 0, the same shape of x = T/T_peak scaled so that its peak
+// (0.33 at x = 1) becomes zeta_peak.  The exponentials are fdlibm_exp (this
+// is reached only where c_s^2 != 1/3, which the reference's EoS never gives).
+template <class GEO>
+__device__ __forceinline__ double zeta_over_s(const GEO &G, double T) {
+    const bool ext = G.zeta_Tpeak_GeV > 0.0;
+    const double x = ext ? T * HBARC / G.zeta_Tpeak_GeV : T / 1.01355;
+    double z;
+    if (x > 1.05) z = 0.9 * fdlibm_exp(-(x - 1) / 0.025) + 0.25 * fdlibm_exp(-(x - 1) / 0.13) + 0.001;
+    else if (x < 0.995) z = 0.9 * fdlibm_exp((x - 1) / 0.0025) + 0.22 * fdlibm_exp((x - 1) / 0.022) + 0.03;
+    else z = -13.77 * x * x + 27.55 * x - 13.45;
+    return ext ? G.zeta_peak / 0.33 * z : z;
 }
 
 // Primitive state on one side of a face, with the reciprocal of u^tau used by
@@ -295,18 +313,23 @@ __device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, 
     const double cs2 = eos_cs2<TABLE>(E, e, div);
     const double a = 1.0 / 3.0 - cs2;
     const double a2 = a * a;
-    const double beta_Pi = 15 * a2 * (e + p);
+    double beta_Pi = 15 * a2 * (e + p);
     const double lambda_Pipi = div(8 * a, r5);
     double tauPiInv = 0.0;
     if constexpr (NC == 15) {
         // bulkViscosityToEntropyDensity (SourceTerms.cpp:40-48); with a == 0 the
-        // quotient is +0 for any positive finite zeta/s, so exp is skipped.
+        // quotient is +0 for any positive finite zeta/s, so it is skipped.
         if (a2 != 0.0) {
-            double x = T / 1.01355, zetabar;
-            if (x > 1.05) zetabar = 0.9 * exp(-(x - 1) / 0.025) + 0.25 * exp(-(x - 1) / 0.13) + 0.001;
-            else if (x < 0.995) zetabar = 0.9 * exp((x - 1) / 0.0025) + 0.22 * exp((x - 1) / 0.022) + 0.03;
-            else zetabar = -13.77 * x * x + 27.55 * x - 13.45;
-            tauPiInv = 15 * a2 * T / zetabar;
+            tauPiInv = 15 * a2 * T / zeta_over_s(G, T);
+            // Bulk relaxation no faster than 1/dt (tau_Pi >= dt), with beta_Pi
+            // scaled alike so that the Navier-Stokes value -zeta theta stays:
+            // the explicit RK2 step is unstable for dt/tau_Pi > 2, which a
+            // small zeta/s reaches.  Never active where c_s^2 = 1/3 (a = 0).
+            const double cap = div(1.0, rdt);
+            if (tauPiInv > cap) {
+                beta_Pi = beta_Pi * (cap / tauPiInv);
+                tauPiInv = cap;
+            }
         }
     }
     const double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;

@@ -73,6 +73,25 @@ typedef struct {
        etabar + slope_lo*(Tk-T) below, T in GeV.  Both slopes 0 reproduces the reference. */
     double etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
     int device;                /* HIP device ordinal */
+    /* Bulk transport extensions (no reference counterpart; the zero values give
+       the reference).  cs2_mode 0: the reference's speedOfSoundSquared, the
+       one-table-segment quotient delta_e/delta_p clamped to [0.01, 1/3]
+       (EquationOfState.cpp:141-153; 1/3 for the bundled table, which makes every
+       bulk coefficient (1/3 - c_s^2)^2 vanish, SourceTerms.cpp:75-82); 1: the
+       table's sound speed delta_p/delta_e over the same segment, same clamp.
+       zeta_Tpeak_GeV > 0: zeta/s(T) = zeta_peak/0.33 * f(T/zeta_Tpeak_GeV), f the
+       reference's shape (SourceTerms.cpp:40-48, peak 0.33 at its argument 1),
+       so zeta/s peaks at zeta_Tpeak_GeV with the value zeta_peak; 0: the
+       reference's zeta/s, f(T/1.01355 fm^-1).  zeta_peak 0 (with zeta_Tpeak_GeV
+       > 0) is zeta/s = 0.  Where the bulk terms are reached (cs2_mode 1) the
+       exponentials of f are vh's own fdlibm-method exp, the same on the device
+       and in tests, and the bulk relaxation rate 15 (1/3 - c_s^2)^2 T / (zeta/s)
+       is held at 1/dt at most, with beta_Pi scaled alike (the Navier-Stokes
+       value -zeta theta is kept): the explicit RK2 step is unstable beyond
+       dt/tau_Pi = 2, which a small zeta/s reaches.  None of this changes a
+       configuration with cs2_mode 0, where (1/3 - c_s^2) = 0. */
+    int cs2_mode;
+    double zeta_Tpeak_GeV, zeta_peak;
 } vh_config;
 
 typedef struct vh_handle vh_handle;
@@ -319,6 +338,7 @@ typedef struct {
     int eos_kind;
     const double *eos_table;   /* as vh_config.eos_table */
     int precision;             /* vh_precision */
+    int cs2_mode;              /* as vh_config.cs2_mode */
 } vh_batch_config;
 
 /* One event's parameters: vh_config's per-event fields. */
@@ -327,6 +347,7 @@ typedef struct {
     double dx, dy, dz, dt, t0;
     double etabar, T_reg_GeV;
     double etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    double zeta_Tpeak_GeV, zeta_peak;  /* as vh_config's */
 } vh_event_params;
 
 typedef struct vh_batch vh_batch;

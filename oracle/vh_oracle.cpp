@@ -30,6 +30,7 @@
 // 25 regFacShear, 26 regFacBulk.  Arrays are x-fastest over the interior: i + nx*(j + ny*k).
 // =============================================================================
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -56,6 +57,11 @@ enum { ERR_NONE = 0, ERR_NAN_E = 1, ERR_NAN_NR = 2, ERR_NAN_REG_SHEAR = 3, ERR_N
 
 struct Eos {
     int kind = 0;  // 0 conformal, 1 HotQCD+HRG table
+    // table only: 0 the reference's speedOfSoundSquared (delta_e/delta_p,
+    // clamped to [0.01, 1/3], hence 1/3 for this table), 1 the sound speed
+    // delta_p/delta_e under the same clamp (an extension: it makes the bulk
+    // coefficients (1/3 - c_s^2) non-zero)
+    int cs2_mode = 0;
     double e_min = 0, e_max = 0, e_spacing = 0;
     std::vector<double> p_tb, T_tb;
 
@@ -83,7 +89,7 @@ struct Eos {
             double p = pressure(e);
             double p_r = pressure(e + delta_e);
             double delta_p = p_r - p;
-            double dpde = delta_e / delta_p;
+            double dpde = cs2_mode ? delta_p / delta_e : delta_e / delta_p;
             double v = dpde;
             v = std::max(v_min, std::min(v_max, v));
             return v;
@@ -145,12 +151,68 @@ inline double approx_deriv(double x, double y, double z) {
     return minmod3(l, c, r);
 }
 
-// ζ/s of rhic/src/SourceTerms.cpp:25-48
-inline double zeta_over_s_ref(double T) {
-    double x = T / 1.01355;
-    if (x > 1.05) return 0.9 * exp(-(x - 1) / 0.025) + 0.25 * exp(-(x - 1) / 0.13) + 0.001;
-    else if (x < 0.995) return 0.9 * exp((x - 1) / 0.0025) + 0.22 * exp((x - 1) / 0.022) + 0.03;
-    else return -13.77 * x * x + 27.55 * x - 13.45;
+// exp by the fdlibm method (Sun Microsystems' e_exp.c, freely
+// distributable): reduce x = k ln2 + r (ln2 in a 32-bit-exact high part and
+// a low part), exp(r) = 1 + r + r c/(2 - c) with c = r - r^2 R(r^2) and R
+// the published degree-4 minimax polynomial, then scale by 2^k.  The
+// reference's ζ/s calls the C library's exp; this one is used instead
+// because every one of its bits follows from the source, so the device can
+// match it.  Within one ulp of exp (tests/test_oracle_zeta.py).
+inline uint64_t dbits(double x) { uint64_t b; std::memcpy(&b, &x, 8); return b; }
+inline double from_dbits(uint64_t b) { double x; std::memcpy(&x, &b, 8); return x; }
+double exp_fdlibm(double x) {
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double inv_ln2 = 1.44269504088896338700e+00;
+    const double R1 = 1.66666666666666019037e-01, R2 = -2.77777777770155933842e-03,
+                 R3 = 6.61375632143793436117e-05, R4 = -1.65339022054652515390e-06,
+                 R5 = 4.13813679705723846039e-08;
+    const uint64_t b = dbits(x);
+    const bool neg = (b >> 63) != 0;
+    const uint32_t ahi = (uint32_t)(b >> 32) & 0x7fffffffu;
+    if (ahi >= 0x40862E42u) {                    // |x| >= 709.78
+        if (ahi >= 0x7ff00000u) return std::isnan(x) ? x + x : (neg ? 0.0 : x);
+        if (x > 7.09782712893383973096e+02) return 1.0e300 * 1.0e300;
+        if (x < -7.45133219101941108420e+02) return 9.33263618503218878990e-302 * 9.33263618503218878990e-302;
+    }
+    int k = 0;
+    double hi = 0.0, lo = 0.0;
+    if (ahi > 0x3fd62e42u) {                     // |x| > ln2/2
+        if (ahi < 0x3FF0A2B2u) {                 // |x| < 3 ln2/2: k = +-1
+            k = neg ? -1 : 1;
+            hi = neg ? x + ln2_hi : x - ln2_hi;
+            lo = neg ? -ln2_lo : ln2_lo;
+        } else {
+            k = (int)(inv_ln2 * x + (neg ? -0.5 : 0.5));
+            hi = x - k * ln2_hi;
+            lo = k * ln2_lo;
+        }
+        x = hi - lo;
+    } else if (ahi < 0x3e300000u) {              // |x| < 2^-28
+        return 1.0 + x;
+    }
+    const double r2 = x * x;
+    const double c = x - r2 * (R1 + r2 * (R2 + r2 * (R3 + r2 * (R4 + r2 * R5))));
+    if (k == 0) return 1.0 - ((x * c) / (c - 2.0) - x);
+    const double y = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
+    if (k >= -1021) return from_dbits(dbits(y) + ((uint64_t)(int64_t)k << 52));
+    return from_dbits(dbits(y) + ((uint64_t)(int64_t)(k + 1000) << 52)) * 9.33263618503218878990e-302;
+}
+
+// ζ/s of rhic/src/SourceTerms.cpp:25-48 (bulkViscosityToEntropyDensity,
+// x = T/1.01355 with T in fm^-1: peak 0.33 at 200 MeV), or, with
+// Tpeak_GeV > 0, the same shape of x = T/T_peak scaled by peak/0.33 (an
+// extension: ζ/s peaked at T_peak with the height peak).  The value enters
+// the step only through 15 (1/3 - c_s^2)^2 T / (ζ/s), which is +0 whenever
+// c_s^2 = 1/3 (conformal EoS, the reference's table c_s^2), so the choice of
+// exp cannot change a reference configuration.
+inline double zeta_over_s(double T, double Tpeak_GeV, double peak) {
+    const bool ext = Tpeak_GeV > 0.0;
+    const double x = ext ? T * HBARC / Tpeak_GeV : T / 1.01355;
+    double z;
+    if (x > 1.05) z = 0.9 * exp_fdlibm(-(x - 1) / 0.025) + 0.25 * exp_fdlibm(-(x - 1) / 0.13) + 0.001;
+    else if (x < 0.995) z = 0.9 * exp_fdlibm((x - 1) / 0.0025) + 0.22 * exp_fdlibm((x - 1) / 0.022) + 0.03;
+    else z = -13.77 * x * x + 27.55 * x - 13.45;
+    return ext ? peak / 0.33 * z : z;
 }
 
 struct Params {
@@ -161,6 +223,9 @@ struct Params {
     // η/s(T) extension (the reference has a constant η/s, SourceTerms.cpp:69).
     // With both slopes zero etabar_of_T returns etas_min bit for bit.
     double etas_min, etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    // ζ/s(T) extension (zeta_over_s above); zeta_Tpeak_GeV = 0 keeps the
+    // reference's shape
+    double zeta_Tpeak_GeV, zeta_peak;
 };
 
 struct Orc {
@@ -444,9 +509,18 @@ struct Orc {
         double beta_Pi = 15 * a2 * (e + p);
         double lambda_Pipi = 8 * a / 5;
         double tauPiInv = 0.0;
-        if (P.bulk) {
-            double zetabar = zeta_over_s_ref(T);
+        if (P.bulk && a2 != 0.0) {  // a2 == 0: the reference's quotient is +0
+            double zetabar = zeta_over_s(T, P.zeta_Tpeak_GeV, P.zeta_peak);
             tauPiInv = 15 * a2 * T / zetabar;
+            // Extension: relaxation no faster than 1/dt (tau_Pi >= dt; explicit
+            // RK2 is unstable beyond dt/tau_Pi = 2), beta_Pi scaled alike so
+            // the Navier-Stokes value -zeta theta stays.  a2 == 0 in every
+            // reference configuration, where this never applies.
+            double cap = 1.0 / P.dt;
+            if (tauPiInv > cap) {
+                beta_Pi = beta_Pi * (cap / tauPiInv);
+                tauPiInv = cap;
+            }
         }
         double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
         double d_dt = P.dt;
@@ -663,6 +737,8 @@ struct orc_params {
     int nx, ny, nz, bulk, eos_kind;
     double dx, dy, dz, dt, t0, etabar, T_reg_GeV;
     double etas_Tk_GeV, etas_slope_lo, etas_slope_hi;
+    int cs2_mode;
+    double zeta_Tpeak_GeV, zeta_peak;
 };
 
 ORC_API void *orc_create(const orc_params *cp, const double *eos_raw) {
@@ -673,9 +749,11 @@ ORC_API void *orc_create(const orc_params *cp, const double *eos_raw) {
     P.etabar = cp->etabar; P.T_reg_GeV = cp->T_reg_GeV;
     P.etas_min = cp->etabar; P.etas_Tk_GeV = cp->etas_Tk_GeV;
     P.etas_slope_lo = cp->etas_slope_lo; P.etas_slope_hi = cp->etas_slope_hi;
+    P.zeta_Tpeak_GeV = cp->zeta_Tpeak_GeV; P.zeta_peak = cp->zeta_peak;
     if (cp->eos_kind == 1) {
         if (!eos_raw) { delete o; return nullptr; }
         o->eos.load(eos_raw);
+        o->eos.cs2_mode = cp->cs2_mode;
     }
     o->nc = P.bulk ? 15 : 14;
     o->N = (long)P.nx * P.ny * P.nz;
@@ -813,6 +891,9 @@ ORC_API void orc_eos_destroy(void *h) { delete static_cast<Eos *>(h); }
 ORC_API double orc_eos_pressure(void *h, double e) { return static_cast<Eos *>(h)->pressure(e); }
 ORC_API double orc_eos_temperature(void *h, double e) { return static_cast<Eos *>(h)->temperature(e); }
 ORC_API double orc_eos_cs2(void *h, double e) { return static_cast<Eos *>(h)->cs2(e); }
+ORC_API void orc_eos_set_cs2_mode(void *h, int mode) { static_cast<Eos *>(h)->cs2_mode = mode; }
+ORC_API double orc_exp_fdlibm(double x) { return exp_fdlibm(x); }
+ORC_API double orc_zeta_over_s(double T, double Tpeak_GeV, double peak) { return zeta_over_s(T, Tpeak_GeV, peak); }
 ORC_API double orc_eos_energy_density(void *h, double T) { return static_cast<Eos *>(h)->energy_density(T); }
 
 // One-cell primitive recovery through a throwaway 1-cell lattice.

@@ -5,6 +5,9 @@
 //                       N arguments: uniform on [0, 25), log-uniform on
 //                       [2^-60, 2^6), negative, and every double within
 //                       2^16 ulps of each branch threshold of tanh and expm1;
+//   libm_check exp N    vh::fdlibm_exp against the host exp on N arguments in
+//                       [-745, 710]: at most one ulp apart (prints the
+//                       largest distance in ulps);
 //   libm_check div N    vh::DivFast (reciprocal + Markstein correction, with
 //                       the division operator outside its proven range)
 //                       against the division operator on N pairs with
@@ -74,6 +77,28 @@ static void check_tanh(long n) {
     for (double x : special) cmp("tanh", x, 0, std::tanh(x), vh::glibc_tanh(x));
 }
 
+static long max_ulps = 0;
+static void check_exp(long n) {
+    for (long i = 0; i < n; ++i) {
+        double x;
+        switch (i & 3) {
+            case 0: x = -745.0 + 1455.0 * unit(); break;  // the whole finite range
+            case 1: x = -30.0 + 40.0 * unit(); break;      // what zeta/s feeds it
+            case 2: x = std::ldexp(unit() - 0.5, -20); break;
+            default: x = -3.0 + 6.0 * unit(); break;
+        }
+        const double want = std::exp(x), got = vh::fdlibm_exp(x);
+        ++checked;
+        const long d = (long)(bits(want) > bits(got) ? bits(want) - bits(got) : bits(got) - bits(want));
+        if (d > max_ulps) max_ulps = d;
+        if (d > 1) {
+            if (bad < 8) std::printf("exp x=%a want=%a got=%a (%ld ulps)\n", x, want, got, d);
+            ++bad;
+        }
+    }
+    std::printf("largest distance %ld ulps\n", max_ulps);
+}
+
 static double spread(int emin, int emax) {
     const uint64_t r = next_u64();
     const double m = 1.0 + unit();
@@ -102,6 +127,7 @@ int main(int argc, char **argv) {
     const long n = std::atol(argv[2]);
     if (!std::strcmp(argv[1], "tanh")) check_tanh(n);
     else if (!std::strcmp(argv[1], "div")) check_div(n);
+    else if (!std::strcmp(argv[1], "exp")) check_exp(n);
     else return 2;
     std::printf("checked %ld mismatches %ld\n", checked, bad);
     return bad ? 1 : 0;

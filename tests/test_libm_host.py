@@ -7,7 +7,11 @@
   bit for bit;
 - the reciprocal division with Markstein's correction (vh::DivFast,
   cpu-vh_amd/csrc/vh_device.hpp) against the division operator on 10^8 pairs:
-  bit for bit.
+  bit for bit;
+- the fdlibm-method exp of the bulk-viscosity extension (vh::fdlibm_exp)
+  against the host exp on 10^7 arguments: never more than one ulp apart
+  (the two are different algorithms; the extension only needs the device and
+  the oracle to share one).
 
 tests/libm_check.cpp is compiled from the library's headers for the host
 (hipcc, no device code, no contraction, like the device build).
@@ -28,9 +32,10 @@ def checker(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("what", ["tanh", "div"])
+@pytest.mark.parametrize("what", ["tanh", "div", "exp"])
 def test_bitwise_against_host(checker, what):
-    r = subprocess.run([checker, what, "100000000"], capture_output=True, text=True, timeout=300)
+    n = "10000000" if what == "exp" else "100000000"
+    r = subprocess.run([checker, what, n], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert r.stdout.strip().endswith("mismatches 0")
@@ -57,4 +62,25 @@ def test_checker_sees_a_wrong_tanh(checker, tmp_path):
     subprocess.run(["/opt/rocm/bin/hipcc", "--offload-host-only", "-O2", "-std=c++17", "-ffp-contract=off", "-mfma",
                     "-o", exe, str(t / "libm_check.cpp")], check=True, capture_output=True)
     r = subprocess.run([exe, "tanh", "1000000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "mismatches 0" not in r.stdout
+
+
+def test_checker_sees_a_wrong_exp(tmp_path):
+    """A plausible slip in the exp (one polynomial coefficient's sign) is
+    caught: the same checker built with it reports mismatches."""
+    src = open(os.path.join(HERE, "..", "cpu-vh_amd", "csrc", "vh_libm.hpp")).read()
+    bad = src.replace("P4 = -1.65339022054652515390e-06", "P4 = 1.65339022054652515390e-06")
+    assert bad != src
+    d = tmp_path / "csrc"
+    d.mkdir()
+    (d / "vh_libm.hpp").write_text(bad)
+    (d / "vh_device.hpp").write_text(open(os.path.join(HERE, "..", "cpu-vh_amd", "csrc", "vh_device.hpp")).read())
+    t = tmp_path / "tests"
+    t.mkdir()
+    (t / "libm_check.cpp").write_text(open(os.path.join(HERE, "libm_check.cpp")).read()
+                                       .replace("../cpu-vh_amd/csrc/", "../csrc/"))
+    exe = str(tmp_path / "bad_check")
+    subprocess.run(["/opt/rocm/bin/hipcc", "--offload-host-only", "-O2", "-std=c++17", "-ffp-contract=off", "-mfma",
+                    "-o", exe, str(t / "libm_check.cpp")], check=True, capture_output=True)
+    r = subprocess.run([exe, "exp", "1000000"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "mismatches 0" not in r.stdout

@@ -33,7 +33,8 @@ def eos_table():
 # ----------------------------------------------------------------- oracle --
 class _OrcParams(ctypes.Structure):
     _fields_ = [(n, I) for n in "nx ny nz bulk eos_kind".split()] + \
-               [(n, D) for n in "dx dy dz dt t0 etabar T_reg_GeV etas_Tk_GeV etas_slope_lo etas_slope_hi".split()]
+               [(n, D) for n in "dx dy dz dt t0 etabar T_reg_GeV etas_Tk_GeV etas_slope_lo etas_slope_hi".split()] + \
+               [("cs2_mode", I), ("zeta_Tpeak_GeV", D), ("zeta_peak", D)]
 
 
 _ORC = None
@@ -73,6 +74,11 @@ def oracle_lib():
             getattr(L, f).argtypes = [VP, D]
             getattr(L, f).restype = D
         L.orc_inferred.argtypes = [VP, D, VP, D, VP]
+        L.orc_eos_set_cs2_mode.argtypes = [VP, I]
+        L.orc_exp_fdlibm.argtypes = [D]
+        L.orc_exp_fdlibm.restype = D
+        L.orc_zeta_over_s.argtypes = [D, D, D]
+        L.orc_zeta_over_s.restype = D
         L.orc_powf_quarter.argtypes = [ctypes.c_float]
         L.orc_powf_quarter.restype = ctypes.c_float
         _ORC = L
@@ -81,13 +87,14 @@ def oracle_lib():
 
 class Oracle:
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, T_reg_GeV=0.0, bulk=False, table=None,
-                 etas_Tk_GeV=0.154, etas_slope_lo=0.0, etas_slope_hi=0.0):
+                 etas_Tk_GeV=0.154, etas_slope_lo=0.0, etas_slope_hi=0.0, cs2_mode=0, zeta_Tpeak_GeV=0.0,
+                 zeta_peak=0.0):
         L = oracle_lib()
         self.N = nx * ny * nz
         self.dims = (nx, ny, nz)
         self._table = table
         p = _OrcParams(nx, ny, nz, int(bool(bulk)), 0 if table is None else 1, dx, dy, dz, dt, t0, etabar,
-                       T_reg_GeV, etas_Tk_GeV, etas_slope_lo, etas_slope_hi)
+                       T_reg_GeV, etas_Tk_GeV, etas_slope_lo, etas_slope_hi, cs2_mode, zeta_Tpeak_GeV, zeta_peak)
         self.h = L.orc_create(ctypes.byref(p), None if table is None else table.ctypes.data)
         if not self.h:
             raise RuntimeError("orc_create failed")
