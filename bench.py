@@ -5,7 +5,8 @@ Headline (value): events/s of the batched mixed-precision form on the
 batching workload -- 256 seeded 2+1D boost-invariant MC-Glauber events with
 square grids drawn from 81^2..201^2 (dx = 0.1 fm, dtau = 0.02 fm/c) and
 Latin-hypercube per-event parameters ((eta/s)_min in [0.04, 0.2], zeta/s peak
-in [0, 0.1], tau0 in [0.3, 1.0] fm/c), all resident in one batch and stepped
+in [0, 0.1] at 160 MeV, tau0 in [0.3, 1.0] fm/c; bulk viscosity acting through
+the EoS table's sound speed), all resident in one batch and stepped
 together until each is below T_F = 150 MeV (the third coarse check, every 5
 steps, with no cell above e_F -- the reference never resets that count;
 finished events leave the batch).  A step of --steps/--warmup is one pass of
@@ -35,6 +36,30 @@ sys.path.insert(0, os.path.join(REPO, "cpu-vh_amd"))
 METRIC = "events/s batched-FP32 vs plain FP64 (≤1e-3 rel err), zone-upd/s, p50/p99"
 HBM_PEAK_GBS = 8000.0
 HBARC = 0.197326938
+
+# Bulk viscosity that acts: c_s^2 from the EoS table (cs2_mode 1; the
+# reference's own c_s^2 is 1/3 on this table, which zeroes every bulk term)
+# and zeta/s(T) peaked at 160 MeV, with the batching workload's per-event
+# height and DROPIN_ZETA_PEAK for the 3+1D events.  --reference-bulk turns it
+# off (the reference's transport).
+BULK_EXT = True
+ZETA_TPEAK_GEV = 0.16
+DROPIN_ZETA_PEAK = 0.1
+
+
+def cs2_mode(reference=False):
+    return 0 if reference or not BULK_EXT else 1
+
+
+def zeta_kw(peak, reference=False):
+    if reference or not BULK_EXT:
+        return {}
+    return dict(zeta_Tpeak_GeV=ZETA_TPEAK_GEV, zeta_peak=peak)
+
+
+def bulk_text():
+    return (f"bulk zeta/s(T) peaked at {ZETA_TPEAK_GEV * 1e3:.0f} MeV with c_s^2 from the table" if BULK_EXT
+            else "bulk on with the reference's c_s^2 = 1/3 (bulk terms zero)")
 
 WORKLOADS = {
     # 3+1D single event at real size (the plain FP64 drop-in configuration)
@@ -170,7 +195,7 @@ def add_event(b, ev):
     return b.add(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], ev["ic"],
                  T_reg_GeV=BATCH["T_reg_GeV"], etas_Tk_GeV=BATCH["etas_Tk_GeV"],
                  etas_slope_lo=0.0 if const else BATCH["etas_slope_lo"],
-                 etas_slope_hi=0.0 if const else BATCH["etas_slope_hi"])
+                 etas_slope_hi=0.0 if const else BATCH["etas_slope_hi"], **zeta_kw(ev["zeta_peak"], const))
 
 
 CHECKS_PER_SYNC = 8
@@ -239,14 +264,14 @@ def one_pass(b, events, coarse, max_batched_steps=None):
 def run_batched(events, table, precision, passes, warmup, max_batched_steps=None, const_etas=False):
     """The batching workload: `warmup` untimed passes, then `passes` timed
     passes, each from admission of all events to the last one's freezeout,
-    through one batch object.  const_etas: eta/s(T) slopes zero (each event's
-    (eta/s)_min as a constant, the reference's transport)."""
+    through one batch object.  const_etas: the reference's transport (each
+    event's (eta/s)_min as a constant, the reference's c_s^2 and zeta/s)."""
     import cpuvh_amd as C
     if const_etas:
         events = [dict(ev, const_etas=True) for ev in events]
     cap = sum(ev["N"] for ev in events)
     kw = dict(max_events=len(events), max_steps=BATCH["max_steps"], bulk=True, eos_table=table,
-              precision=precision)
+              precision=precision, cs2_mode=cs2_mode(const_etas))
     tiles = (-1, -1)
     if precision == C.PREC_MIXED:
         if TILES_OVERRIDE is not None:
@@ -296,7 +321,8 @@ def run_plain_loop(events, table):
         t0 = time.perf_counter()
         h = C.Hydro(ev["n"], ev["n"], 1, d, d, d, BATCH["dt"], ev["t0"], ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"],
                     bulk=True, eos_table=table, etas_Tk_GeV=BATCH["etas_Tk_GeV"],
-                    etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"])
+                    etas_slope_lo=BATCH["etas_slope_lo"], etas_slope_hi=BATCH["etas_slope_hi"],
+                    cs2_mode=cs2_mode(), **zeta_kw(ev["zeta_peak"]))
         ic = ev["ic"]
         h.upload_ic(ic["e"], ic["p"], ic.get("ut"), ic.get("ux"), ic.get("uy"), ic.get("un"), ic.get("pi"),
                     ic.get("Pi"))
@@ -348,7 +374,8 @@ def accuracy_all(events, table, checkpoints=(50, 100, 200, 300)):
                 "pert1": (C.PREC_FP64, 1), "pert2": (C.PREC_FP64, 2)}
     batches = {}
     for name, (prec, pert) in variants.items():
-        b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec)
+        b = C.Batch(cap, max_events=len(events), max_steps=nsteps, bulk=True, eos_table=table, precision=prec,
+                    cs2_mode=cs2_mode())
         if prec == C.PREC_MIXED and FACE_MODE is not None:
             b.set_face_mode(FACE_MODE)
         ids = []
@@ -453,9 +480,11 @@ def run_precision_3d(table, steps, checkpoints=(100, 300)):
     fields = ("e", "ut", "ux", "uy", "un", "pitt", "pixx", "pixy", "piyy", "pinn", "Pi")
     batches, wall = {}, {}
     for prec in (C.PREC_FP64, C.PREC_MIXED):
-        b = C.Batch(N, max_events=1, max_steps=steps, bulk=True, eos_table=table, precision=prec)
+        b = C.Batch(N, max_events=1, max_steps=steps, bulk=True, eos_table=table, precision=prec,
+                    cs2_mode=cs2_mode())
         eid = b.add(nx, ny, nz, dx, dx, dx, w["dt"], w["t0"], w["etabar"], ic, T_reg_GeV=w["T_reg_GeV"],
-                    etas_Tk_GeV=w["etas_Tk_GeV"], etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"])
+                    etas_Tk_GeV=w["etas_Tk_GeV"], etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"],
+                    **zeta_kw(DROPIN_ZETA_PEAK))
         batches[prec] = (b, eid)
         wall[prec] = 0.0
     eF = batches[C.PREC_FP64][0].eos_energy_density(BATCH["T_F_GeV"] / HBARC)
@@ -489,8 +518,9 @@ def run_precision_3d(table, steps, checkpoints=(100, 300)):
                        "fo_cubes_differing": len(sym)})
     for b, _ in batches.values():
         b.close()
-    return {"workload": "3+1D MC-Glauber event 201x201x101, dx=deta=0.08, tau0=0.6, dtau=0.02, table EoS, bulk, "
-                        "eta/s(T) min 0.08", "steps": steps, "cells": N, "glauber": info,
+    return {"workload": "3+1D MC-Glauber event 201x201x101, dx=deta=0.08, tau0=0.6, dtau=0.02, table EoS, "
+                        f"eta/s(T) min 0.08, {bulk_text()}"
+                        + (f" (height {DROPIN_ZETA_PEAK})" if BULK_EXT else ""), "steps": steps, "cells": N, "glauber": info,
             "fp64_s": wall[C.PREC_FP64], "mixed_s": wall[C.PREC_MIXED], "checkpoints": report}
 
 
@@ -528,7 +558,8 @@ def run_dropin_3d(w, table, steps, warmup):
     N = nx * ny * nz
     h = C.Hydro(nx, ny, nz, dx, dx, dx, w["dt"], w["t0"], w["etabar"], T_reg_GeV=w["T_reg_GeV"], bulk=w["bulk"],
                 eos_table=table if w["lattice"] else None, etas_Tk_GeV=w["etas_Tk_GeV"],
-                etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"])
+                etas_slope_lo=w["etas_slope_lo"], etas_slope_hi=w["etas_slope_hi"], cs2_mode=cs2_mode(),
+                **zeta_kw(DROPIN_ZETA_PEAK))
     h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic["Pi"])
     h.init_conserved(w["t0"])
     if warmup > 0:
@@ -547,7 +578,8 @@ def run_dropin_3d(w, table, steps, warmup):
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     return ic, {
         "workload": "3+1D single event 131x131x61, dx=deta=0.1, tau0=0.6, dtau=0.02, MC-Glauber Au+Au, "
-                    "HotQCD+HRG table EoS, eta/s(T) min 0.08 @154 MeV, bulk on",
+                    f"HotQCD+HRG table EoS, eta/s(T) min 0.08 @154 MeV, {bulk_text()}"
+                    + (f" (height {DROPIN_ZETA_PEAK})" if BULK_EXT else ""),
         "cells": N, "glauber": info, "steps": steps, "warmup": warmup, "ms_per_step": elapsed / steps * 1e3,
         "zone_upd_per_s": N * steps / elapsed, "events_per_s_at_600_steps": 1.0 / (elapsed / steps * 600),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -586,6 +618,9 @@ def main():
     ap.add_argument("--tiles", default=None,
                     help="mixed-kernel tile variants 'T2,T3' for the timed batch (counter runs without the autotuner)")
     ap.add_argument("--face-mode", type=int, default=None, help="face semantics of the mixed kernel (0, 1, 2)")
+    ap.add_argument("--reference-bulk", action="store_true",
+                    help="the reference's c_s^2 (1/3 on this table: bulk terms zero) instead of the table's sound "
+                         "speed with zeta/s(T) peaked at 160 MeV")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -610,7 +645,8 @@ def main():
         return 2
 
     import cpuvh_amd as C
-    global TILES_OVERRIDE, FACE_MODE
+    global TILES_OVERRIDE, FACE_MODE, BULK_EXT
+    BULK_EXT = not args.reference_bulk
     if args.face_mode is not None:
         FACE_MODE = args.face_mode
     if args.tiles:
@@ -676,7 +712,7 @@ def main():
                        bulk=True, etabar=ev["etabar"], T_reg_GeV=BATCH["T_reg_GeV"]), full_ic(ev["ic"], ev["N"]))
                  for ev in picks]
         res = cpu_baseline(cases, seconds=10.0, max_steps=400)
-        log("batched mixed with constant eta/s (the CPU comparison's transport)")
+        log("batched mixed with the reference's transport (the CPU comparison's)")
         gpu_const = run_batched(events, table, C.PREC_MIXED, 1, 0, args.max_batch_steps, const_etas=True)
         if res and "zone_upd_per_s" in res:
             work = float(np.mean([ev["N"] * s for ev, s in zip(events, gpu_const["steps_to_freezeout"])
@@ -686,7 +722,8 @@ def main():
                    "events_per_s": res["zone_upd_per_s"] / work,
                    "sample": f"{nproc} single-thread processes of the reference build (oracle/_ref, -O3 "
                              "-ffp-contract=off) at once, one event of the workload each (spread over its sizes), "
-                             "constant eta/s = the event's (eta/s)_min (the reference has no eta/s(T)), ~10 s of "
+                             "constant eta/s = the event's (eta/s)_min and c_s^2 = 1/3 (the reference has neither "
+                             "eta/s(T) nor an acting bulk term), ~10 s of "
                              "steps each after one untimed step; events/s = summed zone-upd/s / the workload's mean "
                              f"zone-updates per event to freezeout ({work:.3g}, from the GPU run with the same "
                              "transport)",
@@ -703,7 +740,8 @@ def main():
         "data": "synthetic (seeded MC-Glauber initial states)",
         "config": {"workload": f"{len(events)} 2+1D MC-Glauber Au+Au events, grids {BATCH['n_min']}^2..{BATCH['n_max']}^2, "
                                "dx=0.1, dtau=0.02, LHS (eta/s)_min [0.04,0.2], zeta/s peak [0,0.1], tau0 [0.3,1.0], "
-                               "HotQCD+HRG table EoS, bulk on, eta/s(T) min at 154 MeV, to freezeout at T=150 MeV "
+                               f"HotQCD+HRG table EoS, {bulk_text()}, eta/s(T) min at 154 MeV, "
+                               "to freezeout at T=150 MeV "
                                "(coarse check every 5 steps, 3 strikes), one batch, mixed precision",
                    "step": "one pass of the workload: every event admitted (initial state uploaded, conserved "
                            "variables formed), all advanced together to freezeout, finished events removed",
@@ -735,7 +773,7 @@ def main():
                      "avg_launch_ms": mixed["graph_ms"] / launches,
                      "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},
         "cpu_baseline": cpu,
-        "batched_mixed_const_etas": None if gpu_const is None else
+        "batched_mixed_reference_transport": None if gpu_const is None else
         {k: v for k, v in gpu_const.items() if k != "steps_to_freezeout"},
     }
     if cpu and "events_per_s" in cpu and gpu_const:

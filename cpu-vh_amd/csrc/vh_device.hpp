@@ -340,13 +340,15 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
 // in single precision from the FP64 moments, with B = M0 (M0 + Pi) - M
 // formed in FP64 (it cancels for fast flow), hardware reciprocals, and the
 // reference's stop test |e - e0| <= 1e-3 |e| evaluated in FP32, as the
-// reference evaluates it (fabsf).  Table EoS with a constant c_s^2 only; the
-// bisection fallback is the FP64 one.
+// reference evaluates it (fabsf).  CONST_CS2: the table's c_s^2 is 1/3 for
+// every e; otherwise c_s^2(e0) is the one-segment quotient of eos_cs2 (the
+// pressure one segment up is the next table entry at the same fraction),
+// clamped alike.  The bisection fallback is the FP64 one.
+template <bool CONST_CS2>
 __device__ __forceinline__ RootResult table_root_f32(const EosDev &E, double ePrev, double M0d, double Md, double Pid) {
     const float M0 = (float)M0d, Pi = (float)Pid;
     const float B = (float)(M0d * (M0d + Pid) - Md);
     const float emin = (float)E.e_min, rsp = (float)E.r_spacing;
-    const float cs2 = 1.0f / 3.0f;
     float e0 = (float)ePrev;
     for (int j = 0; j < MAX_ITERS_NR; ++j) {
         const float x = (e0 - emin) * rsp;
@@ -357,6 +359,22 @@ __device__ __forceinline__ RootResult table_root_f32(const EosDev &E, double ePr
         const float t1 = (float)E.p_tb[idx], t2 = (float)E.p_tb[idx + 1];
         float p = __builtin_fmaf(frac, t2 - t1, t1);
         p = 1.e-15f < p ? p : 1.e-15f;
+        float cs2 = 1.0f / 3.0f;
+        if constexpr (!CONST_CS2) {
+            const float x1 = x + 1.0f;
+            int i1 = (int)x1;
+            i1 = i1 < EOS_LEN - 2 ? i1 : EOS_LEN - 2;
+            i1 = i1 > 0 ? i1 : 0;
+            const float f1 = x1 - (float)i1;
+            const float u1 = (float)E.p_tb[i1], u2 = (float)E.p_tb[i1 + 1];
+            float pr = __builtin_fmaf(f1, u2 - u1, u1);
+            pr = 1.e-15f < pr ? pr : 1.e-15f;
+            const float dpde = (pr - p) * rsp;
+            float v = E.cs2_mode ? dpde : __builtin_amdgcn_rcpf(dpde);
+            v = (1.0f / 3.0f) < v ? (1.0f / 3.0f) : v;
+            v = 0.01f < v ? v : 0.01f;
+            cs2 = v;
+        }
         const float R = __builtin_amdgcn_rcpf(e0 * p);
         const float ie0 = p * R, ip = e0 * R;
         const float cst2 = p * ie0;

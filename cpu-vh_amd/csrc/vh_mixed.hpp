@@ -512,8 +512,9 @@ __device__ __forceinline__ float face_speed_mx(const EosDev &E, const Moments &m
                                                int &nan) {
     double e;
     if (ePrev <= 0.1) e = m.M0 - m.M * QuotFast::rcp1(m.M0);
-    else if (TABLE && E.cs2_const) {  // the Newton iteration in FP32
-        const RootResult r = table_root_f32(E, ePrev, m.M0, m.M, Pi);
+    else if (TABLE) {  // the Newton iteration in FP32
+        const RootResult r = E.cs2_const ? table_root_f32<true>(E, ePrev, m.M0, m.M, Pi)
+                                         : table_root_f32<false>(E, ePrev, m.M0, m.M, Pi);
         if (r.nan) nan = r.nan;
         e = r.e;
     } else e = energy_from_conserved<TABLE, QuotFast, DivRecip>(E, ePrev, m.M0, m.M, Pi, nan);
