@@ -26,7 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag", "vh_fo_interp",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
-           "vh_inferred_variables", "vh_fo_flag_noalloc",
+           "vh_inferred_variables", "vh_fo_flag_noalloc", "vh_fo_surface",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
@@ -132,6 +132,7 @@ def _lib():
         L.vh_count_above.argtypes = [P, D, ctypes.POINTER(ctypes.c_int64)]
         L.vh_fo_flag.argtypes = [P, D, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         L.vh_fo_interp.argtypes = [P, P, P, ctypes.c_int64, P]
+        L.vh_fo_surface.argtypes = [P, P, P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         for f in ("vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature"):
             getattr(L, f).argtypes = [P, D]
             getattr(L, f).restype = D
@@ -300,6 +301,7 @@ class Hydro:
         corners = np.empty(max(cap, 1) * per, dtype=np.float64) if with_corners else None
         n = ctypes.c_int64()
         self._check(_lib().vh_fo_flag(self.h, eF, _ptr(ids), _ptr(corners), cap, ctypes.byref(n)), "vh_fo_flag")
+        self._fo_n = n.value
         ids = ids[:n.value].copy()
         if with_corners:
             return ids, corners[:n.value * per].reshape(n.value, per).copy()
@@ -316,6 +318,30 @@ class Hydro:
         out = np.empty((max(cubes.shape[0], 1), 10), dtype=np.float64)
         self._check(_lib().vh_fo_interp(self.h, _ptr(cubes), _ptr(frac), cubes.shape[0], _ptr(out)), "vh_fo_interp")
         return out[:cubes.shape[0]].copy()
+
+    def fo_surface(self, widths):
+        """The surface elements of the cubes the last fo_flag_ids call flagged,
+        built on the device (vh_fo_surface): returns (counts per cube,
+        elements (n, 2*dim): centroid relative to the cube then normal,
+        vars (n, 10) at the centroids)."""
+        nz = self.cfg.nz
+        dim = 4 if nz > 1 else 3
+        w = np.ascontiguousarray(widths, dtype=np.float64)
+        if w.size < dim:
+            raise ValueError(f"fo_surface: {dim} cell widths")
+        m = (self.cfg.nx - 1) * (self.cfg.ny - 1) * (nz - 1 if nz > 1 else 1)
+        counts = np.zeros(max(m, 1), dtype=np.int32)
+        total = ctypes.c_int64()
+        st = _lib().vh_fo_surface(self.h, _ptr(w), _ptr(counts), None, None, 0, ctypes.byref(total))
+        if st != VH_ERR_CAPACITY:
+            self._check(st, "vh_fo_surface")
+        cap = max(total.value, 1)
+        el = np.empty((cap, 2 * dim), dtype=np.float64)
+        var = np.empty((cap, 10), dtype=np.float64)
+        self._check(_lib().vh_fo_surface(self.h, _ptr(w), _ptr(counts), _ptr(el), _ptr(var), cap,
+                                         ctypes.byref(total)), "vh_fo_surface")
+        n = total.value
+        return counts[:getattr(self, "_fo_n", 0)].copy(), el[:n].copy(), var[:n].copy()
 
     def eos_energy_density(self, T):
         return _lib().vh_eos_energy_density(self.h, T)

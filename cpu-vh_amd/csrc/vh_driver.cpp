@@ -45,8 +45,8 @@
 // y, then eta), each surface element's position, dsigma_mu = t * normal, and
 // u^x u^y u^eta e T P pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi linearly
 // interpolated at its centroid — 20 columns, default stream formatting.  The
-// GPU flags the cut cubes (vh_fo_flag); the elements come from
-// vh_surface_cube / vh_surface_hypercube.
+// GPU flags the cut cubes (vh_fo_flag) and builds their elements and the
+// variables at the centroids (vh_fo_surface); the host orders and formats.
 #include <algorithm>
 #include <cctype>
 #include <charconv>
@@ -458,26 +458,23 @@ void radix_sort_by_key(std::vector<T> &a, std::vector<T> &tmp, int64_t limit) {
 // Per-coarse-step work space of the surface writer, kept across calls.
 struct SurfaceWork {
     std::vector<std::pair<int64_t, int64_t>> order, scratch;  // (reference-order key, index into ids)
-    std::vector<int64_t> cubes;
-    std::vector<double> el, frac, vars;
     std::vector<int> ne;
     std::vector<int64_t> off;
+    std::vector<double> el, vars;
     std::vector<std::string> text;
 };
 
 // callFOFinder2p1D / callFOFinder3p1D (FreezeOut.cpp:200-361) for the nf cut
-// cubes ids (x-fastest cube index, increasing) with their corner energies as
-// vh_fo_flag returns them (previous slice first, x fastest within a slice):
-// visited in the reference's order (x outer, eta inner); the elements come
-// from vh_surface_*, the ten variables at their centroids from vh_fo_interp
-// (interpolateVariable3D/4D on the device's slices), one line per element.
-// Cubes are independent: their elements and text are built on all host
-// threads and written afterwards in the reference's order.  Adds the
-// seconds spent sorting, building elements, interpolating and formatting to
-// phase[0..3].
-void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const std::vector<double> &ecorn, int64_t nf,
-                   int dim, const Lattice &L, const double *spacing, double cell_tau, double t, double eF,
-                   vh_handle *h, SurfaceWork &W, double *phase) {
+// cubes ids (x-fastest cube index, increasing, as vh_fo_flag returns them):
+// the elements and the ten variables at their centroids come from the
+// device (vh_fo_surface, the Cornelius construction and
+// interpolateVariable3D/4D, cube by cube in the ids' order); the cubes are
+// visited in the reference's order (x outer, eta inner), one line per
+// element, the text built on all host threads and written afterwards in that
+// order.  Adds the seconds spent sorting, on the device's elements and
+// variables, and formatting to phase[0], phase[1] and phase[3].
+void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, int64_t nf, int dim, const Lattice &L,
+                   const double *spacing, double cell_tau, double t, vh_handle *h, SurfaceWork &W, double *phase) {
     using clk = std::chrono::steady_clock;
     auto lap = [](clk::time_point &a) {
         const clk::time_point b = clk::now();
@@ -496,65 +493,38 @@ void write_surface(std::ofstream &out, const std::vector<int64_t> &ids, const st
     }
     radix_sort_by_key(W.order, W.scratch, cx * cy * std::max(nz - 1, 1));
     phase[0] += lap(t_lap);
-    constexpr int MAXEL = 16;
-    const int per = dim == 4 ? 16 : 8, half = per / 2, nzc = dim == 4 ? 2 : 1;
-    W.el.resize((size_t)nf * MAXEL * 8);
-    W.ne.assign(nf, 0);
-    int failed = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(max : failed)
-    for (int64_t r = 0; r < nf; ++r) {
-        const int64_t m = W.order[r].second, c = ids[m];
-        const double *ec = ecorn.data() + (size_t)m * per;
-        double corners[16];  // [t][x][y][z]
-        for (int it = 0; it < 2; ++it)
-            for (int x = 0; x < 2; ++x)
-                for (int y = 0; y < 2; ++y)
-                    for (int z = 0; z < nzc; ++z)
-                        corners[((it * 2 + x) * 2 + y) * nzc + z] = ec[it * half + (z * 2 + y) * 2 + x];
-        int ne = 0;
-        double *el = W.el.data() + (size_t)r * MAXEL * 8;
-        const vh_status st = dim == 4 ? vh_surface_hypercube(corners, eF, spacing, el, MAXEL, &ne)
-                                      : vh_surface_cube(corners, eF, spacing, el, MAXEL, &ne);
-        if (st != VH_OK) {
-            std::printf("freezeout surface: %s at cube (%d, %d, %d)\n", vh_status_string(st), (int)(c % cx),
-                        (int)((c / cx) % cy), dim == 4 ? (int)(c / (cx * cy)) : 0);
-            failed = 1;
-            continue;
-        }
-        W.ne[r] = ne;
+    const int per = 2 * dim;
+    W.ne.resize(std::max<int64_t>(nf, 1));
+    int64_t total = 0;
+    const int64_t room = (int64_t)(W.vars.size() / 10);
+    vh_status st = vh_fo_surface(h, spacing, W.ne.data(), W.el.data(), W.vars.data(), room, &total);
+    if (st == VH_ERR_CAPACITY) {  // grow, and build again
+        W.el.resize((size_t)total * per + total / 2 * per);
+        W.vars.resize((size_t)total * 10 + total / 2 * 10);
+        st = vh_fo_surface(h, spacing, W.ne.data(), W.el.data(), W.vars.data(), (int64_t)(W.vars.size() / 10),
+                           &total);
     }
-    if (failed) std::exit(-1);
-    phase[1] += lap(t_lap);
+    if (st != VH_OK) {
+        std::printf("freezeout surface: %s\n", vh_status_string(st));
+        std::exit(-1);
+    }
     W.off.resize(nf + 1);
     W.off[0] = 0;
-    for (int64_t r = 0; r < nf; ++r) W.off[r + 1] = W.off[r] + W.ne[r];
-    const int64_t total = W.off[nf];
-    W.cubes.resize(total);
-    W.frac.assign((size_t)total * 4, 0.0);
-    W.vars.resize((size_t)total * 10);
-#pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < nf; ++r)
-        for (int e = 0; e < W.ne[r]; ++e) {
-            const int64_t k = W.off[r] + e;
-            const double *cen = W.el.data() + (size_t)r * MAXEL * 8 + 2 * dim * e;
-            W.cubes[k] = ids[W.order[r].second];
-            for (int d = 0; d < dim; ++d) W.frac[4 * k + d] = cen[d] / spacing[d];
-        }
-    if (total) check_sync(h, vh_fo_interp(h, W.cubes.data(), W.frac.data(), total, W.vars.data()));
-    phase[2] += lap(t_lap);
+    for (int64_t m = 0; m < nf; ++m) W.off[m + 1] = W.off[m] + W.ne[m];
+    phase[1] += lap(t_lap);
     W.text.resize(nf);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t r = 0; r < nf; ++r) {
         std::string &os = W.text[r];
         os.clear();
-        const int64_t c = ids[W.order[r].second];
+        const int64_t m = W.order[r].second, c = ids[m];
         const int ix = (int)(c % cx), iy = (int)((c / cx) % cy), iz = dim == 4 ? (int)(c / (cx * cy)) : 0;
         const double cell_x = (double)ix * L.dx - (((double)(nx - 1)) / 2.0 * L.dx);
         const double cell_y = (double)iy * L.dy - (((double)(ny - 1)) / 2.0 * L.dy);
         const double cell_z = dim == 4 ? (double)iz * L.dz - (((double)(nz - 1)) / 2.0 * L.dz) : 0.0;
-        for (int e = 0; e < W.ne[r]; ++e) {
-            const double *cen = W.el.data() + (size_t)r * MAXEL * 8 + 2 * dim * e, *nor = cen + dim;
-            const double *var = W.vars.data() + (size_t)(W.off[r] + e) * 10;
+        for (int e = 0; e < W.ne[m]; ++e) {
+            const double *cen = W.el.data() + (size_t)(W.off[m] + e) * per, *nor = cen + dim;
+            const double *var = W.vars.data() + (size_t)(W.off[m] + e) * 10;
             const double tau = cen[0] + cell_tau, x = cen[1] + cell_x, y = cen[2] + cell_y;
             const double eta = dim == 4 ? cen[3] + cell_z : 0.0;
             const double ds0 = t * nor[0], ds1 = t * nor[1], ds2 = t * nor[2], ds3 = dim == 4 ? t * nor[3] : 0.0;
@@ -637,12 +607,8 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     // previous and current coarse slices of e and the interpolated variables
     // (the previous ones start at zero, HydroPlugin.cpp:159-170)
     std::vector<int64_t> ids;
-    std::vector<double> ecorn;
     SurfaceWork work;
-    if (H.do_freezeout && dim) {
-        ids.resize(ncubes);
-        ecorn.resize(ncubes * (dim == 4 ? 16 : 8));
-    }
+    if (H.do_freezeout && dim) ids.resize(ncubes);
     const double spacing[4] = {dt * coarse, L.dx, L.dy, L.dz};  // HydroPlugin.cpp:130-145
     const int sc = nx / 2, sj = ny / 2, sk = nz / 2;  // the reference's centre cell (HydroPlugin.cpp:211-214)
     const size_t sctr = (size_t)sc + (size_t)nx * (sj + (size_t)ny * sk);
@@ -683,11 +649,10 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
                 // swapAndSetHydroVariables (FreezeOut.cpp:57-97) on the device, then the cut cubes
                 const auto a = clk::now();
                 int64_t nf = 0;
-                check_sync(h, vh_fo_flag(h, eF, ids.data(), ecorn.data(), (int64_t)ids.size(), &nf));
+                check_sync(h, vh_fo_flag(h, eF, ids.data(), nullptr, (int64_t)ids.size(), &nf));
                 const auto b = clk::now();
                 const double before = fo_phase[0] + fo_phase[1] + fo_phase[2] + fo_phase[3];
-                write_surface(surface, ids, ecorn, nf, dim, L, spacing, t0 + (double)n * dt, t, eF, h, work,
-                              fo_phase);
+                write_surface(surface, ids, nf, dim, L, spacing, t0 + (double)n * dt, t, h, work, fo_phase);
                 const double build = fo_phase[0] + fo_phase[1] + fo_phase[2] + fo_phase[3] - before;
                 fo_flag_s += std::chrono::duration<double>(b - a).count();
                 fo_write_s += std::chrono::duration<double>(clk::now() - b).count() - build;
@@ -715,9 +680,9 @@ void run(const Lattice &L, const InitCond &I, const HydroPar &H, const Extra &X,
     std::printf("Setup: %.3f s (EoS table, device, initial state)\n", setup_s);
     std::printf("Snapshots: %.3f s\n", snap_ms * 1e-3);
     if (H.do_freezeout && dim)
-        std::printf("Freezeout: %.3f s flagging, %.3f s ordering, %.3f s elements, %.3f s interpolating, "
+        std::printf("Freezeout: %.3f s flagging, %.3f s ordering, %.3f s elements and variables (device), "
                     "%.3f s text, %.3f s writing\n",
-                    fo_flag_s, fo_phase[0], fo_phase[1], fo_phase[2], fo_phase[3], fo_write_s);
+                    fo_flag_s, fo_phase[0], fo_phase[1], fo_phase[3], fo_write_s);
     std::printf("Steps taken: %d (t = %.3f)\n", nsteps, t);
     surface.close();
     check("vh_destroy", vh_destroy(h));

@@ -1,0 +1,128 @@
+// Freezeout surface elements on the device: the elements of every cut cube
+// of a coarse step (callFOFinder2p1D / callFOFinder3p1D,
+// freezeout/FreezeOut.cpp:200-361, with the Cornelius construction of
+// vh_surface.hpp, the same source the host C ABI runs) and the ten
+// variables at their centroids (interpolateVariable3D/4D, :182-198).
+//
+// One thread per cut cube.  A first pass counts each cube's elements, a scan
+// of the counts gives every cube its offset, a second pass builds the
+// elements again and writes them, with the interpolated variables, at that
+// offset — so the elements come out compact and in the flagged cubes' order.
+// The hypercube construction keeps the eight cubes' segments in private
+// memory (about 12 KB a thread), hence blocks of 64 threads.
+#include <hipcub/hipcub.hpp>
+
+#include "vh_fo_interp.hpp"
+#include "vh_kernels.hpp"
+#include "vh_surface.hpp"
+
+namespace vh {
+
+namespace {
+
+constexpr int ELEM_BLOCK = 64;
+
+// A flagged cube's corner energies (fo_corners_kernel's layout: previous
+// slice then current, x fastest within a slice) in the element builder's
+// order [tau][x][y][eta].
+__device__ inline void builder_corners(const double *fc, bool d4, double *c) {
+    const int half = d4 ? 8 : 4, nzc = d4 ? 2 : 1;
+    for (int it = 0; it < 2; ++it)
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y)
+                for (int z = 0; z < nzc; ++z) c[((it * 2 + x) * 2 + y) * nzc + z] = fc[it * half + (z * 2 + y) * 2 + x];
+}
+
+__global__ __launch_bounds__(ELEM_BLOCK) void fo_elem_count_kernel(const double *__restrict__ corners, long long n,
+                                                                   bool d4, double eF, FoCell g,
+                                                                   int *__restrict__ counts, int *__restrict__ err) {
+    const long long m = (long long)blockIdx.x * ELEM_BLOCK + threadIdx.x;
+    if (m >= n) return;
+    double c[16];
+    builder_corners(corners + m * (d4 ? 16 : 8), d4, c);
+    int ne = 0;
+    const int r = d4 ? surf::hypercube_elements(c, eF, g.d, nullptr, 0, &ne)
+                     : surf::cube_elements(c, eF, g.d, nullptr, 0, &ne);
+    counts[m] = ne;
+    if (r != surf::SURF_OK) err[0] = 1;  // any failing cube will do: one plain store
+}
+
+__global__ __launch_bounds__(ELEM_BLOCK) void fo_elem_write_kernel(
+    const double *__restrict__ corners, const long long *__restrict__ cubes, long long n, bool d4, double eF, FoCell g,
+    const int *__restrict__ offsets, FoSlices S, int nx, int ny, double *__restrict__ elements,
+    double *__restrict__ vars) {
+    const long long m = (long long)blockIdx.x * ELEM_BLOCK + threadIdx.x;
+    if (m >= n) return;
+    double c[16];
+    builder_corners(corners + m * (d4 ? 16 : 8), d4, c);
+    const int off = offsets[m], cap = offsets[m + 1] - off;
+    if (cap == 0) return;
+    const int per = d4 ? 8 : 6, dim = d4 ? 4 : 3;
+    double *el = elements + (long long)off * per;
+    int ne = 0;
+    if (d4) surf::hypercube_elements(c, eF, g.d, el, cap, &ne);
+    else surf::cube_elements(c, eF, g.d, el, cap, &ne);
+    // the ten variables at each centroid: the cell fraction cen/width, then
+    // the multilinear interpolation of the two coarse slices
+    const long mx = nx - 1, my = ny - 1;
+    const long cid = cubes[m];
+    const long ix = cid % mx, iy = (cid / mx) % my, iz = cid / (mx * my);
+    const long sy = nx, sz = (long)nx * ny;
+    const long base = ix + sy * iy + sz * iz;
+    for (int e = 0; e < ne; ++e) {
+        const double *cen = el + e * per;
+        double f[4] = {0, 0, 0, 0};
+        for (int k = 0; k < dim; ++k) f[k] = cen[k] / g.d[k];
+        double *out = vars + ((long long)off + e) * FO_NVARS;
+        for (int v = 0; v < FO_NVARS; ++v) {
+            const double *p0 = S.v[0][v], *p1 = S.v[1][v];
+            if (d4) {
+                double a[2][2][2][2];
+                for (int x = 0; x < 2; ++x)
+                    for (int y = 0; y < 2; ++y)
+                        for (int z = 0; z < 2; ++z) {
+                            const long s = base + x + sy * y + sz * z;
+                            a[0][x][y][z] = p0[s];
+                            a[1][x][y][z] = p1[s];
+                        }
+                out[v] = fo_lin4(f[0], f[1], f[2], f[3], a);
+            } else {
+                double a[2][2][2];
+                for (int x = 0; x < 2; ++x)
+                    for (int y = 0; y < 2; ++y) {
+                        const long s = base + x + sy * y;
+                        a[0][x][y] = p0[s];
+                        a[1][x][y] = p1[s];
+                    }
+                out[v] = fo_lin3(f[0], f[1], f[2], a);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_fo_elem_count(const double *corners, long long n, bool d4, double eF, const FoCell &g, int *counts,
+                                int *err, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + ELEM_BLOCK - 1) / ELEM_BLOCK);
+    hipLaunchKernelGGL(fo_elem_count_kernel, dim3(grid), dim3(ELEM_BLOCK), 0, st, corners, n, d4, eF, g, counts, err);
+    return hipGetLastError();
+}
+
+hipError_t fo_elem_scan(void *temp, size_t &temp_bytes, const int *counts, int *offsets_plus1, long long n,
+                        hipStream_t st) {
+    return hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, counts, offsets_plus1, (int)n, st);
+}
+
+hipError_t launch_fo_elem_write(const double *corners, const long long *cubes, long long n, bool d4, double eF,
+                                const FoCell &g, const int *offsets, const FoSlices &S, int nx, int ny,
+                                double *elements, double *vars, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + ELEM_BLOCK - 1) / ELEM_BLOCK);
+    hipLaunchKernelGGL(fo_elem_write_kernel, dim3(grid), dim3(ELEM_BLOCK), 0, st, corners, cubes, n, d4, eF, g,
+                       offsets, S, nx, ny, elements, vars);
+    return hipGetLastError();
+}
+
+}  // namespace vh

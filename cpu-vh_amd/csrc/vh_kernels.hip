@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "vh_kernels.hpp"
+#include "vh_fo_interp.hpp"
 
 namespace vh {
 
@@ -164,27 +165,6 @@ __global__ __launch_bounds__(256) void fo_corners_kernel(const double *__restric
                 corners[m * per + q] = eprev[s];
                 corners[m * per + half + q] = ecur[s];
             }
-}
-
-// Multilinear interpolation inside a cell (FreezeOut.cpp:13-56), the terms
-// summed in the reference's order; a[t][x][y][z] corner values.
-__device__ double fo_lin3(double x0, double x1, double x2, const double (&a)[2][2][2]) {
-    return ((1 - x0) * (1 - x1) * (1 - x2) * a[0][0][0]) + ((x0) * (1 - x1) * (1 - x2) * a[1][0][0]) +
-           ((1 - x0) * (x1) * (1 - x2) * a[0][1][0]) + ((1 - x0) * (1 - x1) * (x2) * a[0][0][1]) +
-           ((x0) * (x1) * (1 - x2) * a[1][1][0]) + ((x0) * (1 - x1) * (x2) * a[1][0][1]) +
-           ((1 - x0) * (x1) * (x2) * a[0][1][1]) + ((x0) * (x1) * (x2) * a[1][1][1]);
-}
-__device__ double fo_lin4(double x0, double x1, double x2, double x3, const double (&a)[2][2][2][2]) {
-    return ((1 - x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[0][0][0][0]) +
-           ((x0) * (1 - x1) * (1 - x2) * (1 - x3) * a[1][0][0][0]) +
-           ((1 - x0) * (x1) * (1 - x2) * (1 - x3) * a[0][1][0][0]) +
-           ((1 - x0) * (1 - x1) * (x2) * (1 - x3) * a[0][0][1][0]) +
-           ((1 - x0) * (1 - x1) * (1 - x2) * (x3) * a[0][0][0][1]) + ((x0) * (x1) * (1 - x2) * (1 - x3) * a[1][1][0][0]) +
-           ((x0) * (1 - x1) * (x2) * (1 - x3) * a[1][0][1][0]) + ((x0) * (1 - x1) * (1 - x2) * (x3) * a[1][0][0][1]) +
-           ((1 - x0) * (x1) * (x2) * (1 - x3) * a[0][1][1][0]) + ((1 - x0) * (x1) * (1 - x2) * (x3) * a[0][1][0][1]) +
-           ((1 - x0) * (1 - x1) * (x2) * (x3) * a[0][0][1][1]) + ((x0) * (x1) * (x2) * (1 - x3) * a[1][1][1][0]) +
-           ((x0) * (x1) * (1 - x2) * (x3) * a[1][1][0][1]) + ((x0) * (1 - x1) * (x2) * (x3) * a[1][0][1][1]) +
-           ((1 - x0) * (x1) * (x2) * (x3) * a[0][1][1][1]) + ((x0) * (x1) * (x2) * (x3) * a[1][1][1][1]);
 }
 
 // interpolateVariable3D/4D (FreezeOut.cpp:182-198) of the ten freezeout

@@ -61,6 +61,22 @@ struct FoSlices {
 };
 hipError_t launch_fo_interp(const FoSlices &S, int nx, int ny, int nz, const long long *cubes, const double *frac,
                             long long n, double *vars, hipStream_t st);
+// Surface elements of the flagged cubes (vh_fo_surface.hip): d = the cell's
+// widths (coarse dtau, dx, dy, deta).  Count pass (counts[m], err[0] = 1 when
+// a cube's surface cannot be closed), inclusive scan of the counts into
+// offsets[1 .. n] (hipcub; temp NULL queries temp_bytes), write pass
+// (elements 2*dim doubles and vars FO_NVARS doubles per element, at
+// offsets[m] with offsets[0] = 0).
+struct FoCell {
+    double d[4];
+};
+hipError_t launch_fo_elem_count(const double *corners, long long n, bool d4, double eF, const FoCell &g, int *counts,
+                                int *err, hipStream_t st);
+hipError_t fo_elem_scan(void *temp, size_t &temp_bytes, const int *counts, int *offsets_plus1, long long n,
+                        hipStream_t st);
+hipError_t launch_fo_elem_write(const double *corners, const long long *cubes, long long n, bool d4, double eF,
+                                const FoCell &g, const int *offsets, const FoSlices &S, int nx, int ny,
+                                double *elements, double *vars, hipStream_t st);
 
 // Mixed-precision form (FP32 storage; FP64 root solve, EoS and accumulation).
 namespace mx {

@@ -162,6 +162,26 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
  * for a cube id out of range.  Synchronising. */
 vh_status vh_fo_interp(vh_handle *h, const int64_t *cubes, const double *frac, int64_t n, double *vars);
 
+/* The surface elements of the cubes the last vh_fo_flag call flagged, built
+ * on the device: replaces the reference's per-cube host loop
+ * (callFOFinder2p1D / callFOFinder3p1D, freezeout/FreezeOut.cpp:200-361, with
+ * Cornelius::find_surface_3d/4d, freezeout/cornelius-c++-1.3/cornelius.cpp:
+ * 2192-2287) and the interpolation at the centroids (:248-262, :327-343).
+ * Each element is the one vh_surface_cube (2+1D) / vh_surface_hypercube
+ * (3+1D) builds from the cube's corner energies at that call's level, bit for
+ * bit.  widths: the cell's widths (coarse dtau, dx, dy[, deta]).  For the
+ * flagged cubes in vh_fo_flag's order, n_el[m] receives cube m's element
+ * count; elements (2*dim doubles per element: centroid relative to the
+ * cube's first corner, then the normal, dim = 3 in 2+1D and 4 in 3+1D) and
+ * vars (the ten variables of vh_fo_interp at the centroid, 10 doubles per
+ * element) follow cube by cube in that order.  cap: capacity in elements;
+ * *n_elements receives the total.  VH_ERR_CAPACITY when cap is too small
+ * (n_el and *n_elements are set: call again with room); VH_ERR_STATE for a
+ * cube whose surface cannot be closed; VH_ERR_ARG before the first vh_fo_flag
+ * call.  Synchronising. */
+vh_status vh_fo_surface(vh_handle *h, const double *widths, int *n_el, double *elements, double *vars, int64_t cap,
+                        int64_t *n_elements);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops): vh_fo_flag's
  * crossing test and ordered compaction on caller-owned DEVICE memory,
