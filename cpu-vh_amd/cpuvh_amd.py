@@ -30,7 +30,7 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
-           "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_event_info",
+           "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_fo_surface", "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
            "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
            "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube",
@@ -52,7 +52,8 @@ class VhConfig(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("capacity_cells", ctypes.c_int64), ("max_events", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("bulk", ctypes.c_int), ("eos_kind", ctypes.c_int),
-                ("eos_table", ctypes.c_void_p), ("precision", ctypes.c_int), ("cs2_mode", ctypes.c_int)]
+                ("eos_table", ctypes.c_void_p), ("precision", ctypes.c_int), ("cs2_mode", ctypes.c_int),
+                ("fo_vars", ctypes.c_int)]
 
 
 class EventParams(ctypes.Structure):
@@ -62,6 +63,10 @@ class EventParams(ctypes.Structure):
                 ("T_reg_GeV", ctypes.c_double), ("etas_Tk_GeV", ctypes.c_double),
                 ("etas_slope_lo", ctypes.c_double), ("etas_slope_hi", ctypes.c_double),
                 ("zeta_Tpeak_GeV", ctypes.c_double), ("zeta_peak", ctypes.c_double)]
+
+
+SURFACE_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                    ctypes.POINTER(ctypes.c_double))
 
 
 class ServeConfig(ctypes.Structure):
@@ -74,14 +79,15 @@ class ServeConfig(ctypes.Structure):
                 ("coarse", ctypes.c_int), ("e0", ctypes.c_double), ("e_floor", ctypes.c_double),
                 ("hbm_budget_bytes", ctypes.c_int64), ("precision", ctypes.c_int), ("bulk", ctypes.c_int),
                 ("max_steps", ctypes.c_int), ("max_resident", ctypes.c_int), ("collect_surface", ctypes.c_int),
-                ("device", ctypes.c_int), ("digest", ctypes.c_int)]
+                ("device", ctypes.c_int), ("digest", ctypes.c_int), ("on_surface", SURFACE_CALLBACK),
+                ("user", ctypes.c_void_p)]
 
 
 class ServeRequest(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
                 ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("steps", ctypes.c_int),
                 ("status", ctypes.c_int), ("glauber_seed", ctypes.c_uint64), ("fo_cubes", ctypes.c_int64),
-                ("e_sum", ctypes.c_double), ("digest", ctypes.c_uint64)]
+                ("e_sum", ctypes.c_double), ("digest", ctypes.c_uint64), ("fo_elements", ctypes.c_int64)]
 
 
 class ServeStats(ctypes.Structure):
@@ -171,6 +177,7 @@ def _lib():
         L.vh_batch_run.argtypes = [P, D, I, I, P, P]
         L.vh_batch_download.argtypes = [P, I, I, P]
         L.vh_batch_fo_flag.argtypes = [P, I, D, P, P, I64, ctypes.POINTER(I64)]
+        L.vh_batch_fo_surface.argtypes = [P, I, P, P, P, P, I64, ctypes.POINTER(I64)]
         L.vh_batch_event_info.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
         L.vh_batch_free_cells.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]
         L.vh_batch_last_ms.argtypes = [P]
@@ -376,7 +383,7 @@ class Batch:
     """Many events resident on one GPU (a vh_batch), stepped together."""
 
     def __init__(self, capacity_cells, max_events, max_steps, bulk=False, eos_table=None, device=0,
-                 precision=PREC_FP64, cs2_mode=0):
+                 precision=PREC_FP64, cs2_mode=0, fo_vars=False):
         L = _lib()
         self.nc = 15 if bulk else 14
         self.precision = precision
@@ -384,7 +391,8 @@ class Batch:
         self._table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)
         cfg = BatchConfig(device, capacity_cells, max_events, max_steps, int(bool(bulk)),
                           EOS_CONFORMAL if self._table is None else EOS_TABLE,
-                          None if self._table is None else self._table.ctypes.data, precision, cs2_mode)
+                          None if self._table is None else self._table.ctypes.data, precision, cs2_mode,
+                          int(bool(fo_vars)))
         b = ctypes.c_void_p()
         Hydro._check(L.vh_batch_create(ctypes.byref(cfg), ctypes.byref(b)), "vh_batch_create")
         self.b = b
@@ -459,14 +467,40 @@ class Batch:
         Hydro._check(_lib().vh_batch_download(self.b, eid, f, _ptr(out)), "vh_batch_download")
         return out
 
-    def fo_flag_ids(self, eid, eF):
+    def fo_flag_ids(self, eid, eF, with_corners=False):
         nx, ny, nz = self.events[eid]
         cap = max(1, (nx - 1) * (ny - 1) * (nz - 1 if nz > 1 else 1))
         ids = np.empty(cap, dtype=np.int64)
+        per = 16 if nz > 1 else 8
+        corners = np.empty(cap * per, dtype=np.float64) if with_corners else None
         n = ctypes.c_int64()
-        Hydro._check(_lib().vh_batch_fo_flag(self.b, eid, eF, _ptr(ids), None, cap, ctypes.byref(n)),
+        Hydro._check(_lib().vh_batch_fo_flag(self.b, eid, eF, _ptr(ids), _ptr(corners), cap, ctypes.byref(n)),
                      "vh_batch_fo_flag")
+        self._fo_n = getattr(self, "_fo_n", {})
+        self._fo_n[eid] = n.value
+        if with_corners:
+            return ids[:n.value].copy(), corners[:n.value * per].reshape(n.value, per).copy()
         return ids[:n.value].copy()
+
+    def fo_surface(self, eid, widths):
+        """As Hydro.fo_surface for one event (batch created with fo_vars=True):
+        (counts per flagged cube, elements (n, 2*dim), vars (n, 10))."""
+        nx, ny, nz = self.events[eid]
+        dim = 4 if nz > 1 else 3
+        w = np.ascontiguousarray(widths, dtype=np.float64)
+        nf = getattr(self, "_fo_n", {}).get(eid, 0)
+        counts = np.zeros(max(nf, 1), dtype=np.int32)
+        total = ctypes.c_int64()
+        st = _lib().vh_batch_fo_surface(self.b, eid, _ptr(w), _ptr(counts), None, None, 0, ctypes.byref(total))
+        if st != VH_ERR_CAPACITY:
+            Hydro._check(st, "vh_batch_fo_surface")
+        cap = max(total.value, 1)
+        el = np.empty((cap, 2 * dim), dtype=np.float64)
+        var = np.empty((cap, 10), dtype=np.float64)
+        Hydro._check(_lib().vh_batch_fo_surface(self.b, eid, _ptr(w), _ptr(counts), _ptr(el), _ptr(var), cap,
+                                                ctypes.byref(total)), "vh_batch_fo_surface")
+        n = total.value
+        return counts[:nf].copy(), el[:n].copy(), var[:n].copy()
 
     def info(self, eid):
         a, s, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
@@ -504,14 +538,21 @@ SERVE_DEFAULTS = dict(rate_per_s=10.0, n_requests=50, seed=1, frac_3d=0.2, side2
                       max_steps=3000, max_resident=256, collect_surface=1, device=0, digest=0)
 
 
-def serve(eos_table=None, **kw):
-    """Runs the serving loop (vh_serve_run); returns (stats dict, list of request dicts)."""
+def serve(eos_table=None, on_surface=None, **kw):
+    """Runs the serving loop (vh_serve_run); returns (stats dict, list of
+    request dicts).  on_surface(request, rows): with collect_surface=2, each
+    resident request's surface elements at each check, rows an (n, 20) array
+    in surface.dat's columns."""
     cfg = dict(SERVE_DEFAULTS)
     unknown = set(kw) - set(cfg)
     if unknown:
         raise TypeError(f"unknown serving options {sorted(unknown)}")
     cfg.update(kw)
     c = ServeConfig(**cfg)
+    if on_surface is not None:
+        def _cb(user, request, n, rows):
+            on_surface(request, np.ctypeslib.as_array(rows, shape=(n, 20)).copy())
+        c.on_surface = SURFACE_CALLBACK(_cb)
     st = ServeStats()
     reqs = (ServeRequest * cfg["n_requests"])()
     table = None if eos_table is None else np.ascontiguousarray(eos_table, dtype=np.float64)

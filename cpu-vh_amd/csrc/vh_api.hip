@@ -54,8 +54,7 @@ struct vh_handle {
     // device surface elements and their variables, grown on demand
     long long fo_n = -1;
     double fo_eF = 0;
-    double *fo_el = nullptr;
-    size_t fo_el_bytes = 0;
+    FoSurfaceWork fo_surf;
     // equation of state
     HostEos heos;
     EosDev eos{};
@@ -286,7 +285,7 @@ vh_status vh_destroy(vh_handle *h) {
     hipFree(h->fo_ids);
     hipFree(h->fo_vars);
     hipFree(h->fo_buf);
-    hipFree(h->fo_el);
+    fo_surface_free(h->fo_surf);
     h->deos.release();
     hipFree(h->d_times);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -660,35 +659,22 @@ vh_status vh_fo_surface(vh_handle *h, const double *widths, int *n_el, double *e
     *n_elements = 0;
     if (n == 0) return VH_OK;
     if (!n_el) return VH_ERR_ARG;
-    const bool d4 = c.nz > 1;
-    const int dim = d4 ? 4 : 3, per = d4 ? 16 : 8;
+    const int dim = c.nz > 1 ? 4 : 3;
     FoCell g{};
     for (int k = 0; k < dim; ++k) {
         if (!(widths[k] > 0)) return VH_ERR_ARG;
         g.d[k] = widths[k];
     }
     HIPCHK(hipSetDevice(h->dev));
-    // staging: corners | element counts | offsets (n + 1) | error word | scan scratch
-    size_t temp_bytes = 0;
-    HIPCHK(fo_elem_scan(nullptr, temp_bytes, nullptr, nullptr, n, h->stream));
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t cb = al(sizeof(double) * per * n), nb = al(sizeof(int) * n), ob = al(sizeof(int) * (n + 1)),
-                 eb = 256;
-    HIPCHK(fo_staging(h, cb + nb + ob + eb + temp_bytes));
-    char *base = (char *)h->fo_buf;
-    double *d_corn = (double *)base;
-    int *d_cnt = (int *)(base + cb), *d_off = (int *)(base + cb + nb), *d_bad = (int *)(base + cb + nb + ob);
-    void *d_tmp = base + cb + nb + ob + eb;
-    HIPCHK(launch_fo_corners(h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, h->fo_ids, n, d_corn, h->stream));
-    HIPCHK(hipMemsetAsync(d_off, 0, sizeof(int), h->stream));
-    HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), h->stream));
-    HIPCHK(launch_fo_elem_count(d_corn, n, d4, h->fo_eF, g, d_cnt, d_bad, h->stream));
-    HIPCHK(fo_elem_scan(d_tmp, temp_bytes, d_cnt, d_off + 1, n, h->stream));
-    int total = 0, bad = 0;
-    HIPCHK(hipMemcpyAsync(n_el, d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(&total, d_off + n, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    FoSlices S;
+    for (int v = 0; v < FO_NVARS; ++v) {
+        S.v[1][v] = h->fo_vars + (h->fo_vslot * FO_NVARS + v) * h->N;
+        S.v[0][v] = h->fo_vars + ((h->fo_vslot ^ 1) * FO_NVARS + v) * h->N;
+    }
+    long long total = 0;
+    int bad = 0;
+    HIPCHK(fo_surface_run(h->fo_surf, h->stream, h->fo_prev, h->fo_cur, c.nx, c.ny, c.nz, h->fo_ids, n, h->fo_eF, g,
+                          S, n_el, elements, vars, cap, &total, &bad));
     vh_status s = check_err(h);
     if (s != VH_OK) return s;
     if (bad) {
@@ -696,31 +682,7 @@ vh_status vh_fo_surface(vh_handle *h, const double *widths, int *n_el, double *e
         return VH_ERR_STATE;
     }
     *n_elements = total;
-    if (total > cap) return VH_ERR_CAPACITY;
-    if (total == 0) return VH_OK;
-    const int epd = 2 * dim;
-    const size_t need = sizeof(double) * (size_t)total * (epd + FO_NVARS);
-    if (need > h->fo_el_bytes) {
-        const size_t want = std::max(need, 2 * h->fo_el_bytes);
-        hipFree(h->fo_el);
-        h->fo_el = nullptr;
-        h->fo_el_bytes = 0;
-        HIPCHK(hipMalloc(&h->fo_el, want));
-        h->fo_el_bytes = want;
-    }
-    double *d_el = h->fo_el, *d_vars = h->fo_el + (size_t)total * epd;
-    FoSlices S;
-    for (int v = 0; v < FO_NVARS; ++v) {
-        S.v[1][v] = h->fo_vars + (h->fo_vslot * FO_NVARS + v) * h->N;
-        S.v[0][v] = h->fo_vars + ((h->fo_vslot ^ 1) * FO_NVARS + v) * h->N;
-    }
-    HIPCHK(launch_fo_elem_write(d_corn, h->fo_ids, n, d4, h->fo_eF, g, d_off, S, c.nx, c.ny, d_el, d_vars,
-                                h->stream));
-    HIPCHK(hipMemcpyAsync(elements, d_el, sizeof(double) * (size_t)total * epd, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(vars, d_vars, sizeof(double) * (size_t)total * FO_NVARS, hipMemcpyDeviceToHost,
-                          h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return check_err(h);
+    return total > cap ? VH_ERR_CAPACITY : VH_OK;
 }
 
 double vh_eos_energy_density(const vh_handle *h, double T) { return h ? h->heos.energy_density(T) : NAN; }

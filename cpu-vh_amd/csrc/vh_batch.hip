@@ -47,6 +47,12 @@ struct Event {
     double e_reg = 0;
     double *fo_prev = nullptr, *fo_cur = nullptr;
     long long *fo_ids = nullptr;
+    // with cfg.fo_vars: the ten surface variables' coarse slices, [2][10][N]
+    // (slot fo_vslot current); the last vh_batch_fo_flag's count and level
+    double *fo_vars = nullptr;
+    int fo_vslot = 0;
+    long long fo_n = -1;
+    double fo_eF = 0;
     bool bi = false;  // boost-invariant 2+1D: nz = 1, u^eta = pi^{tau eta} = pi^{x eta} = pi^{y eta} = 0
     bool stopped = false;  // met the stop rule in vh_batch_run; no longer stepped
     long stop_at = 0;      // the batch's step count when it stopped
@@ -69,6 +75,7 @@ struct vh_batch {
     DeviceEos deos;
     int *d_err = nullptr;
     std::vector<Event> ev;
+    FoSurfaceWork fo_surf;  // vh_batch_fo_surface's device buffers
     std::map<long, long> free_seg;  // offset -> length
     EvDesc *d_ev = nullptr;
     std::vector<EvDesc> h_ev;
@@ -303,6 +310,10 @@ vh_status enqueue_batch_step(vh_batch *b, int off) {
 
 // flip: an odd number of steps since the event stopped (its state is in the
 // buffers that are now the previous ones)
+// vh_download's field numbers of the ten surface variables (u^x u^y u^eta e
+// pi^xx pi^xy pi^xeta pi^yy pi^yeta Pi)
+constexpr int FO_SURF_FIELD[FO_NVARS] = {18, 19, 20, 15, 8, 9, 10, 11, 12, 14};
+
 double *field_of(vh_batch *b, int field, bool flip) {
     const int iq = flip ? b->iq_next : b->iq_cur, iu = flip ? b->iu_prev : b->iu_cur;
     const int iup = flip ? b->iu_cur : b->iu_prev;
@@ -417,7 +428,9 @@ vh_status vh_batch_destroy(vh_batch *b) {
         (void)hipFree(E.fo_prev);
         (void)hipFree(E.fo_cur);
         (void)hipFree(E.fo_ids);
+        (void)hipFree(E.fo_vars);
     }
+    fo_surface_free(b->fo_surf);
     (void)hipFree(b->pool);
     (void)hipFree(b->fpool);
     (void)hipFree(b->stage_buf);
@@ -582,6 +595,7 @@ vh_status vh_batch_remove(vh_batch *b, int id) {
     (void)hipFree(E.fo_prev);
     (void)hipFree(E.fo_cur);
     (void)hipFree(E.fo_ids);
+    (void)hipFree(E.fo_vars);
     E = Event();
     b->h_ev[id].active = 0;
     BCHK(hipMemcpy(b->d_ev + id, &b->h_ev[id], sizeof(EvDesc), hipMemcpyHostToDevice));
@@ -869,6 +883,31 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     std::swap(E.fo_prev, E.fo_cur);
     if (b->mixed) BCHK(launch_widen(b->ef[0] + E.off, nullptr, E.fo_cur, E.N, b->stream));
     else BCHK(hipMemcpyAsync(E.fo_cur, b->e[0] + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
+    if (b->cfg.fo_vars) {
+        // the ten surface variables move with e (the eta components of a
+        // boost-invariant event and Pi without bulk are zero)
+        if (!E.fo_vars) {
+            BCHK(hipMalloc(&E.fo_vars, 2 * FO_NVARS * bytes));
+            BCHK(launch_fill(E.fo_vars, 2 * FO_NVARS * E.N, 0.0, b->stream));
+        }
+        E.fo_vslot ^= 1;
+        const bool flip = E.stopped && ((b->step_count - E.stop_at) & 1);
+        for (int v = 0; v < FO_NVARS; ++v) {
+            double *dst = E.fo_vars + (E.fo_vslot * FO_NVARS + v) * E.N;
+            const int f = FO_SURF_FIELD[v];
+            if (f >= b->nc && f < 15) {
+                BCHK(launch_fill(dst, E.N, 0.0, b->stream));
+            } else if (!b->mixed) {
+                BCHK(hipMemcpyAsync(dst, field_of(b, f, flip) + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
+            } else if (E.bi && (f == 10 || f == 12 || f == 20)) {
+                BCHK(launch_fill(dst, E.N, 0.0, b->stream));
+            } else {
+                const float *hi = nullptr, *lo = nullptr;
+                if (!field_of_mixed(b, f, flip, &hi, &lo)) return VH_ERR_STATE;
+                BCHK(launch_widen(hi + E.off, lo ? lo + E.off : nullptr, dst, E.N, b->stream));
+            }
+        }
+    }
     BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
     BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts,
                         (unsigned long long *)(E.fo_ids + std::max(1L, M)), b->stream));
@@ -877,6 +916,8 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     vh_status s = vh_batch_sync(b);
     if (s != VH_OK) return s;
     *n_flagged = (int64_t)n;
+    E.fo_n = (long long)n;
+    E.fo_eF = eF;
     if ((int64_t)n > cap) return VH_ERR_CAPACITY;
     if (n == 0) return VH_OK;
     // the ids come out in increasing order
@@ -892,6 +933,47 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
         if (le != hipSuccess) return bfail(le, "fo corners");
     }
     return VH_OK;
+}
+
+vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_el, double *elements, double *vars,
+                              int64_t cap, int64_t *n_elements) {
+    if (!b || !widths || !n_elements || cap < 0 || (cap > 0 && (!elements || !vars)) || id < 0 ||
+        id >= (int)b->ev.size() || !b->ev[id].used)
+        return VH_ERR_ARG;
+    Event &E = b->ev[id];
+    const vh_event_params &c = E.prm;
+    if (!b->cfg.fo_vars || E.fo_n < 0) {
+        vh::set_last_error("vh_batch_fo_surface: needs fo_vars in the batch's configuration and a vh_batch_fo_flag "
+                           "call for the event");
+        return VH_ERR_ARG;
+    }
+    *n_elements = 0;
+    if (E.fo_n == 0) return VH_OK;
+    if (!n_el) return VH_ERR_ARG;
+    const int dim = c.nz > 1 ? 4 : 3;
+    FoCell g{};
+    for (int k = 0; k < dim; ++k) {
+        if (!(widths[k] > 0)) return VH_ERR_ARG;
+        g.d[k] = widths[k];
+    }
+    BCHK(hipSetDevice(b->dev));
+    FoSlices S;
+    for (int v = 0; v < FO_NVARS; ++v) {
+        S.v[1][v] = E.fo_vars + (E.fo_vslot * FO_NVARS + v) * E.N;
+        S.v[0][v] = E.fo_vars + ((E.fo_vslot ^ 1) * FO_NVARS + v) * E.N;
+    }
+    long long total = 0;
+    int bad = 0;
+    BCHK(fo_surface_run(b->fo_surf, b->stream, E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, E.fo_ids, E.fo_n, E.fo_eF, g, S,
+                        n_el, elements, vars, cap, &total, &bad));
+    vh_status s = vh_batch_sync(b);
+    if (s != VH_OK) return s;
+    if (bad) {
+        vh::set_last_error("vh_batch_fo_surface: a cut cube whose surface cannot be closed");
+        return VH_ERR_STATE;
+    }
+    *n_elements = total;
+    return total > cap ? VH_ERR_CAPACITY : VH_OK;
 }
 
 double vh_batch_eos_energy_density(const vh_batch *b, double T) { return b ? b->heos.energy_density(T) : NAN; }

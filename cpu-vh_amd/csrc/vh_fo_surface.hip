@@ -10,6 +10,8 @@
 // offset — so the elements come out compact and in the flagged cubes' order.
 // The hypercube construction keeps the eight cubes' segments in private
 // memory (about 12 KB a thread), hence blocks of 64 threads.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "vh_fo_interp.hpp"
@@ -101,6 +103,75 @@ __global__ __launch_bounds__(ELEM_BLOCK) void fo_elem_write_kernel(
 }
 
 }  // namespace
+
+// Device memory grown on demand (the stream is drained before a buffer is
+// replaced).
+static hipError_t grow(void **buf, size_t *have, size_t need, hipStream_t st) {
+    if (need <= *have) return hipSuccess;
+    hipError_t r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return r;
+    const size_t want = std::max(need, 2 * *have);
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if ((r = hipMalloc(buf, want)) != hipSuccess) return r;
+    *have = want;
+    return hipSuccess;
+}
+
+hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev, const double *ecur, int nx, int ny,
+                          int nz, const long long *ids, long long n, double eF, const FoCell &g, const FoSlices &S,
+                          int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad) {
+    *total = 0;
+    *bad = 0;
+    if (n <= 0) return hipSuccess;
+    const bool d4 = nz > 1;
+    const int dim = d4 ? 4 : 3, per = d4 ? 16 : 8;
+    // staging: corners | element counts | offsets (n + 1) | error word | scan scratch
+    size_t temp_bytes = 0;
+    hipError_t r = fo_elem_scan(nullptr, temp_bytes, nullptr, nullptr, n, st);
+    if (r != hipSuccess) return r;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t cb = al(sizeof(double) * per * n), nb = al(sizeof(int) * n), ob = al(sizeof(int) * (n + 1)),
+                 eb = 256;
+    if ((r = grow(&W.buf, &W.buf_bytes, cb + nb + ob + eb + temp_bytes, st)) != hipSuccess) return r;
+    char *base = (char *)W.buf;
+    double *d_corn = (double *)base;
+    int *d_cnt = (int *)(base + cb), *d_off = (int *)(base + cb + nb), *d_bad = (int *)(base + cb + nb + ob);
+    void *d_tmp = base + cb + nb + ob + eb;
+    int h_total = 0;
+#define FOCHK(call)                          \
+    do {                                     \
+        if ((r = (call)) != hipSuccess) return r; \
+    } while (0)
+    FOCHK(launch_fo_corners(eprev, ecur, nx, ny, nz, ids, n, d_corn, st));
+    FOCHK(hipMemsetAsync(d_off, 0, sizeof(int), st));
+    FOCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+    FOCHK(launch_fo_elem_count(d_corn, n, d4, eF, g, d_cnt, d_bad, st));
+    FOCHK(fo_elem_scan(d_tmp, temp_bytes, d_cnt, d_off + 1, n, st));
+    FOCHK(hipMemcpyAsync(n_el, d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    FOCHK(hipMemcpyAsync(&h_total, d_off + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    FOCHK(hipMemcpyAsync(bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    FOCHK(hipStreamSynchronize(st));
+    *total = h_total;
+    if (*bad || h_total == 0 || h_total > cap) return hipSuccess;
+    const int epd = 2 * dim;
+    const size_t need = sizeof(double) * (size_t)h_total * (epd + FO_NVARS);
+    FOCHK(grow((void **)&W.el, &W.el_bytes, need, st));
+    double *d_el = W.el, *d_vars = W.el + (size_t)h_total * epd;
+    FOCHK(launch_fo_elem_write(d_corn, ids, n, d4, eF, g, d_off, S, nx, ny, d_el, d_vars, st));
+    FOCHK(hipMemcpyAsync(elements, d_el, sizeof(double) * (size_t)h_total * epd, hipMemcpyDeviceToHost, st));
+    FOCHK(hipMemcpyAsync(vars, d_vars, sizeof(double) * (size_t)h_total * FO_NVARS, hipMemcpyDeviceToHost, st));
+    FOCHK(hipStreamSynchronize(st));
+#undef FOCHK
+    return hipSuccess;
+}
+
+void fo_surface_free(FoSurfaceWork &W) {
+    (void)hipFree(W.buf);
+    (void)hipFree(W.el);
+    W = FoSurfaceWork{};
+}
 
 hipError_t launch_fo_elem_count(const double *corners, long long n, bool d4, double eF, const FoCell &g, int *counts,
                                 int *err, hipStream_t st) {

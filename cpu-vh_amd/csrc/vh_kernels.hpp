@@ -77,6 +77,20 @@ hipError_t fo_elem_scan(void *temp, size_t &temp_bytes, const int *counts, int *
 hipError_t launch_fo_elem_write(const double *corners, const long long *cubes, long long n, bool d4, double eF,
                                 const FoCell &g, const int *offsets, const FoSlices &S, int nx, int ny,
                                 double *elements, double *vars, hipStream_t st);
+// The whole surface of one coarse step for the n flagged cubes ids (device):
+// corners, counts, scan, then, when there is room (total <= cap) and no cube
+// failed (*bad), the elements and variables copied to the host arrays n_el
+// (n counts), elements and vars.  Synchronising.  W's buffers grow on demand.
+struct FoSurfaceWork {
+    void *buf = nullptr;
+    size_t buf_bytes = 0;
+    double *el = nullptr;
+    size_t el_bytes = 0;
+};
+hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev, const double *ecur, int nx, int ny,
+                          int nz, const long long *ids, long long n, double eF, const FoCell &g, const FoSlices &S,
+                          int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad);
+void fo_surface_free(FoSurfaceWork &W);
 
 // Mixed-precision form (FP32 storage; FP64 root solve, EoS and accumulation).
 namespace mx {

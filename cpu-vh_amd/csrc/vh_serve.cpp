@@ -10,6 +10,7 @@
 #include <cstring>
 #include <deque>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/cpuvh_amd.h"
@@ -53,6 +54,70 @@ uint64_t state_digest(vh_batch *b, int id, int nc, int64_t cells, std::vector<do
     return h;
 }
 
+// One request's surface at a check (vh_batch_fo_surface), as surface.dat's
+// 20 columns per element (FreezeOut.cpp:200-361): cubes in the reference's
+// order (x outer, eta inner), positions from the cube's origin, dsigma =
+// t * normal, T and P from the host EoS at the interpolated e.
+vh_status surface_rows(vh_batch *b, int id, const vh_serve_request &r, const vh_serve_config &c,
+                       const vh::HostEos &heos, const std::vector<int64_t> &ids, int64_t nf, std::vector<int> &ne,
+                       std::vector<double> &el, std::vector<double> &var, std::vector<double> &rows,
+                       std::vector<std::pair<int64_t, int64_t>> &order) {
+    const bool d4 = r.nz > 1;
+    const int dim = d4 ? 4 : 3, per = 2 * dim;
+    int steps = 0;
+    vh_batch_event_info(b, id, nullptr, &steps, nullptr);
+    const double t = c.t0 + steps * c.dt, cell_tau = t + c.dt;
+    const double widths[4] = {c.dt * c.coarse, c.dx, c.dx, c.dx};
+    ne.resize(nf);
+    int64_t total = 0;
+    vh_status s = vh_batch_fo_surface(b, id, widths, ne.data(), el.data(), var.data(), (int64_t)(var.size() / 10),
+                                      &total);
+    if (s == VH_ERR_CAPACITY) {
+        el.resize((size_t)total * per);
+        var.resize((size_t)total * 10);
+        s = vh_batch_fo_surface(b, id, widths, ne.data(), el.data(), var.data(), total, &total);
+    }
+    if (s != VH_OK) return s;
+    std::vector<int64_t> off(nf + 1, 0);
+    for (int64_t m = 0; m < nf; ++m) off[m + 1] = off[m] + ne[m];
+    const int64_t cx = r.nx - 1, cy = r.ny - 1, cz = d4 ? r.nz - 1 : 1;
+    order.resize(nf);
+    for (int64_t m = 0; m < nf; ++m) {
+        const int64_t q = ids[m], ix = q % cx, iy = (q / cx) % cy, iz = q / (cx * cy);
+        order[m] = {(ix * cy + iy) * cz + iz, m};
+    }
+    std::sort(order.begin(), order.end());
+    rows.resize((size_t)total * 20);
+    size_t k = 0;
+    for (const auto &om : order) {
+        const int64_t m = om.second, q = ids[m];
+        const int ix = (int)(q % cx), iy = (int)((q / cx) % cy), iz = d4 ? (int)(q / (cx * cy)) : 0;
+        const double cell_x = (double)ix * c.dx - (((double)(r.nx - 1)) / 2.0 * c.dx);
+        const double cell_y = (double)iy * c.dx - (((double)(r.ny - 1)) / 2.0 * c.dx);
+        const double cell_z = d4 ? (double)iz * c.dx - (((double)(r.nz - 1)) / 2.0 * c.dx) : 0.0;
+        for (int64_t e = off[m]; e < off[m + 1]; ++e, ++k) {
+            const double *cen = el.data() + (size_t)e * per, *nor = cen + dim, *v = var.data() + (size_t)e * 10;
+            double *o = rows.data() + k * 20;
+            o[0] = cen[0] + cell_tau;
+            o[1] = cen[1] + cell_x;
+            o[2] = cen[2] + cell_y;
+            o[3] = d4 ? cen[3] + cell_z : 0.0;
+            o[4] = t * nor[0];
+            o[5] = t * nor[1];
+            o[6] = t * nor[2];
+            o[7] = d4 ? t * nor[3] : 0.0;
+            o[8] = v[0];
+            o[9] = v[1];
+            o[10] = v[2];
+            o[11] = v[3];
+            o[12] = heos.temperature(v[3]);
+            o[13] = heos.pressure(v[3]);
+            for (int j = 0; j < 6; ++j) o[14 + j] = v[4 + j];
+        }
+    }
+    return VH_OK;
+}
+
 double quantile(std::vector<double> v, double q) {  // nearest rank
     if (v.empty()) return 0.0;
     std::sort(v.begin(), v.end());
@@ -72,9 +137,13 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
         c.nz3_min < 2 || c.nz3_max < c.nz3_min || !(c.dx > 0) || !(c.dt > 0) || !(c.t0 > 0) ||
         c.hbm_budget_bytes <= 0)
         return VH_ERR_ARG;
+    if (c.collect_surface < 0 || c.collect_surface > 2) return VH_ERR_ARG;
     const int n = c.n_requests;
     const bool mixed = c.precision == VH_PREC_MIXED;
-    const int64_t bytes_per_cell = mixed ? (3 * 15 + 3 * 4 + 3 * 4 + 4) * 4 : (3 * 15 + 3 * 4 + 4) * 8;
+    // resident state, plus the freezeout slices: e's two, and the ten surface
+    // variables' two when elements are built
+    const int64_t bytes_per_cell = (mixed ? (3 * 15 + 3 * 4 + 3 * 4 + 4) * 4 : (3 * 15 + 3 * 4 + 4) * 8) +
+                                   (c.collect_surface ? 2 * 8 : 0) + (c.collect_surface == 2 ? 20 * 8 : 0);
 
     // requests and their payload (generated before the clock starts)
     Rng rng{c.seed};
@@ -116,6 +185,7 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
     bc.eos_kind = eos_table ? VH_EOS_TABLE : VH_EOS_CONFORMAL;
     bc.eos_table = eos_table;
     bc.precision = c.precision;
+    bc.fo_vars = c.collect_surface == 2;
     vh_batch *b = nullptr;
     vh_status st = vh_batch_create(&bc, &b);
     if (st != VH_OK) return st;
@@ -124,6 +194,12 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
     std::vector<int> owner(c.max_resident, -1), strikes(c.max_resident, 0);
     std::vector<int64_t> counts(c.max_resident);
     std::vector<double> ebuf(nmax);
+    // surface elements of one request's check: cube ids, counts, elements,
+    // variables, and the rows handed to on_surface
+    std::vector<int64_t> fo_ids;
+    std::vector<int> fo_ne;
+    std::vector<double> fo_el, fo_var, fo_rows;
+    std::vector<std::pair<int64_t, int64_t>> fo_order;
     std::deque<int> queue;
     int next = 0, finished = 0, resident = 0;
     int64_t resident_cells = 0;
@@ -218,13 +294,27 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
         for (int id = 0; id < c.max_resident; ++id) {
             if (owner[id] < 0) continue;
             if (c.collect_surface) {
+                vh_serve_request &r = req[owner[id]];
+                const int64_t ncube = (int64_t)(r.nx - 1) * (r.ny - 1) * (r.nz > 1 ? r.nz - 1 : 1);
+                if (c.collect_surface == 2 && (int64_t)fo_ids.size() < ncube) fo_ids.resize(ncube);
                 int64_t nflag = 0;
-                vh_status fs = vh_batch_fo_flag(b, id, eF, nullptr, nullptr, 0, &nflag);
+                vh_status fs = vh_batch_fo_flag(b, id, eF, c.collect_surface == 2 ? fo_ids.data() : nullptr, nullptr,
+                                                c.collect_surface == 2 ? ncube : 0, &nflag);
                 if (fs != VH_OK && fs != VH_ERR_CAPACITY) {
                     st = fs;
                     break;
                 }
-                req[owner[id]].fo_cubes += nflag;
+                r.fo_cubes += nflag;
+                if (c.collect_surface == 2 && nflag > 0) {
+                    fs = surface_rows(b, id, r, c, heos, fo_ids, nflag, fo_ne, fo_el, fo_var, fo_rows, fo_order);
+                    if (fs != VH_OK) {
+                        st = fs;
+                        break;
+                    }
+                    const int64_t ne = (int64_t)(fo_rows.size() / 20);
+                    r.fo_elements += ne;
+                    if (c.on_surface && ne) c.on_surface(c.user, owner[id], ne, fo_rows.data());
+                }
             }
             if (counts[id] == 0) ++strikes[id];
             int active = 0, steps = 0;

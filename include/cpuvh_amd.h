@@ -359,6 +359,8 @@ typedef struct {
     const double *eos_table;   /* as vh_config.eos_table */
     int precision;             /* vh_precision */
     int cs2_mode;              /* as vh_config.cs2_mode */
+    int fo_vars;               /* 1: vh_batch_fo_flag also keeps each event's previous and current slices of
+                                  the ten surface variables (20 doubles a cell), for vh_batch_fo_surface */
 } vh_batch_config;
 
 /* One event's parameters: vh_config's per-event fields. */
@@ -410,6 +412,11 @@ vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t 
 vh_status vh_batch_download(vh_batch *b, int id, int field, double *out);
 vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
                            int64_t *n_flagged);
+/* As vh_fo_surface for one event of a batch created with fo_vars = 1, on the
+ * cubes its last vh_batch_fo_flag call flagged (mixed-precision events'
+ * fields widened to FP64 first). */
+vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_el, double *elements, double *vars,
+                              int64_t cap, int64_t *n_elements);
 vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells);
 vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest_range);
 /* Mixed form: times stage launches of each tile variant (reps launches each)
@@ -458,9 +465,20 @@ typedef struct {
     int bulk;
     int max_steps;             /* per request */
     int max_resident;          /* event slots */
-    int collect_surface;       /* 1: flag freezeout cubes of every resident request at every check */
+    int collect_surface;       /* 1: flag freezeout cubes of every resident request at every check;
+                                  2: also build their surface elements on the device (vh_batch_fo_surface)
+                                  and hand each request's to on_surface */
     int device;
     int digest;                /* 1: a completed request's final state is downloaded and hashed (digest) */
+    /* with collect_surface 2 (may be NULL): called for each resident request
+     * at each check with that check's n surface elements, 20 doubles each in
+     * surface.dat's columns (FreezeOut.cpp:243-270: tau x y eta, dsigma_mu
+     * = t * normal, u^x u^y u^eta e T P pi^xx pi^xy pi^xeta pi^yy pi^yeta
+     * Pi), cubes in the reference's order.  t is the request's state time
+     * t0 + steps * dt and an element's tau is measured from t + dt, as the
+     * reference's are.  The rows are valid during the call only. */
+    void (*on_surface)(void *user, int request, int64_t n, const double *rows);
+    void *user;
 } vh_serve_config;
 
 typedef struct {
@@ -475,6 +493,7 @@ typedef struct {
                                          * h = 14695981039346656037; for each field f of vh_download's
                                          * numbering 0..nc-1, 15, 16, 17..20: h = h * 1099511628211 +
                                          * sum_i bits(x_i) * (2i + 1), all mod 2^64 */
+    int64_t fo_elements;                /* with collect_surface 2: surface elements, summed over checks */
 } vh_serve_request;
 
 typedef struct {

@@ -62,3 +62,118 @@ def test_device_elements_equal_host(nx, ny, nz):
         print(f"coarse step {k}: {ids.size} cubes, {el.shape[0]} elements", flush=True)
     assert total_el > 100
     h.close()
+
+
+def _event_ic(nx, ny, nz, seed):
+    return V.random_ic(nx, ny, nz, 0.1, 0.6, seed=seed, lattice=True, bulk=True, e_scale=6.0, e_floor=0.3)
+
+
+BATCH_EVENTS = [(27, 25, 1), (19, 17, 9), (23, 21, 1)]
+
+
+def test_batch_surface_equals_single_event():
+    """An FP64 batch's events (neighbours in one pool) build the same surface,
+    bit for bit, as the same events alone through the single-event path."""
+    table = V.eos_table()
+    ics = [_event_ic(nx, ny, nz, 30 + k) for k, (nx, ny, nz) in enumerate(BATCH_EVENTS)]
+    b = C.Batch(sum(nx * ny * nz for nx, ny, nz in BATCH_EVENTS), len(BATCH_EVENTS), 64, bulk=True, eos_table=table,
+                fo_vars=True)
+    ids = [b.add(nx, ny, nz, 0.1, 0.1, 0.1, 0.02, 0.6, 0.1, ic, T_reg_GeV=0.1)
+           for (nx, ny, nz), ic in zip(BATCH_EVENTS, ics)]
+    hs = []
+    for (nx, ny, nz), ic in zip(BATCH_EVENTS, ics):
+        h = C.Hydro(nx, ny, nz, 0.1, 0.1, 0.1, 0.02, 0.6, 0.1, T_reg_GeV=0.1, bulk=True, eos_table=table)
+        h.upload_ic(ic["e"], ic["p"], ic["ut"], ic["ux"], ic["uy"], ic["un"], ic["pi"], ic["Pi"])
+        h.init_conserved(0.6)
+        hs.append(h)
+    eF = hs[0].eos_energy_density(0.15 / V.HBARC)
+    n_step = 1
+    for k in range(3):
+        if k:
+            b.steps(2)
+            for h in hs:
+                h.steps(n_step, 2)
+            n_step += 2
+        for (nx, ny, nz), eid, h in zip(BATCH_EVENTS, ids, hs):
+            print(f"case batch surface {nx}x{ny}x{nz} check {k}", flush=True)
+            widths = np.array([0.04, 0.1, 0.1, 0.1])[:4 if nz > 1 else 3]
+            assert np.array_equal(b.fo_flag_ids(eid, eF), h.fo_flag_ids(eF))
+            cb, eb, vb = b.fo_surface(eid, widths)
+            ch, eh, vh = h.fo_surface(widths)
+            assert np.array_equal(cb, ch) and V.bits_equal(eb, eh) and V.bits_equal(vb, vh)
+    b.close()
+    for h in hs:
+        h.close()
+
+
+def test_mixed_batch_elements_from_its_corners():
+    """Mixed-precision events: the device's elements are the host
+    construction's on the corner energies the batch flags with (its FP32 e
+    widened), and the interpolated e at each centroid lies between the
+    cube's smallest and largest corner."""
+    table = V.eos_table()
+    ics = [_event_ic(nx, ny, nz, 50 + k) for k, (nx, ny, nz) in enumerate(BATCH_EVENTS)]
+    b = C.Batch(sum(nx * ny * nz for nx, ny, nz in BATCH_EVENTS), len(BATCH_EVENTS), 64, bulk=True, eos_table=table,
+                precision=C.PREC_MIXED, fo_vars=True)
+    ids = [b.add(nx, ny, nz, 0.1, 0.1, 0.1, 0.02, 0.6, 0.1, ic, T_reg_GeV=0.1)
+           for (nx, ny, nz), ic in zip(BATCH_EVENTS, ics)]
+    eF = b.eos_energy_density(0.15 / V.HBARC)
+    for k in range(3):
+        if k:
+            b.steps(2)
+        for (nx, ny, nz), eid in zip(BATCH_EVENTS, ids):
+            print(f"case mixed surface {nx}x{ny}x{nz} check {k}", flush=True)
+            d4 = nz > 1
+            widths = np.array([0.04, 0.1, 0.1, 0.1])[:4 if d4 else 3]
+            cube_ids, corners = b.fo_flag_ids(eid, eF, with_corners=True)
+            counts, el, var = b.fo_surface(eid, widths)
+            rows = [C.surface_elements(_builder_order(corners[m], d4), eF, widths) for m in range(cube_ids.size)]
+            assert np.array_equal(counts, [r.shape[0] for r in rows])
+            want = np.concatenate(rows) if rows else np.empty((0, 2 * (4 if d4 else 3)))
+            assert V.bits_equal(el, want)
+            lo = np.repeat(corners.min(axis=1), counts)
+            hi = np.repeat(corners.max(axis=1), counts)
+            assert np.all((var[:, 3] >= lo - 1e-12 * hi) & (var[:, 3] <= hi * (1 + 1e-12)))
+    b.close()
+
+
+SERVE_CFG = dict(rate_per_s=4000.0, n_requests=6, seed=11, frac_3d=0.34, side2_min=31, side2_max=41, side3_min=17,
+                 side3_max=21, nz3_min=9, nz3_max=11, dx=0.4, dt=0.05, t0=0.6, etabar=0.12, T_reg_GeV=0.1,
+                 T_F_GeV=0.150, coarse=4, e0=10.0, e_floor=0.01, precision=C.PREC_MIXED, bulk=1, max_steps=400,
+                 max_resident=8, collect_surface=2)
+
+
+def _served_surfaces(budget):
+    got = {}
+
+    def on_surface(request, rows):
+        got.setdefault(request, []).append(rows)
+
+    stats, reqs = C.serve(None, on_surface=on_surface, hbm_budget_bytes=budget, **SERVE_CFG)
+    return stats, reqs, got
+
+
+def test_served_surfaces_same_alone_and_in_the_mix():
+    """Serving with surface collection: each request's surface elements (in
+    surface.dat's columns, handed out at every check) are the same, bit for
+    bit, whether all requests share the batch at once or wait for the memory
+    of earlier ones (room for about one 3+1D request); fo_elements counts
+    them."""
+    per_cell = (3 * 15 + 3 * 4 + 3 * 4 + 4) * 4 + 2 * 8 + 20 * 8
+    big = 21 * 21 * 11
+    s_mix, r_mix, g_mix = _served_surfaces(64 * big * per_cell)
+    s_one, r_one, g_one = _served_surfaces(int(1.05 * big) * per_cell)
+    print(s_mix, s_one, flush=True)
+    assert s_mix["completed"] == s_one["completed"] >= SERVE_CFG["n_requests"] - 1
+    assert max(r["admit_s"] - r["arrival_s"] for r in r_one) > max(r["admit_s"] - r["arrival_s"] for r in r_mix)
+    for k, (a, c) in enumerate(zip(r_mix, r_one)):
+        if a["status"] != C.VH_OK:
+            continue
+        ra, rc = g_mix.get(k, []), g_one.get(k, [])
+        assert len(ra) == len(rc) and len(ra) > 0, k
+        for x, y in zip(ra, rc):
+            assert V.bits_equal(x, y), k
+        n = sum(x.shape[0] for x in ra)
+        assert a["fo_elements"] == c["fo_elements"] == n > 0
+        rows = np.concatenate(ra)
+        assert np.all(np.isfinite(rows))
