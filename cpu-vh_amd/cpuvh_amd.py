@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libcpuvh_amd.so")
 
 FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn",
                "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
-               "up_t", "up_x", "up_y", "up_n", "regfac", "regfac_bulk"]
+               "up_t", "up_x", "up_y", "up_n", "regfac", "regfac_bulk",
+               "wtx", "wty", "wtn", "wxy", "wxn", "wyn", "beta_t", "beta_x", "beta_y", "beta_n"]
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_NOMEM, VH_ERR_NAN, VH_ERR_STATE, VH_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
@@ -26,7 +27,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_step", "vh_steps", "vh_sync",
            "vh_nan_info", "vh_download", "vh_upload_field", "vh_count_above", "vh_fo_flag", "vh_fo_interp",
            "vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature", "vh_last_kernel_ms",
-           "vh_inferred_variables", "vh_fo_flag_noalloc", "vh_fo_surface",
+           "vh_inferred_variables", "vh_fo_flag_noalloc", "vh_fo_surface", "vh_thermal_vorticity",
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
@@ -139,6 +140,7 @@ def _lib():
         L.vh_fo_flag.argtypes = [P, D, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         L.vh_fo_interp.argtypes = [P, P, P, ctypes.c_int64, P]
         L.vh_fo_surface.argtypes = [P, P, P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.vh_thermal_vorticity.argtypes = [P]
         for f in ("vh_eos_energy_density", "vh_eos_pressure", "vh_eos_temperature"):
             getattr(L, f).argtypes = [P, D]
             getattr(L, f).restype = D
@@ -325,6 +327,13 @@ class Hydro:
         out = np.empty((max(cubes.shape[0], 1), 10), dtype=np.float64)
         self._check(_lib().vh_fo_interp(self.h, _ptr(cubes), _ptr(frac), cubes.shape[0], _ptr(out)), "vh_fo_interp")
         return out[:cubes.shape[0]].copy()
+
+    def thermal_vorticity(self):
+        """calculateThermalVorticity on the current state (vh_thermal_vorticity):
+        (10, N) wtx wty wtn wxy wxn wyn beta_t beta_x beta_y beta_n."""
+        self._check(_lib().vh_thermal_vorticity(self.h), "vh_thermal_vorticity")
+        return np.stack([self.get(f) for f in ("wtx", "wty", "wtn", "wxy", "wxn", "wyn", "beta_t", "beta_x",
+                                               "beta_y", "beta_n")])
 
     def fo_surface(self, widths):
         """The surface elements of the cubes the last fo_flag_ids call flagged,

@@ -55,6 +55,8 @@ struct vh_handle {
     long long fo_n = -1;
     double fo_eF = 0;
     FoSurfaceWork fo_surf;
+    // thermal vorticity and velocity, 10 x N, from the last vh_thermal_vorticity
+    double *vort = nullptr;
     // equation of state
     HostEos heos;
     EosDev eos{};
@@ -286,6 +288,7 @@ vh_status vh_destroy(vh_handle *h) {
     hipFree(h->fo_vars);
     hipFree(h->fo_buf);
     fo_surface_free(h->fo_surf);
+    hipFree(h->vort);
     h->deos.release();
     hipFree(h->d_times);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -491,6 +494,7 @@ static double *field_ptr(vh_handle *h, int field) {
     if (field >= 21 && field <= 24) return h->u[h->iu_prev][field - 21];
     if (field == 25) return h->regfac;
     if (field == 26) return h->nc == 15 ? h->regfacb : nullptr;
+    if (field >= 27 && field <= 36) return h->vort ? h->vort + (field - 27) * h->N : nullptr;
     return nullptr;
 }
 
@@ -514,6 +518,29 @@ vh_status vh_download(vh_handle *h, int field, double *out) {
     if (!src) return VH_ERR_ARG;
     HIPCHK(hipMemcpyAsync(out, src, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->stream));
     return check_err(h);
+}
+
+vh_status vh_thermal_vorticity(vh_handle *h) {
+    if (!h) return VH_ERR_ARG;
+    const vh_config &c = h->cfg;
+    HIPCHK(hipSetDevice(h->dev));
+    if (!h->vort) HIPCHK(hipMalloc(&h->vort, sizeof(double) * 10 * h->N));
+    VortArgs A{};
+    for (int m = 0; m < 4; ++m) {
+        A.u[m] = h->u[h->iu_cur][m];
+        A.up[m] = h->u[h->iu_prev][m];
+    }
+    A.e = h->e[0];
+    A.out = h->vort;
+    A.nx = c.nx;
+    A.ny = c.ny;
+    A.nz = c.nz;
+    A.dt = c.dt;
+    A.dx = c.dx;
+    A.dy = c.dy;
+    A.dz = c.dz;
+    HIPCHK(launch_vorticity(A, h->eos, h->stream));
+    return VH_OK;
 }
 
 vh_status vh_upload_field(vh_handle *h, int field, const double *in) {

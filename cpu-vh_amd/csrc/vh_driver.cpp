@@ -415,22 +415,28 @@ void write_snapshot(const std::vector<double> &v, double t, const std::string &d
     std::fclose(fp);
 }
 
-// outputDynamicalQuantities, rhic/src/HydroPlugin.cpp:40-72.  Before the
-// first step the regulation factors are the reference's calloc'd zeros.
+// outputDynamicalQuantities, rhic/src/HydroPlugin.cpp:40-72, with the
+// thermal vorticity the reference computes after every step
+// (THERMAL_VORTICITY is on in its default build, DynamicalVariables.h:15):
+// wxy, beta_x and beta_y of the current state.  Before the first step the
+// regulation factors and the vorticity are the reference's calloc'd zeros.
 void snapshot(vh_handle *h, double t, const std::string &dir, const Lattice &L, int bulk, bool before_first_step,
               std::vector<double> &buf) {
     struct F { const char *name; int id; };
     const F shear[] = {{"e", 15}, {"ux", 18}, {"uy", 19}, {"ut", 17}, {"pixx", 8}, {"pixy", 9}, {"pinn", 13},
                        {"regFacShear", 25}};
     const F bulkf[] = {{"Pi", 14}, {"regFacBulk", 26}};
+    const F vort[] = {{"wxy", 30}, {"beta_x", 34}, {"beta_y", 35}};
     auto one = [&](const F &f) {
-        if (before_first_step && (f.id == 25 || f.id == 26)) std::fill(buf.begin(), buf.end(), 0.0);
+        if (before_first_step && (f.id == 25 || f.id == 26 || f.id >= 27)) std::fill(buf.begin(), buf.end(), 0.0);
         else check_sync(h, vh_download(h, f.id, buf.data()));
         write_snapshot(buf, t, dir, f.name, L);
     };
     for (const F &f : shear) one(f);
     if (bulk)
         for (const F &f : bulkf) one(f);
+    if (!before_first_step) check_sync(h, vh_thermal_vorticity(h));
+    for (const F &f : vort) one(f);
 }
 
 // Sorts a[0..n) by .first (all keys in [0, limit)): least-significant digit

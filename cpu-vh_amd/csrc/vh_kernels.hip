@@ -208,6 +208,62 @@ __global__ __launch_bounds__(256) void fo_interp_kernel(FoSlices S, int nx, int 
     }
 }
 
+// calculateThermalVorticity (rhic/src/Vorticity.cpp:6-171), called after
+// each step (HydroPlugin.cpp:302-305): beta_mu = u_mu / T with
+// u_mu = (u^t, -u^x, -u^y, -u^eta) and T the centre cell's (the neighbours'
+// beta are divided by it as well, as the reference does), the time
+// derivative backwards over dt from the previous step's u, the spatial ones
+// central with the neighbours clamped at the edges (the reference's ghost
+// copies), w_{mu nu} = -(d_mu beta_nu - d_nu beta_mu) / 2; the operations in
+// the reference's order, so the results are its bit for bit.
+template <bool TABLE>
+__global__ __launch_bounds__(256) void vorticity_kernel(VortArgs A, EosDev E) {
+    const long N = (long)A.nx * A.ny * A.nz;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const int i = (int)(s % A.nx), j = (int)((s / A.nx) % A.ny), k = (int)(s / ((long)A.nx * A.ny));
+    const long sy = A.nx, sz = (long)A.nx * A.ny;
+    const long nb[6] = {s + ((i > 0) ? -1 : 0),          s + ((i < A.nx - 1) ? 1 : 0),
+                        s + ((j > 0) ? -sy : 0),         s + ((j < A.ny - 1) ? sy : 0),
+                        s + ((k > 0) ? -sz : 0),         s + ((k < A.nz - 1) ? sz : 0)};
+    DivExact dv;
+    const double T = eos_temperature<TABLE>(E, A.e[s], dv);
+    double b[4], bp[4], bn[6][4];
+    b[0] = A.u[0][s] / T;
+    bp[0] = A.up[0][s] / T;
+    for (int c = 1; c < 4; ++c) {
+        b[c] = (A.u[c][s] * -1.0) / T;
+        bp[c] = (A.up[c][s] * -1.0) / T;
+    }
+    for (int m = 0; m < 6; ++m) {
+        bn[m][0] = A.u[0][nb[m]] / T;
+        for (int c = 1; c < 4; ++c) bn[m][c] = (A.u[c][nb[m]] * -1.0) / T;
+    }
+    double dtb[4], dxb[4], dyb[4], dnb[4];
+    for (int c = 0; c < 4; ++c) {
+        dtb[c] = (b[c] - bp[c]) / A.dt;
+        dxb[c] = (bn[1][c] - bn[0][c]) / (2.0 * A.dx);
+        dyb[c] = (bn[3][c] - bn[2][c]) / (2.0 * A.dy);
+        dnb[c] = (bn[5][c] - bn[4][c]) / (2.0 * A.dz);
+    }
+    double *o = A.out;
+    o[0 * N + s] = (-1.0 / 2.0) * (dtb[1] - dxb[0]);
+    o[1 * N + s] = (-1.0 / 2.0) * (dtb[2] - dyb[0]);
+    o[2 * N + s] = (-1.0 / 2.0) * (dtb[3] - dnb[0]);
+    o[3 * N + s] = (-1.0 / 2.0) * (dxb[2] - dyb[1]);
+    o[4 * N + s] = (-1.0 / 2.0) * (dxb[3] - dnb[1]);
+    o[5 * N + s] = (-1.0 / 2.0) * (dyb[3] - dnb[2]);
+    for (int c = 0; c < 4; ++c) o[(6 + c) * N + s] = b[c];
+}
+
+hipError_t launch_vorticity(const VortArgs &A, const EosDev &E, hipStream_t st) {
+    const long N = (long)A.nx * A.ny * A.nz;
+    const unsigned grid = (unsigned)((N + 255) / 256);
+    if (E.kind == 1) hipLaunchKernelGGL(vorticity_kernel<true>, dim3(grid), dim3(256), 0, st, A, E);
+    else hipLaunchKernelGGL(vorticity_kernel<false>, dim3(grid), dim3(256), 0, st, A, E);
+    return hipGetLastError();
+}
+
 template <int NC, bool TABLE, bool DIM3>
 static hipError_t launch_stage_t(const StageParams &P, const EosDev &E, hipStream_t st) {
     const long N = (long)P.nx * P.ny * P.nz;

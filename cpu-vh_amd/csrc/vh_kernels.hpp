@@ -92,6 +92,17 @@ hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev,
                           int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad);
 void fo_surface_free(FoSurfaceWork &W);
 
+// Thermal vorticity (calculateThermalVorticity): u and up the step's and the
+// previous step's velocity, e the step's energy density; out: 10 arrays of
+// nx*ny*nz, wtx wty wtn wxy wxn wyn then beta_t beta_x beta_y beta_n.
+struct VortArgs {
+    const double *u[4], *up[4], *e;
+    double *out;
+    int nx, ny, nz;
+    double dt, dx, dy, dz;
+};
+hipError_t launch_vorticity(const VortArgs &A, const EosDev &E, hipStream_t st);
+
 // Mixed-precision form (FP32 storage; FP64 root solve, EoS and accumulation).
 namespace mx {
 struct MxFields;

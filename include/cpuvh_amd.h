@@ -15,6 +15,8 @@
  *   15 e   16 p   17..20 ut ux uy un   21..24 previous-step ut ux uy un
  *   25     regulation factor of the last stage (regFacShear)
  *   26     bulk regulation factor of the last stage (regFacBulk; bulk builds only)
+ *   27..32 thermal vorticity wtx wty wtn wxy wxn wyn, 33..36 thermal velocity
+ *          beta_t beta_x beta_y beta_n (from the last vh_thermal_vorticity)
  * Fields 25 and 26 are recorded only once requested: the first vh_download of
  * either returns ones and turns its recording on from the next step on
  * (the reference writes both in regulateDissipativeCurrents,
@@ -126,6 +128,17 @@ vh_status vh_steps(vh_handle *h, int n_first, int count);
 /* Waits for the stream and returns VH_ERR_NAN if any stage hit a NaN. */
 vh_status vh_sync(vh_handle *h);
 vh_status vh_nan_info(vh_handle *h, int *site, int64_t *cell);
+
+/* calculateThermalVorticity (rhic/src/Vorticity.cpp:6-171), which the
+ * reference runs after every step (rhic/src/HydroPlugin.cpp:302-305) and
+ * writes (wxy, beta_x, beta_y) into its snapshots (:63-70): the thermal
+ * vorticity w_{mu nu} = -(d_mu beta_nu - d_nu beta_mu)/2 of beta_mu = u_mu/T
+ * from the current state (u, the previous step's u, e), bit for bit the
+ * reference's, into the handle's fields 27..32 (wtx wty wtn wxy wxn wyn) and
+ * 33..36 (beta_t beta_x beta_y beta_n) for vh_download.  Computing it on
+ * demand gives what the reference holds after the same steps.  Enqueued on
+ * the handle's stream.  Fields 27..36 are VH_ERR_ARG before the first call. */
+vh_status vh_thermal_vorticity(vh_handle *h);
 
 /* Host copy of one field (ids above), nx*ny*nz doubles.  Synchronising. */
 vh_status vh_download(vh_handle *h, int field, double *out);

@@ -26,7 +26,7 @@ HERE := $(dir $(lastword $(MAKEFILE_LIST)))
 
 SRCS = FullyDiscreteKurganovTadmorScheme SemiDiscreteKurganovTadmorScheme HalfSiteExtrapolation \
        FluxLimiter LocalPropagationSpeed SpectralRadius FluxFunctions SourceTerms \
-       EnergyMomentumTensor EquationOfState DynamicalVariables
+       EnergyMomentumTensor EquationOfState DynamicalVariables Vorticity
 REF_CPP = $(addprefix $(REF)/rhic/src/,$(addsuffix .cpp,$(SRCS)))
 HARNESS = $(HERE)ref_harness.cpp $(HERE)ref_fo_unit.cpp
 

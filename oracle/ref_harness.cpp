@@ -10,12 +10,15 @@
 // (rhic/src/DynamicalVariables.cpp:169-218), setGhostCells (:220-227),
 // rungeKutta2 (rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:866-924),
 // setCurrentConservedVariables (rhic/src/DynamicalVariables.cpp:386-388),
-// checkForCellsAboveTc and the freezeout finder (freezeout/FreezeOut.cpp).
+// calculateThermalVorticity (rhic/src/Vorticity.cpp:6-171, called after each
+// step at rhic/src/HydroPlugin.cpp:302-305), checkForCellsAboveTc and the
+// freezeout finder (freezeout/FreezeOut.cpp).
 //
 // Field numbering (shared with the oracle and the product's C-ABI):
 //   0..13 ttt ttx tty ttn pitt pitx pity pitn pixx pixy pixn piyy piyn pinn,
 //   14 Pi, 15 e, 16 p, 17..20 ut ux uy un, 21..24 up (previous-step u),
-//   25 regFacShear.
+//   25 regFacShear, 27..32 the thermal vorticity wtx wty wtn wxy wxn wyn,
+//   33..36 the thermal velocity beta_t beta_x beta_y beta_n.
 // Interior arrays are x-fastest: idx = i + nx*(j + ny*k).
 
 #include <cstdio>
@@ -33,6 +36,7 @@
 #include "FullyDiscreteKurganovTadmorScheme.h"
 #include "EnergyMomentumTensor.h"
 #include "FreezeOut.h"
+#include "Vorticity.h"
 
 // freezeout/FreezeOut.cpp and freezeout/Memory.cpp (compiled in ref_fo_unit.cpp)
 double ****calloc4dArray(double ****array, int dim1, int dim2, int dim3, int dim4);
@@ -77,6 +81,10 @@ PRECISION *field_ptr(int f) {
     case 17: return u->ut; case 18: return u->ux; case 19: return u->uy; case 20: return u->un;
     case 21: return up->ut; case 22: return up->ux; case 23: return up->uy; case 24: return up->un;
     case 25: return regFacShear;
+    case 27: return wmunu->wtx; case 28: return wmunu->wty; case 29: return wmunu->wtn;
+    case 30: return wmunu->wxy; case 31: return wmunu->wxn; case 32: return wmunu->wyn;
+    case 33: return beta_mu->beta_t; case 34: return beta_mu->beta_x; case 35: return beta_mu->beta_y;
+    case 36: return beta_mu->beta_n;
     default: return nullptr;
     }
 }
@@ -185,6 +193,11 @@ REF_API void ref_init_conserved(double t) {
 REF_API void ref_step(double t) {
     rungeKutta2(t, g_lat.latticeSpacingProperTime, q, Q, &g_lat, &g_hydro);
     setCurrentConservedVariables();
+}
+
+// HydroPlugin.cpp:302-305 (the fields land at 27..36 of ref_get)
+REF_API void ref_vorticity(double t) {
+    calculateThermalVorticity(t, g_lat.latticeSpacingProperTime, q, Q, &g_lat, &g_hydro);
 }
 
 REF_API int ref_get(int field, double *out) {

@@ -808,6 +808,56 @@ ORC_API int orc_step(void *h, double t) {
 
 ORC_API long orc_error_cell(void *h) { return static_cast<Orc *>(h)->err_cell; }
 
+// calculateThermalVorticity (rhic/src/Vorticity.cpp:6-171), called after
+// each step (HydroPlugin.cpp:302-305) on the step's u, previous u (up) and e:
+// beta_mu = u_mu / T with u_mu = (u^t, -u^x, -u^y, -u^eta) and T the centre
+// cell's (every neighbour's beta is divided by it too, as the reference
+// does), the time derivative backwards over dt, the spatial ones central
+// (neighbours clamped at the edge, the reference's ghost copies), and
+// w_{mu nu} = -(d_mu beta_nu - d_nu beta_mu)/2.  out: 10 arrays of N,
+// wtx wty wtn wxy wxn wyn then beta_t beta_x beta_y beta_n.
+ORC_API void orc_vorticity(void *h, double *out) {
+    Orc *o = static_cast<Orc *>(h);
+    const Params &P = o->P;
+    const long N = o->N;
+    const double dt = P.dt, dx = P.dx, dy = P.dy, dz = P.dz;
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < P.nz; ++k)
+        for (int j = 0; j < P.ny; ++j)
+            for (int i = 0; i < P.nx; ++i) {
+                const long s = o->lin(i, j, k);
+                const long nb[6] = {o->lin(o->cx(i - 1), j, k), o->lin(o->cx(i + 1), j, k),
+                                    o->lin(i, o->cy(j - 1), k), o->lin(i, o->cy(j + 1), k),
+                                    o->lin(i, j, o->cz(k - 1)), o->lin(i, j, o->cz(k + 1))};
+                const double T = o->eos.temperature(o->e[s]);
+                // beta at the cell (from u and from up) and at the six neighbours
+                auto beta = [&](const std::vector<double> *uu, long a, double *bt) {
+                    bt[0] = uu[0][a] / T;
+                    bt[1] = (uu[1][a] * -1.0) / T;
+                    bt[2] = (uu[2][a] * -1.0) / T;
+                    bt[3] = (uu[3][a] * -1.0) / T;
+                };
+                double b[4], bp[4], bn[6][4];
+                beta(o->u, s, b);
+                beta(o->up, s, bp);
+                for (int m = 0; m < 6; ++m) beta(o->u, nb[m], bn[m]);
+                double dtb[4], dxb[4], dyb[4], dnb[4];
+                for (int c = 0; c < 4; ++c) {
+                    dtb[c] = (b[c] - bp[c]) / dt;
+                    dxb[c] = (bn[1][c] - bn[0][c]) / (2.0 * dx);
+                    dyb[c] = (bn[3][c] - bn[2][c]) / (2.0 * dy);
+                    dnb[c] = (bn[5][c] - bn[4][c]) / (2.0 * dz);
+                }
+                out[0 * N + s] = (-1.0 / 2.0) * (dtb[1] - dxb[0]);
+                out[1 * N + s] = (-1.0 / 2.0) * (dtb[2] - dyb[0]);
+                out[2 * N + s] = (-1.0 / 2.0) * (dtb[3] - dnb[0]);
+                out[3 * N + s] = (-1.0 / 2.0) * (dxb[2] - dyb[1]);
+                out[4 * N + s] = (-1.0 / 2.0) * (dxb[3] - dnb[1]);
+                out[5 * N + s] = (-1.0 / 2.0) * (dyb[3] - dnb[2]);
+                for (int c = 0; c < 4; ++c) out[(6 + c) * N + s] = b[c];
+            }
+}
+
 ORC_API int orc_get(void *h, int field, double *out) {
     Orc *o = static_cast<Orc *>(h);
     const std::vector<double> *src = nullptr;

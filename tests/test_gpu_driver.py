@@ -1,6 +1,7 @@
 """The command-line driver (cpu-vh_amd/cpuvh_run) against the oracle stepped
 through the reference's run() loop (rhic/src/HydroPlugin.cpp:74-321): same
-initial state from the same rhic-conf files, the same snapshot files byte for
+initial state from the same rhic-conf files, the same snapshot files (the
+thermal vorticity's among them) byte for
 byte (rhic/src/FileIO.cpp:14-59 format), the same surface.dat (every coarse
 step; the previous slice starts at zero; freezeout/FreezeOut.cpp:200-361: the
 cut cubes in the reference's order, elements from the surface finder pinned
@@ -169,6 +170,10 @@ def _oracle_loop(lat, hyd, bulk, table, ic):
             for name, f in SHEAR + (BULK if bulk else []):
                 v = np.zeros(N) if (n == 1 and f in (25, 26)) else o.get(f)
                 files["%s_%.3f.dat" % (name, t)] = _snapshot_text(v, nx, ny, nz, dx)
+            # the thermal vorticity of the current state (zeros before the first step)
+            w = np.zeros((10, N)) if n == 1 else o.vorticity()
+            for name, c in (("wxy", 3), ("beta_x", 7), ("beta_y", 8)):
+                files["%s_%.3f.dat" % (name, t)] = _snapshot_text(w[c], nx, ny, nz, dx)
         if n % coarse == 0:
             e = o.get("e")
             if dim:

@@ -75,6 +75,7 @@ def oracle_lib():
             getattr(L, f).restype = D
         L.orc_inferred.argtypes = [VP, D, VP, D, VP]
         L.orc_eos_set_cs2_mode.argtypes = [VP, I]
+        L.orc_vorticity.argtypes = [VP, VP]
         L.orc_exp_fdlibm.argtypes = [D]
         L.orc_exp_fdlibm.restype = D
         L.orc_zeta_over_s.argtypes = [D, D, D]
@@ -113,6 +114,12 @@ class Oracle:
 
     def init_conserved(self, t):
         oracle_lib().orc_init_conserved(self.h, t)
+
+    def vorticity(self):
+        """(10, N): wtx wty wtn wxy wxn wyn beta_t beta_x beta_y beta_n."""
+        out = np.empty((10, self.N))
+        oracle_lib().orc_vorticity(self.h, out.ctypes.data)
+        return out
 
     def step(self, t):
         return oracle_lib().orc_step(self.h, t)
@@ -174,6 +181,7 @@ class Reference:
         L.ref_fo_begin.argtypes = [ctypes.c_char_p]
         L.ref_fo_call.argtypes = [I, D, D]
         L.ref_fo_slice.argtypes = [I, VP]
+        L.ref_vorticity.argtypes = [D]
         self.L = L
         self.N = nx * ny * nz
         self._tmpd = tmpd
@@ -201,6 +209,12 @@ class Reference:
         if self.L.ref_get(f, out.ctypes.data) != 0:
             raise ValueError(f"field {f} not in this build")
         return out
+
+    def vorticity(self, t):
+        """calculateThermalVorticity on the current state: (10, N) arrays wtx
+        wty wtn wxy wxn wyn beta_t beta_x beta_y beta_n."""
+        self.L.ref_vorticity(t)
+        return np.stack([self.get(f) for f in range(27, 37)])
 
 
 # ------------------------------------------------------ initial conditions --
