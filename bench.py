@@ -527,16 +527,17 @@ def run_precision_3d(table, steps, checkpoints=(100, 300)):
 SERVE = dict(frac_3d=0.2, side2_min=101, side2_max=201, side3_min=101, side3_max=161, nz3_min=41, nz3_max=81,
              dx=0.1, dt=0.02, t0=0.6, etabar=0.08, etas_Tk_GeV=0.154, etas_slope_lo=0.5, etas_slope_hi=1.0,
              T_reg_GeV=0.1, T_F_GeV=0.150, coarse=5, e0=10.0, e_floor=0.01, hbm_budget_bytes=8 << 30,
-             bulk=1, max_steps=3000, max_resident=256, collect_surface=1)
+             bulk=1, max_steps=3000, max_resident=256, collect_surface=2)
 
 
-def run_serving(table, rates, n_requests, share_3d, budget):
+def run_serving(table, rates, n_requests, share_3d, budget, surface=2):
     """Open-loop serving at each arrival rate (vh_serve_run)."""
     import cpuvh_amd as C
     out = []
     for k, (rate, nreq) in enumerate(zip(rates, n_requests)):
         stats, reqs = C.serve(table, rate_per_s=float(rate), n_requests=nreq, seed=1000 + k,
-                              precision=C.PREC_MIXED, **dict(SERVE, frac_3d=share_3d, hbm_budget_bytes=budget))
+                              precision=C.PREC_MIXED, **dict(SERVE, frac_3d=share_3d, hbm_budget_bytes=budget,
+                                                             collect_surface=surface))
         done = [r for r in reqs if r["status"] == C.VH_OK]
         out.append({"rate_per_s": rate, "requests": nreq, "completed": stats["completed"],
                     "failed": stats["failed"], "events_per_s": stats["events_per_s"],
@@ -546,6 +547,8 @@ def run_serving(table, rates, n_requests, share_3d, budget):
                     "mean_wait_for_admission_s": float(np.mean([r["admit_s"] - r["arrival_s"] for r in done]))
                     if done else None,
                     "share_3d": float(np.mean([r["nz"] > 1 for r in reqs])),
+                    "surface_elements_per_request_mean": float(np.mean([r["fo_elements"] for r in done]))
+                    if done else None,
                     "failed_requests": [{"nx": r["nx"], "nz": r["nz"], "glauber_seed": r["glauber_seed"],
                                          "status": r["status"]} for r in reqs if r["status"] != C.VH_OK]})
     return out
@@ -615,6 +618,9 @@ def main():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-3d-share", type=float, default=0.2, help="share of 3+1D requests in the serving sweep")
     ap.add_argument("--serve-budget-gib", type=float, default=8.0, help="HBM budget of the serving loop")
+    ap.add_argument("--serve-surface", type=int, default=2,
+                    help="1: flag the freezeout cubes of each request at every check; 2: also build its surface "
+                         "elements (surface.dat's rows) on the device and return them")
     ap.add_argument("--tiles", default=None,
                     help="mixed-kernel tile variants 'T2,T3' for the timed batch (counter runs without the autotuner)")
     ap.add_argument("--face-mode", type=int, default=None, help="face semantics of the mixed kernel (0, 1, 2)")
@@ -677,7 +683,8 @@ def main():
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         nreq = [int(x) for x in args.serve_requests.split(",") if x]
         nreq = (nreq + nreq[-1:] * len(rates))[:len(rates)]
-        serving = run_serving(table, rates, nreq, args.serve_3d_share, int(args.serve_budget_gib * (1 << 30)))
+        serving = run_serving(table, rates, nreq, args.serve_3d_share, int(args.serve_budget_gib * (1 << 30)),
+                              args.serve_surface)
     precision3d = None
     if not args.skip_precision_3d:
         log("precision form 3+1D 201x201x101")
@@ -762,7 +769,9 @@ def main():
                         f"2+1D (101^2-201^2), {100 * args.serve_3d_share:.0f}% 3+1D (101x101x41 - 161x161x81), "
                         f"dx=0.1, table EoS, bulk, mixed precision, admitted into one batch by a "
                         f"{args.serve_budget_gib:g} GiB HBM budget, resident until no cell is above T=150 MeV, "
-                        "freezeout cubes flagged per request at every check", "sweep": serving},
+                        + ("freezeout surface (surface.dat's rows) built on the device and returned per request at "
+                           "every check" if args.serve_surface == 2 else "freezeout cubes flagged per request at every check"),
+                        "sweep": serving},
         "fp64_dropin_3d": dropin,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_mx,

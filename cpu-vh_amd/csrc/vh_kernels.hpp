@@ -62,25 +62,18 @@ struct FoSlices {
 hipError_t launch_fo_interp(const FoSlices &S, int nx, int ny, int nz, const long long *cubes, const double *frac,
                             long long n, double *vars, hipStream_t st);
 // Surface elements of the flagged cubes (vh_fo_surface.hip): d = the cell's
-// widths (coarse dtau, dx, dy, deta).  Count pass (counts[m], err[0] = 1 when
-// a cube's surface cannot be closed), inclusive scan of the counts into
-// offsets[1 .. n] (hipcub; temp NULL queries temp_bytes), write pass
-// (elements 2*dim doubles and vars FO_NVARS doubles per element, at
-// offsets[m] with offsets[0] = 0).
+// widths (coarse dtau, dx, dy, deta); fo_elem_scan: inclusive scan of the
+// element counts into offsets[1 .. n] (hipcub; temp NULL queries temp_bytes).
 struct FoCell {
     double d[4];
 };
-hipError_t launch_fo_elem_count(const double *corners, long long n, bool d4, double eF, const FoCell &g, int *counts,
-                                int *err, hipStream_t st);
 hipError_t fo_elem_scan(void *temp, size_t &temp_bytes, const int *counts, int *offsets_plus1, long long n,
                         hipStream_t st);
-hipError_t launch_fo_elem_write(const double *corners, const long long *cubes, long long n, bool d4, double eF,
-                                const FoCell &g, const int *offsets, const FoSlices &S, int nx, int ny,
-                                double *elements, double *vars, hipStream_t st);
 // The whole surface of one coarse step for the n flagged cubes ids (device):
-// corners, counts, scan, then, when there is room (total <= cap) and no cube
-// failed (*bad), the elements and variables copied to the host arrays n_el
-// (n counts), elements and vars.  Synchronising.  W's buffers grow on demand.
+// corners, the elements and variables into per-cube slots, the counts' scan,
+// then, when there is room (total <= cap) and no cube failed (*bad), the
+// elements and variables compacted and copied to the host arrays n_el (n
+// counts), elements and vars.  Synchronising.  W's buffers grow on demand.
 struct FoSurfaceWork {
     void *buf = nullptr;
     size_t buf_bytes = 0;

@@ -251,18 +251,22 @@ __device__ __forceinline__ void dir_contrib(const StageParams &P, const GEO &G, 
 
 // Divisions by the cell's u^tau, by t, t^2, t^3, 2t, pow(t,2), the lattice
 // spacings, dt, 3 and 5 go through correctly rounded reciprocals (divr),
-// which return the division operator's value.
-template <int NC, bool TABLE, class DV, class GEO>
+// which return the division operator's value.  Each variable's result
+// q + dt*S goes to sink(n, value) as soon as its S is known, with q read
+// again from memory there, so the fifteen inputs are not held in registers
+// across the whole evaluation.
+template <int NC, bool TABLE, class DV, class GEO, class SINK>
 __device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, const EosDev &E, long s, int i, int j, int k,
                                             double t, const Recip &rt2pow, const Recip &rut, double ecell,
-                                            double (&acc)[NC], DV &div) {
+                                            SINK &&sink, DV &div) {
     const Recip rdx = make_recip(G.dx), rdy = make_recip(G.dy), rdz = make_recip(G.dz);
     const Recip rt = make_recip(t), rdt = make_recip(G.dt);
     const Recip r3{3.0, 1.0 / 3.0, true}, r5{5.0, 1.0 / 5.0, true};
-    const double ttt = acc[0], ttx = acc[1], tty = acc[2], ttn = acc[3];
-    const double pitt = acc[4], pitx = acc[5], pity = acc[6], pitn = acc[7], pixx = acc[8], pixy = acc[9],
-                 pixn = acc[10], piyy = acc[11], piyn = acc[12], pinn = acc[13];
-    const double Pi = NC == 15 ? acc[NC - 1] : 0.0;
+    auto put = [&](int n, double Sn) { sink(n, P.q[n][s] + G.dt * Sn); };
+    const double ttt = P.q[0][s], ttx = P.q[1][s], tty = P.q[2][s], ttn = P.q[3][s];
+    const double pitt = P.q[4][s], pitx = P.q[5][s], pity = P.q[6][s], pitn = P.q[7][s], pixx = P.q[8][s],
+                 pixy = P.q[9][s], pixn = P.q[10][s], piyy = P.q[11][s], piyn = P.q[12][s], pinn = P.q[13][s];
+    const double Pi = NC == 15 ? P.q[NC - 1][s] : 0.0;
     const double *__restrict__ utv = P.u[0];
     const double *__restrict__ uxv = P.u[1];
     const double *__restrict__ uyv = P.u[2];
@@ -304,6 +308,9 @@ __device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, 
     S[1] = src0 - div(ttx, rt) - dxp + dkvk * pitx;
     S[2] = src0 - div(tty, rt) - dyp + dkvk * pity;
     S[3] = src0 - div(3 * ttn, rt) - div(dnp, rt2pow) + dkvk * pitn;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) put(n, S[n]);
+    __builtin_amdgcn_sched_barrier(0);
 
     const double e = ecell;
     const double T = eos_temperature<TABLE>(E, e, div);
@@ -368,83 +375,133 @@ __device__ __forceinline__ void source_init(const StageParams &P, const GEO &G, 
     const double wxn = (t2 * dxun - dnux - ux * Dun - t2 * un * dux) / 2.0;
     const double wyn = (t2 * dyun - dnuy - uy * Dun - t2 * un * duy) / 2.0;
     const double wxt = wtx, wyt = wty, wnt = div(wtn, rt2), wyx = -wxy, wnx = div(-wxn, rt2), wny = div(-wyn, rt2);
-    const double I1tt = 2.0 * ut * (pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun);
-    const double I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
-                        (pitn * ux + pixn * ut) * Dun;
-    const double I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy +
-                        (pitn * uy + piyn * ut) * Dun;
-    const double I1tn = (pitt * un + pitn * ut) * Dut + (pitx * un + pixn * ut) * Dux + (pity * un + piyn * ut) * Duy +
-                        (pitn * un + pinn * ut) * Dun;
-    const double I1xx = 2.0 * ux * (pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun);
-    const double I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy +
-                        (pixn * uy + piyn * ux) * Dun;
-    const double I1xn = (pitx * un + pitn * ux) * Dut + (pixx * un + pixn * ux) * Dux + (pixy * un + piyn * ux) * Duy +
-                        (pixn * un + pinn * ux) * Dun;
-    const double I1yy = 2.0 * uy * (pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun);
-    const double I1yn = (pity * un + pitn * uy) * Dut + (pixy * un + pixn * uy) * Dux + (piyy * un + piyn * uy) * Duy +
-                        (piyn * un + pinn * uy) * Dun;
-    const double I1nn = 2.0 * un * (pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun);
-    const double I2tt = theta * pitt, I2tx = theta * pitx, I2ty = theta * pity, I2tn = theta * pitn,
-                 I2xx = theta * pixx, I2xy = theta * pixy, I2xn = theta * pixn, I2yy = theta * piyy,
-                 I2yn = theta * piyn, I2nn = theta * pinn;
-    const double I3tt = 2.0 * (pitx * wtx + pity * wty + pitn * wtn);
-    const double I3tx = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
-    const double I3ty = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
-    const double I3tn = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
-    const double I3xx = 2.0 * (pitx * wxt + pixy * wxy + pixn * wxn);
-    const double I3xy = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
-    const double I3xn = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
-    const double I3yy = 2.0 * (pity * wyt + pixy * wyx + piyn * wyn);
-    const double I3yn = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
-    const double I3nn = 2.0 * (pitn * wnt + pixn * wnx + piyn * wny);
     const double ux2 = ux * ux, uy2 = uy * uy;
     const double ps = pitt * stt - 2.0 * pitx * stx - 2.0 * pity * sty + pixx * sxx + 2.0 * pixy * sxy + piyy * syy -
                       2.0 * pitn * stn * t2 + 2.0 * pixn * sxn * t2 + 2.0 * piyn * syn * t2 + pinn * snn * t2 * t2;
-    const double I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - div((1.0 - ut2) * ps, r3);
-    const double I4tx = (pitt * stx + pitx * stt) / 2.0 - (pitx * sxx + pixx * stx) / 2.0 -
-                        (pity * sxy + pixy * sty) / 2.0 - t2 * (pitn * sxn + pixn * stn) / 2.0 + div((ut * ux) * ps, r3);
-    const double I4ty = (pitt * sty + pity * stt) / 2.0 - (pitx * sxy + pixy * stx) / 2.0 -
-                        (pity * syy + piyy * sty) / 2.0 - t2 * (pitn * syn + piyn * stn) / 2.0 + div((ut * uy) * ps, r3);
-    const double I4tn = (pitt * stn + pitn * stt) / 2.0 - (pitx * sxn + pixn * stx) / 2.0 -
-                        (pity * syn + piyn * sty) / 2.0 - t2 * (pitn * snn + pinn * stn) / 2.0 + div((ut * un) * ps, r3);
-    const double I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + div((1.0 + ux2) * ps, r3);
-    const double I4xy = (pitx * sty + pity * stx) / 2.0 - (pixx * sxy + pixy * sxx) / 2.0 -
-                        (pixy * syy + piyy * sxy) / 2.0 - t2 * (pixn * syn + piyn * sxn) / 2.0 + div((ux * uy) * ps, r3);
-    const double I4xn = (pitx * stn + pitn * stx) / 2.0 - (pixx * sxn + pixn * sxx) / 2.0 -
-                        (pixy * syn + piyn * sxy) / 2.0 - t2 * (pixn * snn + pinn * sxn) / 2.0 + div((ux * un) * ps, r3);
-    const double I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + div((1.0 + uy2) * ps, r3);
-    const double I4yn = (pity * stn + pitn * sty) / 2.0 - (pixy * sxn + pixn * sxy) / 2.0 -
-                        (piyy * syn + piyn * syy) / 2.0 - t2 * (piyn * snn + pinn * syn) / 2.0 + div((uy * un) * ps, r3);
-    const double I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + div((rt2.r + un2) * ps, r3);
-    const double Itt = I1tt + delta_pipi * I2tt - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
-    const double Itx = I1tx + delta_pipi * I2tx - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
-    const double Ity = I1ty + delta_pipi * I2ty - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
-    const double Itn = I1tn + delta_pipi * I2tn - I3tn + tau_pipi * I4tn - lambda_piPi * Pi * stn;
-    const double Ixx = I1xx + delta_pipi * I2xx - I3xx + tau_pipi * I4xx - lambda_piPi * Pi * sxx;
-    const double Ixy = I1xy + delta_pipi * I2xy - I3xy + tau_pipi * I4xy - lambda_piPi * Pi * sxy;
-    const double Ixn = I1xn + delta_pipi * I2xn - I3xn + tau_pipi * I4xn - lambda_piPi * Pi * sxn;
-    const double Iyy = I1yy + delta_pipi * I2yy - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
-    const double Iyn = I1yn + delta_pipi * I2yn - I3yn + tau_pipi * I4yn - lambda_piPi * Pi * syn;
-    const double Inn = I1nn + delta_pipi * I2nn - I3nn + tau_pipi * I4nn - lambda_piPi * Pi * snn;
-    S[4] = div(2.0 * beta_pi * stt - pitt * taupiInv - Itt - 2.0 * un * t * pitn, rut) + pitt * dkvk;
-    S[5] = div(2.0 * beta_pi * stx - pitx * taupiInv - Itx - un * t * pixn, rut) + pitx * dkvk;
-    S[6] = div(2.0 * beta_pi * sty - pity * taupiInv - Ity - un * t * piyn, rut) + pity * dkvk;
-    S[7] = div(2.0 * beta_pi * stn - pitn * taupiInv - Itn - un * t * pinn - div(ut * pitn + un * pitt, rt), rut) +
-           pitn * dkvk;
-    S[8] = div(2.0 * beta_pi * sxx - pixx * taupiInv - Ixx, rut) + pixx * dkvk;
-    S[9] = div(2.0 * beta_pi * sxy - pixy * taupiInv - Ixy, rut) + pixy * dkvk;
-    S[10] = div(2.0 * beta_pi * sxn - pixn * taupiInv - Ixn - div(ut * pixn + un * pitx, rt), rut) + pixn * dkvk;
-    S[11] = div(2.0 * beta_pi * syy - piyy * taupiInv - Iyy, rut) + piyy * dkvk;
-    S[12] = div(2.0 * beta_pi * syn - piyn * taupiInv - Iyn - div(ut * piyn + un * pity, rt), rut) + piyn * dkvk;
-    S[13] = div(2.0 * beta_pi * snn - pinn * taupiInv - Inn - div(2 * (ut * pinn + un * pitn), rt), rut) + pinn * dkvk;
+    // One component of pi^{mu nu} at a time: its I1..I4 terms, I and S, with
+    // a scheduling barrier after each so the ten components' terms are not all
+    // live at once (the same operations in the same order as written out in
+    // the reference, only computed component by component).
+    {
+        const double I1tt = 2.0 * ut * (pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun);
+        const double I2tt = theta * pitt;
+        const double I3tt = 2.0 * (pitx * wtx + pity * wty + pitn * wtn);
+        const double I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - div((1.0 - ut2) * ps, r3);
+        const double Itt = I1tt + delta_pipi * I2tt - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
+        S[4] = div(2.0 * beta_pi * stt - pitt * taupiInv - Itt - 2.0 * un * t * pitn, rut) + pitt * dkvk;
+        put(4, S[4]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
+                            (pitn * ux + pixn * ut) * Dun;
+        const double I2tx = theta * pitx;
+        const double I3tx = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
+        const double I4tx = (pitt * stx + pitx * stt) / 2.0 - (pitx * sxx + pixx * stx) / 2.0 -
+                            (pity * sxy + pixy * sty) / 2.0 - t2 * (pitn * sxn + pixn * stn) / 2.0 + div((ut * ux) * ps, r3);
+        const double Itx = I1tx + delta_pipi * I2tx - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
+        S[5] = div(2.0 * beta_pi * stx - pitx * taupiInv - Itx - un * t * pixn, rut) + pitx * dkvk;
+        put(5, S[5]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy +
+                            (pitn * uy + piyn * ut) * Dun;
+        const double I2ty = theta * pity;
+        const double I3ty = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
+        const double I4ty = (pitt * sty + pity * stt) / 2.0 - (pitx * sxy + pixy * stx) / 2.0 -
+                            (pity * syy + piyy * sty) / 2.0 - t2 * (pitn * syn + piyn * stn) / 2.0 + div((ut * uy) * ps, r3);
+        const double Ity = I1ty + delta_pipi * I2ty - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
+        S[6] = div(2.0 * beta_pi * sty - pity * taupiInv - Ity - un * t * piyn, rut) + pity * dkvk;
+        put(6, S[6]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1tn = (pitt * un + pitn * ut) * Dut + (pitx * un + pixn * ut) * Dux + (pity * un + piyn * ut) * Duy +
+                            (pitn * un + pinn * ut) * Dun;
+        const double I2tn = theta * pitn;
+        const double I3tn = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
+        const double I4tn = (pitt * stn + pitn * stt) / 2.0 - (pitx * sxn + pixn * stx) / 2.0 -
+                            (pity * syn + piyn * sty) / 2.0 - t2 * (pitn * snn + pinn * stn) / 2.0 + div((ut * un) * ps, r3);
+        const double Itn = I1tn + delta_pipi * I2tn - I3tn + tau_pipi * I4tn - lambda_piPi * Pi * stn;
+        S[7] = div(2.0 * beta_pi * stn - pitn * taupiInv - Itn - un * t * pinn - div(ut * pitn + un * pitt, rt), rut) +
+               pitn * dkvk;
+        put(7, S[7]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1xx = 2.0 * ux * (pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun);
+        const double I2xx = theta * pixx;
+        const double I3xx = 2.0 * (pitx * wxt + pixy * wxy + pixn * wxn);
+        const double I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + div((1.0 + ux2) * ps, r3);
+        const double Ixx = I1xx + delta_pipi * I2xx - I3xx + tau_pipi * I4xx - lambda_piPi * Pi * sxx;
+        S[8] = div(2.0 * beta_pi * sxx - pixx * taupiInv - Ixx, rut) + pixx * dkvk;
+        put(8, S[8]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy +
+                            (pixn * uy + piyn * ux) * Dun;
+        const double I2xy = theta * pixy;
+        const double I3xy = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
+        const double I4xy = (pitx * sty + pity * stx) / 2.0 - (pixx * sxy + pixy * sxx) / 2.0 -
+                            (pixy * syy + piyy * sxy) / 2.0 - t2 * (pixn * syn + piyn * sxn) / 2.0 + div((ux * uy) * ps, r3);
+        const double Ixy = I1xy + delta_pipi * I2xy - I3xy + tau_pipi * I4xy - lambda_piPi * Pi * sxy;
+        S[9] = div(2.0 * beta_pi * sxy - pixy * taupiInv - Ixy, rut) + pixy * dkvk;
+        put(9, S[9]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1xn = (pitx * un + pitn * ux) * Dut + (pixx * un + pixn * ux) * Dux + (pixy * un + piyn * ux) * Duy +
+                            (pixn * un + pinn * ux) * Dun;
+        const double I2xn = theta * pixn;
+        const double I3xn = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
+        const double I4xn = (pitx * stn + pitn * stx) / 2.0 - (pixx * sxn + pixn * sxx) / 2.0 -
+                            (pixy * syn + piyn * sxy) / 2.0 - t2 * (pixn * snn + pinn * sxn) / 2.0 + div((ux * un) * ps, r3);
+        const double Ixn = I1xn + delta_pipi * I2xn - I3xn + tau_pipi * I4xn - lambda_piPi * Pi * sxn;
+        S[10] = div(2.0 * beta_pi * sxn - pixn * taupiInv - Ixn - div(ut * pixn + un * pitx, rt), rut) + pixn * dkvk;
+        put(10, S[10]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1yy = 2.0 * uy * (pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun);
+        const double I2yy = theta * piyy;
+        const double I3yy = 2.0 * (pity * wyt + pixy * wyx + piyn * wyn);
+        const double I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + div((1.0 + uy2) * ps, r3);
+        const double Iyy = I1yy + delta_pipi * I2yy - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
+        S[11] = div(2.0 * beta_pi * syy - piyy * taupiInv - Iyy, rut) + piyy * dkvk;
+        put(11, S[11]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1yn = (pity * un + pitn * uy) * Dut + (pixy * un + pixn * uy) * Dux + (piyy * un + piyn * uy) * Duy +
+                            (piyn * un + pinn * uy) * Dun;
+        const double I2yn = theta * piyn;
+        const double I3yn = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
+        const double I4yn = (pity * stn + pitn * sty) / 2.0 - (pixy * sxn + pixn * sxy) / 2.0 -
+                            (piyy * syn + piyn * syy) / 2.0 - t2 * (piyn * snn + pinn * syn) / 2.0 + div((uy * un) * ps, r3);
+        const double Iyn = I1yn + delta_pipi * I2yn - I3yn + tau_pipi * I4yn - lambda_piPi * Pi * syn;
+        S[12] = div(2.0 * beta_pi * syn - piyn * taupiInv - Iyn - div(ut * piyn + un * pity, rt), rut) + piyn * dkvk;
+        put(12, S[12]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const double I1nn = 2.0 * un * (pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun);
+        const double I2nn = theta * pinn;
+        const double I3nn = 2.0 * (pitn * wnt + pixn * wnx + piyn * wny);
+        const double I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + div((rt2.r + un2) * ps, r3);
+        const double Inn = I1nn + delta_pipi * I2nn - I3nn + tau_pipi * I4nn - lambda_piPi * Pi * snn;
+        S[13] = div(2.0 * beta_pi * snn - pinn * taupiInv - Inn - div(2 * (ut * pinn + un * pitn), rt), rut) + pinn * dkvk;
+        put(13, S[13]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     if constexpr (NC == 15) {
         const double dPi = -beta_Pi * theta - Pi * tauPiInv - delta_PiPi * Pi * theta + lambda_Pipi * ps;
         S[14] = div(dPi, rut) + Pi * dkvk;
+        put(14, S[14]);
     } else {
         (void)beta_Pi; (void)lambda_Pipi; (void)tauPiInv;
     }
-#pragma unroll
-    for (int n = 0; n < NC; ++n) acc[n] = acc[n] + G.dt * S[n];
 }
 
 // REG_SCHEME_1, rhic/src/FullyDiscreteKurganovTadmorScheme.cpp:689-860, one cell.
@@ -564,12 +621,8 @@ __device__ __forceinline__ void stage_body(const StageParams &P, const EosDev &E
         const Recip rut = make_recip(P.u[0][s]);
         const Recip rt2pow = make_recip(t2pow);
         if (role == 0) {
-            double acc[NC];
-#pragma unroll
-            for (int n = 0; n < NC; ++n) acc[n] = P.q[n][s];
-            source_init<NC, TABLE>(P, G, E, s, i, j, k, t, rt2pow, rut, ecell, acc, div);
-#pragma unroll
-            for (int n = 0; n < NC; ++n) part[0][n][lane] = acc[n];
+            source_init<NC, TABLE>(P, G, E, s, i, j, k, t, rt2pow, rut, ecell,
+                                   [&](int n, double a) { part[0][n][lane] = a; }, div);
         } else {
             auto put = [&](int n, double r) { part[role][n][lane] = r; };
             const int dir = role - 1;
