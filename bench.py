@@ -417,14 +417,19 @@ def accuracy_all(events, table, checkpoints=(50, 100, 200, 300)):
                 err = {}
                 for f in ACC_FIELDS:
                     a, c = ref[f], got[name][f]
-                    err[f] = float(np.max(np.abs(a - c))) / (float(np.max(np.abs(a))) or 1.0)
+                    d, scale = np.abs(a - c), float(np.max(np.abs(a))) or 1.0
+                    err[f] = float(np.max(d)) / scale
+                    err[f + "_q999"] = float(np.quantile(d, 0.999)) / scale
                     if f in ("e", "ut"):
-                        err[f + "_pt"] = float(np.max(np.abs(a - c) / np.abs(a)))
+                        rel = d / np.abs(a)
+                        err[f + "_pt"] = float(np.max(rel))
+                        err[f + "_pt_q999"] = float(np.quantile(rel, 0.999))
                 err["cubes"] = len(cubes["fp64"] ^ cubes[name])
                 row[name] = err
             row["cubes_fp64"] = len(cubes["fp64"])
             per_event.append(row)
         keys = [f for f in ACC_FIELDS] + ["e_pt", "ut_pt"]
+        keys += [k + "_q999" for k in keys]
         mixed_max = {f: max(r["mixed"][f] for r in per_event) for f in keys}
         env = [{f: max(r[v][f] for v in envs) for f in keys + ["cubes"]} for r in per_event]
         env_max = {f: max(x[f] for x in env) for f in keys}
@@ -451,6 +456,8 @@ def accuracy_all(events, table, checkpoints=(50, 100, 200, 300)):
     pass_metric = pass_metric and all(r["fo_cubes_differing_mixed"] == 0 for r in report)
     pass_env = all(r["events_within_k_envelope"][f] == n for r in report for f in ("e", "e_pt", "ut_pt", "ux", "uy"))
     pass_env = pass_env and all(r["events_mixed_cubes_beyond_envelope"] == 0 for r in report)
+    pass_env_q = all(r["events_within_k_envelope"][f + "_q999"] == n for r in report
+                     for f in ("e", "e_pt", "ut_pt", "ux", "uy"))
     return {"events": n, "events_dropped_nan": {str(k): v for k, v in dropped.items()}, "checkpoints": list(checkpoints),
             "bar_metric": "max relative error <= 1e-3 in e and u^mu (e, u^tau pointwise; u^x, u^y against the "
                           "field's largest magnitude) and identical freezeout cube sets, every event, every checkpoint",
@@ -458,7 +465,11 @@ def accuracy_all(events, table, checkpoints=(50, 100, 200, 300)):
             "bar_attainable": f"per event and field: mixed error <= {ENVELOPE_K:g} x the largest error of FP64 "
                               "from three perturbed starts (FP32-rounded, two 1e-7 e perturbations), and no more "
                               "differing freezeout cubes than the largest of those runs",
-            "pass_attainable_bar": bool(pass_env), "report": report}
+            "pass_attainable_bar": bool(pass_env),
+            "bar_attainable_q999": f"as bar_attainable on the 99.9th percentile of each event's pointwise errors "
+                                   "(a cell whose temperature crosses a join of the zeta/s parametrization, where "
+                                   "it jumps by about 1 %, in a different stage than FP64 changes the maximum only)",
+            "pass_attainable_bar_q999": bool(pass_env_q), "report": report}
 
 
 PRECISION_3D = dict(nx=201, ny=201, nz=101, dx=0.08, dt=0.02, t0=0.6, steps=600, etabar=0.08, etas_Tk_GeV=0.154,

@@ -118,6 +118,10 @@ struct EosDev {
     double r_spacing;  // RN(1/e_spacing)
     const double *p_tb;
     const double *T_tb;
+    // FP32 segments of the pressure table for the mixed form's face solves:
+    // entry k = {p_k, d_k, d_k / de, (d_{k+1} - d_k) / de}, d_k = p_{k+1} - p_k
+    // formed in FP64 (the last segment's slope continues past the table)
+    const float4 *p4;
     // glibc powf(x, 0.25f) exceptions: sorted (x bits, result bits) pairs,
     // bucketed by (xbits - exc_lo) >> exc_shift.
     const uint2 *exc;
@@ -335,41 +339,41 @@ __device__ __forceinline__ RootResult table_root(const EosDev &E, double ePrev, 
     return RootResult{table_bisect<QD, DV>(E, ePrev, M0, M, Pi), NAN_NONE};
 }
 
+// The segment of E.p4 holding e (clamped to the table's ends, where its
+// line extends) and e's fraction within it.
+__device__ __forceinline__ float4 eos_segment_f32(const EosDev &E, float e, float emin, float rsp, float &frac) {
+    const float x = (e - emin) * rsp;
+    int idx = (int)x;
+    idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
+    idx = idx > 0 ? idx : 0;
+    frac = x - (float)idx;
+    return E.p4[idx];
+}
+
 // The mixed form's Newton iteration in FP32: the reference's iteration
 // (EnergyMomentumTensor.cpp:26-47, the formulas of nr_step's QuotFast form)
 // in single precision from the FP64 moments, with B = M0 (M0 + Pi) - M
-// formed in FP64 (it cancels for fast flow), hardware reciprocals, and the
-// reference's stop test |e - e0| <= 1e-3 |e| evaluated in FP32, as the
-// reference evaluates it (fabsf).  CONST_CS2: the table's c_s^2 is 1/3 for
-// every e; otherwise c_s^2(e0) is the one-segment quotient of eos_cs2 (the
-// pressure one segment up is the next table entry at the same fraction),
-// clamped alike.  The bisection fallback is the FP64 one.
+// formed in FP64 (it cancels for fast flow), hardware reciprocals and square
+// root, and the reference's stop test |e - e0| <= 1e-3 |e| evaluated in FP32,
+// as the reference evaluates it (fabsf).  One FP32 segment per step gives p
+// and, unless CONST_CS2 (the table's c_s^2 is 1/3 for every e), c_s^2(e0) as
+// eos_cs2's one-segment quotient: the pressure one table spacing up at the
+// same fraction, clamped alike.  The bisection fallback is the FP64 one.
 template <bool CONST_CS2>
 __device__ __forceinline__ RootResult table_root_f32(const EosDev &E, double ePrev, double M0d, double Md, double Pid) {
     const float M0 = (float)M0d, Pi = (float)Pid;
     const float B = (float)(M0d * (M0d + Pid) - Md);
     const float emin = (float)E.e_min, rsp = (float)E.r_spacing;
     float e0 = (float)ePrev;
+    int how = 0;  // 1: converged, 2: NaN step, 0: iterations spent
     for (int j = 0; j < MAX_ITERS_NR; ++j) {
-        const float x = (e0 - emin) * rsp;
-        int idx = (int)x;
-        idx = idx < EOS_LEN - 2 ? idx : EOS_LEN - 2;
-        idx = idx > 0 ? idx : 0;
-        const float frac = x - (float)idx;
-        const float t1 = (float)E.p_tb[idx], t2 = (float)E.p_tb[idx + 1];
-        float p = __builtin_fmaf(frac, t2 - t1, t1);
+        float frac;
+        const float4 T = eos_segment_f32(E, e0, emin, rsp, frac);
+        float p = __builtin_fmaf(frac, T.y, T.x);
         p = 1.e-15f < p ? p : 1.e-15f;
         float cs2 = 1.0f / 3.0f;
         if constexpr (!CONST_CS2) {
-            const float x1 = x + 1.0f;
-            int i1 = (int)x1;
-            i1 = i1 < EOS_LEN - 2 ? i1 : EOS_LEN - 2;
-            i1 = i1 > 0 ? i1 : 0;
-            const float f1 = x1 - (float)i1;
-            const float u1 = (float)E.p_tb[i1], u2 = (float)E.p_tb[i1 + 1];
-            float pr = __builtin_fmaf(f1, u2 - u1, u1);
-            pr = 1.e-15f < pr ? pr : 1.e-15f;
-            const float dpde = (pr - p) * rsp;
+            const float dpde = __builtin_fmaf(frac, T.w, T.z);
             float v = E.cs2_mode ? dpde : __builtin_amdgcn_rcpf(dpde);
             v = (1.0f / 3.0f) < v ? (1.0f / 3.0f) : v;
             v = 0.01f < v ? v : 0.01f;
@@ -379,18 +383,27 @@ __device__ __forceinline__ RootResult table_root_f32(const EosDev &E, double ePr
         const float ie0 = p * R, ip = e0 * R;
         const float cst2 = p * ie0;
         const float A = __builtin_fmaf(-M0, cst2, M0) + Pi;
-        const float H = __builtin_sqrtf(__builtin_fabsf(__builtin_fmaf(A, A, 4 * cst2 * B)));
+        const float H = __builtin_amdgcn_sqrtf(__builtin_fabsf(__builtin_fmaf(A, A, 4 * cst2 * B)));
         const float D = (A - H) * (0.5f * e0 * ip);
         const float f = e0 + D;
         const float c = cs2 - cst2;
         const float X = c * (B + D * H - c * cst2 * D * M0 * ie0);
         const float pH = p * H;
         const float step = f * pH * __builtin_amdgcn_rcpf(pH - X);
-        if (__builtin_isnan(step)) return RootResult{ePrev, NAN_ROOT};
+        if (__builtin_isnan(step)) {
+            how = 2;
+            break;
+        }
         const float e = e0 - step;
-        if (__builtin_fabsf(e - e0) <= 1.0e-3f * __builtin_fabsf(e)) return RootResult{(double)e, NAN_NONE};
+        const bool done = __builtin_fabsf(e - e0) <= 1.0e-3f * __builtin_fabsf(e);
         e0 = e;
+        if (done) {
+            how = 1;
+            break;
+        }
     }
+    if (how == 1) return RootResult{(double)e0, NAN_NONE};
+    if (how == 2) return RootResult{ePrev, NAN_ROOT};
     return RootResult{table_bisect<QuotFast, DivRecip>(E, ePrev, M0d, Md, Pid), NAN_NONE};
 }
 

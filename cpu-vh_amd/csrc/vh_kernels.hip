@@ -32,6 +32,51 @@ __global__ __launch_bounds__(256) void init_conserved_kernel(double *__restrict_
     ttn[s] = (e_s + pp) * ut_s * un[s] + pitn[s];
 }
 
+// One batched event's admission in one pass: the uploaded primitives (a NULL
+// array is zero, u^tau one) into the event's cells of e, p, u and the
+// previous u, pi and Pi into q, and T^{tau mu} formed as init_conserved_kernel
+// forms them; the first thread also opens the event's stop-rule slots.
+__global__ __launch_bounds__(256) void admit_fp64_kernel(const AdmitSrc S, double *__restrict__ q,
+                                                         double *__restrict__ e0, double *__restrict__ p0,
+                                                         double *__restrict__ uc, double *__restrict__ up, long cap,
+                                                         long N, long off, int nc, int *__restrict__ strikes,
+                                                         int *__restrict__ stop) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) {
+        *strikes = 0;
+        *stop = -1;
+    }
+    if (c >= N) return;
+    const long s = off + c;
+    const double e_s = S.e[c], p_s = S.p[c];
+    double u[4];
+    for (int k = 0; k < 4; ++k) u[k] = S.u[k] ? S.u[k][c] : (k == 0 ? 1.0 : 0.0);
+    double pi[10];
+    for (int n = 0; n < 10; ++n) pi[n] = S.pi10 ? S.pi10[n * N + c] : 0.0;
+    const double Pi_s = (nc == 15 && S.Pi) ? S.Pi[c] : 0.0;
+    e0[s] = e_s;
+    p0[s] = p_s;
+    for (int k = 0; k < 4; ++k) {
+        uc[k * cap + s] = u[k];
+        up[k * cap + s] = u[k];
+    }
+    for (int n = 0; n < 10; ++n) q[(4 + n) * cap + s] = pi[n];
+    if (nc == 15) q[14 * cap + s] = Pi_s;
+    const double pp = p_s + Pi_s;
+    q[s] = (e_s + pp) * u[0] * u[0] - pp + pi[0];
+    q[cap + s] = (e_s + pp) * u[0] * u[1] + pi[1];
+    q[2 * cap + s] = (e_s + pp) * u[0] * u[2] + pi[2];
+    q[3 * cap + s] = (e_s + pp) * u[0] * u[3] + pi[3];
+}
+
+hipError_t launch_admit_fp64(const AdmitSrc &S, double *q, double *e0, double *p0, double *uc, double *up, long cap,
+                             long N, long off, int nc, int *strikes, int *stop, hipStream_t st) {
+    if (N <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(admit_fp64_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, S, q, e0, p0, uc, up,
+                       cap, N, off, nc, strikes, stop);
+    return hipGetLastError();
+}
+
 // checkForCellsAboveTc (freezeout/FreezeOut.cpp:364-381) as a wavefront
 // reduction: one 64-bit atomic per wave.
 // A bounded grid strides over the cells; each wave sums its ballots, the
@@ -497,6 +542,20 @@ hipError_t DeviceEos::init(const HostEos &h, int kind, int cs2_mode) {
             return r;
         E.p_tb = ptb;
         E.T_tb = Ttb;
+        const size_t n = h.p_tb.size();
+        std::vector<float4> seg(n);
+        for (size_t k = 0; k < n; ++k) {
+            const size_t a = std::min(k, n - 2);  // past the last segment its slope continues
+            const double d = h.p_tb[a + 1] - h.p_tb[a];
+            const size_t b = std::min(k + 1, n - 2);
+            const double d1 = h.p_tb[b + 1] - h.p_tb[b];
+            seg[k] = make_float4((float)h.p_tb[k], (float)d, (float)(d * E.r_spacing),
+                                 (float)((d1 - d) * E.r_spacing));
+        }
+        if ((r = hipMalloc(&p4, sizeof(float4) * n)) != hipSuccess ||
+            (r = hipMemcpy(p4, seg.data(), sizeof(float4) * n, hipMemcpyHostToDevice)) != hipSuccess)
+            return r;
+        E.p4 = p4;
     } else {
         const QuarterRootExceptions &X = quarter_root_exceptions();
         std::vector<uint2> pairs(std::max<size_t>(1, X.key.size()));
@@ -520,9 +579,11 @@ hipError_t DeviceEos::init(const HostEos &h, int kind, int cs2_mode) {
 void DeviceEos::release() {
     (void)hipFree(ptb);
     (void)hipFree(Ttb);
+    (void)hipFree(p4);
     (void)hipFree(exc);
     (void)hipFree(bucket);
     ptb = Ttb = nullptr;
+    p4 = nullptr;
     exc = nullptr;
     bucket = nullptr;
 }

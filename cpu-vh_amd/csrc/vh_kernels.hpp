@@ -16,6 +16,7 @@ void set_last_error(const std::string &msg);
 struct DeviceEos {
     EosDev E{};
     double *ptb = nullptr, *Ttb = nullptr;
+    float4 *p4 = nullptr;
     uint2 *exc = nullptr;
     uint32_t *bucket = nullptr;
     hipError_t init(const HostEos &h, int kind, int cs2_mode = 0);  // on the current device
@@ -43,6 +44,16 @@ hipError_t launch_strikes(EvDesc *ev, const unsigned long long *counts, int *str
                           int check, hipStream_t st);
 hipError_t launch_init_conserved(double *const q[15], const double *Pi, const double *e, const double *p,
                                  const double *const u[4], long N, hipStream_t st);
+// One event's initial state as uploaded to the device; a NULL array is absent
+// (u^tau one, every other field zero).
+struct AdmitSrc {
+    const double *e, *p, *u[4], *pi10, *Pi;
+};
+// A batched FP64 event's admission: e, p, u and the previous u, pi and Pi
+// (q[(4 + n) * cap + off ..]) and T^{tau mu} (q[n * cap + off ..]); strikes
+// and stop are the event's stop-rule slots (0 and -1).
+hipError_t launch_admit_fp64(const AdmitSrc &S, double *q, double *e0, double *p0, double *uc, double *up, long cap,
+                             long N, long off, int nc, int *strikes, int *stop, hipStream_t st);
 hipError_t launch_count_above(const double *e, long N, double eF, unsigned long long *out, hipStream_t st);
 // The cut cubes' ids in increasing order into ids (capacity cap), their
 // number into *count; blk is scratch of fo_flag_blocks(cubes) entries.
@@ -107,8 +118,11 @@ extern const int MIXED_TILES_3D[MIXED_NTILE][3];
 // the pi^{. eta} components zero), whose eta components stay zero.
 hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bool dim3d, int tile, bool bi,
                               const EvDesc *ev, const int *blk_ev, const int *blk_first, int nblocks, hipStream_t st);
-hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,
-                             const double *pi10, const double *Pi, long N, long off, int nc, hipStream_t st);
+// A mixed batch's admission: F.qo/F.qol/F.uo/F.eo/F.po of the event's cells
+// from the FP64 upload, the previous u (up, field stride F.cap) equal to u,
+// and the stop-rule slots as launch_admit_fp64 opens them.
+hipError_t launch_init_mixed(const mx::MxFields &F, const AdmitSrc &S, float *up, long N, long off, int nc,
+                             int *strikes, int *stop, hipStream_t st);
 hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
                                       const long *start, int nblocks, unsigned long long *counts, hipStream_t st);
 hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, hipStream_t st);

@@ -8,37 +8,43 @@ namespace mx {
 
 // setConservedVariables (rhic/src/DynamicalVariables.cpp:169-218) in FP64 from
 // FP64 primitives, stored as the mixed form's FP32 fields.
-__global__ __launch_bounds__(256) void init_mixed_kernel(const MxFields F, const double *__restrict__ e,
-                                                         const double *__restrict__ p,
-                                                         const double *__restrict__ u4,
-                                                         const double *__restrict__ pi10,
-                                                         const double *__restrict__ Pi, long N, long off, int nc) {
+__global__ __launch_bounds__(256) void init_mixed_kernel(const MxFields F, const AdmitSrc S, float *__restrict__ up,
+                                                         long N, long off, int nc, int *__restrict__ strikes,
+                                                         int *__restrict__ stop) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) {
+        *strikes = 0;
+        *stop = -1;
+    }
     if (c >= N) return;
     const long s = off + c;
-    const double ut = u4[c], ux = u4[N + c], uy = u4[2 * N + c], un = u4[3 * N + c];
-    const double Pb = (nc == 15 && Pi) ? Pi[c] : 0.0;
-    const double P = p[c] + Pb;
-    const double w = e[c] + P;
+    double u[4];
+    for (int k = 0; k < 4; ++k) u[k] = S.u[k] ? S.u[k][c] : (k == 0 ? 1.0 : 0.0);
+    double pi[10];
+    for (int n = 0; n < 10; ++n) pi[n] = S.pi10 ? S.pi10[n * N + c] : 0.0;
+    const double Pb = (nc == 15 && S.Pi) ? S.Pi[c] : 0.0;
+    const double e = S.e[c], p = S.p[c];
+    const double P = p + Pb;
+    const double w = e + P;
     double T4[4];
-    T4[0] = w * ut * ut - P + pi10[c];
-    T4[1] = w * ut * ux + pi10[N + c];
-    T4[2] = w * ut * uy + pi10[2 * N + c];
-    T4[3] = w * ut * un + pi10[3 * N + c];
+    T4[0] = w * u[0] * u[0] - P + pi[0];
+    T4[1] = w * u[0] * u[1] + pi[1];
+    T4[2] = w * u[0] * u[2] + pi[2];
+    T4[3] = w * u[0] * u[3] + pi[3];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
         const float hi = (float)T4[n];
         F.at(F.qo, n)[s] = hi;
         F.at(F.qol, n)[s] = (float)(T4[n] - (double)hi);
     }
-    for (int n = 0; n < 10; ++n) F.at(F.qo, 4 + n)[s] = (float)pi10[n * N + c];
+    for (int n = 0; n < 10; ++n) F.at(F.qo, 4 + n)[s] = (float)pi[n];
     if (nc == 15) F.at(F.qo, 14)[s] = (float)Pb;
-    F.eo[s] = (float)e[c];
-    F.po[s] = (float)p[c];
-    F.at(F.uo, 0)[s] = (float)ut;
-    F.at(F.uo, 1)[s] = (float)ux;
-    F.at(F.uo, 2)[s] = (float)uy;
-    F.at(F.uo, 3)[s] = (float)un;
+    F.eo[s] = (float)e;
+    F.po[s] = (float)p;
+    for (int k = 0; k < 4; ++k) {
+        F.at(F.uo, k)[s] = (float)u[k];
+        F.at(up, k)[s] = (float)u[k];
+    }
 }
 
 }  // namespace mx
@@ -112,11 +118,11 @@ hipError_t launch_stage_mixed(const mx::MxFields &F, const EosDev &E, int nc, bo
                : launch_mx_tile<15, false>(F, E, dim3d, tile, bi, ev, blk_ev, blk_first, nblocks, st);
 }
 
-hipError_t launch_init_mixed(const mx::MxFields &F, const double *e, const double *p, const double *u4,
-                             const double *pi10, const double *Pi, long N, long off, int nc, hipStream_t st) {
-    if (N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mx::init_mixed_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, e, p, u4, pi10,
-                       Pi, N, off, nc);
+hipError_t launch_init_mixed(const mx::MxFields &F, const AdmitSrc &S, float *up, long N, long off, int nc,
+                             int *strikes, int *stop, hipStream_t st) {
+    if (N <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mx::init_mixed_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, F, S, up, N, off,
+                       nc, strikes, stop);
     return hipGetLastError();
 }
 

@@ -521,8 +521,12 @@ __device__ __forceinline__ float face_speed_mx(const EosDev &E, const Moments &m
     if (__builtin_isnan(e)) nan = NAN_ENERGY;
     DivRecip div;
     double p;
-    if constexpr (TABLE) p = eos_pressure_fast(E, e);
-    else p = eos_pressure<false>(E, e, div);
+    if constexpr (TABLE) {  // the speed is FP32: so is the pressure that sets it
+        float fr;
+        const float4 T = eos_segment_f32(E, (float)e, (float)E.e_min, (float)E.r_spacing, fr);
+        const float pf = __builtin_fmaf(fr, T.y, T.x);
+        p = 1.e-15f < pf ? pf : 1.e-15f;
+    } else p = eos_pressure<false>(E, e, div);
     if (e < 1.e-7) {
         e = 1.e-7;
         p = 1.e-7;

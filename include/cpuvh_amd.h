@@ -432,9 +432,11 @@ vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_
                               int64_t cap, int64_t *n_elements);
 vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells);
 vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest_range);
-/* Mixed form: times stage launches of each tile variant (reps launches each)
- * on the resident events and keeps the fastest 2+1D and 3+1D tiles (the
- * state is not advanced).  FP64 batches: no variants, *tile = -1. */
+/* Mixed form: times a stage launch of each tile variant in turn, reps rounds,
+ * on the resident events (the state is not advanced) and keeps the 2+1D and
+ * 3+1D tiles by median time, a later variant replacing the one kept only
+ * when more than 2 % faster (variant 0 is the fastest over whole runs of 2+1D
+ * Glauber batches).  FP64 batches: no variants, *tile = -1. */
 vh_status vh_batch_autotune(vh_batch *b, int reps, int *tile2d, int *tile3d);
 vh_status vh_batch_set_tiles(vh_batch *b, int tile2d, int tile3d);
 /* Mixed form: how face fluxes are shared.  The reference evaluates each face
