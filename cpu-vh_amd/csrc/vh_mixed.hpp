@@ -69,13 +69,18 @@ struct MxFields {
 // rhic/src/FluxLimiter.cpp:14-32 in FP32: minmod(a, minmod(b, c)) of
 // a = theta (y - x), b = (z - x) / 2, c = theta (z - y) is min(|a|,|b|,|c|)
 // with the common sign when all three share one, else zero (up to the sign
-// of a zero result).  Three-operand min/max instructions make it a few
-// operations.
+// of a zero result).  a and c share a sign exactly when y - x and z - y do,
+// and then z - x has it too (a float difference of distinct values is never
+// zero with subnormals kept), so the test is a c > 0 (products below the
+// smallest subnormal, differences under 1e-22, give zero).
+// S scales the result (minmod is positively homogeneous): the half-site step
+// takes half the slope with the halves folded into the three candidates.
+template <int S2 = 0>
 __device__ __forceinline__ float deriv_f(float x, float y, float z) {
-    const float a = THETA_F * (y - x), b = (z - x) * 0.5f, c = THETA_F * (z - y);
-    const float lo = __builtin_fminf(a, __builtin_fminf(b, c)), hi = __builtin_fmaxf(a, __builtin_fmaxf(b, c));
+    constexpr float s = S2 ? 0.5f : 1.0f;
+    const float a = (s * THETA_F) * (y - x), b = (z - x) * (s * 0.5f), c = (s * THETA_F) * (z - y);
     const float m = __builtin_fminf(__builtin_fabsf(a), __builtin_fminf(__builtin_fabsf(b), __builtin_fabsf(c)));
-    return lo > 0.f ? m : (hi < 0.f ? -m : 0.f);
+    return a * c > 0.f ? __builtin_copysignf(m, a) : 0.f;
 }
 
 __device__ __forceinline__ int clampi(int c, int n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
@@ -519,23 +524,26 @@ __device__ __forceinline__ float face_speed_mx(const EosDev &E, const Moments &m
         e = r.e;
     } else e = energy_from_conserved<TABLE, QuotFast, DivRecip>(E, ePrev, m.M0, m.M, Pi, nan);
     if (__builtin_isnan(e)) nan = NAN_ENERGY;
-    DivRecip div;
-    double p;
-    if constexpr (TABLE) {  // the speed is FP32: so is the pressure that sets it
+    // the speed is FP32, and so is what sets it
+    float ef = (float)e, pf;
+    if constexpr (TABLE) {
         float fr;
-        const float4 T = eos_segment_f32(E, (float)e, (float)E.e_min, (float)E.r_spacing, fr);
-        const float pf = __builtin_fmaf(fr, T.y, T.x);
-        p = 1.e-15f < pf ? pf : 1.e-15f;
-    } else p = eos_pressure<false>(E, e, div);
-    if (e < 1.e-7) {
-        e = 1.e-7;
-        p = 1.e-7;
+        const float4 T = eos_segment_f32(E, ef, (float)E.e_min, (float)E.r_spacing, fr);
+        pf = __builtin_fmaf(fr, T.y, T.x);
+        pf = 1.e-15f < pf ? pf : 1.e-15f;
+    } else {
+        DivRecip div;
+        pf = (float)eos_pressure<false>(E, e, div);
     }
-    const double P = p + Pi;
-    const double A = m.M0 + P;
-    if (__builtin_fabs(A) > 1.0e6 * __builtin_fabs(e + P)) return 0.0f;
-    const double Mi = dir == 0 ? m.M1 : (dir == 1 ? m.M2 : m.M3);
-    return (float)Mi * frcp((float)A);
+    if (e < 1.e-7) {
+        ef = 1.e-7f;
+        pf = 1.e-7f;
+    }
+    const float P = pf + (float)Pi;
+    const float A = (float)m.M0 + P;
+    if (__builtin_fabsf(A) > 1.0e6f * __builtin_fabsf(ef + P)) return 0.0f;
+    const float Mi = (float)(dir == 0 ? m.M1 : (dir == 1 ? m.M2 : m.M3));
+    return Mi * frcp(A);
 }
 
 // One face flux between cells a = b-1 and b along dir, b = (i, j, k).  The
@@ -577,8 +585,8 @@ __device__ __forceinline__ void face_mx(const MxFields &F, const EvDesc &G, cons
         }
         const float *__restrict__ fld = F.at(F.q, n);
         const float x0 = ldf(fld, p0), x1 = ldf(fld, p1), x2 = ldf(fld, p2), x3 = ldf(fld, p3);
-        qL[n] = x1 + 0.5f * deriv_f(x0, x1, x2);
-        qR[n] = x2 - 0.5f * deriv_f(x1, x2, x3);
+        qL[n] = x1 + deriv_f<1>(x0, x1, x2);
+        qR[n] = x2 - deriv_f<1>(x1, x2, x3);
     }
     const double ea = ldf(F.e, p1), eb = ldf(F.e, p2);
     const int mode = F.face_mode;
