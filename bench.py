@@ -35,6 +35,9 @@ sys.path.insert(0, os.path.join(REPO, "cpu-vh_amd"))
 
 METRIC = "events/s batched-FP32 vs plain FP64 (≤1e-3 rel err), zone-upd/s, p50/p99"
 HBM_PEAK_GBS = 8000.0
+# VALU issue: one wave instruction per CU per clock (four SIMDs, a wave64
+# instruction every four clocks each), 256 CUs at 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 2.4e9
 HBARC = 0.197326938
 
 # Bulk viscosity that acts: c_s^2 from the EoS table (cs2_mode 1; the
@@ -139,6 +142,18 @@ def cpu_baseline(cases, seconds=10.0, max_steps=400):
                 return {"error": (err or out)[-300:]}
             results.append(json.loads(lines[-1]))
     return {"zone_upd_per_s": sum(r["zone_upd_per_s"] for r in results), "processes": results}
+
+
+def valu_issue(name, cell_stages, seconds):
+    """The kernel's VALU issue rate against the chip's: wave instructions per
+    cell-stage measured with rocprofv3 (SQ_INSTS_VALU, committed under
+    profiles/) x the cell-stages the timed launches covered, over their time."""
+    per = load_traffic(name, "valu_wave_insts_per_cell_stage")
+    if not per or not seconds:
+        return None
+    rate = per * cell_stages / seconds
+    return {"valu_wave_insts_per_cell_stage": per, "achieved_per_s": rate, "peak_per_s": VALU_ISSUE_PER_S,
+            "frac": rate / VALU_ISSUE_PER_S, "source": name}
 
 
 def load_traffic(name, key="hbm_bytes_per_launch"):
@@ -600,7 +615,8 @@ def run_dropin_3d(w, table, steps, warmup):
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": load_traffic("stage_kernel_glauber3d_pmc.json"),
                      "kernel": "stage_kernel (one RK stage, FP64, fused)", "avg_launch_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "valu_issue": valu_issue("stage_kernel_glauber3d_pmc.json", N, kernel_ms * 1e-3)},
     }
 
 
@@ -791,7 +807,9 @@ def main():
                      "kernel": "stage_mixed_kernel (one RK stage, all 2+1D events"
                                + (", boost-invariant)" if bi else ")"),
                      "avg_launch_ms": mixed["graph_ms"] / launches,
-                     "algorithmic_bytes_per_zone_update": bytes_per_zone_mx},
+                     "algorithmic_bytes_per_zone_update": bytes_per_zone_mx,
+                     "valu_issue": valu_issue("stage_mixed_kernel_batch_pmc.json", 2 * mixed["zone_updates"],
+                                              mixed["graph_ms"] * 1e-3)},
         "cpu_baseline": cpu,
         "batched_mixed_reference_transport": None if gpu_const is None else
         {k: v for k, v in gpu_const.items() if k != "steps_to_freezeout"},
