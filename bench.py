@@ -782,7 +782,8 @@ def main():
                    "cells_total": int(sum(ev["N"] for ev in events)), "form": "batched mixed precision",
                    "tiles": mixed["tiles"]},
         "accuracy": None if acc is None else {k: acc[k] for k in ("bar_metric", "pass_metric_bar", "bar_attainable",
-                                                                   "pass_attainable_bar")},
+                                                                   "pass_attainable_bar", "bar_attainable_q999",
+                                                                   "pass_attainable_bar_q999")},
         "batched_mixed": {k: v for k, v in mixed.items() if k != "steps_to_freezeout"},
         "plain_fp64_loop": None if plain is None else {k: v for k, v in plain.items() if k != "steps_to_freezeout"},
         "speedup_vs_plain_fp64": ratio,
