@@ -626,7 +626,12 @@ vh_status vh_fo_flag(vh_handle *h, double eF, int64_t *ids, double *corners, int
     if (n == 0) return VH_OK;
     // the ids come out in increasing order
     static_assert(sizeof(long long) == sizeof(int64_t), "cube ids are 64-bit");
-    HIPCHK(hipMemcpy(ids, h->fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    {
+        void *const dst[1] = {ids};
+        const void *const src[1] = {h->fo_ids};
+        const size_t nb[1] = {sizeof(long long) * n};
+        HIPCHK(fo_download(h->fo_surf, h->stream, 1, dst, src, nb));
+    }
     if (corners) {
         const int per = c.nz > 1 ? 16 : 8;
         HIPCHK(fo_staging(h, sizeof(double) * per * n));

@@ -64,6 +64,9 @@ struct vh_batch {
     vh_batch_config cfg{};
     int nc = 14, dev = 0;
     hipStream_t stream = nullptr;
+    // vh_batch_fo_surface's stream: it works from the flag's copies, so a
+    // caller may build surfaces while steps run on `stream`
+    hipStream_t side = nullptr;
     long cap = 0;
     double *pool = nullptr;
     double *q[3][15] = {};
@@ -415,7 +418,9 @@ vh_status vh_batch_create(const vh_batch_config *cfg, vh_batch **out) {
         return fail(VH_ERR_NOMEM);
     if (hipMalloc(&b->d_times, sizeof(StepTimes) * (size_t)c.max_events * (c.max_steps + 1)) != hipSuccess)
         return fail(VH_ERR_NOMEM);
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(VH_ERR_HIP);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess)
+        return fail(VH_ERR_HIP);
     if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return fail(VH_ERR_HIP);
     for (auto &H : b->hslot)
         if (hipEventCreateWithFlags(&H.done, hipEventDisableTiming) != hipSuccess) return fail(VH_ERR_HIP);
@@ -433,6 +438,7 @@ vh_status vh_batch_destroy(vh_batch *b) {
     if (!b) return VH_ERR_ARG;
     (void)hipSetDevice(b->dev);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->side) (void)hipStreamSynchronize(b->side);
     drop_graphs(b);
     for (auto &E : b->ev) {
         (void)hipFree(E.fo_prev);
@@ -461,6 +467,7 @@ vh_status vh_batch_destroy(vh_batch *b) {
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->side) (void)hipStreamDestroy(b->side);
     b->deos.release();
     delete b;
     return VH_OK;
@@ -907,21 +914,24 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
         }
         E.fo_vslot ^= 1;
         const bool flip = E.stopped && ((b->step_count - E.stop_at) & 1);
+        WidenSet W{};  // the mixed form's slices: one launch
         for (int v = 0; v < FO_NVARS; ++v) {
             double *dst = E.fo_vars + (E.fo_vslot * FO_NVARS + v) * E.N;
             const int f = FO_SURF_FIELD[v];
-            if (f >= b->nc && f < 15) {
-                BCHK(launch_fill(dst, E.N, 0.0, b->stream));
-            } else if (!b->mixed) {
-                BCHK(hipMemcpyAsync(dst, field_of(b, f, flip) + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
-            } else if (E.bi && (f == 10 || f == 12 || f == 20)) {
+            const bool zero = (f >= b->nc && f < 15) || (b->mixed && E.bi && (f == 10 || f == 12 || f == 20));
+            if (b->mixed) {
+                const float *hi = nullptr, *lo = nullptr;
+                if (!zero && !field_of_mixed(b, f, flip, &hi, &lo)) return VH_ERR_STATE;
+                W.hi[W.count] = zero ? nullptr : hi + E.off;
+                W.lo[W.count] = zero || !lo ? nullptr : lo + E.off;
+                W.out[W.count++] = dst;
+            } else if (zero) {
                 BCHK(launch_fill(dst, E.N, 0.0, b->stream));
             } else {
-                const float *hi = nullptr, *lo = nullptr;
-                if (!field_of_mixed(b, f, flip, &hi, &lo)) return VH_ERR_STATE;
-                BCHK(launch_widen(hi + E.off, lo ? lo + E.off : nullptr, dst, E.N, b->stream));
+                BCHK(hipMemcpyAsync(dst, field_of(b, f, flip) + E.off, bytes, hipMemcpyDeviceToDevice, b->stream));
             }
         }
+        if (b->mixed) BCHK(launch_widen_set(W, E.N, b->stream));
     }
     BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
     BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts,
@@ -936,7 +946,12 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
     if ((int64_t)n > cap) return VH_ERR_CAPACITY;
     if (n == 0) return VH_OK;
     // the ids come out in increasing order
-    BCHK(hipMemcpy(ids, E.fo_ids, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    {
+        void *const dst[1] = {ids};
+        const void *const src[1] = {E.fo_ids};
+        const size_t nb[1] = {sizeof(long long) * n};
+        BCHK(fo_download(b->fo_surf, b->stream, 1, dst, src, nb));
+    }
     if (corners) {
         const int per = c.nz > 1 ? 16 : 8;
         double *dc = nullptr;
@@ -979,10 +994,10 @@ vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_
     }
     long long total = 0;
     int bad = 0;
-    BCHK(fo_surface_run(b->fo_surf, b->stream, E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, E.fo_ids, E.fo_n, E.fo_eF, g, S,
+    // fo_surface_run returns with the side stream drained; the flag call that
+    // made its inputs drained `stream`
+    BCHK(fo_surface_run(b->fo_surf, b->side, E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, E.fo_ids, E.fo_n, E.fo_eF, g, S,
                         n_el, elements, vars, cap, &total, &bad));
-    vh_status s = vh_batch_sync(b);
-    if (s != VH_OK) return s;
     if (bad) {
         vh::set_last_error("vh_batch_fo_surface: a cut cube whose surface cannot be closed");
         return VH_ERR_STATE;

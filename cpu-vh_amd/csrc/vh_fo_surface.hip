@@ -14,6 +14,7 @@
 // hypercube construction keeps the eight cubes' segments in private memory
 // (about 12 KB a thread), hence blocks of 64 threads.
 #include <algorithm>
+#include <cstring>
 
 #include <hipcub/hipcub.hpp>
 
@@ -136,6 +137,35 @@ static hipError_t grow(void **buf, size_t *have, size_t need, hipStream_t st) {
     return hipSuccess;
 }
 
+hipError_t fo_download(FoSurfaceWork &W, hipStream_t st, int n, void *const *dst, const void *const *src,
+                       const size_t *bytes) {
+    size_t total = 0;
+    for (int k = 0; k < n; ++k) total += (bytes[k] + 255) & ~(size_t)255;
+    hipError_t r;
+    if (total > W.host_bytes) {
+        if ((r = hipStreamSynchronize(st)) != hipSuccess) return r;
+        const size_t want = std::max(total, 2 * W.host_bytes);
+        (void)hipHostFree(W.host);
+        W.host = nullptr;
+        W.host_bytes = 0;
+        if ((r = hipHostMalloc((void **)&W.host, want)) != hipSuccess) return r;
+        W.host_bytes = want;
+    }
+    size_t at = 0;
+    for (int k = 0; k < n; ++k) {
+        if (bytes[k] && (r = hipMemcpyAsync(W.host + at, src[k], bytes[k], hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return r;
+        at += (bytes[k] + 255) & ~(size_t)255;
+    }
+    if ((r = hipStreamSynchronize(st)) != hipSuccess) return r;
+    at = 0;
+    for (int k = 0; k < n; ++k) {
+        if (bytes[k]) std::memcpy(dst[k], W.host + at, bytes[k]);
+        at += (bytes[k] + 255) & ~(size_t)255;
+    }
+    return hipSuccess;
+}
+
 hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev, const double *ecur, int nx, int ny,
                           int nz, const long long *ids, long long n, double eF, const FoCell &g, const FoSlices &S,
                           int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad) {
@@ -173,10 +203,12 @@ hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev,
                        nullptr, S, nx, ny, d_sel, d_svar, d_cnt, d_bad, d_ovf);
     FOCHK(hipGetLastError());
     FOCHK(fo_elem_scan(d_tmp, temp_bytes, d_cnt, d_off + 1, n, st));
-    FOCHK(hipMemcpyAsync(n_el, d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-    FOCHK(hipMemcpyAsync(&h_total, d_off + n, sizeof(int), hipMemcpyDeviceToHost, st));
-    FOCHK(hipMemcpyAsync(flags, d_bad, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    FOCHK(hipStreamSynchronize(st));
+    {
+        void *const dst[3] = {n_el, &h_total, flags};
+        const void *const src[3] = {d_cnt, d_off + n, d_bad};
+        const size_t bytes[3] = {sizeof(int) * n, sizeof(int), 2 * sizeof(int)};
+        FOCHK(fo_download(W, st, 3, dst, src, bytes));
+    }
     *total = h_total;
     *bad = flags[0];
     if (*bad || h_total == 0 || h_total > cap) return hipSuccess;
@@ -192,9 +224,12 @@ hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev,
                            d_el, d_vars);
     }
     FOCHK(hipGetLastError());
-    FOCHK(hipMemcpyAsync(elements, d_el, sizeof(double) * (size_t)h_total * epd, hipMemcpyDeviceToHost, st));
-    FOCHK(hipMemcpyAsync(vars, d_vars, sizeof(double) * (size_t)h_total * FO_NVARS, hipMemcpyDeviceToHost, st));
-    FOCHK(hipStreamSynchronize(st));
+    {
+        void *const dst[2] = {elements, vars};
+        const void *const src[2] = {d_el, d_vars};
+        const size_t bytes[2] = {sizeof(double) * (size_t)h_total * epd, sizeof(double) * (size_t)h_total * FO_NVARS};
+        FOCHK(fo_download(W, st, 2, dst, src, bytes));
+    }
 #undef FOCHK
     return hipSuccess;
 }
@@ -202,6 +237,7 @@ hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev,
 void fo_surface_free(FoSurfaceWork &W) {
     (void)hipFree(W.buf);
     (void)hipFree(W.el);
+    (void)hipHostFree(W.host);
     W = FoSurfaceWork{};
 }
 

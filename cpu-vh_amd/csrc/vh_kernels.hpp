@@ -90,7 +90,15 @@ struct FoSurfaceWork {
     size_t buf_bytes = 0;
     double *el = nullptr;
     size_t el_bytes = 0;
+    // pinned host staging of the downloads (a copy into pageable memory is
+    // staged by the runtime in synchronous chunks)
+    char *host = nullptr;
+    size_t host_bytes = 0;
 };
+// Device-to-host copies of n (src, bytes) pieces through W's pinned staging:
+// one synchronisation of st, then each piece copied to its destination.
+hipError_t fo_download(FoSurfaceWork &W, hipStream_t st, int n, void *const *dst, const void *const *src,
+                       const size_t *bytes);
 hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev, const double *ecur, int nx, int ny,
                           int nz, const long long *ids, long long n, double eF, const FoCell &g, const FoSlices &S,
                           int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad);
@@ -126,6 +134,14 @@ hipError_t launch_init_mixed(const mx::MxFields &F, const AdmitSrc &S, float *up
 hipError_t launch_count_above_batch_f(const float *e, double eF, const EvDesc *ev, const int *ev_of,
                                       const long *start, int nblocks, unsigned long long *counts, hipStream_t st);
 hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, hipStream_t st);
+// count arrays widened (or, with hi NULL, zeroed) in one launch
+struct WidenSet {
+    static constexpr int MAX = 10;
+    const float *hi[MAX], *lo[MAX];
+    double *out[MAX];
+    int count;
+};
+hipError_t launch_widen_set(const WidenSet &S, long n, hipStream_t st);
 // out[0 .. n) = v (a fill kernel: usable where a memset node is not)
 hipError_t launch_fill(double *out, long n, double v, hipStream_t st);
 

@@ -193,4 +193,21 @@ hipError_t launch_widen(const float *hi, const float *lo, double *out, long n, h
     return hipGetLastError();
 }
 
+// Several widenings in one launch: blockIdx.y picks the array; a NULL hi
+// fills that output with zeros.
+__global__ __launch_bounds__(256) void widen_set_kernel(const WidenSet S, long n) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (c >= n) return;
+    const float *hi = S.hi[k], *lo = S.lo[k];
+    S.out[k][c] = !hi ? 0.0 : (lo ? (double)hi[c] + (double)lo[c] : (double)hi[c]);
+}
+
+hipError_t launch_widen_set(const WidenSet &S, long n, hipStream_t st) {
+    if (n <= 0 || S.count <= 0) return hipSuccess;
+    if (S.count > WidenSet::MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(widen_set_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)S.count), dim3(256), 0, st, S, n);
+    return hipGetLastError();
+}
+
 }  // namespace vh

@@ -7,8 +7,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -54,18 +57,94 @@ uint64_t state_digest(vh_batch *b, int id, int nc, int64_t cells, std::vector<do
     return h;
 }
 
+// A fixed set of host threads running parts of one job at a time: run(n, f)
+// calls f(0) .. f(n - 1), the caller taking parts too, and returns when all
+// are done.
+class HostPool {
+  public:
+    explicit HostPool(int threads) {
+        for (int k = 0; k < threads; ++k) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            quit_ = true;
+        }
+        go_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int parts, const std::function<void(int)> &f) {
+        if (th_.empty() || parts < 2) {
+            for (int k = 0; k < parts; ++k) f(k);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &f;
+            parts_ = parts;
+            next_ = 0;
+            left_ = parts;
+            ++gen_;
+        }
+        go_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [this] { return left_ == 0; });
+        job_ = nullptr;
+        parts_ = 0;
+    }
+
+  private:
+    // Parts are taken under the lock, so a thread still leaving the last job
+    // can only take parts of the current one.
+    void work() {
+        for (;;) {
+            const std::function<void(int)> *job;
+            int k;
+            {
+                std::lock_guard<std::mutex> g(m_);
+                if (!job_ || next_ >= parts_) return;
+                k = next_++;
+                job = job_;
+            }
+            (*job)(k);
+            std::lock_guard<std::mutex> g(m_);
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                go_.wait(g, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    int parts_ = 0, left_ = 0, next_ = 0;
+    uint64_t gen_ = 0;
+    bool quit_ = false;
+};
+
 // One request's surface at a check (vh_batch_fo_surface), as surface.dat's
 // 20 columns per element (FreezeOut.cpp:200-361): cubes in the reference's
 // order (x outer, eta inner), positions from the cube's origin, dsigma =
-// t * normal, T and P from the host EoS at the interpolated e.
+// t * normal, T and P from the host EoS at the interpolated e; steps is the
+// request's step count at the check.
 vh_status surface_rows(vh_batch *b, int id, const vh_serve_request &r, const vh_serve_config &c,
-                       const vh::HostEos &heos, const std::vector<int64_t> &ids, int64_t nf, std::vector<int> &ne,
-                       std::vector<double> &el, std::vector<double> &var, std::vector<double> &rows,
-                       std::vector<std::pair<int64_t, int64_t>> &order) {
+                       const vh::HostEos &heos, int steps, const std::vector<int64_t> &ids, int64_t nf,
+                       std::vector<int> &ne, std::vector<double> &el, std::vector<double> &var,
+                       std::vector<double> &rows, std::vector<int64_t> &order, std::vector<int64_t> &bucket,
+                       HostPool &pool) {
     const bool d4 = r.nz > 1;
     const int dim = d4 ? 4 : 3, per = 2 * dim;
-    int steps = 0;
-    vh_batch_event_info(b, id, nullptr, &steps, nullptr);
     const double t = c.t0 + steps * c.dt, cell_tau = t + c.dt;
     const double widths[4] = {c.dt * c.coarse, c.dx, c.dx, c.dx};
     ne.resize(nf);
@@ -78,43 +157,62 @@ vh_status surface_rows(vh_batch *b, int id, const vh_serve_request &r, const vh_
         s = vh_batch_fo_surface(b, id, widths, ne.data(), el.data(), var.data(), total, &total);
     }
     if (s != VH_OK) return s;
+    // The cubes come flagged in increasing id (x fastest); the reference lists
+    // them x outermost, then y, then eta.  A stable counting sort on
+    // ix * cy + iy gives that order: within one (ix, iy) the ids increase
+    // with eta.
+    const int64_t cx = r.nx - 1, cy = r.ny - 1, cz = d4 ? r.nz - 1 : 1;
+    (void)cz;
+    bucket.assign((size_t)(cx * cy + 1), 0);
+    for (int64_t m = 0; m < nf; ++m) {
+        const int64_t q = ids[m];
+        ++bucket[(size_t)((q % cx) * cy + (q / cx) % cy) + 1];
+    }
+    for (size_t k = 1; k < bucket.size(); ++k) bucket[k] += bucket[k - 1];
+    order.resize((size_t)nf * 2);  // [cube in the reference's order | its first row]
+    for (int64_t m = 0; m < nf; ++m) {
+        const int64_t q = ids[m];
+        order[(size_t)bucket[(size_t)((q % cx) * cy + (q / cx) % cy)]++] = m;
+    }
     std::vector<int64_t> off(nf + 1, 0);
     for (int64_t m = 0; m < nf; ++m) off[m + 1] = off[m] + ne[m];
-    const int64_t cx = r.nx - 1, cy = r.ny - 1, cz = d4 ? r.nz - 1 : 1;
-    order.resize(nf);
-    for (int64_t m = 0; m < nf; ++m) {
-        const int64_t q = ids[m], ix = q % cx, iy = (q / cx) % cy, iz = q / (cx * cy);
-        order[m] = {(ix * cy + iy) * cz + iz, m};
+    int64_t *first = order.data() + nf;
+    for (int64_t k = 0, at = 0; k < nf; ++k) {
+        first[k] = at;
+        at += ne[order[k]];
     }
-    std::sort(order.begin(), order.end());
     rows.resize((size_t)total * 20);
-    size_t k = 0;
-    for (const auto &om : order) {
-        const int64_t m = om.second, q = ids[m];
-        const int ix = (int)(q % cx), iy = (int)((q / cx) % cy), iz = d4 ? (int)(q / (cx * cy)) : 0;
-        const double cell_x = (double)ix * c.dx - (((double)(r.nx - 1)) / 2.0 * c.dx);
-        const double cell_y = (double)iy * c.dx - (((double)(r.ny - 1)) / 2.0 * c.dx);
-        const double cell_z = d4 ? (double)iz * c.dx - (((double)(r.nz - 1)) / 2.0 * c.dx) : 0.0;
-        for (int64_t e = off[m]; e < off[m + 1]; ++e, ++k) {
-            const double *cen = el.data() + (size_t)e * per, *nor = cen + dim, *v = var.data() + (size_t)e * 10;
-            double *o = rows.data() + k * 20;
-            o[0] = cen[0] + cell_tau;
-            o[1] = cen[1] + cell_x;
-            o[2] = cen[2] + cell_y;
-            o[3] = d4 ? cen[3] + cell_z : 0.0;
-            o[4] = t * nor[0];
-            o[5] = t * nor[1];
-            o[6] = t * nor[2];
-            o[7] = d4 ? t * nor[3] : 0.0;
-            o[8] = v[0];
-            o[9] = v[1];
-            o[10] = v[2];
-            o[11] = v[3];
-            o[12] = heos.temperature(v[3]);
-            o[13] = heos.pressure(v[3]);
-            for (int j = 0; j < 6; ++j) o[14 + j] = v[4 + j];
+    // the rows of consecutive runs of cubes, on the pool's threads
+    const int parts = (int)std::min<int64_t>(nf, 64);
+    pool.run(parts, [&](int part) {
+        const int64_t k0 = nf * part / parts, k1 = nf * (part + 1) / parts;
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t m = order[k], q = ids[m];
+            const int ix = (int)(q % cx), iy = (int)((q / cx) % cy), iz = d4 ? (int)(q / (cx * cy)) : 0;
+            const double cell_x = (double)ix * c.dx - (((double)(r.nx - 1)) / 2.0 * c.dx);
+            const double cell_y = (double)iy * c.dx - (((double)(r.ny - 1)) / 2.0 * c.dx);
+            const double cell_z = d4 ? (double)iz * c.dx - (((double)(r.nz - 1)) / 2.0 * c.dx) : 0.0;
+            double *o = rows.data() + (size_t)first[k] * 20;
+            for (int64_t e = off[m]; e < off[m + 1]; ++e, o += 20) {
+                const double *cen = el.data() + (size_t)e * per, *nor = cen + dim, *v = var.data() + (size_t)e * 10;
+                o[0] = cen[0] + cell_tau;
+                o[1] = cen[1] + cell_x;
+                o[2] = cen[2] + cell_y;
+                o[3] = d4 ? cen[3] + cell_z : 0.0;
+                o[4] = t * nor[0];
+                o[5] = t * nor[1];
+                o[6] = t * nor[2];
+                o[7] = d4 ? t * nor[3] : 0.0;
+                o[8] = v[0];
+                o[9] = v[1];
+                o[10] = v[2];
+                o[11] = v[3];
+                o[12] = heos.temperature(v[3]);
+                o[13] = heos.pressure(v[3]);
+                for (int j = 0; j < 6; ++j) o[14 + j] = v[4 + j];
+            }
         }
-    }
+    });
     return VH_OK;
 }
 
@@ -194,13 +292,19 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
     std::vector<int> owner(c.max_resident, -1), strikes(c.max_resident, 0);
     std::vector<int64_t> counts(c.max_resident);
     std::vector<double> ebuf(nmax);
-    // surface elements of one request's check: cube ids, counts, elements,
-    // variables, and the rows handed to on_surface
-    std::vector<int64_t> fo_ids;
+    // surface elements of one request's check: cube ids (per batch slot, kept
+    // from the flags to the surface), counts, elements, variables, and the
+    // rows handed to on_surface
+    std::vector<std::vector<int64_t>> fo_ids(c.max_resident);
+    std::vector<int64_t> fo_nflag(c.max_resident, 0);
+    std::vector<int> fo_steps(c.max_resident, 0), leaving(c.max_resident, 0);
     std::vector<int> fo_ne;
     std::vector<double> fo_el, fo_var, fo_rows;
-    std::vector<std::pair<int64_t, int64_t>> fo_order;
+    std::vector<int64_t> fo_order, fo_bucket;
+    // threads building the surface rows
+    HostPool pool(c.collect_surface == 2 ? (int)std::min(7u, std::max(1u, std::thread::hardware_concurrency() / 2)) : 0);
     std::deque<int> queue;
+    bool in_flight = false;  // the next check's steps are already enqueued
     int next = 0, finished = 0, resident = 0;
     int64_t resident_cells = 0;
     double occ_int = 0, occ_peak = 0, zone_updates = 0;
@@ -273,9 +377,12 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             }
             continue;
         }
-        st = vh_batch_steps(b, c.coarse);
-        if (st != VH_OK) break;
-        zone_updates += (double)resident_cells * c.coarse;
+        if (!in_flight) {
+            st = vh_batch_steps(b, c.coarse);
+            if (st != VH_OK) break;
+            zone_updates += (double)resident_cells * c.coarse;
+        }
+        in_flight = false;
         // the check; a request whose state hit a NaN leaves as failed
         for (;;) {
             st = vh_batch_counts_above(b, eF, counts.data());
@@ -291,36 +398,66 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             finish(bad, VH_ERR_NAN);
         }
         if (st != VH_OK) break;
+        // flags and the stop rule; a request admitted while the last steps
+        // were in flight has not stepped yet and has its first check next time
         for (int id = 0; id < c.max_resident; ++id) {
+            fo_nflag[id] = 0;
+            leaving[id] = 0;
             if (owner[id] < 0) continue;
+            int active = 0, steps = 0;
+            vh_batch_event_info(b, id, &active, &steps, nullptr);
+            if (steps == 0) continue;
             if (c.collect_surface) {
                 vh_serve_request &r = req[owner[id]];
                 const int64_t ncube = (int64_t)(r.nx - 1) * (r.ny - 1) * (r.nz > 1 ? r.nz - 1 : 1);
-                if (c.collect_surface == 2 && (int64_t)fo_ids.size() < ncube) fo_ids.resize(ncube);
+                std::vector<int64_t> &ids = fo_ids[id];
+                if (c.collect_surface == 2 && (int64_t)ids.size() < ncube) ids.resize(ncube);
                 int64_t nflag = 0;
-                vh_status fs = vh_batch_fo_flag(b, id, eF, c.collect_surface == 2 ? fo_ids.data() : nullptr, nullptr,
+                vh_status fs = vh_batch_fo_flag(b, id, eF, c.collect_surface == 2 ? ids.data() : nullptr, nullptr,
                                                 c.collect_surface == 2 ? ncube : 0, &nflag);
                 if (fs != VH_OK && fs != VH_ERR_CAPACITY) {
                     st = fs;
                     break;
                 }
                 r.fo_cubes += nflag;
-                if (c.collect_surface == 2 && nflag > 0) {
-                    fs = surface_rows(b, id, r, c, heos, fo_ids, nflag, fo_ne, fo_el, fo_var, fo_rows, fo_order);
-                    if (fs != VH_OK) {
-                        st = fs;
-                        break;
-                    }
-                    const int64_t ne = (int64_t)(fo_rows.size() / 20);
-                    r.fo_elements += ne;
-                    if (c.on_surface && ne) c.on_surface(c.user, owner[id], ne, fo_rows.data());
-                }
+                if (c.collect_surface == 2) fo_nflag[id] = nflag;
+                fo_steps[id] = steps;
             }
             if (counts[id] == 0) ++strikes[id];
-            int active = 0, steps = 0;
-            vh_batch_event_info(b, id, &active, &steps, nullptr);
-            if (strikes[id] >= 3 || steps + c.coarse > c.max_steps) finish(id, VH_OK);
+            leaving[id] = strikes[id] >= 3 || steps + c.coarse > c.max_steps;
         }
+        if (st != VH_OK) break;
+        // surfaces: the leaving requests' before their events leave the batch,
+        // then the next check's steps go in flight while the others' are built
+        // (vh_batch_fo_surface works from the flag's copies, on a stream of
+        // its own)
+        auto surfaces = [&](bool of_leaving) {
+            for (int id = 0; id < c.max_resident && st == VH_OK; ++id) {
+                if (owner[id] < 0 || fo_nflag[id] <= 0 || (leaving[id] != 0) != of_leaving) continue;
+                vh_serve_request &r = req[owner[id]];
+                vh_status fs = surface_rows(b, id, r, c, heos, fo_steps[id], fo_ids[id], fo_nflag[id], fo_ne, fo_el,
+                                            fo_var, fo_rows, fo_order, fo_bucket, pool);
+                if (fs != VH_OK) {
+                    st = fs;
+                    break;
+                }
+                const int64_t ne = (int64_t)(fo_rows.size() / 20);
+                r.fo_elements += ne;
+                if (c.on_surface && ne) c.on_surface(c.user, owner[id], ne, fo_rows.data());
+            }
+        };
+        surfaces(true);
+        if (st != VH_OK) break;
+        for (int id = 0; id < c.max_resident; ++id)
+            if (owner[id] >= 0 && leaving[id]) finish(id, VH_OK);
+        if (resident > 0 && c.collect_surface == 2) {
+            st = vh_batch_steps(b, c.coarse);
+            if (st != VH_OK) break;
+            zone_updates += (double)resident_cells * c.coarse;
+            in_flight = true;
+        }
+        surfaces(false);
+        if (st != VH_OK) break;
         const double t = now_s();
         const double occ = (double)(resident_cells * bytes_per_cell) / (double)c.hbm_budget_bytes;
         occ_int += occ * (t - t_prev);

@@ -427,7 +427,10 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
                            int64_t *n_flagged);
 /* As vh_fo_surface for one event of a batch created with fo_vars = 1, on the
  * cubes its last vh_batch_fo_flag call flagged (mixed-precision events'
- * fields widened to FP64 first). */
+ * fields widened to FP64 first).  It reads only the copies that flag call
+ * made and runs on a stream of its own, so steps enqueued after the flag
+ * (vh_batch_steps) proceed while it builds; the event's next
+ * vh_batch_fo_flag must come after it returns. */
 vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_el, double *elements, double *vars,
                               int64_t cap, int64_t *n_elements);
 vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells);
@@ -482,7 +485,9 @@ typedef struct {
     int max_resident;          /* event slots */
     int collect_surface;       /* 1: flag freezeout cubes of every resident request at every check;
                                   2: also build their surface elements on the device (vh_batch_fo_surface)
-                                  and hand each request's to on_surface */
+                                  and hand each request's to on_surface; the next check's steps then run
+                                  while the surfaces of requests that stay are built, and a request
+                                  admitted meanwhile joins the steps after them */
     int device;
     int digest;                /* 1: a completed request's final state is downloaded and hashed (digest) */
     /* with collect_surface 2 (may be NULL): called for each resident request
