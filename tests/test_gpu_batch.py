@@ -295,3 +295,57 @@ def test_workload_events_fp64_bitwise_and_mixed_within_envelope():
         q = f + "_q999"
         assert r["mixed_vs_fp64_max_over_events"][q] <= 2 * r["fp64_envelope_max_over_events"][q], q
     assert r["fo_cubes_differing_mixed"] <= r["fo_cubes_differing_fp64_envelope"]
+
+
+PI_NAMES = ["pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_admission_state_before_any_step(precision):
+    """vh_batch_add enqueues its uploads and the initial-state kernel and
+    returns without waiting.  Events admitted back to back (sizes that grow,
+    so the staging buffers are replaced; more than two, so the pinned slots
+    are reused; some with every array, some with e and p only), one removed
+    and admitted again into the freed slot and cells, all read back before
+    any step: e, p, u and the previous u, pi and Pi as given (u^tau one and
+    the rest zero where an array is not given), and T^{tau mu} =
+    (e + p + Pi) u^tau u^mu - (p + Pi) g^{tau mu} + pi^{tau mu} as
+    setConservedVariables forms it (DynamicalVariables.cpp:169-218) - bit for
+    bit in FP64; in the mixed form the fields rounded to FP32 and T^{tau mu}
+    to 1e-13 (its FP32 high and low parts)."""
+    print(f"case admission precision={precision}", flush=True)
+    order = sorted(range(len(EVENTS)), key=lambda k: np.prod(EVENTS[k][:3]))
+    evs = [EVENTS[k] for k in order]
+    ics = []
+    for k, ev in enumerate(evs):
+        ic = _ic(ev, True, True)
+        ics.append(ic if k % 2 == 0 else {"e": ic["e"], "p": ic["p"]})
+    cap = sum(ev[0] * ev[1] * ev[2] for ev in evs) + evs[1][0] * evs[1][1] * evs[1][2]
+    b = Batch(cap, max_events=len(evs) + 1, max_steps=16, bulk=True, eos_table=V.eos_table(), precision=precision)
+    ids = [_add(b, ev, ic) for ev, ic in zip(evs, ics)]
+    b.remove(ids[1])
+    ids[1] = _add(b, evs[1], ics[1])
+    f32 = (lambda a: np.asarray(a, np.float32).astype(np.float64)) if precision else (lambda a: a)
+    for ev, ic, eid in zip(evs, ics, ids):
+        N = ev[0] * ev[1] * ev[2]
+        z = np.zeros(N)
+        u = [ic.get(k, np.ones(N) if k == "ut" else z) for k in ("ut", "ux", "uy", "un")]
+        pi = ic["pi"].reshape(10, N) if "pi" in ic else np.zeros((10, N))
+        Pi = ic.get("Pi", z)
+        want = {"e": ic["e"], "p": ic["p"], "Pi": Pi}
+        for n, k in enumerate(("t", "x", "y", "n")):
+            want["u" + k] = u[n]
+            want["up_" + k] = u[n]
+        for n, name in enumerate(PI_NAMES):
+            want[name] = pi[n]
+        for f, a in want.items():
+            assert V.bits_equal(b.get(eid, f), f32(a)), (eid, f)
+        pp = ic["p"] + Pi
+        T = [(ic["e"] + pp) * u[0] * u[0] - pp + pi[0]] + [(ic["e"] + pp) * u[0] * u[n] + pi[n] for n in (1, 2, 3)]
+        for n, name in enumerate(("ttt", "ttx", "tty", "ttn")):
+            got = b.get(eid, name)
+            if precision:
+                assert np.all(np.abs(got - T[n]) <= 1e-13 * np.maximum(np.abs(T[n]), 1e-300)), (eid, name)
+            else:
+                assert V.bits_equal(got, T[n]), (eid, name)
+    b.close()
