@@ -347,7 +347,8 @@ void initial_state(vh_handle *h, const Lattice &L, const InitCond &I, const Hydr
                 ux[s] = cut(vec->ux[s]);
                 uy[s] = cut(vec->uy[s]);
                 un[s] = cut(vec->un[s]);
-                ut[s] = std::sqrt(1.0 + ux[s] * ux[s] + uy[s] * uy[s] + H.t0 * H.t0 * un[s] * un[s]);
+                ut[s] = vec->reference_ut ? 1.0 + ux[s] * ux[s] + uy[s] * uy[s] + un[s] * un[s]
+                                          : std::sqrt(1.0 + ux[s] * ux[s] + uy[s] * uy[s] + H.t0 * H.t0 * un[s] * un[s]);
                 for (int c = 0; c < 10; ++c) pi[c * N + s] = (*piv[c])[s] / HBARC;
                 if (bulk) Pi[s] = vec->Pi[s] / HBARC;
             }

@@ -38,6 +38,7 @@ cpuvh::InitialTmunu HYDRO::vectors() const {
     v.pixx = initial_pixx; v.pixy = initial_pixy; v.pixn = initial_pixn;
     v.piyy = initial_piyy; v.piyn = initial_piyn; v.pinn = initial_pinn;
     v.Pi = initial_Pi;
+    v.reference_ut = reference_ut;
     return v;
 }
 

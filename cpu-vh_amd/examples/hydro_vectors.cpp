@@ -3,12 +3,13 @@
 // run_hydro_no_cli() reads rhic-conf/ and writes cpu-vh_output/ under the
 // working directory (initialConditionType 13 or 14 in rhic-conf/ic.properties).
 //
-//   hydro_vectors <file>
+//   hydro_vectors <file> [--reference-ut]
 //
 // <file>: one line per cell (x fastest), either one column, the energy
 // density (initialize_from_vector), or sixteen, e ut ux uy un pitt pitx pity
 // pitn pixx pixy pixn piyy piyn pinn Pi (initialize_from_vectors); energy
-// density and pi in GeV/fm^3.
+// density and pi in GeV/fm^3.  --reference-ut: type 14 stores the
+// reference's u^tau = 1 + ux^2 + uy^2 + un^2 (HYDRO::reference_ut).
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -18,8 +19,9 @@
 #include "../../include/cpuvh_hydro.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: %s <vectors file>\n", argv[0]);
+    const bool reference_ut = argc == 3 && std::string(argv[2]) == "--reference-ut";
+    if (argc != 2 && !reference_ut) {
+        std::fprintf(stderr, "usage: %s <vectors file> [--reference-ut]\n", argv[0]);
         return 64;
     }
     std::ifstream in(argv[1]);
@@ -43,6 +45,7 @@ int main(int argc, char **argv) {
         for (size_t c = 0; c < row.size(); ++c) col[c].push_back(row[c]);
     }
     HYDRO hydro;
+    hydro.reference_ut = reference_ut;
     if (col.size() == 1)
         hydro.initialize_from_vector(col[0]);
     else

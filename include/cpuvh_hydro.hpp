@@ -14,8 +14,9 @@
  * Vectors are x-fastest over the interior (i + nx (j + ny k)); e and the
  * pi/Pi components in GeV/fm^3 as the reference takes them (divided by
  * hbar c there and here, e plus the reference's 1e-3 fm^-4 floor).  Type 14
- * normalises u^tau = sqrt(1 + ux^2 + uy^2 + tau0^2 un^2), where the
- * reference stores 1 + ux^2 + uy^2 + un^2 (InitialConditions.cpp:1681).
+ * normalises u^tau = sqrt(1 + ux^2 + uy^2 + tau0^2 un^2) by default; with
+ * reference_ut set it stores 1 + ux^2 + uy^2 + un^2, as the reference does
+ * (InitialConditions.cpp:1681), for runs that must follow the reference.
  * Both methods return 0; a configuration the reference stops on ends the
  * process with the reference's message, as there.  The EoS table and bulk
  * viscosity come from rhic-conf/gpu.properties (the reference's
@@ -30,6 +31,7 @@ struct InitialTmunu {
     std::vector<double> e, ut, ux, uy, un;
     std::vector<double> pitt, pitx, pity, pitn, pixx, pixy, pixn, piyy, piyn, pinn;
     std::vector<double> Pi;
+    bool reference_ut = false;  // type 14: u^tau = 1 + ux^2 + uy^2 + un^2, the reference's value
 };
 
 // The command-line run of cpuvh_run (argv as its main receives it) and the
@@ -59,6 +61,9 @@ class HYDRO {
     std::vector<double> initial_piyn;
     std::vector<double> initial_pinn;
     std::vector<double> initial_Pi;
+    // type 14: false (default) normalises u^tau; true stores the reference's
+    // 1 + ux^2 + uy^2 + un^2 (InitialConditions.cpp:1681)
+    bool reference_ut = false;
     int run_hydro(int argc, char **argv);
     int run_hydro_no_cli();
     void initialize_from_vector(std::vector<double> e);

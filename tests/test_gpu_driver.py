@@ -314,14 +314,15 @@ def test_driver_type10_field_files_table_bulk(tmp_path):
 HYDRO_DEMO = os.path.join(os.path.dirname(C.LIB_PATH), "hydro_vectors")
 
 
-@pytest.mark.parametrize("ic_type", [13, 14])
-def test_hydro_wrapper_vectors(tmp_path, ic_type):
+@pytest.mark.parametrize("ic_type,reference_ut", [(13, False), (14, False), (14, True)])
+def test_hydro_wrapper_vectors(tmp_path, ic_type, reference_ut):
     """The reference's HYDRO class (include/cpuvh_hydro.hpp) with its vectors
     handed over in memory (cpu-vh_amd/examples/hydro_vectors): type 13, an
     energy density in GeV/fm^3 at rest; type 14, all sixteen vectors; run by
     run_hydro_no_cli() from rhic-conf/ into cpu-vh_output/.  Same files as the
     oracle loop from the state the reference builds from those vectors
-    (rhic/src/InitialConditions.cpp:1593-1711; u^tau normalised)."""
+    (rhic/src/InitialConditions.cpp:1593-1711; u^tau normalised, or with
+    HYDRO::reference_ut the reference's 1 + ux^2 + uy^2 + un^2 of :1681)."""
     nx, ny, nz, dx, t0 = 17, 15, 1, 0.2, 0.6
     N = nx * ny * nz
     ref = V.random_ic(nx, ny, nz, dx, t0, seed=40 + ic_type, bulk=True)
@@ -343,12 +344,13 @@ def test_hydro_wrapper_vectors(tmp_path, ic_type):
         pi_gev = ref["pi"].reshape(10, N) * hbarc
         Pi_gev = ref["Pi"] * hbarc
         rows = np.column_stack([e_gev, np.ones(N), ux, uy, un, pi_gev.T, Pi_gev])
-        ic.update(ux=ux, uy=uy, un=un, ut=np.sqrt(1.0 + ux * ux + uy * uy + t0 * t0 * un * un),
-                  pi=(pi_gev / hbarc).ravel(), Pi=Pi_gev / hbarc)
+        ut = (1.0 + ux * ux + uy * uy + un * un if reference_ut
+              else np.sqrt(1.0 + ux * ux + uy * uy + t0 * t0 * un * un))
+        ic.update(ux=ux, uy=uy, un=un, ut=ut, pi=(pi_gev / hbarc).ravel(), Pi=Pi_gev / hbarc)
     np.savetxt(tmp_path / "vectors.txt", rows, fmt="%.17g")
-    print(f"case hydro wrapper type={ic_type}", flush=True)
-    r = subprocess.run([HYDRO_DEMO, str(tmp_path / "vectors.txt")], cwd=str(tmp_path), capture_output=True, text=True,
-                       timeout=120)
+    print(f"case hydro wrapper type={ic_type} reference_ut={reference_ut}", flush=True)
+    r = subprocess.run([HYDRO_DEMO, str(tmp_path / "vectors.txt")] + (["--reference-ut"] if reference_ut else []),
+                       cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     steps = int(re.search(r"Steps taken: (\d+)", r.stdout).group(1))
     want, want_steps = _oracle_loop(lat, hyd, True, None, ic)
