@@ -566,7 +566,8 @@ vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e,
         *dst[k] = b->stage_buf + at * N;
         at += len[k];
     }
-    // the event's clock (HydroPlugin.cpp:226,307) with glibc pow(t,2)
+    // the event's clock (HydroPlugin.cpp:226,307): pow(t, 2) as the reference
+    // writes it (both compilers make it t * t, which is exact)
     StepTimes *tab = (StepTimes *)(H.h + tab_at);
     tab[0] = StepTimes{0, 0, 0, 0};
     for (size_t n = 1; n < tab_n; ++n) {

@@ -392,7 +392,10 @@ vh_status vh_batch_destroy(vh_batch *b);
 /* Admits one event (vh_upload_ic + vh_init_conserved at t0 for it, with
  * vh_upload_ic's NULL arrays): first-fit placement of its nx*ny*nz cells;
  * VH_ERR_CAPACITY when no free range or event slot is large enough.
- * Synchronising. */
+ * Returns once the arrays are copied (the caller may reuse them at once);
+ * the upload and the initial-state kernel are queued on the batch's stream
+ * without waiting, so every later call sees the event's state, and a
+ * device fault in them is reported by the next call that waits. */
 vh_status vh_batch_add(vh_batch *b, const vh_event_params *prm, const double *e, const double *p, const double *ut,
                        const double *ux, const double *uy, const double *un, const double *pi10, const double *Pi,
                        int *id_out);
