@@ -153,12 +153,26 @@ def _served_surfaces(budget):
     return stats, reqs, got
 
 
+def _cube_order_keys(rows, req, dx):
+    """Each row's cube (x, then y, then eta: the reference's loop order) from
+    its position: an element's centroid lies in its cube, which starts at
+    i*dx - (n - 1)*dx/2.  Rows on a cube face (ambiguous) are left out."""
+    dims = (req["nx"], req["ny"], req["nz"]) if req["nz"] > 1 else (req["nx"], req["ny"])
+    frac = [(rows[:, 1 + k] + (n - 1) / 2 * dx) / dx for k, n in enumerate(dims)]
+    inside = np.all([np.abs(f - np.round(f)) > 1e-9 for f in frac], axis=0)
+    key = np.zeros(rows.shape[0], dtype=np.int64)
+    for f, n in zip(frac, dims):
+        key = key * n + np.floor(f).astype(np.int64)
+    return key[inside], int(inside.sum())
+
+
 def test_served_surfaces_same_alone_and_in_the_mix():
     """Serving with surface collection: each request's surface elements (in
     surface.dat's columns, handed out at every check) are the same, bit for
     bit, whether all requests share the batch at once or wait for the memory
-    of earlier ones (room for about one 3+1D request); fo_elements counts
-    them."""
+    of earlier ones (room for about one 3+1D request), and each check's rows
+    come in the reference's cube order (x outermost, eta innermost); fo_elements
+    counts them."""
     per_cell = (3 * 15 + 3 * 4 + 3 * 4 + 4) * 4 + 2 * 8 + 20 * 8
     big = 21 * 21 * 11
     s_mix, r_mix, g_mix = _served_surfaces(64 * big * per_cell)
@@ -177,3 +191,9 @@ def test_served_surfaces_same_alone_and_in_the_mix():
         assert a["fo_elements"] == c["fo_elements"] == n > 0
         rows = np.concatenate(ra)
         assert np.all(np.isfinite(rows))
+        placed = 0
+        for x in ra:
+            keys, m = _cube_order_keys(x, a, SERVE_CFG["dx"])
+            assert np.all(np.diff(keys) >= 0), k
+            placed += m
+        assert placed > 0.99 * n, (k, placed, n)
