@@ -66,15 +66,23 @@ struct EvDesc {
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
-// and each XCD has its own L2.  Renumbering so that XCD k works through the
-// k-th contiguous eighth of the blocks keeps a block's stencil neighbours
-// (the blocks just before and after it) in the same L2.  A bijection on
-// [0, nblocks).
-constexpr int NUM_XCD = 8;
+// and each XCD has its own L2.  The renumbering hands each XCD runs of
+// consecutive blocks, the runs dealt round-robin: within a run a block's
+// stencil neighbours (the blocks just before and after it) share its L2, and
+// the interleaved runs spread a launch whose blocks cost unequally (the
+// fireball's root solves against the closed form outside it) evenly over the
+// XCDs, where one contiguous eighth per XCD leaves the XCD with the costliest
+// eighth last.  Runs are XCD_RUN blocks long, or an eighth of a smaller
+// launch.  A bijection on [0, nblocks): the blocks past the last whole round
+// of runs keep their order.
+constexpr int NUM_XCD = 8, XCD_RUN = 64;
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
-    const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
-    const int k = b % NUM_XCD, j = b / NUM_XCD;
-    return k < r ? k * (q + 1) + j : r * (q + 1) + (k - r) * q + j;
+    const int run = nblocks / NUM_XCD < XCD_RUN ? nblocks / NUM_XCD : XCD_RUN;
+    if (run == 0) return b;
+    const int round = NUM_XCD * run, full = nblocks / round * round;
+    if (b >= full) return b;
+    const int k = b % NUM_XCD, j = b / NUM_XCD;  // the j-th block XCD k receives
+    return (j / run * NUM_XCD + k) * run + j % run;
 }
 
 __device__ __forceinline__ void report_nan(int *err, int site, long s) {
