@@ -31,7 +31,8 @@ EXPORTS = ["vh_create", "vh_destroy", "vh_upload_ic", "vh_init_conserved", "vh_s
            "vh_status_string", "vh_build_info", "vh_ic_mc_glauber", "vh_ic_read_tmunu", "vh_ic_write_tmunu",
            "vh_prepare_steps", "vh_batch_create", "vh_batch_destroy", "vh_batch_add", "vh_batch_remove",
            "vh_batch_steps", "vh_batch_sync", "vh_batch_nan_info", "vh_batch_clear_error",
-           "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_fo_surface", "vh_batch_event_info",
+           "vh_batch_counts_above", "vh_batch_download", "vh_batch_fo_flag", "vh_batch_fo_flags", "vh_batch_fo_surface", "vh_batch_fo_surfaces",
+           "vh_batch_event_info",
            "vh_batch_free_cells", "vh_batch_last_ms", "vh_batch_eos_energy_density", "vh_batch_autotune",
            "vh_batch_set_tiles", "vh_batch_set_face_mode", "vh_serve_run", "vh_ic_mc_glauber_classic",
            "vh_surface_square", "vh_surface_cube", "vh_surface_hypercube",
@@ -180,6 +181,8 @@ def _lib():
         L.vh_batch_download.argtypes = [P, I, I, P]
         L.vh_batch_fo_flag.argtypes = [P, I, D, P, P, I64, ctypes.POINTER(I64)]
         L.vh_batch_fo_surface.argtypes = [P, I, P, P, P, P, I64, ctypes.POINTER(I64)]
+        L.vh_batch_fo_surfaces.argtypes = [P, I, P, P, P, P, P, P, P]
+        L.vh_batch_fo_flags.argtypes = [P, I, P, ctypes.c_double, P, P, P]
         L.vh_batch_event_info.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I64)]
         L.vh_batch_free_cells.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]
         L.vh_batch_last_ms.argtypes = [P]
@@ -491,6 +494,24 @@ class Batch:
             return ids[:n.value].copy(), corners[:n.value * per].reshape(n.value, per).copy()
         return ids[:n.value].copy()
 
+    def fo_flags(self, eids, eF):
+        """The cut cubes of several events in one pass (vh_batch_fo_flags):
+        each event's ids, in increasing order."""
+        n = len(eids)
+        ids = np.asarray(eids, dtype=np.int32)
+        bufs = []
+        for e in eids:
+            nx, ny, nz = self.events[e]
+            bufs.append(np.empty(max(1, (nx - 1) * (ny - 1) * (nz - 1 if nz > 1 else 1)), dtype=np.int64))
+        cap = np.array([x.size for x in bufs], dtype=np.int64)
+        nf = np.zeros(n, dtype=np.int64)
+        arr = (ctypes.c_void_p * max(n, 1))(*[x.ctypes.data for x in bufs])
+        Hydro._check(_lib().vh_batch_fo_flags(self.b, n, _ptr(ids), eF, arr, _ptr(cap), _ptr(nf)), "vh_batch_fo_flags")
+        self._fo_n = getattr(self, "_fo_n", {})
+        for e, k in zip(eids, nf):
+            self._fo_n[e] = int(k)
+        return [x[:k].copy() for x, k in zip(bufs, nf)]
+
     def fo_surface(self, eid, widths):
         """As Hydro.fo_surface for one event (batch created with fo_vars=True):
         (counts per flagged cube, elements (n, 2*dim), vars (n, 10))."""
@@ -510,6 +531,48 @@ class Batch:
                                                 ctypes.byref(total)), "vh_batch_fo_surface")
         n = total.value
         return counts[:nf].copy(), el[:n].copy(), var[:n].copy()
+
+    def fo_surfaces(self, eids, widths, caps=None):
+        """Several events' surfaces in one pass (vh_batch_fo_surfaces), each on
+        the cubes its last fo_flag_ids flagged: widths[k] the k-th event's cell
+        widths; caps[k] its room in elements (default: a first call with no
+        room gives every count).  Returns (status, totals, results) with
+        results[k] = (counts per flagged cube, elements (n, 2*dim), vars
+        (n, 10)), or None for an event whose elements exceed its room."""
+        n = len(eids)
+        ids = np.asarray(eids, dtype=np.int32)
+        w = np.zeros(4 * n, dtype=np.float64)
+        for k, wk in enumerate(widths):
+            w[4 * k:4 * k + len(wk)] = wk
+        nfs = [getattr(self, "_fo_n", {}).get(e, 0) for e in eids]
+        counts = [np.zeros(max(nf, 1), dtype=np.int32) for nf in nfs]
+        tot = np.zeros(n, dtype=np.int64)
+
+        def call(cap):
+            el = [np.empty(max(int(c), 1) * 8, dtype=np.float64) for c in cap]
+            var = [np.empty(max(int(c), 1) * 10, dtype=np.float64) for c in cap]
+            arr = lambda xs: (ctypes.c_void_p * max(n, 1))(*[x.ctypes.data for x in xs])
+            st = _lib().vh_batch_fo_surfaces(self.b, n, _ptr(ids), _ptr(w), arr(counts), arr(el), arr(var),
+                                             _ptr(cap), _ptr(tot))
+            if st not in (VH_OK, VH_ERR_CAPACITY):
+                Hydro._check(st, "vh_batch_fo_surfaces")
+            return st, el, var
+
+        if caps is None:
+            call(np.zeros(n, dtype=np.int64))
+            caps = tot.copy()
+        cap = np.asarray(caps, dtype=np.int64)
+        st, el, var = call(cap)
+        out = []
+        for k, e in enumerate(eids):
+            t = int(tot[k])
+            if t > cap[k]:
+                out.append(None)
+                continue
+            dim = 4 if self.events[e][2] > 1 else 3
+            out.append((counts[k][:nfs[k]].copy(), el[k][:t * 2 * dim].reshape(t, 2 * dim).copy(),
+                        var[k][:t * 10].reshape(t, 10).copy()))
+        return st, tot.copy(), out
 
     def info(self, eid):
         a, s, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()

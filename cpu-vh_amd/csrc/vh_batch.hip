@@ -885,10 +885,11 @@ vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest
     return VH_OK;
 }
 
-vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
-                           int64_t *n_flagged) {
-    if (!b || !n_flagged || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used || (cap > 0 && !ids))
-        return VH_ERR_ARG;
+// The device work of one event's flag call, queued on the batch's stream:
+// the slices move on (e and, with fo_vars, the ten surface variables) and the
+// cut cubes' ids go to the event's id buffer in increasing order, their count
+// to *count (device).
+static vh_status fo_flag_enqueue(vh_batch *b, int id, double eF, unsigned long long *count) {
     Event &E = b->ev[id];
     const vh_event_params &c = E.prm;
     if (c.nx < 2 || c.ny < 2) return VH_ERR_ARG;
@@ -934,9 +935,22 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
         }
         if (b->mixed) BCHK(launch_widen_set(W, E.N, b->stream));
     }
-    BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
-    BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, b->d_counts,
+    BCHK(launch_fo_flag(E.fo_prev, E.fo_cur, c.nx, c.ny, c.nz, eF, E.fo_ids, M, count,
                         (unsigned long long *)(E.fo_ids + std::max(1L, M)), b->stream));
+    return VH_OK;
+}
+
+vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
+                           int64_t *n_flagged) {
+    if (!b || !n_flagged || id < 0 || id >= (int)b->ev.size() || !b->ev[id].used || (cap > 0 && !ids))
+        return VH_ERR_ARG;
+    Event &E = b->ev[id];
+    const vh_event_params &c = E.prm;
+    if (c.nx < 2 || c.ny < 2) return VH_ERR_ARG;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long), b->stream));
+    vh_status fs = fo_flag_enqueue(b, id, eF, b->d_counts);
+    if (fs != VH_OK) return fs;
     unsigned long long n = 0;
     BCHK(hipMemcpyAsync(&n, b->d_counts, sizeof(n), hipMemcpyDeviceToHost, b->stream));
     vh_status s = vh_batch_sync(b);
@@ -964,6 +978,58 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
         if (le != hipSuccess) return bfail(le, "fo corners");
     }
     return VH_OK;
+}
+
+vh_status vh_batch_fo_flags(vh_batch *b, int n, const int *ids, double eF, int64_t *const *cube_ids,
+                            const int64_t *cap, int64_t *n_flagged) {
+    if (!b || n < 0 || n > (int)b->ev.size() || (n > 0 && (!ids || !cube_ids || !cap || !n_flagged)))
+        return VH_ERR_ARG;
+    for (int k = 0; k < n; ++k) {
+        const int id = ids[k];
+        if (id < 0 || id >= (int)b->ev.size() || !b->ev[id].used || cap[k] < 0 || (cap[k] > 0 && !cube_ids[k]))
+            return VH_ERR_ARG;
+        const vh_event_params &c = b->ev[id].prm;
+        if (c.nx < 2 || c.ny < 2) return VH_ERR_ARG;
+        for (int j = 0; j < k; ++j)
+            if (ids[j] == id) return VH_ERR_ARG;  // an event's slices move once per pass
+    }
+    if (n == 0) return VH_OK;
+    BCHK(hipSetDevice(b->dev));
+    BCHK(hipMemsetAsync(b->d_counts, 0, sizeof(unsigned long long) * n, b->stream));
+    for (int k = 0; k < n; ++k) {
+        vh_status fs = fo_flag_enqueue(b, ids[k], eF, b->d_counts + k);
+        if (fs != VH_OK) return fs;
+    }
+    std::vector<unsigned long long> cnt(n);
+    {
+        void *const dst[1] = {cnt.data()};
+        const void *const src[1] = {b->d_counts};
+        const size_t nb[1] = {sizeof(unsigned long long) * n};
+        BCHK(fo_download(b->fo_surf, b->stream, 1, dst, src, nb));
+    }
+    vh_status s = vh_batch_sync(b);  // a NaN or other device error reported by the kernels
+    if (s != VH_OK) return s;
+    std::vector<void *> dst;
+    std::vector<const void *> src;
+    std::vector<size_t> nb;
+    bool room = true;
+    for (int k = 0; k < n; ++k) {
+        Event &E = b->ev[ids[k]];
+        n_flagged[k] = (int64_t)cnt[k];
+        E.fo_n = (long long)cnt[k];
+        E.fo_eF = eF;
+        if ((int64_t)cnt[k] > cap[k]) {
+            room = false;
+            continue;
+        }
+        if (cnt[k] == 0) continue;
+        dst.push_back(cube_ids[k]);
+        src.push_back(E.fo_ids);
+        nb.push_back(sizeof(long long) * cnt[k]);
+    }
+    // the ids come out in increasing order
+    if (!dst.empty()) BCHK(fo_download(b->fo_surf, b->stream, (int)dst.size(), dst.data(), src.data(), nb.data()));
+    return room ? VH_OK : VH_ERR_CAPACITY;
 }
 
 vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_el, double *elements, double *vars,
@@ -1005,6 +1071,72 @@ vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_
     }
     *n_elements = total;
     return total > cap ? VH_ERR_CAPACITY : VH_OK;
+}
+
+vh_status vh_batch_fo_surfaces(vh_batch *b, int n, const int *ids, const double *widths, int *const *n_el,
+                               double *const *elements, double *const *vars, const int64_t *cap,
+                               int64_t *n_elements) {
+    if (!b || n < 0 || (n > 0 && (!ids || !widths || !n_el || !elements || !vars || !cap || !n_elements)))
+        return VH_ERR_ARG;
+    if (!b->cfg.fo_vars) {
+        vh::set_last_error("vh_batch_fo_surfaces: needs fo_vars in the batch's configuration");
+        return VH_ERR_ARG;
+    }
+    std::vector<FoItem> items(n);
+    std::vector<long long> total(n, 0);
+    for (int k = 0; k < n; ++k) {
+        const int id = ids[k];
+        if (id < 0 || id >= (int)b->ev.size() || !b->ev[id].used || cap[k] < 0 ||
+            (cap[k] > 0 && (!elements[k] || !vars[k])))
+            return VH_ERR_ARG;
+        for (int j = 0; j < k; ++j)
+            if (ids[j] == id) return VH_ERR_ARG;  // one pass builds an event's surface once
+        const Event &E = b->ev[id];
+        const vh_event_params &c = E.prm;
+        if (E.fo_n < 0) {
+            vh::set_last_error("vh_batch_fo_surfaces: an event without a vh_batch_fo_flag call");
+            return VH_ERR_ARG;
+        }
+        FoItem &I = items[k];
+        const int dim = c.nz > 1 ? 4 : 3;
+        for (int j = 0; j < dim; ++j) {
+            if (!(widths[4 * k + j] > 0)) return VH_ERR_ARG;
+            I.g.d[j] = widths[4 * k + j];
+        }
+        if (E.fo_n > 0 && !n_el[k]) return VH_ERR_ARG;
+        I.eprev = E.fo_prev;
+        I.ecur = E.fo_cur;
+        I.ids = E.fo_ids;
+        I.n = E.fo_n;
+        I.nx = c.nx;
+        I.ny = c.ny;
+        I.nz = c.nz;
+        I.eF = E.fo_eF;
+        for (int v = 0; v < FO_NVARS; ++v) {
+            I.S.v[1][v] = E.fo_vars + (E.fo_vslot * FO_NVARS + v) * E.N;
+            I.S.v[0][v] = E.fo_vars + ((E.fo_vslot ^ 1) * FO_NVARS + v) * E.N;
+        }
+        I.n_el = n_el[k];
+        I.elements = elements[k];
+        I.vars = vars[k];
+        I.cap = cap[k];
+        I.total = &total[k];
+    }
+    BCHK(hipSetDevice(b->dev));
+    int bad = 0;
+    // returns with the side stream drained; the flag calls that made its
+    // inputs drained `stream`
+    BCHK(fo_surface_many(b->fo_surf, b->side, items.data(), n, &bad));
+    if (bad) {
+        vh::set_last_error("vh_batch_fo_surfaces: a cut cube whose surface cannot be closed");
+        return VH_ERR_STATE;
+    }
+    bool room = true;
+    for (int k = 0; k < n; ++k) {
+        n_elements[k] = total[k];
+        room &= total[k] <= cap[k];
+    }
+    return room ? VH_OK : VH_ERR_CAPACITY;
 }
 
 double vh_batch_eos_energy_density(const vh_batch *b, double T) { return b ? b->heos.energy_density(T) : NAN; }

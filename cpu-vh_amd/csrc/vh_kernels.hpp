@@ -94,6 +94,10 @@ struct FoSurfaceWork {
     // staged by the runtime in synchronous chunks)
     char *host = nullptr;
     size_t host_bytes = 0;
+    // pinned staging of the pass's item table (uploaded; reused only after
+    // the pass has synchronised)
+    char *up = nullptr;
+    size_t up_bytes = 0;
 };
 // Device-to-host copies of n (src, bytes) pieces through W's pinned staging:
 // one synchronisation of st, then each piece copied to its destination.
@@ -102,6 +106,31 @@ hipError_t fo_download(FoSurfaceWork &W, hipStream_t st, int n, void *const *dst
 hipError_t fo_surface_run(FoSurfaceWork &W, hipStream_t st, const double *eprev, const double *ecur, int nx, int ny,
                           int nz, const long long *ids, long long n, double eF, const FoCell &g, const FoSlices &S,
                           int *n_el, double *elements, double *vars, long long cap, long long *total, int *bad);
+// One event's part of a surface pass over several events (fo_surface_many):
+// its coarse slices of e, its n flagged cubes ids (device), lattice, level,
+// cell widths and variable slices; the host arrays its results go to
+// (n_el: n counts; elements and vars: room for cap elements) and its element
+// count (*total).  cube0 is set by the pass (the event's first cube in it).
+struct FoItem {
+    const double *eprev = nullptr, *ecur = nullptr;
+    const long long *ids = nullptr;
+    long long n = 0;
+    int nx = 0, ny = 0, nz = 1;
+    double eF = 0;
+    FoCell g{};
+    FoSlices S{};
+    long long cube0 = 0;
+    int *n_el = nullptr;
+    double *elements = nullptr, *vars = nullptr;
+    long long cap = 0;
+    long long *total = nullptr;
+};
+// The surfaces of several events in one pass: one element launch over all
+// their cubes, one scan, two synchronisations.  Every item's *total and n_el
+// are set; the elements and variables of each item with room (total <= cap)
+// are copied, unless a cube failed (*bad: nothing copied).  Items with n = 0
+// are skipped (total 0).
+hipError_t fo_surface_many(FoSurfaceWork &W, hipStream_t st, FoItem *items, int n_items, int *bad);
 void fo_surface_free(FoSurfaceWork &W);
 
 // Thermal vorticity (calculateThermalVorticity): u and up the step's and the

@@ -133,30 +133,67 @@ class HostPool {
     bool quit_ = false;
 };
 
-// One request's surface at a check (vh_batch_fo_surface), as surface.dat's
-// 20 columns per element (FreezeOut.cpp:200-361): cubes in the reference's
-// order (x outer, eta inner), positions from the cube's origin, dsigma =
-// t * normal, T and P from the host EoS at the interpolated e; steps is the
-// request's step count at the check.
-vh_status surface_rows(vh_batch *b, int id, const vh_serve_request &r, const vh_serve_config &c,
-                       const vh::HostEos &heos, int steps, const std::vector<int64_t> &ids, int64_t nf,
-                       std::vector<int> &ne, std::vector<double> &el, std::vector<double> &var,
-                       std::vector<double> &rows, std::vector<int64_t> &order, std::vector<int64_t> &bucket,
-                       HostPool &pool) {
+// The surfaces of the requests in batch slots sel at one check, in one pass
+// (vh_batch_fo_surfaces): each slot's element counts ne, elements el,
+// variables var and element count tot.  A slot whose arrays were too small
+// gets larger ones and is built again in a second pass.
+vh_status surface_pass(vh_batch *b, const std::vector<int> &sel, const vh_serve_config &c,
+                       const std::vector<int64_t> &nflag, std::vector<std::vector<int>> &ne,
+                       std::vector<std::vector<double>> &el, std::vector<std::vector<double>> &var,
+                       std::vector<int64_t> &tot) {
+    std::vector<int> ids(sel), again;
+    for (int pass = 0; pass < 2 && !ids.empty(); ++pass) {
+        const size_t n = ids.size();
+        std::vector<double> widths(4 * n);
+        std::vector<int *> pn(n);
+        std::vector<double *> pe(n), pv(n);
+        std::vector<int64_t> cap(n), got(n);
+        for (size_t k = 0; k < n; ++k) {
+            const int id = ids[k];
+            widths[4 * k] = c.dt * c.coarse;
+            widths[4 * k + 1] = widths[4 * k + 2] = widths[4 * k + 3] = c.dx;
+            ne[id].resize((size_t)nflag[id]);
+            if (pass == 0 && var[id].size() < (size_t)nflag[id] * 20) {  // about one element a cube
+                el[id].resize((size_t)nflag[id] * 16);
+                var[id].resize((size_t)nflag[id] * 20);
+            }
+            pn[k] = ne[id].data();
+            pe[k] = el[id].data();
+            pv[k] = var[id].data();
+            cap[k] = (int64_t)(var[id].size() / 10);  // el holds 8 doubles an element
+        }
+        vh_status s = vh_batch_fo_surfaces(b, (int)n, ids.data(), widths.data(), pn.data(), pe.data(), pv.data(),
+                                           cap.data(), got.data());
+        if (s != VH_OK && s != VH_ERR_CAPACITY) return s;
+        again.clear();
+        for (size_t k = 0; k < n; ++k) {
+            const int id = ids[k];
+            tot[id] = got[k];
+            if (got[k] > cap[k]) {
+                el[id].resize((size_t)got[k] * 8);
+                var[id].resize((size_t)got[k] * 10);
+                again.push_back(id);
+            }
+        }
+        ids.swap(again);
+    }
+    return ids.empty() ? VH_OK : VH_ERR_STATE;
+}
+
+// One request's surface at a check as surface.dat's 20 columns per element
+// (FreezeOut.cpp:200-361): cubes in the reference's order (x outer, eta
+// inner), positions from the cube's origin, dsigma = t * normal, T and P from
+// the host EoS at the interpolated e; steps is the request's step count at
+// the check; ids, ne, el, var and total as the flag and surface_pass left
+// them.
+void surface_rows(const vh_serve_request &r, const vh_serve_config &c, const vh::HostEos &heos, int steps,
+                  const std::vector<int64_t> &ids, int64_t nf, const std::vector<int> &ne,
+                  const std::vector<double> &el, const std::vector<double> &var, int64_t total,
+                  std::vector<double> &rows, std::vector<int64_t> &order, std::vector<int64_t> &bucket,
+                  HostPool &pool) {
     const bool d4 = r.nz > 1;
     const int dim = d4 ? 4 : 3, per = 2 * dim;
     const double t = c.t0 + steps * c.dt, cell_tau = t + c.dt;
-    const double widths[4] = {c.dt * c.coarse, c.dx, c.dx, c.dx};
-    ne.resize(nf);
-    int64_t total = 0;
-    vh_status s = vh_batch_fo_surface(b, id, widths, ne.data(), el.data(), var.data(), (int64_t)(var.size() / 10),
-                                      &total);
-    if (s == VH_ERR_CAPACITY) {
-        el.resize((size_t)total * per);
-        var.resize((size_t)total * 10);
-        s = vh_batch_fo_surface(b, id, widths, ne.data(), el.data(), var.data(), total, &total);
-    }
-    if (s != VH_OK) return s;
     // The cubes come flagged in increasing id (x fastest); the reference lists
     // them x outermost, then y, then eta.  A stable counting sort on
     // ix * cy + iy gives that order: within one (ix, iy) the ids increase
@@ -213,7 +250,6 @@ vh_status surface_rows(vh_batch *b, int id, const vh_serve_request &r, const vh_
             }
         }
     });
-    return VH_OK;
 }
 
 double quantile(std::vector<double> v, double q) {  // nearest rank
@@ -298,8 +334,10 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
     std::vector<std::vector<int64_t>> fo_ids(c.max_resident);
     std::vector<int64_t> fo_nflag(c.max_resident, 0);
     std::vector<int> fo_steps(c.max_resident, 0), leaving(c.max_resident, 0);
-    std::vector<int> fo_ne;
-    std::vector<double> fo_el, fo_var, fo_rows;
+    std::vector<std::vector<int>> fo_ne(c.max_resident);
+    std::vector<std::vector<double>> fo_el(c.max_resident), fo_var(c.max_resident);
+    std::vector<int64_t> fo_tot(c.max_resident, 0);
+    std::vector<double> fo_rows;
     std::vector<int64_t> fo_order, fo_bucket;
     // threads building the surface rows
     HostPool pool(c.collect_surface == 2 ? (int)std::min(7u, std::max(1u, std::thread::hardware_concurrency() / 2)) : 0);
@@ -400,6 +438,7 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
         if (st != VH_OK) break;
         // flags and the stop rule; a request admitted while the last steps
         // were in flight has not stepped yet and has its first check next time
+        std::vector<int> chk;
         for (int id = 0; id < c.max_resident; ++id) {
             fo_nflag[id] = 0;
             leaving[id] = 0;
@@ -407,40 +446,55 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             int active = 0, steps = 0;
             vh_batch_event_info(b, id, &active, &steps, nullptr);
             if (steps == 0) continue;
-            if (c.collect_surface) {
-                vh_serve_request &r = req[owner[id]];
+            fo_steps[id] = steps;
+            chk.push_back(id);
+        }
+        if (c.collect_surface && !chk.empty()) {
+            // every checked request's cut cubes in one pass
+            const size_t n = chk.size();
+            std::vector<int64_t *> pid(n, nullptr);
+            std::vector<int64_t> cap(n, 0), nflag(n, 0);
+            for (size_t k = 0; k < n && c.collect_surface == 2; ++k) {
+                const vh_serve_request &r = req[owner[chk[k]]];
                 const int64_t ncube = (int64_t)(r.nx - 1) * (r.ny - 1) * (r.nz > 1 ? r.nz - 1 : 1);
-                std::vector<int64_t> &ids = fo_ids[id];
-                if (c.collect_surface == 2 && (int64_t)ids.size() < ncube) ids.resize(ncube);
-                int64_t nflag = 0;
-                vh_status fs = vh_batch_fo_flag(b, id, eF, c.collect_surface == 2 ? ids.data() : nullptr, nullptr,
-                                                c.collect_surface == 2 ? ncube : 0, &nflag);
-                if (fs != VH_OK && fs != VH_ERR_CAPACITY) {
-                    st = fs;
-                    break;
-                }
-                r.fo_cubes += nflag;
-                if (c.collect_surface == 2) fo_nflag[id] = nflag;
-                fo_steps[id] = steps;
+                std::vector<int64_t> &ids = fo_ids[chk[k]];
+                if ((int64_t)ids.size() < ncube) ids.resize(ncube);
+                pid[k] = ids.data();
+                cap[k] = ncube;
             }
+            vh_status fs = vh_batch_fo_flags(b, (int)n, chk.data(), eF, pid.data(), cap.data(), nflag.data());
+            if (fs != VH_OK && fs != VH_ERR_CAPACITY) {
+                st = fs;
+                break;
+            }
+            for (size_t k = 0; k < n; ++k) {
+                req[owner[chk[k]]].fo_cubes += nflag[k];
+                if (c.collect_surface == 2) fo_nflag[chk[k]] = nflag[k];
+            }
+        }
+        for (int id : chk) {
             if (counts[id] == 0) ++strikes[id];
-            leaving[id] = strikes[id] >= 3 || steps + c.coarse > c.max_steps;
+            leaving[id] = strikes[id] >= 3 || fo_steps[id] + c.coarse > c.max_steps;
         }
         if (st != VH_OK) break;
         // surfaces: the leaving requests' before their events leave the batch,
         // then the next check's steps go in flight while the others' are built
-        // (vh_batch_fo_surface works from the flag's copies, on a stream of
+        // (vh_batch_fo_surfaces works from the flag's copies, on a stream of
         // its own)
         auto surfaces = [&](bool of_leaving) {
-            for (int id = 0; id < c.max_resident && st == VH_OK; ++id) {
-                if (owner[id] < 0 || fo_nflag[id] <= 0 || (leaving[id] != 0) != of_leaving) continue;
+            std::vector<int> sel;
+            for (int id = 0; id < c.max_resident; ++id)
+                if (owner[id] >= 0 && fo_nflag[id] > 0 && (leaving[id] != 0) == of_leaving) sel.push_back(id);
+            if (sel.empty()) return;
+            vh_status fs = surface_pass(b, sel, c, fo_nflag, fo_ne, fo_el, fo_var, fo_tot);
+            if (fs != VH_OK) {
+                st = fs;
+                return;
+            }
+            for (int id : sel) {
                 vh_serve_request &r = req[owner[id]];
-                vh_status fs = surface_rows(b, id, r, c, heos, fo_steps[id], fo_ids[id], fo_nflag[id], fo_ne, fo_el,
-                                            fo_var, fo_rows, fo_order, fo_bucket, pool);
-                if (fs != VH_OK) {
-                    st = fs;
-                    break;
-                }
+                surface_rows(r, c, heos, fo_steps[id], fo_ids[id], fo_nflag[id], fo_ne[id], fo_el[id], fo_var[id],
+                             fo_tot[id], fo_rows, fo_order, fo_bucket, pool);
                 const int64_t ne = (int64_t)(fo_rows.size() / 20);
                 r.fo_elements += ne;
                 if (c.on_surface && ne) c.on_surface(c.user, owner[id], ne, fo_rows.data());

@@ -428,6 +428,16 @@ vh_status vh_batch_run(vh_batch *b, double eF, int coarse, int nchecks, int32_t 
 vh_status vh_batch_download(vh_batch *b, int id, int field, double *out);
 vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double *corners, int64_t cap,
                            int64_t *n_flagged);
+/* vh_batch_fo_flag for several events in one pass (ids[0..n), distinct;
+ * no corners): every event's flag kernels queued, then one download of the
+ * counts and one of the ids, so two synchronisations in all.  cube_ids[k]
+ * receives event k's cut cubes' ids when they fit its cap[k];
+ * n_flagged[k] their count.  VH_ERR_CAPACITY when an event's cubes exceed
+ * its cap (every count is set, every event's slices have moved and its
+ * surface can be built; only its ids are not copied).  No counterpart in
+ * the reference (one event, one check). */
+vh_status vh_batch_fo_flags(vh_batch *b, int n, const int *ids, double eF, int64_t *const *cube_ids,
+                            const int64_t *cap, int64_t *n_flagged);
 /* As vh_fo_surface for one event of a batch created with fo_vars = 1, on the
  * cubes its last vh_batch_fo_flag call flagged (mixed-precision events'
  * fields widened to FP64 first).  It reads only the copies that flag call
@@ -436,6 +446,19 @@ vh_status vh_batch_fo_flag(vh_batch *b, int id, double eF, int64_t *ids, double 
  * vh_batch_fo_flag must come after it returns. */
 vh_status vh_batch_fo_surface(vh_batch *b, int id, const double *widths, int *n_el, double *elements, double *vars,
                               int64_t cap, int64_t *n_elements);
+/* Surfaces of several events of such a batch in one pass: as
+ * vh_batch_fo_surface for each of ids[0..n) (distinct), with one element
+ * launch over all their cut cubes and two synchronisations in all, on the
+ * same stream of its own.  Event k: widths[4k..4k+3] its cell widths (the
+ * fourth ignored for nz = 1), n_el[k] room for its flag count, elements[k]
+ * and vars[k] room for cap[k] elements, n_elements[k] its element count.
+ * VH_ERR_CAPACITY when an event's elements exceed its cap: every count is
+ * set and the events with room have their elements; a second call for the
+ * others, with larger arrays, builds theirs.  No counterpart in the
+ * reference (one event, one surface call). */
+vh_status vh_batch_fo_surfaces(vh_batch *b, int n, const int *ids, const double *widths, int *const *n_el,
+                               double *const *elements, double *const *vars, const int64_t *cap,
+                               int64_t *n_elements);
 vh_status vh_batch_event_info(vh_batch *b, int id, int *active, int *steps_done, int64_t *cells);
 vh_status vh_batch_free_cells(vh_batch *b, int64_t *free_cells, int64_t *largest_range);
 /* Mixed form: times a stage launch of each tile variant in turn, reps rounds,

@@ -106,6 +106,55 @@ def test_batch_surface_equals_single_event():
         h.close()
 
 
+def _same(a, b):
+    return np.array_equal(a[0], b[0]) and V.bits_equal(a[1], b[1]) and V.bits_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("prec", [C.PREC_FP64, C.PREC_MIXED])
+def test_batch_flags_and_surfaces_in_one_pass_equal_per_event(prec):
+    """vh_batch_fo_flags and vh_batch_fo_surfaces: the cut cubes and surfaces
+    of several events (2+1D and 3+1D together) found in one pass each are,
+    bit for bit, those of the same events in a twin batch flagged and built
+    one event at a time; an event given too little room for its elements
+    reports its count and gets none while the others get theirs, and a call
+    for it alone then builds it."""
+    table = V.eos_table()
+    ics = [_event_ic(nx, ny, nz, 70 + k) for k, (nx, ny, nz) in enumerate(BATCH_EVENTS)]
+    cells = sum(nx * ny * nz for nx, ny, nz in BATCH_EVENTS)
+    twins = [C.Batch(cells, len(BATCH_EVENTS), 64, bulk=True, eos_table=table, precision=prec, fo_vars=True)
+             for _ in range(2)]
+    ids = [[t.add(nx, ny, nz, 0.1, 0.1, 0.1, 0.02, 0.6, 0.1, ic, T_reg_GeV=0.1)
+            for (nx, ny, nz), ic in zip(BATCH_EVENTS, ics)] for t in twins]
+    b, one = twins
+    widths = [np.array([0.04, 0.1, 0.1, 0.1])[:4 if nz > 1 else 3] for nx, ny, nz in BATCH_EVENTS]
+    eF = b.eos_energy_density(0.15 / V.HBARC)
+    checked = 0
+    for k in range(3):
+        if k:
+            for t in twins:
+                t.steps(2)
+        print(f"case flags and surfaces in one pass, precision {prec}, check {k}", flush=True)
+        flagged = b.fo_flags(ids[0], eF)
+        for f, eid in zip(flagged, ids[1]):
+            assert np.array_equal(f, one.fo_flag_ids(eid, eF))
+        st, tot, res = b.fo_surfaces(ids[0], widths)
+        assert st == C.VH_OK
+        for eid, w, r in zip(ids[1], widths, res):
+            assert _same(r, one.fo_surface(eid, w))
+            checked += r[1].shape[0]
+        if tot[1] > 0:
+            caps = tot.copy()
+            caps[1] -= 1
+            st2, tot2, res2 = b.fo_surfaces(ids[0], widths, caps)
+            assert st2 == C.VH_ERR_CAPACITY and res2[1] is None and np.array_equal(tot2, tot)
+            assert _same(res2[0], res[0]) and _same(res2[2], res[2])
+            st3, _, res3 = b.fo_surfaces([ids[0][1]], [widths[1]])
+            assert st3 == C.VH_OK and _same(res3[0], res[1])
+    assert checked > 100
+    for t in twins:
+        t.close()
+
+
 def test_mixed_batch_elements_from_its_corners():
     """Mixed-precision events: the device's elements are the host
     construction's on the corner energies the batch flags with (its FP32 e
