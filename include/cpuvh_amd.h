@@ -353,7 +353,8 @@ vh_status vh_prepare_steps(vh_handle *h, int count);
  * clock, eta/s and T_reg, and vh_batch_steps advances every resident event by
  * the same number of steps (two launches per RK stage: all 2+1D events, then
  * all 3+1D events).  Results of an event are bit-identical to running it
- * alone through vh_handle.  bulk and the EoS are per batch. */
+ * alone through vh_handle.  bulk and the EoS are per batch.  One host thread
+ * at a time per batch: its calls share staging buffers and streams. */
 /* Arithmetic of a batch.  VH_PREC_FP64: the drop-in path (each event
  * bit-identical to vh_handle).  VH_PREC_MIXED: FP32 storage (with an FP32
  * low part kept for T^{tau mu}) and FP32 slopes, face fluxes, source terms
