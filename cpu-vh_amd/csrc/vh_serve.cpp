@@ -364,6 +364,12 @@ extern "C" vh_status vh_serve_run(const vh_serve_config *cfg, const double *eos_
             if (c.digest) r.digest = state_digest(b, id, c.bulk ? 15 : 14, cells, ebuf);
         }
         vh_batch_remove(b, id);
+        // the slot's surface arrays leave with the request (a large 3+1D
+        // request's first check alone holds millions of elements)
+        std::vector<int64_t>().swap(fo_ids[id]);
+        std::vector<int>().swap(fo_ne[id]);
+        std::vector<double>().swap(fo_el[id]);
+        std::vector<double>().swap(fo_var[id]);
         r.done_s = now_s();
         owner[id] = -1;
         --resident;
